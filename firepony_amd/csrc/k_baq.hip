@@ -1,0 +1,478 @@
+/*
+ * K_baq: the BAQ glocal banded HMM (device/baq.cu:214-697) plus MAP quality,
+ * capping and BQ-tag recoding (baq.cu:842-1081).
+ *
+ * One lane per BAQ read.  The band of the current row lives in registers in
+ * band-relative coordinates r = k - i + bw, which makes every dependency a
+ * fixed offset: M[i][r] <- row i-1 at r, I[i][r] <- row i-1 at r+1,
+ * D[i][r] <- row i at r-1 (forward), mirrored for backward.  The forward pass
+ * spills each scaled row's M and I states to a per-wave scratch slot,
+ * interleaved by lane so every store/load moves 64 consecutive doubles; the
+ * backward pass recomputes nothing and consumes those rows in reverse for the
+ * MAP step, so the backward matrix is never stored.
+ *
+ * Arithmetic is FP64 with the reference's operation order and association
+ * (this file must be compiled with -ffp-contract=off).  qual2prob and the gap
+ * open probability come from host tables computed with the host libm; the MAP
+ * quality (int)(-4.343*log(1-max)+.499) is evaluated through host-computed
+ * thresholds on 1-max, which reproduces the host libm bit for bit.
+ */
+#include "fp_device.h"
+
+namespace fpd {
+
+__constant__ double c_qual2prob[256];
+__constant__ double c_map_thr[128];     // thresholds T[v], v = 5..101
+__constant__ double c_gap_open;
+
+static constexpr double EM = 0.33333333333;
+static constexpr double EI = 0.25;
+static constexpr double GEP = 0.1;
+
+// MAP quality from y = 1 - max/sum (baq.cu:673-674)
+__device__ inline uint8_t map_quality(double y)
+{
+    if (y == 0.0) return 4;                 // log(0): (int)+inf is INT_MIN on the host
+    if (!(y <= c_map_thr[5])) return 4;     // k <= 4
+    // largest v in [5, 101] with y <= T[v] (T is decreasing)
+    int lo = 5, hi = 101;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (y <= c_map_thr[mid]) lo = mid; else hi = mid - 1;
+    }
+    if (lo >= 101) return 99;
+    return (uint8_t)lo;
+}
+
+struct BaqArgs {
+    BatchDev b;
+    GenomeDev g;
+    ReadStateDev s;
+    const uint32_t *list;     // BAQ read ids of this class
+    const uint32_t *count;    // number of ids
+    double *scratch;          // per-wave slots
+    uint64_t slot_doubles;    // doubles per wave slot
+    uint32_t rows_cap;        // >= max query length
+    uint32_t wmax_runtime;    // band cells for the generic kernel
+};
+
+template <int WMAX>
+__global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
+{
+    constexpr bool REG = WMAX > 0;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = blockIdx.x;
+    const uint32_t nwaves = gridDim.x;
+    const uint32_t total = *a.count;
+    const int W = REG ? WMAX : (int)a.wmax_runtime;
+
+    double *slot = a.scratch + (uint64_t)wave * a.slot_doubles;
+    double *Fst = slot;                                              // rows x W x 2 x 64
+    double *Sst = slot + (uint64_t)(a.rows_cap + 1) * W * 2 * 64;    // (rows+2) x 64
+    double *Gst = Sst + (uint64_t)(a.rows_cap + 2) * 64;             // generic rows: 3 x W x 64
+
+    double Mr[REG ? WMAX : 1], Ir[REG ? WMAX : 1], Dr[REG ? WMAX : 1];
+
+    auto M = [&](int r) -> double & { if constexpr (REG) return Mr[r]; else return Gst[(0 * W + r) * 64 + lane]; };
+    auto I = [&](int r) -> double & { if constexpr (REG) return Ir[r]; else return Gst[(1 * W + r) * 64 + lane]; };
+    auto D = [&](int r) -> double & { if constexpr (REG) return Dr[r]; else return Gst[(2 * W + r) * 64 + lane]; };
+    auto F = [&](int i, int r, int st) -> double & { return Fst[(((uint64_t)i * W + r) * 2 + st) * 64 + lane]; };
+    auto S = [&](int i) -> double & { return Sst[(uint64_t)i * 64 + lane]; };
+
+    for (uint32_t chunk = wave; (uint64_t)chunk * 64 < total; chunk += nwaves) {
+        const uint32_t idx = chunk * 64 + lane;
+        if (idx >= total) continue;
+        const uint32_t rd = a.list[idx];
+        const BatchDev &b = a.b;
+
+        // ---- setup (baq.cu:260-354)
+        const uint32_t hw = a.s.hmmw[rd];
+        const int hx = (int16_t)lo16(hw), hy = (int16_t)hi16(hw);
+        const uint32_t rwv = a.s.rwc[rd];
+        const int qStart = lo16(rwv);
+        const int qLen = (int)hi16(rwv) - qStart + 1;
+        const int bw = a.s.bandwidth[rd];
+        const uint32_t astart = b.alignment_start[rd];
+        const uint32_t chrom = b.chromosome[rd];
+        const int refLenW = hy - hx + 1;
+        const uint32_t ahx = hx < 0 ? (uint32_t)(-hx) : (uint32_t)hx;
+        const bool clamp = hx < 0 && astart < ahx;
+        const int refLen = clamp ? refLenW - 1 : refLenW;
+        const uint32_t hoff = clamp ? 0u : (uint32_t)(astart + hx);
+
+        const double GOP = c_gap_open;
+        const double sM = 1.0 / (2 * qLen + 2);
+        const double sI = sM;
+        const double bM = (1 - GOP) / refLenW;
+        const double bI = GOP / refLenW;
+        const double m0 = (1 - GOP - GOP) * (1 - sM);
+        const double m1 = GOP * (1 - sM);
+        const double m2 = m1;
+        const double m3 = (1 - GEP) * (1 - sI);
+        const double m4 = GEP * (1 - sI);
+        const double m6 = 1 - GEP;
+        const double m8 = GEP;
+        const double EIm1 = EI * m1;
+        const double EIm4 = EI * m4;
+
+        const uint64_t qbase = (uint64_t)b.read_start[rd] + qStart;
+        const uint32_t qqual = b.qual_start[rd] + qStart;
+        auto refbase = [&](int idx0) -> uint8_t {   // 0-based index into the HMM window
+            if (idx0 < 0 || idx0 >= refLen) return B_N;
+            return contig_base(a.g, chrom, hoff + (uint32_t)idx0);
+        };
+
+        // ---- forward row 1 (baq.cu:396-442)
+        {
+            const uint8_t qb = read_base(b.reads, qbase);
+            const uint8_t qq = b.quals[qqual];
+            const double p = c_qual2prob[qq < 4 ? 4 : qq];
+            const double em_eq = 1 - p, em_ne = p * EM;
+            const int khi = refLen < bw + 1 ? refLen : bw + 1;
+            double sum = 0.0;
+#pragma unroll
+            for (int r = 0; r < (REG ? WMAX : 1) * 0 + W; r++) {
+                const int k = 1 - bw + r;
+                const bool in = k >= 1 && k <= khi;
+                double mv = 0.0, iv = 0.0;
+                if (in) {
+                    const uint8_t rb = refbase(k - 1);
+                    const double e = (rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne);
+                    mv = e * bM;
+                    iv = EI * bI;
+                    sum += mv + iv;
+                }
+                M(r) = mv; I(r) = iv; D(r) = 0.0;
+            }
+            S(0) = 1.0;
+            S(1) = sum;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                M(r) = M(r) / sum; I(r) = I(r) / sum; D(r) = D(r) / sum;
+                F(1, r, 0) = M(r); F(1, r, 1) = I(r);
+            }
+        }
+        // ---- forward rows 2..qLen (baq.cu:444-512)
+        for (int i = 2; i <= qLen; i++) {
+            const uint8_t qb = read_base(b.reads, qbase + i - 1);
+            const uint8_t qq = b.quals[qqual + i - 1];
+            const double p = c_qual2prob[qq < 4 ? 4 : qq];
+            const double em_eq = 1 - p, em_ne = p * EM;
+            const int klo = 1 > i - bw ? 1 : i - bw;
+            const int khi = refLen < i + bw ? refLen : i + bw;
+            double sum = 0.0, cM = 0.0, cD = 0.0;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = i - bw + r;
+                const bool in = k >= klo && k <= khi;
+                const double pM = M(r), pI = I(r), pD = D(r);
+                const double nM = (r + 1 < W) ? M(r + 1) : 0.0;
+                const double nI = (r + 1 < W) ? I(r + 1) : 0.0;
+                double mv = 0.0, iv = 0.0, dv = 0.0;
+                if (in) {
+                    const uint8_t rb = refbase(k - 1);
+                    const double e = (rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne);
+                    mv = e * (m0 * pM + m3 * pI + m6 * pD);
+                    iv = EI * (m1 * nM + m4 * nI);
+                    dv = m2 * cM + m8 * cD;
+                    sum += mv + iv + dv;
+                }
+                M(r) = mv; I(r) = iv; D(r) = dv;
+                cM = mv; cD = dv;
+            }
+            S(i) = sum;
+            const double inv = 1. / sum;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                M(r) = M(r) * inv; I(r) = I(r) * inv; D(r) = D(r) * inv;
+                F(i, r, 0) = M(r); F(i, r, 1) = I(r);
+            }
+        }
+        // ---- s[qLen+1] (baq.cu:514-525)
+        {
+            double sum = 0.0;
+            const int klo = 1 > qLen - bw ? 1 : qLen - bw;
+            const int khi = refLen < qLen + bw ? refLen : qLen + bw;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = qLen - bw + r;
+                if (k >= klo && k <= khi) sum += M(r) * sM + I(r) * sI;
+            }
+            S(qLen + 1) = sum;
+        }
+
+        // ---- backward + MAP (baq.cu:528-681)
+        const uint32_t qout = qqual;
+        auto map_row = [&](int i) {
+            double sum = 0.0, mx = 0.0;
+            int32_t st = -1;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = i - bw + r;
+                double z = F(i, r, 0) * M(r);
+                sum += z;
+                if (z > mx) { mx = z; st = (k - 1) << 2 | 0; }
+                z = F(i, r, 1) * I(r);
+                sum += z;
+                if (z > mx) { mx = z; st = (k - 1) << 2 | 1; }
+            }
+            mx /= sum;
+            a.s.baq[qout + i - 1] = map_quality(1.0 - mx);
+            a.s.baq_state[qout + i - 1] = (uint32_t)st;
+        };
+        {
+            const double sq = S(qLen), sq1 = S(qLen + 1);
+            const int klo = 1 > qLen - bw ? 1 : qLen - bw;
+            const int khi = refLen < qLen + bw ? refLen : qLen + bw;
+            const double vM = sM / sq / sq1, vI = sI / sq / sq1;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = qLen - bw + r;
+                const bool in = k >= klo && k <= khi;
+                M(r) = in ? vM : 0.0; I(r) = in ? vI : 0.0; D(r) = 0.0;
+            }
+            map_row(qLen);
+        }
+        for (int i = qLen - 1; i >= 1; i--) {
+            const double y = (i > 1) ? 1. : 0.;
+            const uint8_t qb = read_base(b.reads, qbase + i);
+            const uint8_t qq = b.quals[qqual + i];
+            const double p = c_qual2prob[qq < 4 ? 4 : qq];
+            const double em_eq = 1 - p, em_ne = p * EM;
+            const int klo = 1 > i - bw ? 1 : i - bw;
+            const int khi = refLen < i + bw ? refLen : i + bw;
+            double cD = 0.0;
+#pragma unroll
+            for (int r = W - 1; r >= 0; r--) {
+                const int k = i - bw + r;
+                const bool in = k >= klo && k <= khi;
+                const double pM = M(r);
+                const double pIl = r >= 1 ? I(r - 1) : 0.0;
+                double mv = 0.0, iv = 0.0, dv = 0.0;
+                if (in) {
+                    double e;
+                    if (k >= refLen) e = 0;
+                    else {
+                        const uint8_t rb = refbase(k);
+                        e = ((rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne)) * pM;
+                    }
+                    mv = e * m0 + EIm1 * pIl + m2 * cD;
+                    iv = e * m3 + EIm4 * pIl;
+                    dv = (e * m6 + m8 * cD) * y;
+                }
+                M(r) = mv; I(r) = iv; D(r) = dv;
+                cD = dv;
+            }
+            const double yy = 1.0 / S(i);
+#pragma unroll
+            for (int r = 0; r < W; r++) { M(r) = M(r) * yy; I(r) = I(r) * yy; D(r) = D(r) * yy; }
+            map_row(i);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// flat BAQ, capping and BQ-tag recoding, one lane per read
+// (read_flat_baq baq.cu:842-868, cap_baq_qualities :870-1056, recode :1058-1081)
+
+struct PostArgs {
+    BatchDev b;
+    ReadStateDev s;
+};
+
+__device__ inline uint32_t ev_type_of(uint32_t op) { return op_event(op); }
+
+__global__ void __launch_bounds__(256) k_baq_post(PostArgs a)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const BatchDev &b = a.b;
+    if (r >= b.num_reads) return;
+    if (a.s.status[r] != RS_ACTIVE) return;
+    const uint32_t rw = a.s.rwc[r];
+    const uint16_t rwx = lo16(rw), rwy = hi16(rw);
+    const uint32_t qs = b.qual_start[r];
+    uint8_t *baq = a.s.baq;
+    if (a.s.num_errors[r] == 0) {
+        for (uint32_t i = rwx; i <= rwy; i++) baq[qs + i] = 64;
+        return;
+    }
+    const uint32_t *ops = b.cigars + b.cigar_start[r];
+    const uint32_t nops = b.cigar_len[r];
+    const uint32_t hw = a.s.hmmw[r];
+    const int hx = (int16_t)lo16(hw);
+    const uint16_t fwx = lo16(a.s.refw[r]);
+    const uint32_t astart = b.alignment_start[r];
+
+    // walk the events once to find baq_start, then run the check and update passes
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < nops; i++) total += op_len(ops[i]);
+
+    int16_t reference_shift, read_shift;
+    const uint32_t ahx = hx < 0 ? (uint32_t)(-hx) : (uint32_t)hx;
+    if (hx < 0 && astart < ahx) { reference_shift = (int16_t)ahx; read_shift = (int16_t)astart; }
+    else { reference_shift = 0; read_shift = 0; }
+    const int16_t refOffset = (int16_t)(hx - (int)fwx + reference_shift);
+
+    // pass 0: baq_start
+    uint32_t baq_start = 0;
+    {
+        uint32_t e = 0;
+        uint16_t ro = 0;
+        bool done = false;
+        for (uint32_t i = 0; i < nops && !done; i++) {
+            const uint32_t op = op_code(ops[i]), l = op_len(ops[i]);
+            if (op_reads(op)) {
+                for (uint32_t k = 0; k < l; k++) {
+                    const uint16_t rc = (uint16_t)(ro + k);
+                    if (rc >= rwx) { baq_start = e + k; done = true; break; }
+                }
+                ro = (uint16_t)(ro + l);
+            }
+            e += l;
+        }
+    }
+    bool good = true;
+    for (int pass = 0; pass < 2 && good; pass++) {
+        const bool update = pass == 1;
+        uint32_t readI = 0, refI = 0, numD = 0;
+        uint32_t e = 0;
+        uint16_t ro = 0;
+        bool stop = false;
+        for (uint32_t oi = 0; oi < nops && !stop; oi++) {
+            const uint32_t op = op_code(ops[oi]), l = op_len(ops[oi]);
+            const uint32_t t = ev_type_of(op);
+            const bool reads = op_reads(op);
+            for (uint32_t k = 0; k < l; k++) {
+                const uint32_t i = e + k;
+                if (i < baq_start) continue;
+                const uint16_t rb = reads ? (uint16_t)(ro + k) : NONE16;
+                const uint32_t qi = qs + rb;
+                if (rb != NONE16) {
+                    if (rb < rwx) continue;
+                    if (rb > rwy) { stop = true; break; }
+                }
+                if (t == EV_S) refI++;
+                else if (t == EV_I) {
+                    if (update) baq[qi] = b.quals[qi];
+                    readI++;
+                } else if (t == EV_D) {
+                    refI++; numD++;
+                } else {
+                    if (update) {
+                        const uint32_t expectedPos = refI - refOffset + (i - numD - baq_start - readI) + read_shift;
+                        const uint8_t oq = b.quals[qi];
+                        const uint8_t bq = baq[qi];
+                        const uint32_t state = a.s.baq_state[qi];
+                        uint8_t v;
+                        if ((state & 3) != 0 || (state >> 2) != expectedPos) v = 4;
+                        else v = bq < oq ? bq : oq;
+                        baq[qi] = v;
+                    }
+                    readI++; refI++;
+                }
+            }
+            if (reads) ro = (uint16_t)(ro + l);
+            e += l;
+        }
+        if (!update && readI != (uint32_t)(rwy - rwx + 1)) good = false;
+    }
+    if (!good) {
+        for (uint32_t i = rwx; i <= rwy; i++) baq[qs + i] = b.quals[qs + i];
+    }
+    // recode
+    const uint32_t ql = b.qual_len[r];
+    for (uint32_t i = qs; i < qs + ql; i++) {
+        const uint8_t bi = baq[i];
+        if (bi == 0xff) continue;
+        const uint8_t bq = (uint8_t)(b.quals[i] + 64);
+        baq[i] = (uint8_t)(bq - bi);
+    }
+}
+
+// BAQ read lists per band class
+struct ClassifyArgs {
+    uint32_t num_reads;
+    const uint8_t *status;
+    const uint16_t *num_errors;
+    const uint16_t *bandwidth;
+    const uint32_t *rwc;
+    uint32_t *lists;       // 4 lists of capacity num_reads
+    uint32_t *counts;      // 4 counters
+    uint32_t capacity;
+};
+
+__global__ void __launch_bounds__(256) k_baq_classify(ClassifyArgs a)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.num_reads) return;
+    if (a.status[r] != RS_ACTIVE || a.num_errors[r] == 0) return;
+    const uint32_t bw = a.bandwidth[r];
+    const uint32_t w = 2 * bw + 1;
+    const int cls = w <= 15 ? 0 : w <= 23 ? 1 : w <= 33 ? 2 : 3;
+    const uint32_t pos = atomicAdd(&a.counts[cls], 1u);
+    a.lists[(uint64_t)cls * a.capacity + pos] = r;
+}
+
+__global__ void __launch_bounds__(256) k_fill_u8(uint8_t *p, uint64_t n, uint8_t v)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+} // namespace fpd
+
+extern "C" hipError_t fpd_baq_set_constants(const double *qual2prob, const double *thr, double gap_open)
+{
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(fpd::c_qual2prob), qual2prob, 256 * sizeof(double));
+    if (e != hipSuccess) return e;
+    e = hipMemcpyToSymbol(HIP_SYMBOL(fpd::c_map_thr), thr, 128 * sizeof(double));
+    if (e != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(fpd::c_gap_open), &gap_open, sizeof(double));
+}
+
+extern "C" hipError_t fpd_launch_baq_classify(uint32_t num_reads, const fpd::ReadStateDev *s, uint32_t *lists,
+                                              uint32_t *counts, uint32_t capacity, hipStream_t st)
+{
+    if (num_reads == 0) return hipSuccess;
+    fpd::ClassifyArgs a;
+    a.num_reads = num_reads; a.status = s->status; a.num_errors = s->num_errors; a.bandwidth = s->bandwidth;
+    a.rwc = s->rwc; a.lists = lists; a.counts = counts; a.capacity = capacity;
+    hipLaunchKernelGGL(fpd::k_baq_classify, dim3((num_reads + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+// class: 0 -> 15 cells, 1 -> 23, 2 -> 33, 3 -> generic (wmax_runtime)
+extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const fpd::GenomeDev *g,
+                                         const fpd::ReadStateDev *s, const uint32_t *list,
+                                         const uint32_t *count, double *scratch, uint64_t slot_doubles,
+                                         uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves,
+                                         hipStream_t st)
+{
+    fpd::BaqArgs a;
+    a.b = *b; a.g = *g; a.s = *s; a.list = list; a.count = count; a.scratch = scratch;
+    a.slot_doubles = slot_doubles; a.rows_cap = rows_cap; a.wmax_runtime = wmax_runtime;
+    if (waves == 0) return hipSuccess;
+    switch (cls) {
+    case 0: hipLaunchKernelGGL(fpd::k_baq_hmm<15>, dim3(waves), dim3(64), 0, st, a); break;
+    case 1: hipLaunchKernelGGL(fpd::k_baq_hmm<23>, dim3(waves), dim3(64), 0, st, a); break;
+    case 2: hipLaunchKernelGGL(fpd::k_baq_hmm<33>, dim3(waves), dim3(64), 0, st, a); break;
+    default: hipLaunchKernelGGL(fpd::k_baq_hmm<0>, dim3(waves), dim3(64), 0, st, a); break;
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *s, hipStream_t st)
+{
+    if (b->num_reads == 0) return hipSuccess;
+    fpd::PostArgs a;
+    a.b = *b; a.s = *s;
+    hipLaunchKernelGGL(fpd::k_baq_post, dim3((b->num_reads + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fpd_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fpd::k_fill_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}

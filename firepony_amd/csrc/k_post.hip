@@ -1,0 +1,158 @@
+/*
+ * Postprocess on the device: observation rows -> empirical rows
+ * (covariate_table.cu:109-136, expected_error.h:37-76,
+ * empirical_quality.cu:35-197) and the read-group table
+ * (read_group_table.cu:47-93).
+ *
+ * pow/log10 terms that depend only on small integers (qualities, posterior
+ * bins, prior distances) come from host-computed tables; per-row terms use
+ * the device libm (log10 for the estimated quality, lgammaf for the binomial
+ * coefficient).
+ */
+#include "fp_device.h"
+
+namespace fpd {
+
+__constant__ double c_qual_err[256];    // pow(10.0, q / -10.0)
+__constant__ double c_prior[41];        // log10(gaussian(d)), d = 0..40
+__constant__ double c_log1mp[61];       // log10(1 - pow(10.0, bin / -10.0))
+__constant__ double c_log10e;           // log10(M_E)
+
+__device__ inline double log10_factorial(uint64_t x)
+{
+    return (double)lgammaf((float)(double)(x + 1)) * c_log10e;
+}
+
+__device__ double empirical_quality(uint64_t obs, double mism, double expected, bool need_rounding, double *est_out)
+{
+    const double estimated = double(-10.0 * log10(expected / double(obs)));
+    *est_out = estimated;
+    uint64_t nErr = uint64_t(mism + 0.5) + 1;
+    uint64_t nObs = obs + 1 + 1;
+    const uint32_t MAXOBS = 2147483647u - 1;
+    if (nObs > MAXOBS) {
+        const double fraction = double(MAXOBS) / double(nObs);
+        nErr = (uint64_t)round(double(nErr) * fraction);
+        nObs = MAXOBS;
+    }
+    const double coeff = log10_factorial(nObs) - log10_factorial(nErr) - log10_factorial(nObs - nErr);
+    int mle = 0;
+    double best = 0.0;
+    for (int bin = 0; bin < 61; bin++) {
+        double delta = (double)bin - estimated;
+        if (need_rounding) delta = round(delta);
+        int d = abs(int(delta));
+        d = d < 40 ? d : 40;
+        const double log10p = (bin / 1.0) / -10.0;
+        const double lik = coeff + log10p * nErr + c_log1mp[bin] * (nObs - nErr);
+        const double post = c_prior[d] + lik;
+        if (bin == 0 || post > best) {
+            if (bin == 0 || post > best) { best = post; mle = bin; }
+        }
+    }
+    const double q = (double)mle;
+    return q < 93.0 ? q : 93.0;
+}
+
+struct EmpArgs {
+    uint64_t n;
+    const uint32_t *key;
+    const uint64_t *count;
+    const uint64_t *lo;
+    const uint64_t *hi;
+    int qual_shift;            // bit offset of the quality field
+    double *mism, *expected, *estimated, *empirical;
+};
+
+__global__ void __launch_bounds__(256) k_empirical(EmpArgs a)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t q = (a.key[i] >> a.qual_shift) & 255;
+    const double m = fixed_to_double(a.lo[i], a.hi[i]);
+    const uint64_t obs = a.count[i];
+    const double ex = obs * c_qual_err[q];
+    double est;
+    const double emp = empirical_quality(obs, m, ex, true, &est);
+    a.mism[i] = m;
+    a.expected[i] = ex;
+    a.estimated[i] = est;
+    a.empirical[i] = emp;
+}
+
+// one lane per (read group, event): fold the quality rows in key order
+struct RgArgs {
+    uint32_t num_rg;
+    const uint64_t *count;   // dense quality table
+    const uint64_t *lo;
+    const uint64_t *hi;
+    uint32_t *valid;
+    uint64_t *obs;
+    double *mism, *expected, *estimated, *empirical;
+};
+
+__global__ void __launch_bounds__(256) k_rg_table(RgArgs a)
+{
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    if (slot >= a.num_rg * 4) return;
+    const uint32_t rg = slot >> 2, ev = slot & 3;
+    bool any = false;
+    uint64_t obs = 0;
+    double m = 0.0, ex = 0.0;
+    for (uint32_t q = 0; q < 256; q++) {
+        const uint32_t key = ev | (q << 2) | (rg << 10);
+        const uint64_t c = a.count[key];
+        if (c == 0) continue;
+        const double mm = fixed_to_double(a.lo[key], a.hi[key]);
+        const double ee = c * c_qual_err[q];
+        if (!any) { obs = c; m = mm; ex = ee; any = true; }
+        else { obs += c; m = m + mm; ex = ex + ee; }
+    }
+    a.valid[slot] = any ? 1 : 0;
+    if (!any) return;
+    double est;
+    const double emp = empirical_quality(obs, m, ex, false, &est);
+    a.obs[slot] = obs;
+    a.mism[slot] = m;
+    a.expected[slot] = ex;
+    a.estimated[slot] = est;
+    a.empirical[slot] = emp;
+}
+
+} // namespace fpd
+
+extern "C" hipError_t fpd_post_set_constants(const double *qual_err, const double *prior, const double *log1mp,
+                                             double log10e)
+{
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(fpd::c_qual_err), qual_err, 256 * sizeof(double));
+    if (e != hipSuccess) return e;
+    e = hipMemcpyToSymbol(HIP_SYMBOL(fpd::c_prior), prior, 41 * sizeof(double));
+    if (e != hipSuccess) return e;
+    e = hipMemcpyToSymbol(HIP_SYMBOL(fpd::c_log1mp), log1mp, 61 * sizeof(double));
+    if (e != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(fpd::c_log10e), &log10e, sizeof(double));
+}
+
+extern "C" hipError_t fpd_launch_empirical(uint64_t n, const uint32_t *key, const uint64_t *count, const uint64_t *lo,
+                                           const uint64_t *hi, int qual_shift, double *mism, double *expected,
+                                           double *estimated, double *empirical, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    fpd::EmpArgs a;
+    a.n = n; a.key = key; a.count = count; a.lo = lo; a.hi = hi; a.qual_shift = qual_shift;
+    a.mism = mism; a.expected = expected; a.estimated = estimated; a.empirical = empirical;
+    hipLaunchKernelGGL(fpd::k_empirical, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fpd_launch_rg_table(uint32_t num_rg, const uint64_t *count, const uint64_t *lo, const uint64_t *hi,
+                                          uint32_t *valid, uint64_t *obs, double *mism, double *expected,
+                                          double *estimated, double *empirical, hipStream_t st)
+{
+    if (num_rg == 0) return hipSuccess;
+    fpd::RgArgs a;
+    a.num_rg = num_rg; a.count = count; a.lo = lo; a.hi = hi; a.valid = valid; a.obs = obs;
+    a.mism = mism; a.expected = expected; a.estimated = estimated; a.empirical = empirical;
+    hipLaunchKernelGGL(fpd::k_rg_table, dim3((num_rg * 4 + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}

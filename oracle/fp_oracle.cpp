@@ -18,7 +18,10 @@
  *       left-tail search (cigar.cu:191-201) treats stepping before the first
  *       event as "no coordinate";
  *   D5  the BAQ MAP quality of a position whose posterior is exactly 1
- *       (log(0) = -inf) follows x86 cvttsd2si: INT_MIN, hence clamped to 4.
+ *       (log(0) = -inf) follows x86 cvttsd2si: INT_MIN, hence clamped to 4;
+ *   D6  zero-length I or D ops mark no error bit (the reference would read the
+ *       next op's first event), and op codes above 8 make a read malformed
+ *       (htslib never produces them; the reference leaves their events unset).
  */
 #include "fp_oracle.h"
 
@@ -309,6 +312,7 @@ static bool filter_cigar(const fpo_ctx &c, uint32 r)
     for (uint32 i = cs; i < cs + cl; i++) {
         const uint32 op = c.op_code(i);
         if (op == OP_N) return false;
+        if (op > OP_X) return false;   // D6
         if (op == OP_H) {
             switch (st) {
             case LEADING_HARD: break;
@@ -733,6 +737,7 @@ static void expand_cigars(fpo_ctx &c)
                 }
                 break;
             case EV_I: {   // == OP_I
+                if (es == ee) break;   // D6
                 cur = c.ev_read_coord[es];
                 if (cur >= rwc.x && cur <= rwc.y) {
                     int off;
@@ -746,6 +751,7 @@ static void expand_cigars(fpo_ctx &c)
                 break;
             }
             case EV_D: {   // == OP_D
+                if (es == ee) break;   // D6
                 const uint16 cri = c.ev_ref_coord[es];
                 if (cri >= fwc.x && cri <= fwc.y) {
                     if (!neg) {

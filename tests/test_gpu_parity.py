@@ -1,0 +1,153 @@
+"""Engine (HIP, through the C ABI) vs the CPU oracle on seeded workloads.
+
+Stage-by-stage: read filters, windows, error bits, base masks, BAQ tags,
+fractional errors, high-quality windows, the raw covariate tables (counts and
+exact fixed-point mismatch sums), the postprocessed tables and the report.
+Integer and index results must match bit for bit; the estimated quality
+(device log10) within 1e-9 relative.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from firepony_amd import abi, synth
+from firepony_amd.engine import Engine
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def windows(u32):
+    return (u32 & 0xffff).astype(np.int64), (u32 >> 16).astype(np.int64)
+
+
+def compare_batch(o, e, batch):
+    arr = batch.arrays()
+    act_o = o.debug(abi.FP_DBG_ACTIVE_READS, np.uint32)
+    act_e = e.debug(abi.FP_DBG_ACTIVE_READS, np.uint32)
+    np.testing.assert_array_equal(act_e, act_o, err_msg="active read list")
+    act = act_o.astype(np.int64)
+    for item, dt, name in [(abi.FP_DBG_READ_WINDOW, np.uint32, "read window"),
+                           (abi.FP_DBG_NOINS_WINDOW, np.uint32, "no-insertion window"),
+                           (abi.FP_DBG_REF_WINDOW, np.uint32, "reference window"),
+                           (abi.FP_DBG_NUM_ERRORS, np.uint16, "num_errors"),
+                           (abi.FP_DBG_HQ_WINDOW, np.uint32, "high quality window")]:
+        a = o.debug(item, dt)[act]
+        b = e.debug(item, dt)[act]
+        bad = np.nonzero(a != b)[0]
+        assert bad.size == 0, f"{name}: {bad.size} reads differ, first read {act[bad[0]]}: oracle {a[bad[0]]:#x} engine {b[bad[0]]:#x}"
+    nerr = o.debug(abi.FP_DBG_NUM_ERRORS, np.uint16)
+    baq_reads = act[nerr[act] != 0]
+    for item, dt, name in [(abi.FP_DBG_HMM_WINDOW, np.uint32, "hmm window"),
+                           (abi.FP_DBG_BANDWIDTH, np.uint16, "bandwidth")]:
+        a = o.debug(item, dt)[baq_reads]
+        b = e.debug(item, dt)[baq_reads]
+        bad = np.nonzero(a != b)[0]
+        assert bad.size == 0, f"{name}: {bad.size} reads differ, first read {baq_reads[bad[0]]}"
+    # per-base and per-quality arrays over active reads
+    rs, rl = arr["read_start"].astype(np.int64), arr["read_len"].astype(np.int64)
+    qs = arr["qual_start"].astype(np.int64)
+    bidx = np.concatenate([np.arange(rs[r], rs[r] + rl[r]) for r in act]) if act.size else np.zeros(0, np.int64)
+    qidx = np.concatenate([np.arange(qs[r], qs[r] + rl[r]) for r in act]) if act.size else np.zeros(0, np.int64)
+    for item, name in [(abi.FP_DBG_ERROR_FLAGS, "error bits"), (abi.FP_DBG_ACTIVE_MASK, "active mask")]:
+        a = o.debug(item, np.uint8)[bidx]
+        b = e.debug(item, np.uint8)[bidx]
+        bad = np.nonzero(a != b)[0]
+        assert bad.size == 0, f"{name}: {bad.size} bases differ, first base {bidx[bad[0]]}: {a[bad[0]]} vs {b[bad[0]]}"
+    a = o.debug(abi.FP_DBG_BAQ, np.uint8)[qidx]
+    b = e.debug(abi.FP_DBG_BAQ, np.uint8)[qidx]
+    bad = np.nonzero(a != b)[0]
+    assert bad.size == 0, f"BAQ: {bad.size} positions differ, first quality index {qidx[bad[0]]}: {a[bad[0]]} vs {b[bad[0]]}"
+    fo = o.debug(abi.FP_DBG_FRAC_ERRORS, np.float64).reshape(-1, 3)[qidx]
+    fe = e.debug(abi.FP_DBG_FRAC_ERRORS, np.float64).reshape(-1, 3)[qidx]
+    np.testing.assert_array_equal(fe, fo, err_msg="fractional errors")
+
+
+def compare_obs_tables(o, e):
+    for t in (abi.FP_TABLE_QUALITY, abi.FP_TABLE_CYCLE, abi.FP_TABLE_CONTEXT):
+        a, b = o.table(t), e.table(t)
+        np.testing.assert_array_equal(b["keys"], a["keys"], err_msg=f"table {t} keys")
+        np.testing.assert_array_equal(b["observations"], a["observations"], err_msg=f"table {t} counts")
+        np.testing.assert_array_equal(b["mismatches_fixed_lo"], a["mismatches_fixed_lo"], err_msg=f"table {t} sums lo")
+        np.testing.assert_array_equal(b["mismatches_fixed_hi"], a["mismatches_fixed_hi"], err_msg=f"table {t} sums hi")
+        np.testing.assert_array_equal(b["mismatches"], a["mismatches"], err_msg=f"table {t} sums")
+
+
+def compare_final(o, e, tmp_path):
+    for t in range(4):
+        a, b = o.table(t), e.table(t)
+        np.testing.assert_array_equal(b["keys"], a["keys"], err_msg=f"final table {t} keys")
+        np.testing.assert_array_equal(b["observations"], a["observations"])
+        np.testing.assert_array_equal(b["mismatches"], a["mismatches"], err_msg=f"final table {t} mismatches")
+        np.testing.assert_array_equal(b["expected_errors"], a["expected_errors"], err_msg=f"final table {t} expected")
+        np.testing.assert_allclose(b["estimated_quality"], a["estimated_quality"], rtol=1e-9, atol=0,
+                                   err_msg=f"final table {t} estimated quality")
+        bad = np.nonzero(b["empirical_quality"] != a["empirical_quality"])[0]
+        assert bad.size == 0, (f"table {t}: {bad.size} empirical qualities differ; key {a['keys'][bad[0]]:#x} "
+                               f"oracle {a['empirical_quality'][bad[0]]} engine {b['empirical_quality'][bad[0]]}")
+    ro = o.report(str(tmp_path / "oracle.txt"))
+    re_ = e.report(str(tmp_path / "engine.txt"))
+    assert ro == re_, "report text differs"
+
+
+def run_parity(tmp_path, genome, rs, n, batch_size, per_batch=True, **engine_opts):
+    f64 = engine_opts.get("frac_error_f64", 0)
+    o = Oracle(genome, frac_error_f64=bool(f64))
+    e = Engine(genome, **engine_opts)
+    for first in range(0, n, batch_size):
+        b = synth.Batch(genome, rs, first, min(batch_size, n - first))
+        say(f"  batch {first}..{first + b.view.num_reads}")
+        o.process(b)
+        e.process(b)
+        if per_batch:
+            compare_batch(o, e, b)
+    compare_obs_tables(o, e)
+    so, se = o.stats(), e.stats()
+    assert (se.total_reads, se.filtered_reads, se.baq_reads) == (so.total_reads, so.filtered_reads, so.baq_reads)
+    o.postprocess()
+    e.postprocess()
+    compare_final(o, e, tmp_path)
+    e.close()
+
+
+def test_config0_slice(tmp_path):
+    say("parity: config 0 first 4000 reads")
+    gs, rs, n = synth.named_spec(0)
+    g = synth.Genome(gs, rs.num_read_groups)
+    run_parity(tmp_path, g, rs, 4000, 1000)
+
+
+def test_edge_cases(tmp_path):
+    say("parity: edge-case menu, paired, N runs, multi-base sites")
+    g, rs, n = synth.small_workload(seed=7, num_reads=3000, edge=0.35, lengths=(36, 100, 151), read_groups=3,
+                                    contigs=(60000, 40000), sites=300, clip=0.3, indel=0.2, paired=1,
+                                    n_runs=0.01, multi_sites=0.3)
+    run_parity(tmp_path, g, rs, n, 1000)
+
+
+def test_mixed_lengths_f64(tmp_path):
+    say("parity: mixed lengths, large indels, fp64 fractional errors")
+    g, rs, n = synth.small_workload(seed=11, num_reads=3000, lengths=(50, 76, 101, 151, 250), read_groups=8,
+                                    contigs=(300000,), sites=600, clip=1.0, indel=0.3)
+    run_parity(tmp_path, g, rs, n, 1500, frac_error_f64=1)
+
+
+def test_batch_size_invariance(tmp_path):
+    say("invariance: one batch vs many batches give identical tables")
+    g, rs, n = synth.small_workload(seed=5, num_reads=3000, edge=0.1)
+    e1 = Engine(g)
+    e1.process(synth.Batch(g, rs, 0, n))
+    e2 = Engine(g)
+    for first in range(0, n, 700):
+        e2.process(synth.Batch(g, rs, first, min(700, n - first)))
+    for t in (abi.FP_TABLE_QUALITY, abi.FP_TABLE_CYCLE, abi.FP_TABLE_CONTEXT):
+        a, b = e1.table(t), e2.table(t)
+        for k in ("keys", "observations", "mismatches_fixed_lo", "mismatches_fixed_hi"):
+            np.testing.assert_array_equal(a[k], b[k])
