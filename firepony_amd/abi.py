@@ -139,7 +139,14 @@ class FpStats(C.Structure):
         ("covariates_sort_ms", C.c_double),
         ("postprocessing_ms", C.c_double),
         ("output_ms", C.c_double),
-        ("reserved", C.c_double * 8),
+        ("prep_kernel_ms", C.c_double),
+        ("baq_hmm_kernel_ms", C.c_double),
+        ("cov_kernel_ms", C.c_double),
+        ("prep_launches", C.c_uint64),
+        ("baq_hmm_launches", C.c_uint64),
+        ("cov_launches", C.c_uint64),
+        ("baq_cells", C.c_uint64),
+        ("reserved", C.c_uint64 * 5),
     ]
 
 

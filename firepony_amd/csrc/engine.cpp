@@ -32,7 +32,8 @@ hipError_t fpd_launch_baq_classify(uint32_t num_reads, const fpd::ReadStateDev *
                                    uint32_t *counts, uint32_t capacity, hipStream_t st);
 hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const fpd::GenomeDev *g, const fpd::ReadStateDev *s,
                               const uint32_t *list, const uint32_t *count, double *scratch, uint64_t slot_doubles,
-                              uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves, hipStream_t st);
+                              uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves, unsigned long long *cells,
+                              hipStream_t st);
 hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *s, hipStream_t st);
 hipError_t fpd_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st);
 hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
@@ -196,7 +197,7 @@ struct fp_engine {
     TableRows rows[4];
 
     fp_stats stats;
-    hipEvent_t ev[8];
+    hipEvent_t ev[16];
 };
 
 namespace {
@@ -301,7 +302,9 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     HIP_TRY(hipEventRecord(e->ev[0], st));
     HIP_TRY(hipMemsetAsync(e->counters.p, 0, 64, st));
     HIP_TRY(hipMemsetAsync(e->baq_counts.p, 0, 64, st));
+    HIP_TRY(hipEventRecord(e->ev[6], st));
     HIP_TRY(fpd_launch_prep(&b, &e->genome, &e->rs, e->counters.as<uint32_t>(), st));
+    HIP_TRY(hipEventRecord(e->ev[7], st));
     HIP_TRY(hipEventRecord(e->ev[1], st));
     HIP_TRY(fpd_fill_u8(e->rs.baq, v->num_qualities, 0xff, st));
     HIP_TRY(hipMemsetAsync(e->rs.frac, 0, 24 * v->num_qualities, st));
@@ -334,25 +337,40 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
         if (cnt[c]) need = std::max(need, slot[c] * waves[c]);
     }
     HIP_TRY(e->scratch.ensure(need * 8 + 8));
+    uint32_t hmm_launches = 0;
+    HIP_TRY(hipEventRecord(e->ev[8], st));
     for (int c = 0; c < 4; c++) {
         if (!cnt[c]) continue;
         HIP_TRY(fpd_launch_baq_hmm(c, &b, &e->genome, &e->rs,
                                    e->baq_lists.as<uint32_t>() + (uint64_t)c * std::max<uint32_t>(n, 1),
                                    e->baq_counts.as<uint32_t>() + c, e->scratch.as<double>(), slot[c], rows_cap,
-                                   c == 3 ? wmax_generic : BAQ_CLASS_W[c], waves[c], st));
+                                   c == 3 ? wmax_generic : BAQ_CLASS_W[c], waves[c],
+                                   (unsigned long long *)(e->counters.as<uint32_t>() + 8), st));
+        hmm_launches++;
     }
+    HIP_TRY(hipEventRecord(e->ev[9], st));
     HIP_TRY(fpd_launch_baq_post(&b, &e->rs, st));
     HIP_TRY(hipEventRecord(e->ev[2], st));
     HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, e->counters.as<uint32_t>(), st));
     HIP_TRY(hipEventRecord(e->ev[3], st));
-    HIP_TRY(hipMemcpyAsync(cnt + 8, e->counters.p, 32, hipMemcpyDeviceToHost, st));
+    uint32_t post[16];
+    HIP_TRY(hipMemcpyAsync(post, e->counters.p, 64, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if (cnt[8 + 4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
+    if (post[4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
 
     e->stats.total_reads += n;
     e->stats.num_batches += 1;
-    e->stats.filtered_reads += cnt[8 + 1];
-    e->stats.baq_reads += cnt[8 + 2];
+    e->stats.filtered_reads += post[1];
+    e->stats.baq_reads += post[2];
+    uint64_t cells;
+    memcpy(&cells, post + 8, 8);
+    e->stats.baq_cells += cells;
+    e->stats.prep_kernel_ms += elapsed(e->ev[6], e->ev[7]);
+    e->stats.prep_launches += 1;
+    if (hmm_launches) e->stats.baq_hmm_kernel_ms += elapsed(e->ev[8], e->ev[9]);
+    e->stats.baq_hmm_launches += hmm_launches;
+    e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[3]);
+    e->stats.cov_launches += 1;
     e->stats.read_filter_ms += elapsed(e->ev[0], e->ev[1]);
     e->stats.baq_ms += elapsed(e->ev[1], e->ev[2]);
     e->stats.covariates_ms += elapsed(e->ev[2], e->ev[3]);
@@ -525,7 +543,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) e->name = prop.name;
     hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (he != hipSuccess) { delete e; return fail(FP_ERR_HIP, hipGetErrorString(he)); }
-    for (int i = 0; i < 8; i++) (void)hipEventCreate(&e->ev[i]);
+    for (int i = 0; i < 16; i++) (void)hipEventCreate(&e->ev[i]);
     compute_constants(e->consts);
     he = fpd_baq_set_constants(e->consts.qual2prob, e->consts.map_thr, e->consts.gap_open);
     if (he == hipSuccess)
@@ -925,7 +943,7 @@ void fp_engine_destroy(fp_engine *e)
     for (DevBuf *b : bufs) b->release();
     for (auto &b : e->in) b.release();
     for (auto &t : e->tab) for (auto &b : t) b.release();
-    for (int i = 0; i < 8; i++) if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+    for (int i = 0; i < 16; i++) if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }

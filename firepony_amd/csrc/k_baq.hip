@@ -54,6 +54,7 @@ struct BaqArgs {
     uint64_t slot_doubles;    // doubles per wave slot
     uint32_t rows_cap;        // >= max query length
     uint32_t wmax_runtime;    // band cells for the generic kernel
+    unsigned long long *cells;  // total band cells evaluated (algorithmic work)
 };
 
 template <int WMAX>
@@ -122,6 +123,7 @@ __global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
             return contig_base(a.g, chrom, hoff + (uint32_t)idx0);
         };
 
+        uint64_t ncell = 0;
         // ---- forward row 1 (baq.cu:396-442)
         {
             const uint8_t qb = read_base(b.reads, qbase);
@@ -129,6 +131,7 @@ __global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
             const double p = c_qual2prob[qq < 4 ? 4 : qq];
             const double em_eq = 1 - p, em_ne = p * EM;
             const int khi = refLen < bw + 1 ? refLen : bw + 1;
+            ncell += khi >= 1 ? (uint64_t)khi : 0;
             double sum = 0.0;
 #pragma unroll
             for (int r = 0; r < (REG ? WMAX : 1) * 0 + W; r++) {
@@ -160,6 +163,7 @@ __global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
             const double em_eq = 1 - p, em_ne = p * EM;
             const int klo = 1 > i - bw ? 1 : i - bw;
             const int khi = refLen < i + bw ? refLen : i + bw;
+            if (khi >= klo) ncell += (uint64_t)(khi - klo + 1);
             double sum = 0.0, cM = 0.0, cD = 0.0;
 #pragma unroll
             for (int r = 0; r < W; r++) {
@@ -200,6 +204,7 @@ __global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
             }
             S(qLen + 1) = sum;
         }
+        atomicAdd(a.cells, (unsigned long long)ncell);
 
         // ---- backward + MAP (baq.cu:528-681)
         const uint32_t qout = qqual;
@@ -446,11 +451,11 @@ extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const 
                                          const fpd::ReadStateDev *s, const uint32_t *list,
                                          const uint32_t *count, double *scratch, uint64_t slot_doubles,
                                          uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves,
-                                         hipStream_t st)
+                                         unsigned long long *cells, hipStream_t st)
 {
     fpd::BaqArgs a;
     a.b = *b; a.g = *g; a.s = *s; a.list = list; a.count = count; a.scratch = scratch;
-    a.slot_doubles = slot_doubles; a.rows_cap = rows_cap; a.wmax_runtime = wmax_runtime;
+    a.slot_doubles = slot_doubles; a.rows_cap = rows_cap; a.wmax_runtime = wmax_runtime; a.cells = cells;
     if (waves == 0) return hipSuccess;
     switch (cls) {
     case 0: hipLaunchKernelGGL(fpd::k_baq_hmm<15>, dim3(waves), dim3(64), 0, st, a); break;

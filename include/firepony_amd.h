@@ -178,7 +178,15 @@ typedef struct fp_stats {
     double covariates_sort_ms;
     double postprocessing_ms;
     double output_ms;
-    double reserved[8];
+    /* per-kernel device time (HIP events on the engine stream) and launches */
+    double prep_kernel_ms;
+    double baq_hmm_kernel_ms;
+    double cov_kernel_ms;
+    uint64_t prep_launches;
+    uint64_t baq_hmm_launches;
+    uint64_t cov_launches;
+    uint64_t baq_cells;        /* band cells evaluated by the BAQ HMM (algorithmic work) */
+    uint64_t reserved[5];
 } fp_stats;
 
 /* Per-batch intermediates of the last processed batch, for stage-by-stage
