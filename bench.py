@@ -121,6 +121,7 @@ def main():
     ap.add_argument("--batch-reads", type=int, default=20000)
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--wpe", type=int, default=0, help="BAQ waves per SIMD (0 = engine default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path would shard by batch, "
@@ -132,6 +133,8 @@ def main():
     genome = synth.Genome(gs, rs.num_read_groups)
     B = args.batch_reads
     K = min(args.steps, (total + B - 1) // B)
+    if B != 20000 and args.steps == 500:
+        K = (total + B - 1) // B
     log(f"generating {K} batches of {B} reads")
     t0 = time.perf_counter()
     dev = []
@@ -143,7 +146,7 @@ def main():
     torch.cuda.synchronize()
     log(f"generated + uploaded in {time.perf_counter() - t0:.1f} s")
 
-    e = Engine(genome, device=0)
+    e = Engine(genome, device=0, baq_waves_per_simd=args.wpe)
     for i in range(args.warmup):
         e.process_device(dev[i % K].view)
     e.synchronize()

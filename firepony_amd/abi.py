@@ -20,6 +20,9 @@ FP_ERR_INTERNAL = 8
 FP_ENGINE_PLAIN = 0
 FP_ENGINE_BATCHED = 1
 
+FP_AGG_DIRECT = 0
+FP_AGG_SORT = 1
+
 FP_TABLE_READ_GROUP = 0
 FP_TABLE_QUALITY = 1
 FP_TABLE_CYCLE = 2
@@ -45,9 +48,11 @@ class FpOptions(C.Structure):
         ("max_batch_bases", C.c_uint32),
         ("frac_error_f64", C.c_uint32),
         ("disable_output_rounding", C.c_uint32),
-        ("baq_lanes_per_read", C.c_uint32),
+        ("baq_waves_per_simd", C.c_uint32),
         ("sort_digit_bits", C.c_uint32),
-        ("reserved", C.c_uint32 * 8),
+        ("max_read_length", C.c_uint32),
+        ("aggregation", C.c_uint32),
+        ("reserved", C.c_uint32 * 6),
     ]
 
 

@@ -33,11 +33,12 @@ hipError_t fpd_launch_baq_classify(uint32_t num_reads, const fpd::ReadStateDev *
 hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const fpd::GenomeDev *g, const fpd::ReadStateDev *s,
                               const uint32_t *list, const uint32_t *count, double *scratch, uint64_t slot_doubles,
                               uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves, unsigned long long *cells,
-                              hipStream_t st);
+                              int wpe, hipStream_t st);
 hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *s, hipStream_t st);
 hipError_t fpd_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st);
 hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
-                          uint32_t frac_f64, uint32_t *counters, hipStream_t st);
+                          uint32_t frac_f64, uint32_t *counters, uint32_t *rg_buf, uint32_t max_read_len,
+                          hipStream_t st);
 hipError_t fpd_compact_count(const uint64_t *count, uint64_t n, uint32_t *block_counts, hipStream_t st);
 hipError_t fpd_compact_scatter(const uint64_t *count, const uint64_t *lo, const uint64_t *hi, uint64_t n,
                                const uint64_t *block_offsets, uint32_t *okey, uint64_t *ocount, uint64_t *olo,
@@ -170,6 +171,7 @@ struct fp_engine {
     hipStream_t stream = nullptr;
     std::string name;
     HostConstants consts;
+    int wpe = 1;   // BAQ HMM waves per SIMD
 
     // setup state
     bool is_setup = false;
@@ -182,7 +184,7 @@ struct fp_engine {
     DevBuf in[18];
     // per-batch state
     DevBuf st_status, st_rwc, st_noins, st_refw, st_nerr, st_hmmw, st_bw, st_hqw, st_bflags, st_baq, st_state, st_frac;
-    DevBuf counters, baq_lists, baq_counts, scratch;
+    DevBuf counters, baq_lists, baq_counts, scratch, rg_buf;
     uint32_t last_num_reads = 0;
     uint64_t last_num_bases = 0, last_num_quals = 0;
     fpd::ReadStateDev rs{};
@@ -237,6 +239,7 @@ fp_status ensure_state(fp_engine *e, uint32_t n, uint64_t nbases, uint64_t nqual
     HIP_TRY(e->st_frac.ensure(24 * (nquals + 8)));
     HIP_TRY(e->counters.ensure(64));
     HIP_TRY(e->baq_lists.ensure(4ull * 4 * std::max<uint32_t>(n, 1)));
+    HIP_TRY(e->rg_buf.ensure(4ull * (4ull * std::max<uint32_t>(e->num_rg, 1) + 2 + n)));
     HIP_TRY(e->baq_counts.ensure(64));
     fpd::ReadStateDev &s = e->rs;
     s.status = e->st_status.as<uint8_t>();
@@ -345,13 +348,14 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
                                    e->baq_lists.as<uint32_t>() + (uint64_t)c * std::max<uint32_t>(n, 1),
                                    e->baq_counts.as<uint32_t>() + c, e->scratch.as<double>(), slot[c], rows_cap,
                                    c == 3 ? wmax_generic : BAQ_CLASS_W[c], waves[c],
-                                   (unsigned long long *)(e->counters.as<uint32_t>() + 8), st));
+                                   (unsigned long long *)(e->counters.as<uint32_t>() + 8), e->wpe, st));
         hmm_launches++;
     }
     HIP_TRY(hipEventRecord(e->ev[9], st));
     HIP_TRY(fpd_launch_baq_post(&b, &e->rs, st));
     HIP_TRY(hipEventRecord(e->ev[2], st));
-    HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, e->counters.as<uint32_t>(), st));
+    HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, e->counters.as<uint32_t>(),
+                           e->rg_buf.as<uint32_t>(), std::min<uint32_t>(rows_cap, 499), st));
     HIP_TRY(hipEventRecord(e->ev[3], st));
     uint32_t post[16];
     HIP_TRY(hipMemcpyAsync(post, e->counters.p, 64, hipMemcpyDeviceToHost, st));
@@ -538,6 +542,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     fp_engine *e = new fp_engine();
     e->device = hip_device;
     e->opts = o;
+    e->wpe = o.baq_waves_per_simd == 2 ? 2 : 1;
     memset(&e->stats, 0, sizeof(e->stats));
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) e->name = prop.name;
@@ -939,7 +944,7 @@ void fp_engine_destroy(fp_engine *e)
     DevBuf *bufs[] = {&e->g_word_offset, &e->g_length, &e->g_words, &e->g_foff, &e->g_fstart, &e->g_fstop,
                       &e->g_mel, &e->st_status, &e->st_rwc, &e->st_noins, &e->st_refw, &e->st_nerr, &e->st_hmmw,
                       &e->st_bw, &e->st_hqw, &e->st_bflags, &e->st_baq, &e->st_state, &e->st_frac, &e->counters,
-                      &e->baq_lists, &e->baq_counts, &e->scratch};
+                      &e->baq_lists, &e->baq_counts, &e->scratch, &e->rg_buf};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : e->in) b.release();
     for (auto &t : e->tab) for (auto &b : t) b.release();

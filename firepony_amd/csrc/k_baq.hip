@@ -29,19 +29,20 @@ static constexpr double EM = 0.33333333333;
 static constexpr double EI = 0.25;
 static constexpr double GEP = 0.1;
 
-// MAP quality from y = 1 - max/sum (baq.cu:673-674)
+// MAP quality from y = 1 - max/sum (baq.cu:673-674).  The exact value is the
+// largest v in [5, 101] with y <= T[v] (T decreasing, computed with the host
+// libm); the device log only provides the starting guess, which the threshold
+// comparisons then correct, so the result never depends on the device log.
 __device__ inline uint8_t map_quality(double y)
 {
     if (y == 0.0) return 4;                 // log(0): (int)+inf is INT_MIN on the host
-    if (!(y <= c_map_thr[5])) return 4;     // k <= 4
-    // largest v in [5, 101] with y <= T[v] (T is decreasing)
-    int lo = 5, hi = 101;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (y <= c_map_thr[mid]) lo = mid; else hi = mid - 1;
-    }
-    if (lo >= 101) return 99;
-    return (uint8_t)lo;
+    int k = (int)(-4.343 * log(y) + 0.499);
+    k = k < 4 ? 4 : (k > 101 ? 101 : k);
+    while (k >= 5 && !(y <= c_map_thr[k])) k--;
+    while (k + 1 <= 101 && y <= c_map_thr[k + 1]) k++;
+    if (k <= 4) return 4;
+    if (k >= 101) return 99;
+    return (uint8_t)k;
 }
 
 struct BaqArgs {
@@ -57,8 +58,10 @@ struct BaqArgs {
     unsigned long long *cells;  // total band cells evaluated (algorithmic work)
 };
 
-template <int WMAX>
-__global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
+// WPE: waves per SIMD the register budget is sized for (1: band fully in
+// registers; 2: twice the latency hiding at the cost of a few spills)
+template <int WMAX, int WPE>
+__global__ void __launch_bounds__(64, WPE) k_baq_hmm(BaqArgs a)
 {
     constexpr bool REG = WMAX > 0;
     const uint32_t lane = threadIdx.x & 63;
@@ -118,9 +121,20 @@ __global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
 
         const uint64_t qbase = (uint64_t)b.read_start[rd] + qStart;
         const uint32_t qqual = b.qual_start[rd] + qStart;
+        const uint32_t *cw = a.g.words + a.g.word_offset[chrom];
+        const uint32_t clen = a.g.length[chrom];
         auto refbase = [&](int idx0) -> uint8_t {   // 0-based index into the HMM window
             if (idx0 < 0 || idx0 >= refLen) return B_N;
-            return contig_base(a.g, chrom, hoff + (uint32_t)idx0);
+            const uint32_t p = hoff + (uint32_t)idx0;
+            if (p >= clen) return B_N;
+            return (uint8_t)((cw[p >> 3] >> ((p & 7) * 4)) & 0xf);
+        };
+        // 15-cell bands keep the band's reference bases as 16 packed nibbles
+        constexpr bool WIN = REG && WMAX <= 16;
+        uint64_t win = 0;
+        auto win_fill = [&](int idx0) {
+            win = 0;
+            for (int j = 0; j < 16; j++) win |= (uint64_t)refbase(idx0 + j) << (4 * j);
         };
 
         uint64_t ncell = 0;
@@ -132,6 +146,7 @@ __global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
             const double em_eq = 1 - p, em_ne = p * EM;
             const int khi = refLen < bw + 1 ? refLen : bw + 1;
             ncell += khi >= 1 ? (uint64_t)khi : 0;
+            if constexpr (WIN) win_fill(1 - bw - 1);   // nibble r = base at k - 1, k = 1 - bw + r
             double sum = 0.0;
 #pragma unroll
             for (int r = 0; r < (REG ? WMAX : 1) * 0 + W; r++) {
@@ -139,7 +154,7 @@ __global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
                 const bool in = k >= 1 && k <= khi;
                 double mv = 0.0, iv = 0.0;
                 if (in) {
-                    const uint8_t rb = refbase(k - 1);
+                    const uint8_t rb = WIN ? (uint8_t)((win >> (4 * r)) & 15) : refbase(k - 1);
                     const double e = (rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne);
                     mv = e * bM;
                     iv = EI * bI;
@@ -156,14 +171,19 @@ __global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
             }
         }
         // ---- forward rows 2..qLen (baq.cu:444-512)
+        // the next row's query base and quality are loaded one row ahead
+        uint8_t nqb = qLen >= 2 ? read_base(b.reads, qbase + 1) : 0;
+        uint8_t nqq = qLen >= 2 ? b.quals[qqual + 1] : 0;
         for (int i = 2; i <= qLen; i++) {
-            const uint8_t qb = read_base(b.reads, qbase + i - 1);
-            const uint8_t qq = b.quals[qqual + i - 1];
+            const uint8_t qb = nqb;
+            const uint8_t qq = nqq;
+            if (i < qLen) { nqb = read_base(b.reads, qbase + i); nqq = b.quals[qqual + i]; }
             const double p = c_qual2prob[qq < 4 ? 4 : qq];
             const double em_eq = 1 - p, em_ne = p * EM;
             const int klo = 1 > i - bw ? 1 : i - bw;
             const int khi = refLen < i + bw ? refLen : i + bw;
             if (khi >= klo) ncell += (uint64_t)(khi - klo + 1);
+            if constexpr (WIN) win = (win >> 4) | ((uint64_t)refbase(i - bw - 1 + 15) << 60);
             double sum = 0.0, cM = 0.0, cD = 0.0;
 #pragma unroll
             for (int r = 0; r < W; r++) {
@@ -174,7 +194,7 @@ __global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
                 const double nI = (r + 1 < W) ? I(r + 1) : 0.0;
                 double mv = 0.0, iv = 0.0, dv = 0.0;
                 if (in) {
-                    const uint8_t rb = refbase(k - 1);
+                    const uint8_t rb = WIN ? (uint8_t)((win >> (4 * r)) & 15) : refbase(k - 1);
                     const double e = (rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne);
                     mv = e * (m0 * pM + m3 * pI + m6 * pD);
                     iv = EI * (m1 * nM + m4 * nI);
@@ -234,14 +254,20 @@ __global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
             for (int r = 0; r < W; r++) {
                 const int k = qLen - bw + r;
                 const bool in = k >= klo && k <= khi;
-                M(r) = in ? vM : 0.0; I(r) = in ? vI : 0.0; D(r) = 0.0;
+                M(r) = in ? vM : 0.0; I(r) = in ? vI : 0.0;
             }
             map_row(qLen);
         }
+        if constexpr (WIN) win_fill(qLen - bw);
+        uint8_t bqb = qLen >= 2 ? read_base(b.reads, qbase + qLen - 1) : 0;
+        uint8_t bqq = qLen >= 2 ? b.quals[qqual + qLen - 1] : 0;
         for (int i = qLen - 1; i >= 1; i--) {
+            // nibble r = base at k = i - bw + r
+            if constexpr (WIN) win = (win << 4) | (uint64_t)refbase(i - bw);
             const double y = (i > 1) ? 1. : 0.;
-            const uint8_t qb = read_base(b.reads, qbase + i);
-            const uint8_t qq = b.quals[qqual + i];
+            const uint8_t qb = bqb;
+            const uint8_t qq = bqq;
+            if (i > 1) { bqb = read_base(b.reads, qbase + i - 1); bqq = b.quals[qqual + i - 1]; }
             const double p = c_qual2prob[qq < 4 ? 4 : qq];
             const double em_eq = 1 - p, em_ne = p * EM;
             const int klo = 1 > i - bw ? 1 : i - bw;
@@ -258,19 +284,20 @@ __global__ void __launch_bounds__(64) k_baq_hmm(BaqArgs a)
                     double e;
                     if (k >= refLen) e = 0;
                     else {
-                        const uint8_t rb = refbase(k);
+                        const uint8_t rb = WIN ? (uint8_t)((win >> (4 * r)) & 15) : refbase(k);
                         e = ((rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne)) * pM;
                     }
                     mv = e * m0 + EIm1 * pIl + m2 * cD;
                     iv = e * m3 + EIm4 * pIl;
                     dv = (e * m6 + m8 * cD) * y;
                 }
-                M(r) = mv; I(r) = iv; D(r) = dv;
+                // the previous row's D is never read by the backward pass: only the carry
+                M(r) = mv; I(r) = iv;
                 cD = dv;
             }
             const double yy = 1.0 / S(i);
 #pragma unroll
-            for (int r = 0; r < W; r++) { M(r) = M(r) * yy; I(r) = I(r) * yy; D(r) = D(r) * yy; }
+            for (int r = 0; r < W; r++) { M(r) = M(r) * yy; I(r) = I(r) * yy; }
             map_row(i);
         }
     }
@@ -308,10 +335,7 @@ __global__ void __launch_bounds__(256) k_baq_post(PostArgs a)
     const uint16_t fwx = lo16(a.s.refw[r]);
     const uint32_t astart = b.alignment_start[r];
 
-    // walk the events once to find baq_start, then run the check and update passes
-    uint32_t total = 0;
-    for (uint32_t i = 0; i < nops; i++) total += op_len(ops[i]);
-
+    // find baq_start, then run the check and update passes
     int16_t reference_shift, read_shift;
     const uint32_t ahx = hx < 0 ? (uint32_t)(-hx) : (uint32_t)hx;
     if (hx < 0 && astart < ahx) { reference_shift = (int16_t)ahx; read_shift = (int16_t)astart; }
@@ -409,13 +433,25 @@ struct ClassifyArgs {
 __global__ void __launch_bounds__(256) k_baq_classify(ClassifyArgs a)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.num_reads) return;
-    if (a.status[r] != RS_ACTIVE || a.num_errors[r] == 0) return;
-    const uint32_t bw = a.bandwidth[r];
-    const uint32_t w = 2 * bw + 1;
-    const int cls = w <= 15 ? 0 : w <= 23 ? 1 : w <= 33 ? 2 : 3;
-    const uint32_t pos = atomicAdd(&a.counts[cls], 1u);
-    a.lists[(uint64_t)cls * a.capacity + pos] = r;
+    const bool act = r < a.num_reads && a.status[r] == RS_ACTIVE && a.num_errors[r] != 0;
+    uint32_t cls = 0;
+    if (act) {
+        const uint32_t w = 2u * a.bandwidth[r] + 1;
+        cls = w <= 15 ? 0 : w <= 23 ? 1 : w <= 33 ? 2 : 3;
+    }
+    // wave-aggregated append: one counter atomic per band class present in the wavefront
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t pos = 0;
+    for (uint32_t c = 0; c < 4; c++) {
+        const uint64_t m = __ballot(act && cls == c);
+        if (!m) continue;
+        const int leader = __builtin_ctzll(m);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&a.counts[c], (uint32_t)__popcll(m));
+        base = __shfl(base, leader);
+        if (act && cls == c) pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+    }
+    if (act) a.lists[(uint64_t)cls * a.capacity + pos] = r;
 }
 
 __global__ void __launch_bounds__(256) k_fill_u8(uint8_t *p, uint64_t n, uint8_t v)
@@ -451,17 +487,26 @@ extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const 
                                          const fpd::ReadStateDev *s, const uint32_t *list,
                                          const uint32_t *count, double *scratch, uint64_t slot_doubles,
                                          uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves,
-                                         unsigned long long *cells, hipStream_t st)
+                                         unsigned long long *cells, int wpe, hipStream_t st)
 {
     fpd::BaqArgs a;
     a.b = *b; a.g = *g; a.s = *s; a.list = list; a.count = count; a.scratch = scratch;
     a.slot_doubles = slot_doubles; a.rows_cap = rows_cap; a.wmax_runtime = wmax_runtime; a.cells = cells;
     if (waves == 0) return hipSuccess;
-    switch (cls) {
-    case 0: hipLaunchKernelGGL(fpd::k_baq_hmm<15>, dim3(waves), dim3(64), 0, st, a); break;
-    case 1: hipLaunchKernelGGL(fpd::k_baq_hmm<23>, dim3(waves), dim3(64), 0, st, a); break;
-    case 2: hipLaunchKernelGGL(fpd::k_baq_hmm<33>, dim3(waves), dim3(64), 0, st, a); break;
-    default: hipLaunchKernelGGL(fpd::k_baq_hmm<0>, dim3(waves), dim3(64), 0, st, a); break;
+    if (wpe >= 2) {
+        switch (cls) {
+        case 0: hipLaunchKernelGGL((fpd::k_baq_hmm<15, 2>), dim3(waves), dim3(64), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((fpd::k_baq_hmm<23, 2>), dim3(waves), dim3(64), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((fpd::k_baq_hmm<33, 2>), dim3(waves), dim3(64), 0, st, a); break;
+        default: hipLaunchKernelGGL((fpd::k_baq_hmm<0, 2>), dim3(waves), dim3(64), 0, st, a); break;
+        }
+    } else {
+        switch (cls) {
+        case 0: hipLaunchKernelGGL((fpd::k_baq_hmm<15, 1>), dim3(waves), dim3(64), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((fpd::k_baq_hmm<23, 1>), dim3(waves), dim3(64), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((fpd::k_baq_hmm<33, 1>), dim3(waves), dim3(64), 0, st, a); break;
+        default: hipLaunchKernelGGL((fpd::k_baq_hmm<0, 1>), dim3(waves), dim3(64), 0, st, a); break;
+        }
     }
     return hipGetLastError();
 }

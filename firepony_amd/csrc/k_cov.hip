@@ -1,7 +1,8 @@
 /*
  * K_cov: fractional errors (device/fractional_errors.cu:39-141), the high
  * quality window (device/covariates.cu:215-246) and covariate key packing
- * (covariates.cu:50-103, covariates/packer_*.h, covariates/bit_packers/*.h),
+ * (covariates.cu:50-103, covariates/packer_quality_score.h and siblings,
+ * covariates/bit_packers/cycle_illumina.h, context.h),
  * accumulated straight into the device-resident covariate tables.
  *
  * The tables are direct-indexed by covariate key: the read group is the most
@@ -11,6 +12,15 @@
  * are u64; mismatch sums are exact u128 fixed point (units of 2^-80), so the
  * accumulated table does not depend on batch size, launch order or the order
  * atomics land in.
+ *
+ * Work layout: reads are grouped by read group; a workgroup takes a tile of up
+ * to 256 reads of one group and keeps that group's hot table entries in LDS.
+ * Each wavefront handles one read at a time with its 64 lanes spread over the
+ * read's bases (positions lane, lane+64, lane+128, lane+192 of the clipped
+ * window), so every per-base load is coalesced.  The GATK block-averaged
+ * fractional errors are evaluated per base from 256-bit ballot masks of the
+ * BAQ blocks and error bits, with no per-read serial pass.  Windows longer
+ * than 256 bases take a serial path on one lane.
  */
 #include "fp_device.h"
 
@@ -21,10 +31,13 @@ struct CovArgs {
     ReadStateDev s;
     TablesDev t;
     uint32_t frac_f64;
-    uint32_t *counters;   // [4] reads with an out-of-range read group
+    uint32_t ncodes;             // cycle codes kept in LDS (2 * max read length + 2)
+    const uint32_t *rg_list;     // active reads grouped by read group
+    const uint32_t *rg_offset;   // num_rg + 1
+    const uint32_t *tile_offset; // num_rg + 1 (tiles of 256 reads)
 };
 
-// calculateFractionalErrorArray for one error class over the clipped window
+// calculateFractionalErrorArray for one error class over the clipped window (serial form)
 __device__ void fractional_errors(const uint8_t *baq, const uint8_t *flags, int len, uint8_t bit,
                                   double *out /* stride 3 */)
 {
@@ -56,18 +69,6 @@ __device__ void fractional_errors(const uint8_t *baq, const uint8_t *flags, int 
         for (int j = s0; j <= end; j++) tot += (flags[j] & bit) ? 1 : 0;
         const double v = ((double)tot) / ((double)(end - s0 + 1));
         for (int j = s0; j <= end; j++) out[3 * j] = v;
-    }
-}
-
-__device__ inline void table_add(const TablesDev &t, int tab, uint32_t key, double v, uint32_t frac_f64)
-{
-    atomicAdd((unsigned long long *)&t.count[tab][key], 1ull);
-    if (v != 0.0) {
-        const double vv = frac_f64 ? v : (double)(float)v;
-        const u128 f = fixed_from_double(vv);
-        const unsigned long long old = atomicAdd((unsigned long long *)&t.sum_lo[tab][key], (unsigned long long)f.lo);
-        const uint64_t carry = (old + f.lo < old) ? 1 : 0;
-        if (f.hi + carry) atomicAdd((unsigned long long *)&t.sum_hi[tab][key], (unsigned long long)(f.hi + carry));
     }
 }
 
@@ -112,15 +113,98 @@ __device__ inline void context_keys(const uint32_t *reads, uint64_t rs, uint16_t
     if (n >= 3) kid = ((ctx >> ((n - 3) * 2)) << 4) | 3;
 }
 
-__global__ void __launch_bounds__(256) k_cov(CovArgs a)
-{
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    const BatchDev &b = a.b;
-    if (r >= b.num_reads) return;
-    if (a.s.status[r] != RS_ACTIVE) return;
-    const uint32_t rg = b.read_group[r];
-    if (rg >= a.t.num_rg) { atomicAdd(&a.counters[4], 1u); return; }
+// LDS-privatised accumulation: an entry holds a count and an exact u128 sum
+struct LdsAcc {
+    unsigned long long *lo;
+    unsigned long long *hi;
+    uint32_t *cnt;
+};
 
+__device__ inline void lds_add_n(const LdsAcc &L, uint32_t e, uint32_t n, uint64_t lo, uint64_t hi)
+{
+    atomicAdd(&L.cnt[e], n);
+    if (lo | hi) {
+        const unsigned long long old = atomicAdd(&L.lo[e], (unsigned long long)lo);
+        const uint64_t carry = (old + lo < old) ? 1 : 0;
+        if (hi + carry) atomicAdd(&L.hi[e], (unsigned long long)(hi + carry));
+    }
+}
+
+__device__ inline void lds_add(const LdsAcc &L, uint32_t e, const u128 &f) { lds_add_n(L, e, 1, f.lo, f.hi); }
+
+__device__ inline void global_add(const TablesDev &t, int tab, uint32_t key, uint64_t count, uint64_t lo, uint64_t hi)
+{
+    atomicAdd((unsigned long long *)&t.count[tab][key], (unsigned long long)count);
+    if (lo | hi) {
+        const unsigned long long old = atomicAdd((unsigned long long *)&t.sum_lo[tab][key], (unsigned long long)lo);
+        const uint64_t carry = (old + lo < old) ? 1 : 0;
+        if (hi + carry) atomicAdd((unsigned long long *)&t.sum_hi[tab][key], (unsigned long long)(hi + carry));
+    }
+}
+
+__device__ inline u128 to_fixed(double v, uint32_t f64)
+{
+    return fixed_from_double(f64 ? v : (double)(float)v);
+}
+
+__device__ inline void add128(uint64_t &lo, uint64_t &hi, uint64_t blo, uint64_t bhi)
+{
+    const uint64_t s = lo + blo;
+    hi += bhi + (s < lo ? 1 : 0);
+    lo = s;
+}
+
+// LDS entry map of one read-group tile
+//   [0, 64)                quality table, event M, quality q < 64
+//   64, 65                 quality table, events I and D (quality 45)
+//   [66, 66+1024)          context M: q < 64 x 16 two-base contexts
+//   [1090, 1090+128)       context I/D: 2 x 64 three-base contexts
+//   [1218, 1218+2*ncodes)  cycle I/D: 2 x cycle codes
+static constexpr uint32_t E_CM = 66, E_CID = 66 + 1024, E_YID = 66 + 1024 + 128;
+
+// the covariate keys of one base and their accumulation
+__device__ inline void emit_base(const CovArgs &a, const LdsAcc &L, uint32_t rg, uint32_t ncodes, uint16_t flags,
+                                 bool neg, uint64_t rs, uint16_t rwx, uint16_t rwy, uint16_t hx, uint16_t hy,
+                                 uint16_t bp, uint32_t q, const u128 &f0, const u128 &f1, const u128 &f2,
+                                 bool id_in_lds)
+{
+    // RecalTable1: EventTracker(2) | QualityScore(8) | ReadGroup(8)
+    if (q < 64) lds_add(L, q, f0);
+    else global_add(a.t, 0, 0u | (q << 2) | (rg << 10), 1, f0.lo, f0.hi);
+    if (id_in_lds) {   // the wave path reduces these across lanes itself
+        lds_add(L, 64, f1);
+        lds_add(L, 65, f2);
+    }
+    // cycle: EventTracker(2) | Cycle(10) | QualityScore(8) | ReadGroup(8)
+    uint32_t cm, cid;
+    cycle_keys(flags, rwx, rwy, bp, cm, cid);
+    global_add(a.t, 1, 0u | (cm << 2) | (q << 12) | (rg << 20), 1, f0.lo, f0.hi);
+    if (cid != 1023u) {
+        if (cid < ncodes) {
+            lds_add(L, E_YID + cid, f1);
+            lds_add(L, E_YID + ncodes + cid, f2);
+        } else {
+            global_add(a.t, 1, 1u | (cid << 2) | (45u << 12) | (rg << 20), 1, f1.lo, f1.hi);
+            global_add(a.t, 1, 2u | (cid << 2) | (45u << 12) | (rg << 20), 1, f2.lo, f2.hi);
+        }
+    }
+    // context: EventTracker(2) | Context(11) | QualityScore(8) | ReadGroup(8)
+    uint32_t xm, xid;
+    context_keys(a.b.reads, rs, hx, hy, neg, bp, xm, xid);
+    if (xm != 2047u) {
+        if (q < 64) lds_add(L, E_CM + q * 16 + (xm >> 4), f0);
+        else global_add(a.t, 2, 0u | (xm << 2) | (q << 13) | (rg << 21), 1, f0.lo, f0.hi);
+    }
+    if (xid != 2047u) {
+        lds_add(L, E_CID + (xid >> 4), f1);
+        lds_add(L, E_CID + 64 + (xid >> 4), f2);
+    }
+}
+
+// one read on one lane (windows longer than 256 bases)
+__device__ void cov_read_serial(const CovArgs &a, const LdsAcc &L, uint32_t r, uint32_t rg, uint32_t ncodes)
+{
+    const BatchDev &b = a.b;
     const uint32_t rw = a.s.rwc[r];
     const uint16_t rwx = lo16(rw), rwy = hi16(rw);
     const uint32_t rs = b.read_start[r];
@@ -129,11 +213,12 @@ __global__ void __launch_bounds__(256) k_cov(CovArgs a)
     const uint8_t *baq = a.s.baq + qs + rwx;
     const uint8_t *bfl = a.s.base_flags + rs + rwx;
     double *frac = a.s.frac + 3ull * (qs + rwx);
+    for (uint32_t k = 0; k < b.read_len[r]; k++)
+        if (k < rwx || k > rwy) { double *z = a.s.frac + 3ull * (qs + k); z[0] = 0.0; z[1] = 0.0; z[2] = 0.0; }
     fractional_errors(baq, bfl, len, BF_SNP, frac + 0);
     fractional_errors(baq, bfl, len, BF_INS, frac + 1);
     fractional_errors(baq, bfl, len, BF_DEL, frac + 2);
 
-    // high quality window
     uint16_t hx = rwx, hy = rwy;
     while (b.quals[qs + hx] <= 2 && hx < hy) hx++;
     while (b.quals[qs + hy] <= 2 && hy > hx) hy--;
@@ -147,48 +232,325 @@ __global__ void __launch_bounds__(256) k_cov(CovArgs a)
     for (uint32_t oi = 0; oi < nops; oi++) {
         const uint32_t op = op_code(ops[oi]), l = op_len(ops[oi]);
         if (!op_reads(op)) continue;
-        const uint32_t ev = op_event(op);
-        if (ev != EV_S) {
+        if (op_event(op) != EV_S) {
             for (uint32_t k = 0; k < l; k++) {
                 const uint16_t bp = (uint16_t)(ro + k);
                 if (bp < rwx || bp > rwy) continue;
                 if (a.s.base_flags[rs + bp] & BF_MASKED) continue;
-                const uint32_t q = b.quals[qs + bp];
                 const double *fe = a.s.frac + 3ull * (qs + bp);
-                // RecalTable1: EventTracker(2) | QualityScore(8) | ReadGroup(8)
-                table_add(a.t, 0, 0u | (q << 2) | (rg << 10), fe[0], a.frac_f64);
-                table_add(a.t, 0, 1u | (45u << 2) | (rg << 10), fe[1], a.frac_f64);
-                table_add(a.t, 0, 2u | (45u << 2) | (rg << 10), fe[2], a.frac_f64);
-                // cycle: EventTracker(2) | Cycle(10) | QualityScore(8) | ReadGroup(8)
-                uint32_t cm, cid;
-                cycle_keys(flags, rwx, rwy, bp, cm, cid);
-                table_add(a.t, 1, 0u | (cm << 2) | (q << 12) | (rg << 20), fe[0], a.frac_f64);
-                if (cid != 1023u) {
-                    table_add(a.t, 1, 1u | (cid << 2) | (45u << 12) | (rg << 20), fe[1], a.frac_f64);
-                    table_add(a.t, 1, 2u | (cid << 2) | (45u << 12) | (rg << 20), fe[2], a.frac_f64);
-                }
-                // context: EventTracker(2) | Context(11) | QualityScore(8) | ReadGroup(8)
-                uint32_t xm, xid;
-                context_keys(b.reads, rs, hx, hy, neg, bp, xm, xid);
-                if (xm != 2047u) table_add(a.t, 2, 0u | (xm << 2) | (q << 13) | (rg << 21), fe[0], a.frac_f64);
-                if (xid != 2047u) {
-                    table_add(a.t, 2, 1u | (xid << 2) | (45u << 13) | (rg << 21), fe[1], a.frac_f64);
-                    table_add(a.t, 2, 2u | (xid << 2) | (45u << 13) | (rg << 21), fe[2], a.frac_f64);
-                }
+                emit_base(a, L, rg, ncodes, flags, neg, rs, rwx, rwy, hx, hy, bp, b.quals[qs + bp],
+                          to_fixed(fe[0], a.frac_f64), to_fixed(fe[1], a.frac_f64), to_fixed(fe[2], a.frac_f64),
+                          true);
             }
         }
         ro = (uint16_t)(ro + l);
     }
 }
 
+// ---- 256-bit masks (4 x u64, bit j = window position j) -------------------
+
+__device__ inline bool mtest(const uint64_t *m, int j) { return (m[j >> 6] >> (j & 63)) & 1; }
+
+// highest index < j whose bit is clear, or -1
+__device__ inline int prev_zero(const uint64_t *m, int j)
+{
+    for (int w = (j - 1) >> 6; w >= 0; w--) {
+        uint64_t z = ~m[w];
+        if (w == ((j - 1) >> 6)) {
+            const int top = (j - 1) & 63;          // keep bits 0..top
+            z &= (top == 63) ? ~0ull : ((1ull << (top + 1)) - 1);
+        }
+        if (z) return w * 64 + 63 - __builtin_clzll(z);
+    }
+    return -1;
+}
+
+// lowest index >= j whose bit is clear, or 256
+__device__ inline int next_zero(const uint64_t *m, int j)
+{
+    for (int w = j >> 6; w < 4; w++) {
+        uint64_t z = ~m[w];
+        if (w == (j >> 6)) z &= ~0ull << (j & 63);
+        if (z) return w * 64 + __builtin_ctzll(z);
+    }
+    return 256;
+}
+
+// number of set bits in [a, b]
+__device__ inline int pop_range(const uint64_t *m, int a, int b)
+{
+    int n = 0;
+    for (int w = a >> 6; w <= (b >> 6); w++) {
+        uint64_t v = m[w];
+        if (w == (a >> 6)) v &= ~0ull << (a & 63);
+        if (w == (b >> 6)) v &= ((b & 63) == 63) ? ~0ull : ((1ull << ((b & 63) + 1)) - 1);
+        n += __popcll(v);
+    }
+    return n;
+}
+
+// one read on one wavefront (clipped window of at most 256 bases)
+__device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, uint32_t r, uint32_t rg, uint32_t ncodes)
+{
+    const BatchDev &b = a.b;
+    const int lane = threadIdx.x & 63;
+    const uint32_t rw = a.s.rwc[r];
+    const uint16_t rwx = lo16(rw), rwy = hi16(rw);
+    const uint32_t rs = b.read_start[r];
+    const uint32_t qs = b.qual_start[r];
+    const uint32_t L_read = b.read_len[r];
+    const int len = (int)rwy - (int)rwx + 1;
+
+    // zero the fractional errors outside the window (defined output for every quality of the read)
+    for (uint32_t k = lane; k < L_read; k += 64)
+        if (k < rwx || k > rwy) { double *z = a.s.frac + 3ull * (qs + k); z[0] = 0.0; z[1] = 0.0; z[2] = 0.0; }
+
+    uint8_t tag[4], fl[4], qv[4];
+    uint64_t IB[4], ES[4], EI_[4], ED[4], LQ[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int j = lane + 64 * c;
+        const bool v = j < len;
+        const uint32_t p = rwx + j;
+        tag[c] = v ? a.s.baq[qs + p] : 64;
+        fl[c] = v ? a.s.base_flags[rs + p] : 0;
+        qv[c] = v ? b.quals[qs + p] : 255;
+        IB[c] = __ballot(v && tag[c] != 64);
+        ES[c] = __ballot(v && (fl[c] & BF_SNP));
+        EI_[c] = __ballot(v && (fl[c] & BF_INS));
+        ED[c] = __ballot(v && (fl[c] & BF_DEL));
+        LQ[c] = __ballot(v && qv[c] <= 2);
+    }
+    // high quality window (covariates.cu:215-246)
+    int hxr = next_zero(LQ, 0);
+    if (hxr > len - 1) hxr = len - 1;
+    int hyr = prev_zero(LQ, len);
+    if (hyr < hxr) hyr = hxr;
+    const uint16_t hx = (uint16_t)(rwx + hxr), hy = (uint16_t)(rwx + hyr);
+    if (lane == 0) a.s.hqw[r] = pack2(hx, hy);
+
+    const uint16_t flags = b.flags[r];
+    const bool neg = (flags & F_REVERSE) != 0;
+    const uint32_t *ops = b.cigars + b.cigar_start[r];
+    const uint32_t nops = b.cigar_len[r];
+
+    uint32_t nid = 0;
+    uint64_t ilo = 0, ihi = 0, dlo = 0, dhi = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int j = lane + 64 * c;
+        if (j >= len) continue;
+        // the block (if any) whose average this base carries
+        double vs, vi, vd;
+        int s0 = -1, e = -1;
+        if (mtest(IB, j)) {
+            const int bs = prev_zero(IB, j) + 1;
+            const int be = next_zero(IB, j) - 1;
+            e = be + 1 < len ? be + 1 : be;
+            s0 = bs - 1 > 0 ? bs - 1 : 0;
+        } else if (j + 1 < len && mtest(IB, j + 1)) {
+            const int be = next_zero(IB, j + 1) - 1;
+            e = be + 1 < len ? be + 1 : be;
+            s0 = j;
+        } else if (j > 0 && mtest(IB, j - 1)) {
+            const int bs = prev_zero(IB, j - 1) + 1;
+            s0 = bs - 1 > 0 ? bs - 1 : 0;
+            e = j;
+        }
+        if (s0 >= 0) {
+            const double n = (double)(e - s0 + 1);
+            vs = (double)pop_range(ES, s0, e) / n;
+            vi = (double)pop_range(EI_, s0, e) / n;
+            vd = (double)pop_range(ED, s0, e) / n;
+        } else {
+            vs = mtest(ES, j) ? 1.0 : 0.0;
+            vi = mtest(EI_, j) ? 1.0 : 0.0;
+            vd = mtest(ED, j) ? 1.0 : 0.0;
+        }
+        const uint16_t bp = (uint16_t)(rwx + j);
+        double *fo = a.s.frac + 3ull * (qs + bp);
+        fo[0] = vs; fo[1] = vi; fo[2] = vd;
+
+        if (fl[c] & BF_MASKED) continue;
+        // soft-clipped bases emit nothing
+        bool soft = false;
+        {
+            uint32_t ro = 0;
+            for (uint32_t oi = 0; oi < nops; oi++) {
+                const uint32_t op = op_code(ops[oi]), l = op_len(ops[oi]);
+                if (!op_reads(op)) continue;
+                if (bp >= ro && bp < ro + l) { soft = op_event(op) == EV_S; break; }
+                ro += l;
+            }
+        }
+        if (soft) continue;
+        const u128 f0 = to_fixed(vs, a.frac_f64), f1 = to_fixed(vi, a.frac_f64), f2 = to_fixed(vd, a.frac_f64);
+        nid++;
+        add128(ilo, ihi, f1.lo, f1.hi);
+        add128(dlo, dhi, f2.lo, f2.hi);
+        emit_base(a, L, rg, ncodes, flags, neg, rs, rwx, rwy, hx, hy, bp, qv[c], f0, f1, f2, false);
+    }
+    // the quality-table I and D keys are the same for every base of the read: one LDS update per read
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        nid += __shfl_xor(nid, off);
+        const uint64_t ol = __shfl_xor(ilo, off), oh = __shfl_xor(ihi, off);
+        add128(ilo, ihi, ol, oh);
+        const uint64_t pl = __shfl_xor(dlo, off), ph = __shfl_xor(dhi, off);
+        add128(dlo, dhi, pl, ph);
+    }
+    if (lane == 0 && nid) {
+        lds_add_n(L, 64, nid, ilo, ihi);
+        lds_add_n(L, 65, nid, dlo, dhi);
+    }
+}
+
+// one workgroup per (read group, tile of 256 reads of that group)
+__global__ void __launch_bounds__(256) k_cov(CovArgs a)
+{
+    extern __shared__ unsigned long long lds_raw[];
+    const uint32_t ncodes = a.ncodes;
+    const uint32_t E = E_YID + 2 * ncodes;
+    LdsAcc L;
+    L.lo = lds_raw;
+    L.hi = lds_raw + E;
+    L.cnt = (uint32_t *)(lds_raw + 2 * E);
+
+    // which read group / tile this block covers
+    const uint32_t nrg = a.t.num_rg;
+    const uint32_t blk = blockIdx.x;
+    if (blk >= a.tile_offset[nrg]) return;
+    uint32_t lo_rg = 0, hi_rg = nrg - 1;
+    while (lo_rg < hi_rg) {      // last rg with tile_offset[rg] <= blk
+        const uint32_t mid = (lo_rg + hi_rg + 1) >> 1;
+        if (a.tile_offset[mid] <= blk) lo_rg = mid; else hi_rg = mid - 1;
+    }
+    const uint32_t rg = lo_rg;
+    const uint32_t tile = blk - a.tile_offset[rg];
+    const uint32_t first = a.rg_offset[rg] + tile * 256;
+    const uint32_t last = min(a.rg_offset[rg + 1], first + 256);
+
+    for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) { L.lo[e] = 0; L.hi[e] = 0; L.cnt[e] = 0; }
+    __syncthreads();
+
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint32_t li = first + wave; li < last; li += 4) {
+        const uint32_t r = a.rg_list[li];
+        const uint32_t rw = a.s.rwc[r];
+        const int len = (int)hi16(rw) - (int)lo16(rw) + 1;
+        if (len <= 256) cov_read_wave(a, L, r, rg, ncodes);
+        else if (lane == 0) cov_read_serial(a, L, r, rg, ncodes);
+    }
+    __syncthreads();
+
+    // flush the tile's entries into the resident tables
+    for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) {
+        const uint32_t c = L.cnt[e];
+        if (!c) continue;
+        int tab;
+        uint32_t key;
+        if (e < 64) { tab = 0; key = 0u | (e << 2) | (rg << 10); }
+        else if (e < 66) { tab = 0; key = (e - 63) | (45u << 2) | (rg << 10); }
+        else if (e < E_CID) {
+            const uint32_t q = (e - E_CM) >> 4, bits = (e - E_CM) & 15;
+            tab = 2; key = 0u | (((bits << 4) | 2) << 2) | (q << 13) | (rg << 21);
+        } else if (e < E_YID) {
+            const uint32_t ev = 1 + (e - E_CID) / 64, bits = (e - E_CID) & 63;
+            tab = 2; key = ev | (((bits << 4) | 3) << 2) | (45u << 13) | (rg << 21);
+        } else {
+            const uint32_t ev = 1 + (e - E_YID) / ncodes, code = (e - E_YID) % ncodes;
+            tab = 1; key = ev | (code << 2) | (45u << 12) | (rg << 20);
+        }
+        global_add(a.t, tab, key, c, L.lo[e], L.hi[e]);
+    }
+}
+
+// wave-aggregated append: one atomic per distinct key in the wavefront.
+// Returns this lane's slot (counter value before its own increment).
+__device__ inline uint32_t wave_append(uint32_t *counters, uint32_t key, bool active)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t slot = 0;
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const uint32_t k = __shfl(key, leader);
+        const uint64_t same = __ballot(active && key == k);
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&counters[k], (uint32_t)__popcll(same));
+        base = __shfl(base, leader);
+        if (active && key == k) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1));
+        pending &= ~same;
+    }
+    return slot;
+}
+
+// per-read-group read lists: counts -> offsets -> scatter
+__global__ void __launch_bounds__(256) k_rg_count(uint32_t n, const uint8_t *status, const uint32_t *read_group,
+                                                  uint32_t num_rg, uint32_t *rg_count, uint32_t *errors)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = r < n && status[r] == RS_ACTIVE;
+    const uint32_t rg = act ? read_group[r] : 0;
+    const bool bad = act && rg >= num_rg;
+    if (__ballot(bad) && bad) atomicAdd(errors, 1u);
+    wave_append(rg_count, rg, act && !bad);
+}
+
+__global__ void __launch_bounds__(256) k_rg_offsets(uint32_t num_rg, const uint32_t *rg_count, uint32_t *rg_offset,
+                                                    uint32_t *tile_offset, uint32_t *rg_cursor)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t o = 0, t = 0;
+    for (uint32_t g = 0; g < num_rg; g++) {
+        rg_offset[g] = o;
+        tile_offset[g] = t;
+        rg_cursor[g] = 0;
+        o += rg_count[g];
+        t += (rg_count[g] + 255) / 256;
+    }
+    rg_offset[num_rg] = o;
+    tile_offset[num_rg] = t;
+}
+
+__global__ void __launch_bounds__(256) k_rg_scatter(uint32_t n, const uint8_t *status, const uint32_t *read_group,
+                                                    uint32_t num_rg, const uint32_t *rg_offset, uint32_t *rg_cursor,
+                                                    uint32_t *rg_list)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = r < n && status[r] == RS_ACTIVE;
+    const uint32_t rg = act ? read_group[r] : 0;
+    const bool ok = act && rg < num_rg;
+    const uint32_t slot = wave_append(rg_cursor, rg, ok);
+    if (ok) rg_list[rg_offset[rg] + slot] = r;
+}
+
 } // namespace fpd
 
 extern "C" hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
-                                     uint32_t frac_f64, uint32_t *counters, hipStream_t st)
+                                     uint32_t frac_f64, uint32_t *counters, uint32_t *rg_buf, uint32_t max_read_len,
+                                     hipStream_t st)
 {
-    if (b->num_reads == 0) return hipSuccess;
+    const uint32_t n = b->num_reads;
+    if (n == 0) return hipSuccess;
+    const uint32_t nrg = t->num_rg;
+    // rg_buf layout: count[nrg] | cursor[nrg] | offset[nrg+1] | tile_offset[nrg+1] | list[n]
+    uint32_t *cnt = rg_buf, *cur = rg_buf + nrg, *off = rg_buf + 2 * nrg, *toff = rg_buf + 3 * nrg + 1;
+    uint32_t *list = rg_buf + 4 * nrg + 2;
+    hipError_t e = hipMemsetAsync(cnt, 0, 4ull * nrg, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fpd::k_rg_count, dim3((n + 255) / 256), dim3(256), 0, st, n, s->status, b->read_group, nrg, cnt,
+                       counters + 4);
+    hipLaunchKernelGGL(fpd::k_rg_offsets, dim3(1), dim3(64), 0, st, nrg, cnt, off, toff, cur);
+    hipLaunchKernelGGL(fpd::k_rg_scatter, dim3((n + 255) / 256), dim3(256), 0, st, n, s->status, b->read_group, nrg,
+                       off, cur, list);
     fpd::CovArgs a;
-    a.b = *b; a.s = *s; a.t = *t; a.frac_f64 = frac_f64; a.counters = counters;
-    hipLaunchKernelGGL(fpd::k_cov, dim3((b->num_reads + 255) / 256), dim3(256), 0, st, a);
+    a.b = *b; a.s = *s; a.t = *t; a.frac_f64 = frac_f64;
+    a.ncodes = 2 * max_read_len + 2;
+    a.rg_list = list; a.rg_offset = off; a.tile_offset = toff;
+    const uint32_t E = fpd::E_YID + 2 * a.ncodes;
+    const size_t lds = (size_t)E * 20;
+    const uint32_t blocks = (n + 255) / 256 + nrg;
+    hipLaunchKernelGGL(fpd::k_cov, dim3(blocks), dim3(256), lds, st, a);
     return hipGetLastError();
 }

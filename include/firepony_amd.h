@@ -60,10 +60,20 @@ typedef struct fp_options {
                                         before accumulation, as the reference does
                                         (device/covariates.cu:91-101); 1: kept in fp64 */
     uint32_t disable_output_rounding;/* report floats as %.64f (table_formatter.cu:102) */
-    uint32_t baq_lanes_per_read;     /* BAQ tile shape: 1 (lane per read); 0 = autotune */
-    uint32_t sort_digit_bits;        /* radix digit width 4..8; 0 = autotune           */
-    uint32_t reserved[8];
+    uint32_t baq_waves_per_simd;     /* BAQ HMM register budget: 1 or 2 waves per SIMD;
+                                        0 = chosen by the engine's autotuner            */
+    uint32_t sort_digit_bits;        /* FP_AGG_SORT radix digit width 4..8; 0 = autotune */
+    uint32_t max_read_length;        /* upper bound of read lengths; lets batches run with
+                                        no host round trip; 0 = measured per batch      */
+    uint32_t aggregation;            /* fp_aggregation                                 */
+    uint32_t reserved[6];
 } fp_options;
+
+/* how covariate observations reach the resident tables (identical results) */
+typedef enum fp_aggregation {
+    FP_AGG_DIRECT = 0,   /* LDS-privatised direct-indexed tables (default)            */
+    FP_AGG_SORT = 1      /* key/value pairs -> LDS-tiled radix sort -> segmented reduce */
+} fp_aggregation;
 
 /* Fill *opts with defaults. */
 void fp_options_init(fp_options *opts);

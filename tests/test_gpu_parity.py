@@ -117,11 +117,12 @@ def run_parity(tmp_path, genome, rs, n, batch_size, per_batch=True, **engine_opt
     e.close()
 
 
-def test_config0_slice(tmp_path):
-    say("parity: config 0 first 4000 reads")
+@pytest.mark.parametrize("wpe", [1, 2])
+def test_config0_slice(tmp_path, wpe):
+    say(f"parity: config 0 first 4000 reads, BAQ register budget for {wpe} waves/SIMD")
     gs, rs, n = synth.named_spec(0)
     g = synth.Genome(gs, rs.num_read_groups)
-    run_parity(tmp_path, g, rs, 4000, 1000)
+    run_parity(tmp_path, g, rs, 4000, 1000, baq_waves_per_simd=wpe)
 
 
 def test_edge_cases(tmp_path):
