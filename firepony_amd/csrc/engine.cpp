@@ -34,7 +34,8 @@ hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const fpd::Genome
                               const uint32_t *list, const uint32_t *count, double *scratch, uint64_t slot_doubles,
                               uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves, unsigned long long *cells,
                               int wpe, hipStream_t st);
-hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *s, hipStream_t st);
+hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *s, uint32_t band_done,
+                               hipStream_t st);
 hipError_t fpd_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st);
 hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
                           uint32_t frac_f64, uint32_t *counters, uint32_t *rg_buf, uint32_t max_read_len,
@@ -171,7 +172,7 @@ struct fp_engine {
     hipStream_t stream = nullptr;
     std::string name;
     HostConstants consts;
-    int wpe = 1;   // BAQ HMM waves per SIMD
+    int wpe = 2;   // waves per SIMD of the 15-cell BAQ band kernel (1-4)
 
     // setup state
     bool is_setup = false;
@@ -329,14 +330,21 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
         HIP_TRY(hipMemcpy(bwv.data(), e->rs.bandwidth, 2ull * n, hipMemcpyDeviceToHost));
         for (uint32_t id : ids) wmax_generic = std::max<uint32_t>(wmax_generic, 2u * bwv[id] + 1);
     }
-    const uint32_t max_waves = 2048;
     uint64_t need = 0;
     uint32_t waves[4];
     uint64_t slot[4];
     for (int c = 0; c < 4; c++) {
         const uint32_t W = c < 3 ? BAQ_CLASS_W[c] : wmax_generic;
-        waves[c] = std::min<uint32_t>((cnt[c] + 63) / 64, max_waves);
-        slot[c] = ((uint64_t)(rows_cap + 1) * W * 2 + (rows_cap + 2) + (c == 3 ? 3ull * W : 0)) * 64;
+        if (c < 3) {
+            // band kernels: one resident set of 4-wave workgroups (256 CUs x 4 SIMDs x waves per SIMD)
+            const uint32_t wpe = c == 0 ? (uint32_t)e->wpe : 1u;
+            waves[c] = std::min<uint32_t>((cnt[c] + 63) / 64, 1024u * wpe);
+            waves[c] = (waves[c] + 3) / 4 * 4;
+        } else {
+            waves[c] = std::min<uint32_t>((cnt[c] + 63) / 64, 2048u);
+        }
+        // forward rows (M, I) | scale factors | staged MAP results (band kernels) or band state (generic)
+        slot[c] = ((uint64_t)(rows_cap + 1) * W * 2 + (rows_cap + 2) + (c == 3 ? 3ull * W : rows_cap + 1ull)) * 64;
         if (cnt[c]) need = std::max(need, slot[c] * waves[c]);
     }
     HIP_TRY(e->scratch.ensure(need * 8 + 8));
@@ -352,7 +360,7 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
         hmm_launches++;
     }
     HIP_TRY(hipEventRecord(e->ev[9], st));
-    HIP_TRY(fpd_launch_baq_post(&b, &e->rs, st));
+    HIP_TRY(fpd_launch_baq_post(&b, &e->rs, 1u, st));
     HIP_TRY(hipEventRecord(e->ev[2], st));
     HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, e->counters.as<uint32_t>(),
                            e->rg_buf.as<uint32_t>(), std::min<uint32_t>(rows_cap, 499), st));
@@ -542,7 +550,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     fp_engine *e = new fp_engine();
     e->device = hip_device;
     e->opts = o;
-    e->wpe = o.baq_waves_per_simd == 2 ? 2 : 1;
+    e->wpe = o.baq_waves_per_simd >= 1 && o.baq_waves_per_simd <= 4 ? (int)o.baq_waves_per_simd : 2;
     memset(&e->stats, 0, sizeof(e->stats));
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) e->name = prop.name;

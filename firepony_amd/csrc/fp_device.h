@@ -122,6 +122,9 @@ __device__ inline uint8_t read_base(const uint32_t *words, uint64_t i)
     return (uint8_t)((words[i >> 3] >> ((i & 7) * 4)) & 0xf);
 }
 
+// the packed word holding base i (8 bases per word)
+__device__ inline uint32_t read_word(const uint32_t *words, uint64_t i) { return words[i >> 3]; }
+
 __device__ inline uint8_t contig_base(const GenomeDev &g, uint32_t c, uint32_t i)
 {
     if (c >= g.num_contigs || i >= g.length[c]) return B_N;   // past the contig: N

@@ -304,15 +304,441 @@ __global__ void __launch_bounds__(64, WPE) k_baq_hmm(BaqArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// cap_baq_qualities + BQ-tag recoding of one BAQ read (baq.cu:870-1056, :1058-1081).
+//
+// The reference caps the MAP qualities in place, then recodes every quality
+// that carries a BAQ value into the tag (q + 64) - baq.  When the check pass
+// succeeds the update pass visits every window position exactly once, in
+// order, so both steps fold into one walk that emits each final tag directly.
+// mapq(i) / mstate(i) give the MAP quality and state of read position i;
+// put(i, tag) stores the tag of read position i.
+template <class MQ, class MS, class PUT>
+__device__ inline void cap_recode(const BatchDev &b, const ReadStateDev &s, uint32_t r, MQ mapq, MS mstate, PUT put)
+{
+    const uint32_t rw = s.rwc[r];
+    const uint16_t rwx = lo16(rw), rwy = hi16(rw);
+    const uint32_t qs = b.qual_start[r];
+    const uint32_t *ops = b.cigars + b.cigar_start[r];
+    const uint32_t nops = b.cigar_len[r];
+    const int hx = (int16_t)lo16(s.hmmw[r]);
+    const uint16_t fwx = lo16(s.refw[r]);
+    const uint32_t astart = b.alignment_start[r];
+
+    int16_t reference_shift, read_shift;
+    const uint32_t ahx = hx < 0 ? (uint32_t)(-hx) : (uint32_t)hx;
+    if (hx < 0 && astart < ahx) { reference_shift = (int16_t)ahx; read_shift = (int16_t)astart; }
+    else { reference_shift = 0; read_shift = 0; }
+    const int16_t refOffset = (int16_t)(hx - (int)fwx + reference_shift);
+
+    // baq_start: event index of the first read position inside the window
+    uint32_t baq_start = 0;
+    {
+        uint32_t e = 0;
+        uint16_t ro = 0;
+        for (uint32_t i = 0; i < nops; i++) {
+            const uint32_t op = op_code(ops[i]), l = op_len(ops[i]);
+            if (op_reads(op)) {
+                if ((uint32_t)ro + l > rwx) { baq_start = e + (rwx > ro ? rwx - ro : 0); break; }
+                ro = (uint16_t)(ro + l);
+            }
+            e += l;
+        }
+    }
+    bool good = true;
+    for (int pass = 0; pass < 2 && good; pass++) {
+        const bool update = pass == 1;
+        uint32_t readI = 0, refI = 0, numD = 0;
+        uint32_t e = 0;
+        uint16_t ro = 0;
+        bool stop = false;
+        for (uint32_t oi = 0; oi < nops && !stop; oi++) {
+            const uint32_t op = op_code(ops[oi]), l = op_len(ops[oi]);
+            const uint32_t t = op_event(op);
+            const bool reads = op_reads(op);
+            for (uint32_t k = 0; k < l; k++) {
+                const uint32_t i = e + k;
+                if (i < baq_start) continue;
+                const uint16_t rb = reads ? (uint16_t)(ro + k) : NONE16;
+                if (rb != NONE16) {
+                    if (rb < rwx) continue;
+                    if (rb > rwy) { stop = true; break; }
+                }
+                if (t == EV_S) {
+                    if (update) put(rb, (uint8_t)((uint8_t)(b.quals[qs + rb] + 64) - mapq(rb)));
+                    refI++;
+                } else if (t == EV_I) {
+                    if (update) put(rb, (uint8_t)64);
+                    readI++;
+                } else if (t == EV_D) {
+                    refI++; numD++;
+                } else {
+                    if (update) {
+                        const uint32_t expectedPos = refI - refOffset + (i - numD - baq_start - readI) + read_shift;
+                        const uint8_t oq = b.quals[qs + rb];
+                        const uint8_t bq = mapq(rb);
+                        const uint32_t state = mstate(rb);
+                        uint8_t v;
+                        if ((state & 3) != 0 || (state >> 2) != expectedPos) v = 4;
+                        else v = bq < oq ? bq : oq;
+                        put(rb, (uint8_t)((uint8_t)(oq + 64) - v));
+                    }
+                    readI++; refI++;
+                }
+            }
+            if (reads) ro = (uint16_t)(ro + l);
+            e += l;
+        }
+        if (!update && readI != (uint32_t)(rwy - rwx + 1)) good = false;
+    }
+    // a failed check keeps the original qualities: tag 64 across the window
+    if (!good)
+        for (uint32_t i = rwx; i <= rwy; i++) put((uint16_t)i, (uint8_t)64);
+}
+
+// ---------------------------------------------------------------------------
+// Band kernels for the common band widths (15, 23 and 33 cells).
+//
+// Same arithmetic as above, laid out for occupancy: 256-lane workgroups with
+// the qual2prob and MAP threshold tables in LDS; each forward row is stored as
+// W (M, I) pairs through one row pointer (16-byte lane-interleaved stores);
+// the next row's query base, quality, reference base and scale factor and,
+// in the backward pass, the forward row that the MAP step consumes are all
+// loaded one row ahead so their latency overlaps the current row's FP64 work.
+
+template <int W, int WPE>
+__global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
+{
+    constexpr int NWIN = (W + 15) / 16;       // u64 words of band reference nibbles
+    constexpr int NREF = 2 * NWIN + 1;        // u32 reference words a band window can touch
+    constexpr int TOP = (W - 1) - 16 * (NWIN - 1);   // last used nibble of the last window word
+    constexpr uint64_t TOPMASK = TOP == 15 ? ~0ull : ((1ull << (4 * (TOP + 1))) - 1);
+
+    __shared__ double s_q2p[256];
+    __shared__ double s_thr[128];
+    for (uint32_t t = threadIdx.x; t < 256; t += blockDim.x) s_q2p[t] = c_qual2prob[t];
+    for (uint32_t t = threadIdx.x; t < 128; t += blockDim.x) s_thr[t] = c_map_thr[t];
+    __syncthreads();
+
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t total = *a.count;
+    double *slot = a.scratch + (uint64_t)gwave * a.slot_doubles;
+    double2 *Fst = (double2 *)slot;                                  // [row][r][lane] (M, I)
+    double *Sst = slot + (uint64_t)(a.rows_cap + 1) * W * 2 * 64;    // [row][lane]
+    uint64_t *Ost = (uint64_t *)(Sst + (uint64_t)(a.rows_cap + 2) * 64);  // [row][lane] MAP state | quality << 32
+    const BatchDev &b = a.b;
+    uint64_t ncell = 0;
+
+    auto qprob = [&](uint8_t q) -> double { return s_q2p[q < 4 ? 4 : q]; };
+    // MAP quality through the thresholds (largest v in [5, 101] with y <= T[v])
+    auto mapq = [&](double y) -> uint8_t {
+        if (y == 0.0) return 4;
+        int k = (int)(-4.343 * log(y) + 0.499);
+        k = k < 4 ? 4 : (k > 100 ? 100 : k);
+        const double t0 = s_thr[k], t1 = s_thr[k + 1];
+        if (k >= 5 && !(y <= t0)) {
+            k--;
+            while (k >= 5 && !(y <= s_thr[k])) k--;
+        } else if (y <= t1) {
+            k++;
+            while (k + 1 <= 101 && y <= s_thr[k + 1]) k++;
+        }
+        if (k <= 4) return 4;
+        if (k >= 101) return 99;
+        return (uint8_t)k;
+    };
+
+    for (uint32_t chunk = gwave; (uint64_t)chunk * 64 < total; chunk += nwaves) {
+        const uint32_t idx = chunk * 64 + lane;
+        if (idx >= total) continue;
+        const uint32_t rd = a.list[idx];
+
+        // ---- setup (baq.cu:260-354)
+        const uint32_t hw = a.s.hmmw[rd];
+        const int hx = (int16_t)lo16(hw), hy = (int16_t)hi16(hw);
+        const uint32_t rwv = a.s.rwc[rd];
+        const int qStart = lo16(rwv);
+        const int qLen = (int)hi16(rwv) - qStart + 1;
+        const int bw = a.s.bandwidth[rd];
+        const uint32_t astart = b.alignment_start[rd];
+        const uint32_t chrom = b.chromosome[rd];
+        const int refLenW = hy - hx + 1;
+        const uint32_t ahx = hx < 0 ? (uint32_t)(-hx) : (uint32_t)hx;
+        const bool clamp = hx < 0 && astart < ahx;
+        const int refLen = clamp ? refLenW - 1 : refLenW;
+        const int64_t hoff = clamp ? 0 : (int64_t)astart + hx;
+
+        const double GOP = c_gap_open;
+        const double sM = 1.0 / (2 * qLen + 2);
+        const double sI = sM;
+        const double bM = (1 - GOP) / refLenW;
+        const double bI = GOP / refLenW;
+        const double m0 = (1 - GOP - GOP) * (1 - sM);
+        const double m1 = GOP * (1 - sM);
+        const double m2 = m1;
+        const double m3 = (1 - GEP) * (1 - sI);
+        const double m4 = GEP * (1 - sI);
+        const double m6 = 1 - GEP;
+        const double m8 = GEP;
+        const double EIm1 = EI * m1;
+        const double EIm4 = EI * m4;
+
+        const uint64_t qbase = (uint64_t)b.read_start[rd] + qStart;
+        const uint32_t qqual = b.qual_start[rd] + qStart;
+        const uint32_t *cw = a.g.words + a.g.word_offset[chrom];
+        const int64_t clen = a.g.length[chrom];
+        const int64_t wlast = clen > 0 ? ((clen + 7) >> 3) - 1 : 0;
+        // reference word holding contig position p, index clamped into the contig (loads never branch)
+        auto ref_word = [&](int64_t p) -> uint32_t {
+            int64_t wi = p >> 3;
+            wi = wi < 0 ? 0 : (wi > wlast ? wlast : wi);
+            return cw[wi];
+        };
+        // cells of the HMM window [0, refLen) that lie inside the contig carry a base; others are N
+        auto ref_ok = [&](int idx0) -> bool { return idx0 >= 0 && idx0 < refLen && hoff + idx0 < clen; };
+        auto nibble = [&](uint32_t w, int idx0) -> uint8_t {
+            return ref_ok(idx0) ? (uint8_t)((w >> (((hoff + idx0) & 7) * 4)) & 15) : (uint8_t)B_N;
+        };
+
+        // band reference bases: nibble r of the window = HMM window index idx0 + r
+        uint64_t win[NWIN];
+        auto rb_of = [&](int r) -> uint8_t { return (uint8_t)((win[r >> 4] >> (4 * (r & 15))) & 15); };
+        auto win_fill = [&](int idx0) {
+            const int64_t p0 = hoff + idx0;
+            const int64_t w0 = p0 >> 3;               // floor
+            const int sh = (int)(p0 & 7) * 4;
+            uint32_t rw[NREF];
+#pragma unroll
+            for (int j = 0; j < NREF; j++) rw[j] = ref_word((w0 + j) * 8);
+            // valid cells [jlo, jhi)
+            int64_t jlo = -idx0, jhi = W;
+            if (jlo < 0) jlo = 0;
+            if (refLen - idx0 < jhi) jhi = refLen - idx0;
+            if (clen - p0 < jhi) jhi = clen - p0;
+#pragma unroll
+            for (int w = 0; w < NWIN; w++) {
+                const uint64_t lo = (uint64_t)rw[2 * w] | ((uint64_t)rw[2 * w + 1] << 32);
+                uint64_t v = lo >> sh;
+                if (sh) v |= (uint64_t)rw[2 * w + 2] << (64 - sh);
+                int64_t a0 = jlo - 16 * w, a1 = jhi - 16 * w;
+                a0 = a0 < 0 ? 0 : (a0 > 16 ? 16 : a0);
+                a1 = a1 < 0 ? 0 : (a1 > 16 ? 16 : a1);
+                const uint64_t below1 = a1 >= 16 ? ~0ull : ((1ull << (4 * a1)) - 1);
+                const uint64_t below0 = a0 >= 16 ? ~0ull : ((1ull << (4 * a0)) - 1);
+                const uint64_t m = a1 > a0 ? (below1 & ~below0) : 0ull;
+                v = (v & m) | (~m & 0xffffffffffffffffull);   // N = 15 in every invalid cell
+                win[w] = w == NWIN - 1 ? (v & TOPMASK) : v;
+            }
+        };
+        auto win_shr = [&](uint8_t nb) {     // band moves right: drop cell 0, append cell W-1
+#pragma unroll
+            for (int w = 0; w < NWIN; w++) {
+                const uint64_t in = (w + 1 < NWIN) ? (win[w + 1 < NWIN ? w + 1 : w] & 15) : (uint64_t)nb;
+                const int top = (w + 1 < NWIN) ? 15 : TOP;
+                win[w] = (win[w] >> 4) | (in << (4 * top));
+            }
+        };
+        auto win_shl = [&](uint8_t nb) {     // band moves left: drop cell W-1, insert cell 0
+#pragma unroll
+            for (int w = NWIN - 1; w >= 0; w--) {
+                const uint64_t in = w > 0 ? (win[w > 0 ? w - 1 : 0] >> 60) & 15 : (uint64_t)nb;
+                const uint64_t keep = (w + 1 < NWIN) ? ~0ull : TOPMASK;
+                win[w] = ((win[w] << 4) | in) & keep;
+            }
+        };
+
+        double M[W], I[W], D[W];
+
+        // ---- forward row 1 (baq.cu:396-442)
+        {
+            const uint8_t qb = read_base(b.reads, qbase);
+            const double p = qprob(b.quals[qqual]);
+            const double em_eq = 1 - p, em_ne = p * EM;
+            const int khi = refLen < bw + 1 ? refLen : bw + 1;
+            ncell += khi >= 1 ? (uint64_t)khi : 0;
+            win_fill(-bw);                       // cell r: k = 1 - bw + r, window index k - 1
+            double sum = 0.0;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = 1 - bw + r;
+                const bool in = k >= 1 && k <= khi;
+                const uint8_t rb = rb_of(r);
+                const double e = (rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne);
+                const double mv = in ? e * bM : 0.0;
+                const double iv = in ? EI * bI : 0.0;
+                sum += in ? mv + iv : 0.0;
+                M[r] = mv; I[r] = iv; D[r] = 0.0;
+            }
+            Sst[64 + lane] = sum;
+            double2 *fr = Fst + (uint64_t)1 * W * 64 + lane;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                M[r] = M[r] / sum; I[r] = I[r] / sum; D[r] = D[r] / sum;
+                fr[r * 64] = make_double2(M[r], I[r]);
+            }
+        }
+        // ---- forward rows 2..qLen (baq.cu:444-512)
+        // the next row's query base, quality and entering reference word are loaded one row ahead
+        const int qlast = qLen - 1;
+        uint32_t nqw = read_word(b.reads, qbase + (1 < qlast ? 1 : qlast));
+        uint8_t nqq = b.quals[qqual + (1 < qlast ? 1 : qlast)];
+        uint32_t nrw = ref_word(hoff + (2 - bw - 1 + (W - 1)));
+        for (int i = 2; i <= qLen; i++) {
+            const uint8_t qb = (uint8_t)((nqw >> (((qbase + i - 1) & 7) * 4)) & 15);
+            const uint8_t qq = nqq;
+            const uint8_t rbin = nibble(nrw, i - bw - 1 + (W - 1));
+            {
+                const int in_ = i < qlast ? i : qlast;
+                nqw = read_word(b.reads, qbase + in_);
+                nqq = b.quals[qqual + in_];
+                nrw = ref_word(hoff + (i + 1 - bw - 1 + (W - 1)));
+            }
+            win_shr(rbin);
+            const double p = qprob(qq);
+            const double em_eq = 1 - p, em_ne = p * EM;
+            const int klo = 1 > i - bw ? 1 : i - bw;
+            const int khi = refLen < i + bw ? refLen : i + bw;
+            if (khi >= klo) ncell += (uint64_t)(khi - klo + 1);
+            double sum = 0.0, cM = 0.0, cD = 0.0;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = i - bw + r;
+                const bool in = k >= klo && k <= khi;
+                const double pM = M[r], pI = I[r], pD = D[r];
+                const double nM = (r + 1 < W) ? M[r + 1 < W ? r + 1 : r] : 0.0;
+                const double nI = (r + 1 < W) ? I[r + 1 < W ? r + 1 : r] : 0.0;
+                const uint8_t rb = rb_of(r);
+                const double e = (rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne);
+                const double mv = in ? e * (m0 * pM + m3 * pI + m6 * pD) : 0.0;
+                const double iv = in ? EI * (m1 * nM + m4 * nI) : 0.0;
+                const double dv = in ? m2 * cM + m8 * cD : 0.0;
+                sum += in ? mv + iv + dv : 0.0;
+                M[r] = mv; I[r] = iv; D[r] = dv;
+                cM = mv; cD = dv;
+            }
+            Sst[(uint64_t)i * 64 + lane] = sum;
+            const double inv = 1. / sum;
+            double2 *fr = Fst + (uint64_t)i * W * 64 + lane;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                M[r] = M[r] * inv; I[r] = I[r] * inv; D[r] = D[r] * inv;
+                fr[r * 64] = make_double2(M[r], I[r]);
+            }
+        }
+        // ---- s[qLen+1] (baq.cu:514-525)
+        double sq1 = 0.0;
+        {
+            const int klo = 1 > qLen - bw ? 1 : qLen - bw;
+            const int khi = refLen < qLen + bw ? refLen : qLen + bw;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = qLen - bw + r;
+                const double t = M[r] * sM + I[r] * sI;
+                sq1 += (k >= klo && k <= khi) ? t : 0.0;
+            }
+        }
+
+        // ---- backward + MAP (baq.cu:528-681)
+        auto map_row = [&](int i) {
+            const double2 *fr = Fst + (uint64_t)i * W * 64 + lane;
+            double sum = 0.0, mx = 0.0;
+            int32_t stt = -1;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = i - bw + r;
+                const double2 f = fr[r * 64];
+                double z = f.x * M[r];
+                sum += z;
+                if (z > mx) { mx = z; stt = (k - 1) << 2 | 0; }
+                z = f.y * I[r];
+                sum += z;
+                if (z > mx) { mx = z; stt = (k - 1) << 2 | 1; }
+            }
+            mx /= sum;
+            // staged in the slot (one coalesced row store); capped and recoded after the backward pass
+            Ost[(uint64_t)i * 64 + lane] = (uint64_t)(uint32_t)stt | ((uint64_t)mapq(1.0 - mx) << 32);
+        };
+        {
+            const double sq = Sst[(uint64_t)qLen * 64 + lane];
+            const int klo = 1 > qLen - bw ? 1 : qLen - bw;
+            const int khi = refLen < qLen + bw ? refLen : qLen + bw;
+            const double vM = sM / sq / sq1, vI = sI / sq / sq1;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = qLen - bw + r;
+                const bool in = k >= klo && k <= khi;
+                M[r] = in ? vM : 0.0; I[r] = in ? vI : 0.0;
+            }
+            map_row(qLen);
+        }
+        win_fill(qLen - bw);                     // backward cell r of row i: window index i - bw + r
+        // backward row i emits query index i (the row below it)
+        uint32_t bqw = read_word(b.reads, qbase + qlast);
+        uint8_t bqq = b.quals[qqual + qlast];
+        uint32_t brw = ref_word(hoff + (qLen - 1 - bw));
+        double bS = Sst[(uint64_t)(qLen >= 2 ? qLen - 1 : qLen) * 64 + lane];
+        for (int i = qLen - 1; i >= 1; i--) {
+            const uint8_t rbin = nibble(brw, i - bw);
+            win_shl(rbin);
+            const double y = (i > 1) ? 1. : 0.;
+            const uint8_t qb = (uint8_t)((bqw >> (((qbase + i) & 7) * 4)) & 15);
+            const uint8_t qq = bqq;
+            const double Si = bS;
+            {
+                const int in_ = i - 1;
+                bqw = read_word(b.reads, qbase + in_);
+                bqq = b.quals[qqual + in_];
+                brw = ref_word(hoff + (i - 1 - bw));
+                bS = Sst[(uint64_t)(i >= 2 ? i - 1 : i) * 64 + lane];
+            }
+            const double p = qprob(qq);
+            const double em_eq = 1 - p, em_ne = p * EM;
+            const int klo = 1 > i - bw ? 1 : i - bw;
+            const int khi = refLen < i + bw ? refLen : i + bw;
+            double cD = 0.0;
+#pragma unroll
+            for (int r = W - 1; r >= 0; r--) {
+                const int k = i - bw + r;
+                const bool in = k >= klo && k <= khi;
+                const double pM = M[r];
+                const double pIl = r >= 1 ? I[r >= 1 ? r - 1 : 0] : 0.0;
+                const uint8_t rb = rb_of(r);
+                const double e0 = ((rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne)) * pM;
+                const double e = k >= refLen ? 0.0 : e0;
+                const double mv = in ? e * m0 + EIm1 * pIl + m2 * cD : 0.0;
+                const double iv = in ? e * m3 + EIm4 * pIl : 0.0;
+                const double dv = in ? (e * m6 + m8 * cD) * y : 0.0;
+                M[r] = mv; I[r] = iv;
+                cD = dv;
+            }
+            const double yy = 1.0 / Si;
+#pragma unroll
+            for (int r = 0; r < W; r++) { M[r] = M[r] * yy; I[r] = I[r] * yy; }
+            map_row(i);
+        }
+        // cap + recode from the staged MAP results; each lane writes its read's tags in one burst
+        const uint32_t qs = b.qual_start[rd];
+        // read position p was row p - qStart + 1
+        auto staged = [&](uint16_t p) { return Ost[(uint64_t)((int)p - qStart + 1) * 64 + lane]; };
+        cap_recode(b, a.s, rd, [&](uint16_t p) { return (uint8_t)(staged(p) >> 32); },
+                   [&](uint16_t p) { return (uint32_t)staged(p); },
+                   [&](uint16_t p, uint8_t v) { a.s.baq[qs + p] = v; });
+    }
+    // one counter update per wave
+    for (int off = 32; off >= 1; off >>= 1) ncell += __shfl_xor(ncell, off);
+    if (lane == 0 && ncell) atomicAdd(a.cells, (unsigned long long)ncell);
+}
+
+// ---------------------------------------------------------------------------
 // flat BAQ, capping and BQ-tag recoding, one lane per read
 // (read_flat_baq baq.cu:842-868, cap_baq_qualities :870-1056, recode :1058-1081)
 
 struct PostArgs {
     BatchDev b;
     ReadStateDev s;
+    uint32_t band_done;   // the band kernels already wrote the tags of reads with bands of <= 33 cells
 };
-
-__device__ inline uint32_t ev_type_of(uint32_t op) { return op_event(op); }
 
 __global__ void __launch_bounds__(256) k_baq_post(PostArgs a)
 {
@@ -328,94 +754,10 @@ __global__ void __launch_bounds__(256) k_baq_post(PostArgs a)
         for (uint32_t i = rwx; i <= rwy; i++) baq[qs + i] = 64;
         return;
     }
-    const uint32_t *ops = b.cigars + b.cigar_start[r];
-    const uint32_t nops = b.cigar_len[r];
-    const uint32_t hw = a.s.hmmw[r];
-    const int hx = (int16_t)lo16(hw);
-    const uint16_t fwx = lo16(a.s.refw[r]);
-    const uint32_t astart = b.alignment_start[r];
-
-    // find baq_start, then run the check and update passes
-    int16_t reference_shift, read_shift;
-    const uint32_t ahx = hx < 0 ? (uint32_t)(-hx) : (uint32_t)hx;
-    if (hx < 0 && astart < ahx) { reference_shift = (int16_t)ahx; read_shift = (int16_t)astart; }
-    else { reference_shift = 0; read_shift = 0; }
-    const int16_t refOffset = (int16_t)(hx - (int)fwx + reference_shift);
-
-    // pass 0: baq_start
-    uint32_t baq_start = 0;
-    {
-        uint32_t e = 0;
-        uint16_t ro = 0;
-        bool done = false;
-        for (uint32_t i = 0; i < nops && !done; i++) {
-            const uint32_t op = op_code(ops[i]), l = op_len(ops[i]);
-            if (op_reads(op)) {
-                for (uint32_t k = 0; k < l; k++) {
-                    const uint16_t rc = (uint16_t)(ro + k);
-                    if (rc >= rwx) { baq_start = e + k; done = true; break; }
-                }
-                ro = (uint16_t)(ro + l);
-            }
-            e += l;
-        }
-    }
-    bool good = true;
-    for (int pass = 0; pass < 2 && good; pass++) {
-        const bool update = pass == 1;
-        uint32_t readI = 0, refI = 0, numD = 0;
-        uint32_t e = 0;
-        uint16_t ro = 0;
-        bool stop = false;
-        for (uint32_t oi = 0; oi < nops && !stop; oi++) {
-            const uint32_t op = op_code(ops[oi]), l = op_len(ops[oi]);
-            const uint32_t t = ev_type_of(op);
-            const bool reads = op_reads(op);
-            for (uint32_t k = 0; k < l; k++) {
-                const uint32_t i = e + k;
-                if (i < baq_start) continue;
-                const uint16_t rb = reads ? (uint16_t)(ro + k) : NONE16;
-                const uint32_t qi = qs + rb;
-                if (rb != NONE16) {
-                    if (rb < rwx) continue;
-                    if (rb > rwy) { stop = true; break; }
-                }
-                if (t == EV_S) refI++;
-                else if (t == EV_I) {
-                    if (update) baq[qi] = b.quals[qi];
-                    readI++;
-                } else if (t == EV_D) {
-                    refI++; numD++;
-                } else {
-                    if (update) {
-                        const uint32_t expectedPos = refI - refOffset + (i - numD - baq_start - readI) + read_shift;
-                        const uint8_t oq = b.quals[qi];
-                        const uint8_t bq = baq[qi];
-                        const uint32_t state = a.s.baq_state[qi];
-                        uint8_t v;
-                        if ((state & 3) != 0 || (state >> 2) != expectedPos) v = 4;
-                        else v = bq < oq ? bq : oq;
-                        baq[qi] = v;
-                    }
-                    readI++; refI++;
-                }
-            }
-            if (reads) ro = (uint16_t)(ro + l);
-            e += l;
-        }
-        if (!update && readI != (uint32_t)(rwy - rwx + 1)) good = false;
-    }
-    if (!good) {
-        for (uint32_t i = rwx; i <= rwy; i++) baq[qs + i] = b.quals[qs + i];
-    }
-    // recode
-    const uint32_t ql = b.qual_len[r];
-    for (uint32_t i = qs; i < qs + ql; i++) {
-        const uint8_t bi = baq[i];
-        if (bi == 0xff) continue;
-        const uint8_t bq = (uint8_t)(b.quals[i] + 64);
-        baq[i] = (uint8_t)(bq - bi);
-    }
+    if (a.band_done && 2u * a.s.bandwidth[r] + 1 <= 33) return;
+    const uint32_t *st = a.s.baq_state;
+    cap_recode(b, a.s, r, [&](uint16_t i) { return baq[qs + i]; }, [&](uint16_t i) { return st[qs + i]; },
+               [&](uint16_t i, uint8_t v) { baq[qs + i] = v; });
 }
 
 // BAQ read lists per band class
@@ -493,29 +835,30 @@ extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const 
     a.b = *b; a.g = *g; a.s = *s; a.list = list; a.count = count; a.scratch = scratch;
     a.slot_doubles = slot_doubles; a.rows_cap = rows_cap; a.wmax_runtime = wmax_runtime; a.cells = cells;
     if (waves == 0) return hipSuccess;
-    if (wpe >= 2) {
-        switch (cls) {
-        case 0: hipLaunchKernelGGL((fpd::k_baq_hmm<15, 2>), dim3(waves), dim3(64), 0, st, a); break;
-        case 1: hipLaunchKernelGGL((fpd::k_baq_hmm<23, 2>), dim3(waves), dim3(64), 0, st, a); break;
-        case 2: hipLaunchKernelGGL((fpd::k_baq_hmm<33, 2>), dim3(waves), dim3(64), 0, st, a); break;
-        default: hipLaunchKernelGGL((fpd::k_baq_hmm<0, 2>), dim3(waves), dim3(64), 0, st, a); break;
+    // band kernels: 4 waves per workgroup (the caller sizes waves as a multiple of 4)
+    const dim3 g4((waves + 3) / 4), b4(256);
+    switch (cls) {
+    case 0:
+        switch (wpe) {
+        case 1: hipLaunchKernelGGL((fpd::k_baq_band<15, 1>), g4, b4, 0, st, a); break;
+        case 3: hipLaunchKernelGGL((fpd::k_baq_band<15, 3>), g4, b4, 0, st, a); break;
+        case 4: hipLaunchKernelGGL((fpd::k_baq_band<15, 4>), g4, b4, 0, st, a); break;
+        default: hipLaunchKernelGGL((fpd::k_baq_band<15, 2>), g4, b4, 0, st, a); break;
         }
-    } else {
-        switch (cls) {
-        case 0: hipLaunchKernelGGL((fpd::k_baq_hmm<15, 1>), dim3(waves), dim3(64), 0, st, a); break;
-        case 1: hipLaunchKernelGGL((fpd::k_baq_hmm<23, 1>), dim3(waves), dim3(64), 0, st, a); break;
-        case 2: hipLaunchKernelGGL((fpd::k_baq_hmm<33, 1>), dim3(waves), dim3(64), 0, st, a); break;
-        default: hipLaunchKernelGGL((fpd::k_baq_hmm<0, 1>), dim3(waves), dim3(64), 0, st, a); break;
-        }
+        break;
+    case 1: hipLaunchKernelGGL((fpd::k_baq_band<23, 1>), g4, b4, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((fpd::k_baq_band<33, 1>), g4, b4, 0, st, a); break;
+    default: hipLaunchKernelGGL((fpd::k_baq_hmm<0, 2>), dim3(waves), dim3(64), 0, st, a); break;
     }
     return hipGetLastError();
 }
 
-extern "C" hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *s, hipStream_t st)
+extern "C" hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *s, uint32_t band_done,
+                                          hipStream_t st)
 {
     if (b->num_reads == 0) return hipSuccess;
     fpd::PostArgs a;
-    a.b = *b; a.s = *s;
+    a.b = *b; a.s = *s; a.band_done = band_done;
     hipLaunchKernelGGL(fpd::k_baq_post, dim3((b->num_reads + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError();
 }

@@ -60,7 +60,7 @@ typedef struct fp_options {
                                         before accumulation, as the reference does
                                         (device/covariates.cu:91-101); 1: kept in fp64 */
     uint32_t disable_output_rounding;/* report floats as %.64f (table_formatter.cu:102) */
-    uint32_t baq_waves_per_simd;     /* BAQ HMM register budget: 1 or 2 waves per SIMD;
+    uint32_t baq_waves_per_simd;     /* BAQ HMM register budget: 1 to 4 waves per SIMD;
                                         0 = chosen by the engine's autotuner            */
     uint32_t sort_digit_bits;        /* FP_AGG_SORT radix digit width 4..8; 0 = autotune */
     uint32_t max_read_length;        /* upper bound of read lengths; lets batches run with

@@ -117,7 +117,7 @@ def run_parity(tmp_path, genome, rs, n, batch_size, per_batch=True, **engine_opt
     e.close()
 
 
-@pytest.mark.parametrize("wpe", [1, 2])
+@pytest.mark.parametrize("wpe", [1, 2, 3, 4])
 def test_config0_slice(tmp_path, wpe):
     say(f"parity: config 0 first 4000 reads, BAQ register budget for {wpe} waves/SIMD")
     gs, rs, n = synth.named_spec(0)
