@@ -304,20 +304,24 @@ __global__ void __launch_bounds__(64, WPE) k_baq_hmm(BaqArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// cap_baq_qualities + BQ-tag recoding of one BAQ read (baq.cu:870-1056, :1058-1081).
+// The event walk of cap_baq_qualities (baq.cu:870-1056).
 //
-// The reference caps the MAP qualities in place, then recodes every quality
-// that carries a BAQ value into the tag (q + 64) - baq.  When the check pass
-// succeeds the update pass visits every window position exactly once, in
-// order, so both steps fold into one walk that emits each final tag directly.
-// mapq(i) / mstate(i) give the MAP quality and state of read position i;
-// put(i, tag) stores the tag of read position i.
-template <class MQ, class MS, class PUT>
-__device__ inline void cap_recode(const BatchDev &b, const ReadStateDev &s, uint32_t r, MQ mapq, MS mstate, PUT put)
+// Visits the read positions of the BAQ window in order, from the first event
+// inside the window (baq_start) to the end of the window, calling
+// visit(rb, kind, expectedPos) with kind CAP_M (match/mismatch: the MAP state
+// must be a match state at expectedPos), CAP_I (insertion: the original
+// quality is kept) or CAP_S (soft clip: the MAP quality is kept).  Returns the
+// reference's check: the window holds exactly its M and I events.  The
+// reference applies its update pass only when the check passes, and then that
+// pass visits every window position exactly once; a failed check keeps the
+// original qualities across the window.
+enum : uint32_t { CAP_M = 0, CAP_I = 1, CAP_S = 3 };
+
+template <class V>
+__device__ inline bool cap_walk(const BatchDev &b, const ReadStateDev &s, uint32_t r, V visit)
 {
     const uint32_t rw = s.rwc[r];
     const uint16_t rwx = lo16(rw), rwy = hi16(rw);
-    const uint32_t qs = b.qual_start[r];
     const uint32_t *ops = b.cigars + b.cigar_start[r];
     const uint32_t nops = b.cigar_len[r];
     const int hx = (int16_t)lo16(s.hmmw[r]);
@@ -344,55 +348,50 @@ __device__ inline void cap_recode(const BatchDev &b, const ReadStateDev &s, uint
             e += l;
         }
     }
-    bool good = true;
-    for (int pass = 0; pass < 2 && good; pass++) {
-        const bool update = pass == 1;
-        uint32_t readI = 0, refI = 0, numD = 0;
-        uint32_t e = 0;
-        uint16_t ro = 0;
-        bool stop = false;
-        for (uint32_t oi = 0; oi < nops && !stop; oi++) {
-            const uint32_t op = op_code(ops[oi]), l = op_len(ops[oi]);
-            const uint32_t t = op_event(op);
-            const bool reads = op_reads(op);
-            for (uint32_t k = 0; k < l; k++) {
-                const uint32_t i = e + k;
-                if (i < baq_start) continue;
-                const uint16_t rb = reads ? (uint16_t)(ro + k) : NONE16;
-                if (rb != NONE16) {
-                    if (rb < rwx) continue;
-                    if (rb > rwy) { stop = true; break; }
-                }
-                if (t == EV_S) {
-                    if (update) put(rb, (uint8_t)((uint8_t)(b.quals[qs + rb] + 64) - mapq(rb)));
-                    refI++;
-                } else if (t == EV_I) {
-                    if (update) put(rb, (uint8_t)64);
-                    readI++;
-                } else if (t == EV_D) {
-                    refI++; numD++;
-                } else {
-                    if (update) {
-                        const uint32_t expectedPos = refI - refOffset + (i - numD - baq_start - readI) + read_shift;
-                        const uint8_t oq = b.quals[qs + rb];
-                        const uint8_t bq = mapq(rb);
-                        const uint32_t state = mstate(rb);
-                        uint8_t v;
-                        if ((state & 3) != 0 || (state >> 2) != expectedPos) v = 4;
-                        else v = bq < oq ? bq : oq;
-                        put(rb, (uint8_t)((uint8_t)(oq + 64) - v));
-                    }
-                    readI++; refI++;
-                }
+    uint32_t readI = 0, refI = 0, numD = 0;
+    uint32_t e = 0;
+    uint16_t ro = 0;
+    bool stop = false;
+    for (uint32_t oi = 0; oi < nops && !stop; oi++) {
+        const uint32_t op = op_code(ops[oi]), l = op_len(ops[oi]);
+        const uint32_t t = op_event(op);
+        const bool reads = op_reads(op);
+        for (uint32_t k = 0; k < l; k++) {
+            const uint32_t i = e + k;
+            if (i < baq_start) continue;
+            const uint16_t rb = reads ? (uint16_t)(ro + k) : NONE16;
+            if (rb != NONE16) {
+                if (rb < rwx) continue;
+                if (rb > rwy) { stop = true; break; }
             }
-            if (reads) ro = (uint16_t)(ro + l);
-            e += l;
+            if (t == EV_S) {
+                visit(rb, CAP_S, 0u);
+                refI++;
+            } else if (t == EV_I) {
+                visit(rb, CAP_I, 0u);
+                readI++;
+            } else if (t == EV_D) {
+                refI++; numD++;
+            } else {
+                visit(rb, CAP_M, refI - refOffset + (i - numD - baq_start - readI) + read_shift);
+                readI++; refI++;
+            }
         }
-        if (!update && readI != (uint32_t)(rwy - rwx + 1)) good = false;
+        if (reads) ro = (uint16_t)(ro + l);
+        e += l;
     }
-    // a failed check keeps the original qualities: tag 64 across the window
-    if (!good)
-        for (uint32_t i = rwx; i <= rwy; i++) put((uint16_t)i, (uint8_t)64);
+    return readI == (uint32_t)(rwy - rwx + 1);
+}
+
+// the capped quality of one window position (cap_baq_qualities) recoded into its BQ tag (recode, baq.cu:1058-1081)
+__device__ inline uint8_t cap_tag(uint32_t kind, uint32_t expectedPos, uint8_t oq, uint8_t mapq, uint32_t state)
+{
+    uint8_t v;
+    if (kind == CAP_I) v = oq;
+    else if (kind == CAP_S) v = mapq;
+    else if ((state & 3) != 0 || (state >> 2) != expectedPos) v = 4;
+    else v = mapq < oq ? mapq : oq;
+    return (uint8_t)((uint8_t)(oq + 64) - v);
 }
 
 // ---------------------------------------------------------------------------
@@ -431,6 +430,10 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
     uint64_t ncell = 0;
 
     auto qprob = [&](uint8_t q) -> double { return s_q2p[q < 4 ? 4 : q]; };
+    // Compiler fence: makes a value materialise here (emits no instruction).  The row loops decode
+    // the next row's prefetched inputs before their own stores; left alone the compiler sinks the
+    // decode into the next row, behind those stores, where it waits for them.
+    auto pin = [](auto &v) { asm volatile("" : "+v"(v)); };
     // MAP quality through the thresholds (largest v in [5, 101] with y <= T[v])
     auto mapq = [&](double y) -> uint8_t {
         if (y == 0.0) return 4;
@@ -496,9 +499,11 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             return cw[wi];
         };
         // cells of the HMM window [0, refLen) that lie inside the contig carry a base; others are N
-        auto ref_ok = [&](int idx0) -> bool { return idx0 >= 0 && idx0 < refLen && hoff + idx0 < clen; };
+        // (bitwise tests and an arithmetic select: no branch, so the row loops stay straight-line)
         auto nibble = [&](uint32_t w, int idx0) -> uint8_t {
-            return ref_ok(idx0) ? (uint8_t)((w >> (((hoff + idx0) & 7) * 4)) & 15) : (uint8_t)B_N;
+            const uint32_t ok = (uint32_t)(idx0 >= 0) & (uint32_t)(idx0 < refLen) & (uint32_t)(hoff + idx0 < clen);
+            const uint32_t nb = (w >> (((uint32_t)(hoff + idx0) & 7) * 4)) & 15;
+            return (uint8_t)((nb & (0u - ok)) | (15u & (ok - 1u)));
         };
 
         // band reference bases: nibble r of the window = HMM window index idx0 + r
@@ -548,6 +553,12 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             }
         };
 
+        // expected MAP state of every window position (the cap_baq_qualities walk), staged per row;
+        // the MAP step turns it into the position's final tag
+        const bool good = cap_walk(b, a.s, rd, [&](uint16_t p, uint32_t kind, uint32_t ep) {
+            Ost[(uint64_t)((int)p - qStart + 1) * 64 + lane] = (uint64_t)ep | ((uint64_t)kind << 32);
+        });
+
         double M[W], I[W], D[W];
 
         // ---- forward row 1 (baq.cu:396-442)
@@ -579,24 +590,23 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             }
         }
         // ---- forward rows 2..qLen (baq.cu:444-512)
-        // the next row's query base, quality and entering reference word are loaded one row ahead
+        // Each row issues the loads of the next row's query word, quality and entering reference word
+        // first and decodes them after its FP64 work but before its own row stores.  Loads and stores
+        // retire in order, so a decode placed behind the stores would wait for them.
         const int qlast = qLen - 1;
-        uint32_t nqw = read_word(b.reads, qbase + (1 < qlast ? 1 : qlast));
-        uint8_t nqq = b.quals[qqual + (1 < qlast ? 1 : qlast)];
-        uint32_t nrw = ref_word(hoff + (2 - bw - 1 + (W - 1)));
+        const int q1 = 1 < qlast ? 1 : qlast;
+        uint32_t cqb = (read_word(b.reads, qbase + q1) >> (((qbase + q1) & 7) * 4)) & 15;
+        double cp = qprob(b.quals[qqual + q1]);
+        uint32_t crb = nibble(ref_word(hoff + (2 - bw - 1 + (W - 1))), 2 - bw - 1 + (W - 1));
         for (int i = 2; i <= qLen; i++) {
-            const uint8_t qb = (uint8_t)((nqw >> (((qbase + i - 1) & 7) * 4)) & 15);
-            const uint8_t qq = nqq;
-            const uint8_t rbin = nibble(nrw, i - bw - 1 + (W - 1));
-            {
-                const int in_ = i < qlast ? i : qlast;
-                nqw = read_word(b.reads, qbase + in_);
-                nqq = b.quals[qqual + in_];
-                nrw = ref_word(hoff + (i + 1 - bw - 1 + (W - 1)));
-            }
-            win_shr(rbin);
-            const double p = qprob(qq);
-            const double em_eq = 1 - p, em_ne = p * EM;
+            const int in_ = i < qlast ? i : qlast;
+            const uint32_t nqw = read_word(b.reads, qbase + in_);
+            const uint8_t nqq = b.quals[qqual + in_];
+            const int nidx = i + 1 - bw - 1 + (W - 1);
+            const uint32_t nrw = ref_word(hoff + nidx);
+            const uint8_t qb = (uint8_t)cqb;
+            win_shr((uint8_t)crb);
+            const double em_eq = 1 - cp, em_ne = cp * EM;
             const int klo = 1 > i - bw ? 1 : i - bw;
             const int khi = refLen < i + bw ? refLen : i + bw;
             if (khi >= klo) ncell += (uint64_t)(khi - klo + 1);
@@ -617,6 +627,10 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
                 M[r] = mv; I[r] = iv; D[r] = dv;
                 cM = mv; cD = dv;
             }
+            cqb = (nqw >> (((qbase + in_) & 7) * 4)) & 15;
+            cp = qprob(nqq);
+            crb = nibble(nrw, nidx);
+            pin(cqb); pin(cp); pin(crb);
             Sst[(uint64_t)i * 64 + lane] = sum;
             const double inv = 1. / sum;
             double2 *fr = Fst + (uint64_t)i * W * 64 + lane;
@@ -640,14 +654,29 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         }
 
         // ---- backward + MAP (baq.cu:528-681)
-        auto map_row = [&](int i) {
+        // with the register budget of one wave per SIMD the forward row that the next MAP step
+        // consumes is loaded right after the current one is used, a full backward row ahead
+        constexpr bool PREF = WPE == 1;
+        double2 Fp[PREF ? W : 1];
+        uint64_t Cp = 0;
+        auto load_f = [&](int i) {
+            if constexpr (PREF) {
+                const double2 *fr = Fst + (uint64_t)i * W * 64 + lane;
+#pragma unroll
+                for (int r = 0; r < W; r++) Fp[r] = fr[r * 64];
+                Cp = Ost[(uint64_t)i * 64 + lane];
+            }
+        };
+        auto map_row = [&](int i, uint8_t oq) {
             const double2 *fr = Fst + (uint64_t)i * W * 64 + lane;
             double sum = 0.0, mx = 0.0;
             int32_t stt = -1;
 #pragma unroll
             for (int r = 0; r < W; r++) {
                 const int k = i - bw + r;
-                const double2 f = fr[r * 64];
+                double2 f;
+                if constexpr (PREF) f = Fp[r < W ? r : 0];
+                else f = fr[r * 64];
                 double z = f.x * M[r];
                 sum += z;
                 if (z > mx) { mx = z; stt = (k - 1) << 2 | 0; }
@@ -656,9 +685,14 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
                 if (z > mx) { mx = z; stt = (k - 1) << 2 | 1; }
             }
             mx /= sum;
-            // staged in the slot (one coalesced row store); capped and recoded after the backward pass
-            Ost[(uint64_t)i * 64 + lane] = (uint64_t)(uint32_t)stt | ((uint64_t)mapq(1.0 - mx) << 32);
+            const uint64_t code = PREF ? Cp : Ost[(uint64_t)i * 64 + lane];
+            // the final tag replaces the code in the slot (one coalesced row store)
+            Ost[(uint64_t)i * 64 + lane] =
+                cap_tag((uint32_t)(code >> 32), (uint32_t)code, oq, mapq(1.0 - mx), (uint32_t)stt);
         };
+        load_f(qLen);
+        // backward row i emits query index i (the row below it); map_row(i) needs quality index i - 1
+        const uint8_t lastq = b.quals[qqual + qlast];
         {
             const double sq = Sst[(uint64_t)qLen * 64 + lane];
             const int klo = 1 > qLen - bw ? 1 : qLen - bw;
@@ -670,30 +704,26 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
                 const bool in = k >= klo && k <= khi;
                 M[r] = in ? vM : 0.0; I[r] = in ? vI : 0.0;
             }
-            map_row(qLen);
+            map_row(qLen, lastq);
         }
+        if (qLen >= 2) load_f(qLen - 1);
         win_fill(qLen - bw);                     // backward cell r of row i: window index i - bw + r
-        // backward row i emits query index i (the row below it)
-        uint32_t bqw = read_word(b.reads, qbase + qlast);
-        uint8_t bqq = b.quals[qqual + qlast];
-        uint32_t brw = ref_word(hoff + (qLen - 1 - bw));
-        double bS = Sst[(uint64_t)(qLen >= 2 ? qLen - 1 : qLen) * 64 + lane];
+        // inputs of backward row qLen - 1, then loaded a row ahead and decoded as in the forward pass
+        cqb = (read_word(b.reads, qbase + qlast) >> (((qbase + qlast) & 7) * 4)) & 15;
+        cp = qprob(lastq);
+        crb = nibble(ref_word(hoff + (qLen - 1 - bw)), qLen - 1 - bw);
+        double cyy = 1.0 / Sst[(uint64_t)(qLen >= 2 ? qLen - 1 : qLen) * 64 + lane];   // 1 / s[i]
         for (int i = qLen - 1; i >= 1; i--) {
-            const uint8_t rbin = nibble(brw, i - bw);
-            win_shl(rbin);
+            const int in_ = i - 1;
+            const uint32_t nqw = read_word(b.reads, qbase + in_);
+            const uint8_t nqq = b.quals[qqual + in_];
+            const int nidx = i - 1 - bw;
+            const uint32_t nrw = ref_word(hoff + nidx);
+            const double nS = Sst[(uint64_t)(i >= 2 ? i - 1 : i) * 64 + lane];
+            win_shl((uint8_t)crb);
             const double y = (i > 1) ? 1. : 0.;
-            const uint8_t qb = (uint8_t)((bqw >> (((qbase + i) & 7) * 4)) & 15);
-            const uint8_t qq = bqq;
-            const double Si = bS;
-            {
-                const int in_ = i - 1;
-                bqw = read_word(b.reads, qbase + in_);
-                bqq = b.quals[qqual + in_];
-                brw = ref_word(hoff + (i - 1 - bw));
-                bS = Sst[(uint64_t)(i >= 2 ? i - 1 : i) * 64 + lane];
-            }
-            const double p = qprob(qq);
-            const double em_eq = 1 - p, em_ne = p * EM;
+            const uint8_t qb = (uint8_t)cqb;
+            const double em_eq = 1 - cp, em_ne = cp * EM;
             const int klo = 1 > i - bw ? 1 : i - bw;
             const int khi = refLen < i + bw ? refLen : i + bw;
             double cD = 0.0;
@@ -712,18 +742,30 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
                 M[r] = mv; I[r] = iv;
                 cD = dv;
             }
-            const double yy = 1.0 / Si;
+            const double yy = cyy;
 #pragma unroll
             for (int r = 0; r < W; r++) { M[r] = M[r] * yy; I[r] = I[r] * yy; }
-            map_row(i);
+            cqb = (nqw >> (((qbase + in_) & 7) * 4)) & 15;
+            cp = qprob(nqq);
+            crb = nibble(nrw, nidx);
+            cyy = 1.0 / nS;
+            pin(cqb); pin(cp); pin(crb); pin(cyy);
+            map_row(i, nqq);
+            if (i > 1) load_f(i - 1);
         }
-        // cap + recode from the staged MAP results; each lane writes its read's tags in one burst
-        const uint32_t qs = b.qual_start[rd];
-        // read position p was row p - qStart + 1
-        auto staged = [&](uint16_t p) { return Ost[(uint64_t)((int)p - qStart + 1) * 64 + lane]; };
-        cap_recode(b, a.s, rd, [&](uint16_t p) { return (uint8_t)(staged(p) >> 32); },
-                   [&](uint16_t p) { return (uint32_t)staged(p); },
-                   [&](uint16_t p, uint8_t v) { a.s.baq[qs + p] = v; });
+        // the read's tags, row by row in lockstep across the wave: coalesced slot loads, and each lane
+        // writes its read's window as one burst of consecutive bytes
+        uint8_t *out = a.s.baq + b.qual_start[rd] + qStart;
+        const uint64_t *orow = Ost + 64 + lane;                 // row j + 1 holds position qStart + j
+        int j = 0;
+        for (; j + 8 <= qLen; j += 8) {
+            uint64_t t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) t[u] = orow[(uint64_t)(j + u) * 64];
+#pragma unroll
+            for (int u = 0; u < 8; u++) out[j + u] = good ? (uint8_t)t[u] : (uint8_t)64;
+        }
+        for (; j < qLen; j++) out[j] = good ? (uint8_t)orow[(uint64_t)j * 64] : (uint8_t)64;
     }
     // one counter update per wave
     for (int off = 32; off >= 1; off >>= 1) ncell += __shfl_xor(ncell, off);
@@ -756,8 +798,13 @@ __global__ void __launch_bounds__(256) k_baq_post(PostArgs a)
     }
     if (a.band_done && 2u * a.s.bandwidth[r] + 1 <= 33) return;
     const uint32_t *st = a.s.baq_state;
-    cap_recode(b, a.s, r, [&](uint16_t i) { return baq[qs + i]; }, [&](uint16_t i) { return st[qs + i]; },
-               [&](uint16_t i, uint8_t v) { baq[qs + i] = v; });
+    if (cap_walk(b, a.s, r, [](uint16_t, uint32_t, uint32_t) {})) {
+        cap_walk(b, a.s, r, [&](uint16_t i, uint32_t kind, uint32_t ep) {
+            baq[qs + i] = cap_tag(kind, ep, b.quals[qs + i], baq[qs + i], st[qs + i]);
+        });
+    } else {
+        for (uint32_t i = rwx; i <= rwy; i++) baq[qs + i] = 64;
+    }
 }
 
 // BAQ read lists per band class
