@@ -158,7 +158,7 @@ def main():
     torch.cuda.synchronize()
     log(f"generated + uploaded in {time.perf_counter() - t0:.1f} s")
 
-    e = Engine(genome, device=0, baq_waves_per_simd=args.wpe)
+    e = Engine(genome, device=0, baq_waves_per_simd=args.wpe, debug_arrays=0)
     for i in range(args.warmup):
         e.process_device(dev[i % K].view)
     e.synchronize()

@@ -52,7 +52,8 @@ class FpOptions(C.Structure):
         ("sort_digit_bits", C.c_uint32),
         ("max_read_length", C.c_uint32),
         ("aggregation", C.c_uint32),
-        ("reserved", C.c_uint32 * 6),
+        ("debug_arrays", C.c_uint32),
+        ("reserved", C.c_uint32 * 5),
     ]
 
 

@@ -38,19 +38,19 @@ hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *
                                hipStream_t st);
 hipError_t fpd_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st);
 hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
-                          uint32_t frac_f64, uint32_t *counters, uint32_t *rg_buf, uint32_t max_read_len,
-                          hipStream_t st);
+                          uint32_t frac_f64, uint32_t keep_frac, uint32_t *counters, uint32_t *rg_buf,
+                          uint32_t max_read_len, hipStream_t st);
 hipError_t fpd_compact_count(const uint64_t *count, uint64_t n, uint32_t *block_counts, hipStream_t st);
-hipError_t fpd_compact_scatter(const uint64_t *count, const uint64_t *lo, const uint64_t *hi, uint64_t n,
+hipError_t fpd_compact_scatter(const uint64_t *count, const uint64_t *const *parts, uint64_t n,
                                const uint64_t *block_offsets, uint32_t *okey, uint64_t *ocount, uint64_t *olo,
                                uint64_t *ohi, hipStream_t st);
-hipError_t fpd_table_add(uint64_t *dc, uint64_t *dlo, uint64_t *dhi, const uint64_t *sc, const uint64_t *slo,
-                         const uint64_t *shi, uint64_t n, hipStream_t st);
+hipError_t fpd_table_add(uint64_t *dc, uint64_t *const *dparts, const uint64_t *sc, const uint64_t *const *sparts,
+                         uint64_t n, hipStream_t st);
 hipError_t fpd_post_set_constants(const double *qual_err, const double *prior, const double *log1mp, double log10e);
 hipError_t fpd_launch_empirical(uint64_t n, const uint32_t *key, const uint64_t *count, const uint64_t *lo,
                                 const uint64_t *hi, int qual_shift, double *mism, double *expected, double *estimated,
                                 double *empirical, hipStream_t st);
-hipError_t fpd_launch_rg_table(uint32_t num_rg, const uint64_t *count, const uint64_t *lo, const uint64_t *hi,
+hipError_t fpd_launch_rg_table(uint32_t num_rg, const uint64_t *count, const uint64_t *const *parts,
                                uint32_t *valid, uint64_t *obs, double *mism, double *expected, double *estimated,
                                double *empirical, hipStream_t st);
 }
@@ -172,7 +172,7 @@ struct fp_engine {
     hipStream_t stream = nullptr;
     std::string name;
     HostConstants consts;
-    int wpe = 2;   // waves per SIMD of the 15-cell BAQ band kernel (1-4)
+    int wpe = 1;   // waves per SIMD of the 15-cell BAQ band kernel (1-4)
 
     // setup state
     bool is_setup = false;
@@ -191,7 +191,7 @@ struct fp_engine {
     fpd::ReadStateDev rs{};
 
     // resident tables
-    DevBuf tab[3][3];
+    DevBuf tab[3][4];     // per table: count, then the three limb sums
     uint64_t tab_size[3] = {0, 0, 0};
     fpd::TablesDev tables{};
 
@@ -211,13 +211,12 @@ fp_status alloc_tables(fp_engine *e)
 {
     for (int t = 0; t < 3; t++) {
         e->tab_size[t] = table_entries(e->num_rg, t);
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < 4; k++) {
             HIP_TRY(e->tab[t][k].ensure(e->tab_size[t] * 8));
             HIP_TRY(hipMemsetAsync(e->tab[t][k].p, 0, e->tab_size[t] * 8, e->stream));
         }
         e->tables.count[t] = e->tab[t][0].as<uint64_t>();
-        e->tables.sum_lo[t] = e->tab[t][1].as<uint64_t>();
-        e->tables.sum_hi[t] = e->tab[t][2].as<uint64_t>();
+        for (int k = 0; k < 3; k++) e->tables.part[t][k] = e->tab[t][k + 1].as<uint64_t>();
     }
     e->tables.num_rg = e->num_rg;
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -362,8 +361,9 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     HIP_TRY(hipEventRecord(e->ev[9], st));
     HIP_TRY(fpd_launch_baq_post(&b, &e->rs, 1u, st));
     HIP_TRY(hipEventRecord(e->ev[2], st));
-    HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, e->counters.as<uint32_t>(),
-                           e->rg_buf.as<uint32_t>(), std::min<uint32_t>(rows_cap, 499), st));
+    HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, e->opts.debug_arrays ? 1u : 0u,
+                           e->counters.as<uint32_t>(), e->rg_buf.as<uint32_t>(), std::min<uint32_t>(rows_cap, 499),
+                           st));
     HIP_TRY(hipEventRecord(e->ev[3], st));
     uint32_t post[16];
     HIP_TRY(hipMemcpyAsync(post, e->counters.p, 64, hipMemcpyDeviceToHost, st));
@@ -410,7 +410,7 @@ fp_status compact(fp_engine *e, int t, DevBuf &okey, DevBuf &ocount, DevBuf &olo
     HIP_TRY(ocount.ensure(8 * total + 8));
     HIP_TRY(olo.ensure(8 * total + 8));
     HIP_TRY(ohi.ensure(8 * total + 8));
-    HIP_TRY(fpd_compact_scatter(e->tables.count[t], e->tables.sum_lo[t], e->tables.sum_hi[t], n, bo.as<uint64_t>(),
+    HIP_TRY(fpd_compact_scatter(e->tables.count[t], e->tables.part[t], n, bo.as<uint64_t>(),
                                 okey.as<uint32_t>(), ocount.as<uint64_t>(), olo.as<uint64_t>(), ohi.as<uint64_t>(),
                                 e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -461,7 +461,7 @@ fp_status postprocess(fp_engine *e)
         HIP_TRY(ex.ensure(8 * ns + 8));
         HIP_TRY(es.ensure(8 * ns + 8));
         HIP_TRY(em.ensure(8 * ns + 8));
-        HIP_TRY(fpd_launch_rg_table(e->num_rg, e->tables.count[0], e->tables.sum_lo[0], e->tables.sum_hi[0],
+        HIP_TRY(fpd_launch_rg_table(e->num_rg, e->tables.count[0], e->tables.part[0],
                                     valid.as<uint32_t>(), obs.as<uint64_t>(), m.as<double>(), ex.as<double>(),
                                     es.as<double>(), em.as<double>(), st));
         std::vector<uint32_t> hv(ns);
@@ -514,6 +514,7 @@ void fp_options_init(fp_options *o)
     o->abi_version = FP_ABI_VERSION;
     o->engine_kind = FP_ENGINE_PLAIN;
     o->max_batch_reads = 20000;
+    o->debug_arrays = 1;
 }
 
 const char *fp_last_error(void) { return g_last_error.c_str(); }
@@ -550,7 +551,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     fp_engine *e = new fp_engine();
     e->device = hip_device;
     e->opts = o;
-    e->wpe = o.baq_waves_per_simd >= 1 && o.baq_waves_per_simd <= 4 ? (int)o.baq_waves_per_simd : 2;
+    e->wpe = o.baq_waves_per_simd >= 1 && o.baq_waves_per_simd <= 4 ? (int)o.baq_waves_per_simd : 1;
     memset(&e->stats, 0, sizeof(e->stats));
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) e->name = prop.name;
@@ -727,9 +728,9 @@ fp_status fp_engine_gather(fp_engine *dst, fp_engine *src)
     HIP_TRY(hipStreamSynchronize(src->stream));
     for (int t = 0; t < 3; t++) {
         const uint64_t n = dst->tab_size[t];
-        DevBuf tmp[3];
-        const uint64_t *sp[3];
-        for (int k = 0; k < 3; k++) {
+        DevBuf tmp[4];
+        const uint64_t *sp[4];
+        for (int k = 0; k < 4; k++) {
             if (src->device == dst->device) {
                 sp[k] = src->tab[t][k].as<uint64_t>();
             } else {
@@ -738,10 +739,9 @@ fp_status fp_engine_gather(fp_engine *dst, fp_engine *src)
                 sp[k] = tmp[k].as<uint64_t>();
             }
         }
-        HIP_TRY(fpd_table_add(dst->tables.count[t], dst->tables.sum_lo[t], dst->tables.sum_hi[t], sp[0], sp[1], sp[2],
-                              n, dst->stream));
+        HIP_TRY(fpd_table_add(dst->tables.count[t], dst->tables.part[t], sp[0], sp + 1, n, dst->stream));
         HIP_TRY(hipStreamSynchronize(dst->stream));
-        for (int k = 0; k < 3; k++) tmp[k].release();
+        for (int k = 0; k < 4; k++) tmp[k].release();
     }
     dst->stats.total_reads += src->stats.total_reads;
     dst->stats.filtered_reads += src->stats.filtered_reads;
@@ -924,7 +924,10 @@ fp_status fp_engine_debug_fetch(fp_engine *e, int item, void *dst, uint64_t byte
     case FP_DBG_NUM_ERRORS: src = e->rs.num_errors; break;
     case FP_DBG_BANDWIDTH: src = e->rs.bandwidth; break;
     case FP_DBG_BAQ: src = e->rs.baq; break;
-    case FP_DBG_FRAC_ERRORS: src = e->rs.frac; break;
+    case FP_DBG_FRAC_ERRORS:
+        if (!e->opts.debug_arrays) return fail(FP_ERR_STATE, "fractional errors are kept only with debug_arrays = 1");
+        src = e->rs.frac;
+        break;
     default: return fail(FP_ERR_INVALID_ARG, "unknown debug item");
     }
     if (need) HIP_TRY(hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
@@ -936,7 +939,7 @@ fp_status fp_engine_reset(fp_engine *e)
     fp_status rc = check_engine(e, true);
     if (rc != FP_OK) return rc;
     for (int t = 0; t < 3; t++)
-        for (int k = 0; k < 3; k++) HIP_TRY(hipMemsetAsync(e->tab[t][k].p, 0, e->tab_size[t] * 8, e->stream));
+        for (int k = 0; k < 4; k++) HIP_TRY(hipMemsetAsync(e->tab[t][k].p, 0, e->tab_size[t] * 8, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     memset(&e->stats, 0, sizeof(e->stats));
     e->postprocessed = false;

@@ -8,8 +8,9 @@
  *   per padded base     flags u8: bit0 snp, bit1 insertion, bit2 deletion,
  *                       bit3 masked (non-ACGT, Q<6 or known site)
  *   per quality         BAQ tag u8, MAP state u32, fractional errors 3 x f64
- *   tables (resident)   direct-indexed by covariate key: count u64,
- *                       mismatch sum as u128 fixed point (units of 2^-80)
+ *   tables (resident)   direct-indexed by covariate key: count u64 and the
+ *                       exact mismatch sum (fixed point, units of 2^-80) as
+ *                       three u64 limb sums
  */
 #ifndef FP_DEVICE_H
 #define FP_DEVICE_H
@@ -90,10 +91,15 @@ struct ReadStateDev {
     double *frac;         // per quality x3 (snp, ins, del), interleaved
 };
 
+// Dense covariate tables indexed by key.  The exact mismatch sum of an entry
+// (fixed point, units of 2^-80) is kept as three u64 sums of 27-bit limbs,
+// value = part[0] + part[1] * 2^27 + part[2] * 2^54, so every update is a
+// plain atomic add that needs no carry and returns nothing.  A per-base value
+// (<= 1.0, i.e. <= 2^80) has limbs < 2^27, which leaves room for 2^37 updates
+// per entry.
 struct TablesDev {
     uint64_t *count[3];
-    uint64_t *sum_lo[3];
-    uint64_t *sum_hi[3];
+    uint64_t *part[3][3];     // [table][limb]
     uint32_t num_rg;
 };
 
@@ -159,6 +165,31 @@ __host__ __device__ inline u128 fixed_from_double(double v)
         r.lo = (-sh >= 64) ? 0 : (mant >> (-sh));
         r.hi = 0;
     }
+    return r;
+}
+
+static constexpr uint64_t LIMB_MASK = (1ull << 27) - 1;
+
+// limbs of a fixed-point value (value < 2^118)
+__host__ __device__ inline void fixed_split(const u128 &v, uint64_t &p0, uint64_t &p1, uint64_t &p2)
+{
+    p0 = v.lo & LIMB_MASK;
+    p1 = (v.lo >> 27) & LIMB_MASK;
+    p2 = (v.lo >> 54) | (v.hi << 10);
+}
+
+// p0 + p1 * 2^27 + p2 * 2^54 as a u128
+__host__ __device__ inline u128 fixed_combine(uint64_t p0, uint64_t p1, uint64_t p2)
+{
+    u128 r;
+    r.lo = p0;
+    r.hi = 0;
+    const uint64_t a = p1 << 27;
+    r.lo += a;
+    r.hi += (r.lo < a ? 1 : 0) + (p1 >> 37);
+    const uint64_t b = p2 << 54;
+    r.lo += b;
+    r.hi += (r.lo < b ? 1 : 0) + (p2 >> 10);
     return r;
 }
 

@@ -2,25 +2,33 @@
  * K_cov: fractional errors (device/fractional_errors.cu:39-141), the high
  * quality window (device/covariates.cu:215-246) and covariate key packing
  * (covariates.cu:50-103, covariates/packer_quality_score.h and siblings,
- * covariates/bit_packers/cycle_illumina.h, context.h),
- * accumulated straight into the device-resident covariate tables.
+ * covariates/bit_packers/cycle_illumina.h, context.h), accumulated straight
+ * into the device-resident covariate tables.
  *
  * The tables are direct-indexed by covariate key: the read group is the most
  * significant field of every key layout, so the dense array index of a key is
  * the key itself and the rows of a table come out in key order -- the order
  * the reference obtains by radix-sorting (covariate_table.cu:43-58).  Counts
- * are u64; mismatch sums are exact u128 fixed point (units of 2^-80), so the
- * accumulated table does not depend on batch size, launch order or the order
- * atomics land in.
+ * are u64; mismatch sums are exact fixed point kept as three 27-bit limb sums
+ * (fp_device.h), so the accumulated table does not depend on batch size,
+ * launch order or the order atomics land in, and no update waits for a
+ * returned value.
  *
- * Work layout: reads are grouped by read group; a workgroup takes a tile of up
- * to 256 reads of one group and keeps that group's hot table entries in LDS.
+ * Work layout: reads are grouped by read group; a workgroup of 16 waves takes a
+ * tile of up to 1024 reads of one group and keeps that group's table entries in
+ * LDS: every count, and the exact sums of the quality, context and cycle I/D
+ * entries.  Only the sums of the cycle table's mismatch entries (few nonzero,
+ * thousands of distinct keys) go to the resident tables during the tile.
+ * Global atomics on a handful of hot keys serialise at the coherence point and
+ * stall the whole memory system, so the per-base counts never leave LDS
+ * until the tile's flush.
  * Each wavefront handles one read at a time with its 64 lanes spread over the
  * read's bases (positions lane, lane+64, lane+128, lane+192 of the clipped
  * window), so every per-base load is coalesced.  The GATK block-averaged
  * fractional errors are evaluated per base from 256-bit ballot masks of the
- * BAQ blocks and error bits, with no per-read serial pass.  Windows longer
- * than 256 bases take a serial path on one lane.
+ * BAQ blocks and error bits; the context keys read neighbouring bases from a
+ * per-wave LDS copy of the window.  Windows longer than 256 bases take a
+ * serial path on one lane.
  */
 #include "fp_device.h"
 
@@ -31,10 +39,12 @@ struct CovArgs {
     ReadStateDev s;
     TablesDev t;
     uint32_t frac_f64;
-    uint32_t ncodes;             // cycle codes kept in LDS (2 * max read length + 2)
+    uint32_t keep_frac;          // also store the per-base fractional errors (inspection output)
+    uint32_t ncodes;             // cycle codes whose I/D entries are kept in LDS
+    uint32_t mcodes;             // cycle codes whose M counts are kept in LDS
     const uint32_t *rg_list;     // active reads grouped by read group
     const uint32_t *rg_offset;   // num_rg + 1
-    const uint32_t *tile_offset; // num_rg + 1 (tiles of 256 reads)
+    const uint32_t *tile_offset; // num_rg + 1 (tiles of TILE reads)
 };
 
 // calculateFractionalErrorArray for one error class over the clipped window (serial form)
@@ -90,9 +100,11 @@ __device__ inline void cycle_keys(uint16_t flags, uint16_t wx, uint16_t wy, uint
     kid = (i < 4 || i > readLength - 4 - 1) ? 1023u : key;
 }
 
-// covariate_Context with 2 mismatch / 3 indel bases (bit_packers/context.h:104-174)
-__device__ inline void context_keys(const uint32_t *reads, uint64_t rs, uint16_t hx, uint16_t hy, bool neg,
-                                    uint16_t bp, uint32_t &km, uint32_t &kid)
+// covariate_Context with 2 mismatch / 3 indel bases (bit_packers/context.h:104-174);
+// base(i) returns the read base at read position i
+template <class BASE>
+__device__ inline void context_keys(BASE base, uint16_t hx, uint16_t hy, bool neg, uint16_t bp, uint32_t &km,
+                                    uint32_t &kid)
 {
     km = 2047u; kid = 2047u;
     const int dir = neg ? 1 : -1;
@@ -102,10 +114,10 @@ __device__ inline void context_keys(const uint32_t *reads, uint64_t rs, uint16_t
     uint32_t ctx = 0;
     int n = 0;
     for (int i = bp, stop = (int)bp + cb * dir; i != stop; i += dir) {
-        const uint8_t base = read_base(reads, rs + (uint32_t)i);
-        if (base != B_A && base != B_C && base != B_T && base != B_G) break;
+        const uint8_t bb = base(i);
+        if (bb != B_A && bb != B_C && bb != B_T && bb != B_G) break;
         ctx <<= 2;
-        ctx |= 31 - __builtin_clz((uint32_t)base);
+        ctx |= 31 - __builtin_clz((uint32_t)bb);
         n++;
     }
     if (neg) ctx = ~ctx & ((1u << (n * 2)) - 1);
@@ -113,45 +125,59 @@ __device__ inline void context_keys(const uint32_t *reads, uint64_t rs, uint16_t
     if (n >= 3) kid = ((ctx >> ((n - 3) * 2)) << 4) | 3;
 }
 
-// LDS-privatised accumulation: an entry holds a count and an exact u128 sum
-struct LdsAcc {
-    unsigned long long *lo;
-    unsigned long long *hi;
-    uint32_t *cnt;
+// an exact per-base value as limbs
+struct Limbs {
+    uint64_t p0, p1, p2;
 };
 
-__device__ inline void lds_add_n(const LdsAcc &L, uint32_t e, uint32_t n, uint64_t lo, uint64_t hi)
+__device__ inline Limbs to_limbs(double v, uint32_t f64)
+{
+    Limbs l;
+    fixed_split(fixed_from_double(f64 ? v : (double)(float)v), l.p0, l.p1, l.p2);
+    return l;
+}
+
+__device__ inline bool nonzero(const Limbs &l) { return (l.p0 | l.p1 | l.p2) != 0; }
+
+// LDS-privatised accumulation: an entry holds a count and three limb sums; the
+// cycle table's M entries keep only a 16-bit count, two per word, quality-major
+// (word index (q * mcodes + code) / 2) so lanes at consecutive cycles hit
+// consecutive banks.  A tile adds at most one count per read to such an entry.
+struct LdsAcc {
+    unsigned long long *p0, *p1, *p2;
+    uint32_t *cnt;
+    uint32_t *mcnt;
+    uint32_t mcodes;
+};
+
+__device__ inline bool lds_cycle_m(const LdsAcc &L, uint32_t code, uint32_t q)
+{
+    if (code >= L.mcodes || q >= 64) return false;
+    const uint32_t i = q * L.mcodes + code;
+    atomicAdd(&L.mcnt[i >> 1], 1u << (16 * (i & 1)));
+    return true;
+}
+
+__device__ inline void lds_add_n(const LdsAcc &L, uint32_t e, uint32_t n, const Limbs &v)
 {
     atomicAdd(&L.cnt[e], n);
-    if (lo | hi) {
-        const unsigned long long old = atomicAdd(&L.lo[e], (unsigned long long)lo);
-        const uint64_t carry = (old + lo < old) ? 1 : 0;
-        if (hi + carry) atomicAdd(&L.hi[e], (unsigned long long)(hi + carry));
+    if (nonzero(v)) {
+        atomicAdd(&L.p0[e], (unsigned long long)v.p0);
+        atomicAdd(&L.p1[e], (unsigned long long)v.p1);
+        atomicAdd(&L.p2[e], (unsigned long long)v.p2);
     }
 }
 
-__device__ inline void lds_add(const LdsAcc &L, uint32_t e, const u128 &f) { lds_add_n(L, e, 1, f.lo, f.hi); }
+__device__ inline void lds_add(const LdsAcc &L, uint32_t e, const Limbs &v) { lds_add_n(L, e, 1, v); }
 
-__device__ inline void global_add(const TablesDev &t, int tab, uint32_t key, uint64_t count, uint64_t lo, uint64_t hi)
+__device__ inline void global_add(const TablesDev &t, int tab, uint32_t key, uint64_t count, const Limbs &v)
 {
     atomicAdd((unsigned long long *)&t.count[tab][key], (unsigned long long)count);
-    if (lo | hi) {
-        const unsigned long long old = atomicAdd((unsigned long long *)&t.sum_lo[tab][key], (unsigned long long)lo);
-        const uint64_t carry = (old + lo < old) ? 1 : 0;
-        if (hi + carry) atomicAdd((unsigned long long *)&t.sum_hi[tab][key], (unsigned long long)(hi + carry));
+    if (nonzero(v)) {
+        atomicAdd((unsigned long long *)&t.part[tab][0][key], (unsigned long long)v.p0);
+        atomicAdd((unsigned long long *)&t.part[tab][1][key], (unsigned long long)v.p1);
+        atomicAdd((unsigned long long *)&t.part[tab][2][key], (unsigned long long)v.p2);
     }
-}
-
-__device__ inline u128 to_fixed(double v, uint32_t f64)
-{
-    return fixed_from_double(f64 ? v : (double)(float)v);
-}
-
-__device__ inline void add128(uint64_t &lo, uint64_t &hi, uint64_t blo, uint64_t bhi)
-{
-    const uint64_t s = lo + blo;
-    hi += bhi + (s < lo ? 1 : 0);
-    lo = s;
 }
 
 // LDS entry map of one read-group tile
@@ -161,16 +187,21 @@ __device__ inline void add128(uint64_t &lo, uint64_t &hi, uint64_t blo, uint64_t
 //   [1090, 1090+128)       context I/D: 2 x 64 three-base contexts
 //   [1218, 1218+2*ncodes)  cycle I/D: 2 x cycle codes
 static constexpr uint32_t E_CM = 66, E_CID = 66 + 1024, E_YID = 66 + 1024 + 128;
+static constexpr uint32_t LDS_ENTRY_BYTES = 3 * 8 + 4;
+static constexpr uint32_t COV_WAVES = 16;
+static constexpr uint32_t TILE = 1024;                     // reads per workgroup tile
+static constexpr uint32_t STAGE_BYTES = COV_WAVES * 256;   // per-wave copies of the window's read bases
 
 // the covariate keys of one base and their accumulation
+template <class BASE>
 __device__ inline void emit_base(const CovArgs &a, const LdsAcc &L, uint32_t rg, uint32_t ncodes, uint16_t flags,
-                                 bool neg, uint64_t rs, uint16_t rwx, uint16_t rwy, uint16_t hx, uint16_t hy,
-                                 uint16_t bp, uint32_t q, const u128 &f0, const u128 &f1, const u128 &f2,
+                                 bool neg, BASE base, uint16_t rwx, uint16_t rwy, uint16_t hx, uint16_t hy,
+                                 uint16_t bp, uint32_t q, const Limbs &f0, const Limbs &f1, const Limbs &f2,
                                  bool id_in_lds)
 {
     // RecalTable1: EventTracker(2) | QualityScore(8) | ReadGroup(8)
     if (q < 64) lds_add(L, q, f0);
-    else global_add(a.t, 0, 0u | (q << 2) | (rg << 10), 1, f0.lo, f0.hi);
+    else global_add(a.t, 0, 0u | (q << 2) | (rg << 10), 1, f0);
     if (id_in_lds) {   // the wave path reduces these across lanes itself
         lds_add(L, 64, f1);
         lds_add(L, 65, f2);
@@ -178,22 +209,33 @@ __device__ inline void emit_base(const CovArgs &a, const LdsAcc &L, uint32_t rg,
     // cycle: EventTracker(2) | Cycle(10) | QualityScore(8) | ReadGroup(8)
     uint32_t cm, cid;
     cycle_keys(flags, rwx, rwy, bp, cm, cid);
-    global_add(a.t, 1, 0u | (cm << 2) | (q << 12) | (rg << 20), 1, f0.lo, f0.hi);
+    {
+        const uint32_t key = 0u | (cm << 2) | (q << 12) | (rg << 20);
+        if (lds_cycle_m(L, cm, q)) {
+            if (nonzero(f0)) {
+                atomicAdd((unsigned long long *)&a.t.part[1][0][key], (unsigned long long)f0.p0);
+                atomicAdd((unsigned long long *)&a.t.part[1][1][key], (unsigned long long)f0.p1);
+                atomicAdd((unsigned long long *)&a.t.part[1][2][key], (unsigned long long)f0.p2);
+            }
+        } else {
+            global_add(a.t, 1, key, 1, f0);
+        }
+    }
     if (cid != 1023u) {
         if (cid < ncodes) {
             lds_add(L, E_YID + cid, f1);
             lds_add(L, E_YID + ncodes + cid, f2);
         } else {
-            global_add(a.t, 1, 1u | (cid << 2) | (45u << 12) | (rg << 20), 1, f1.lo, f1.hi);
-            global_add(a.t, 1, 2u | (cid << 2) | (45u << 12) | (rg << 20), 1, f2.lo, f2.hi);
+            global_add(a.t, 1, 1u | (cid << 2) | (45u << 12) | (rg << 20), 1, f1);
+            global_add(a.t, 1, 2u | (cid << 2) | (45u << 12) | (rg << 20), 1, f2);
         }
     }
     // context: EventTracker(2) | Context(11) | QualityScore(8) | ReadGroup(8)
     uint32_t xm, xid;
-    context_keys(a.b.reads, rs, hx, hy, neg, bp, xm, xid);
+    context_keys(base, hx, hy, neg, bp, xm, xid);
     if (xm != 2047u) {
         if (q < 64) lds_add(L, E_CM + q * 16 + (xm >> 4), f0);
-        else global_add(a.t, 2, 0u | (xm << 2) | (q << 13) | (rg << 21), 1, f0.lo, f0.hi);
+        else global_add(a.t, 2, 0u | (xm << 2) | (q << 13) | (rg << 21), 1, f0);
     }
     if (xid != 2047u) {
         lds_add(L, E_CID + (xid >> 4), f1);
@@ -212,6 +254,7 @@ __device__ void cov_read_serial(const CovArgs &a, const LdsAcc &L, uint32_t r, u
     const int len = (int)rwy - (int)rwx + 1;
     const uint8_t *baq = a.s.baq + qs + rwx;
     const uint8_t *bfl = a.s.base_flags + rs + rwx;
+    // the serial path works in the fractional-error array itself
     double *frac = a.s.frac + 3ull * (qs + rwx);
     for (uint32_t k = 0; k < b.read_len[r]; k++)
         if (k < rwx || k > rwy) { double *z = a.s.frac + 3ull * (qs + k); z[0] = 0.0; z[1] = 0.0; z[2] = 0.0; }
@@ -228,6 +271,7 @@ __device__ void cov_read_serial(const CovArgs &a, const LdsAcc &L, uint32_t r, u
     const bool neg = (flags & F_REVERSE) != 0;
     const uint32_t *ops = b.cigars + b.cigar_start[r];
     const uint32_t nops = b.cigar_len[r];
+    auto base = [&](int i) { return read_base(b.reads, (uint64_t)rs + (uint32_t)i); };
     uint16_t ro = 0;
     for (uint32_t oi = 0; oi < nops; oi++) {
         const uint32_t op = op_code(ops[oi]), l = op_len(ops[oi]);
@@ -238,8 +282,8 @@ __device__ void cov_read_serial(const CovArgs &a, const LdsAcc &L, uint32_t r, u
                 if (bp < rwx || bp > rwy) continue;
                 if (a.s.base_flags[rs + bp] & BF_MASKED) continue;
                 const double *fe = a.s.frac + 3ull * (qs + bp);
-                emit_base(a, L, rg, ncodes, flags, neg, rs, rwx, rwy, hx, hy, bp, b.quals[qs + bp],
-                          to_fixed(fe[0], a.frac_f64), to_fixed(fe[1], a.frac_f64), to_fixed(fe[2], a.frac_f64),
+                emit_base(a, L, rg, ncodes, flags, neg, base, rwx, rwy, hx, hy, bp, b.quals[qs + bp],
+                          to_limbs(fe[0], a.frac_f64), to_limbs(fe[1], a.frac_f64), to_limbs(fe[2], a.frac_f64),
                           true);
             }
         }
@@ -289,8 +333,31 @@ __device__ inline int pop_range(const uint64_t *m, int a, int b)
     return n;
 }
 
-// one read on one wavefront (clipped window of at most 256 bases)
-__device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, uint32_t r, uint32_t rg, uint32_t ncodes)
+// set bits [lo, hi) of a 256-bit mask (wave-uniform values)
+__device__ inline void mset_range(uint64_t *m, int lo, int hi)
+{
+    lo = lo < 0 ? 0 : lo;
+    hi = hi > 256 ? 256 : hi;
+    for (int w = 0; w < 4; w++) {
+        const int a0 = lo - 64 * w, a1 = hi - 64 * w;
+        if (a1 <= 0 || a0 >= 64) continue;
+        const int x0 = a0 < 0 ? 0 : a0, x1 = a1 > 64 ? 64 : a1;
+        const uint64_t hm = x1 >= 64 ? ~0ull : ((1ull << x1) - 1);
+        const uint64_t lm = (1ull << x0) - 1;
+        m[w] |= hm & ~lm;
+    }
+}
+
+__device__ inline uint64_t wave_sum(uint64_t v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// one read on one wavefront (clipped window of at most 256 bases); r is wave-uniform
+__device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, uint8_t *stage, uint32_t r, uint32_t rg,
+                              uint32_t ncodes)
 {
     const BatchDev &b = a.b;
     const int lane = threadIdx.x & 63;
@@ -302,8 +369,9 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, uint32_t r, uin
     const int len = (int)rwy - (int)rwx + 1;
 
     // zero the fractional errors outside the window (defined output for every quality of the read)
-    for (uint32_t k = lane; k < L_read; k += 64)
-        if (k < rwx || k > rwy) { double *z = a.s.frac + 3ull * (qs + k); z[0] = 0.0; z[1] = 0.0; z[2] = 0.0; }
+    if (a.keep_frac)
+        for (uint32_t k = lane; k < L_read; k += 64)
+            if (k < rwx || k > rwy) { double *z = a.s.frac + 3ull * (qs + k); z[0] = 0.0; z[1] = 0.0; z[2] = 0.0; }
 
     uint8_t tag[4], fl[4], qv[4];
     uint64_t IB[4], ES[4], EI_[4], ED[4], LQ[4];
@@ -315,6 +383,7 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, uint32_t r, uin
         tag[c] = v ? a.s.baq[qs + p] : 64;
         fl[c] = v ? a.s.base_flags[rs + p] : 0;
         qv[c] = v ? b.quals[qs + p] : 255;
+        if (v) stage[j] = read_base(b.reads, (uint64_t)rs + p);
         IB[c] = __ballot(v && tag[c] != 64);
         ES[c] = __ballot(v && (fl[c] & BF_SNP));
         EI_[c] = __ballot(v && (fl[c] & BF_INS));
@@ -331,11 +400,26 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, uint32_t r, uin
 
     const uint16_t flags = b.flags[r];
     const bool neg = (flags & F_REVERSE) != 0;
-    const uint32_t *ops = b.cigars + b.cigar_start[r];
-    const uint32_t nops = b.cigar_len[r];
+    // soft-clipped window positions (S ops emit nothing), from a wave-uniform walk of the CIGAR
+    uint64_t SM[4] = {0, 0, 0, 0};
+    {
+        const uint32_t *ops = b.cigars + b.cigar_start[r];
+        const uint32_t nops = b.cigar_len[r];
+        int ro = 0;
+        for (uint32_t oi = 0; oi < nops; oi++) {
+            const uint32_t w = ops[oi];
+            const uint32_t op = op_code(w), l = op_len(w);
+            if (!op_reads(op)) continue;
+            if (op_event(op) == EV_S) mset_range(SM, ro - (int)rwx, ro + (int)l - (int)rwx);
+            ro += (int)l;
+        }
+    }
+    // the stage holds the window's bases; context positions stay inside the high-quality window
+    __builtin_amdgcn_wave_barrier();
+    auto base = [&](int i) -> uint8_t { return stage[i - (int)rwx]; };
 
     uint32_t nid = 0;
-    uint64_t ilo = 0, ihi = 0, dlo = 0, dhi = 0;
+    uint64_t i0 = 0, i1 = 0, i2 = 0, d0 = 0, d1 = 0, d2 = 0;
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const int j = lane + 64 * c;
@@ -368,53 +452,45 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, uint32_t r, uin
             vd = mtest(ED, j) ? 1.0 : 0.0;
         }
         const uint16_t bp = (uint16_t)(rwx + j);
-        double *fo = a.s.frac + 3ull * (qs + bp);
-        fo[0] = vs; fo[1] = vi; fo[2] = vd;
-
-        if (fl[c] & BF_MASKED) continue;
-        // soft-clipped bases emit nothing
-        bool soft = false;
-        {
-            uint32_t ro = 0;
-            for (uint32_t oi = 0; oi < nops; oi++) {
-                const uint32_t op = op_code(ops[oi]), l = op_len(ops[oi]);
-                if (!op_reads(op)) continue;
-                if (bp >= ro && bp < ro + l) { soft = op_event(op) == EV_S; break; }
-                ro += l;
-            }
+        if (a.keep_frac) {
+            double *fo = a.s.frac + 3ull * (qs + bp);
+            fo[0] = vs; fo[1] = vi; fo[2] = vd;
         }
-        if (soft) continue;
-        const u128 f0 = to_fixed(vs, a.frac_f64), f1 = to_fixed(vi, a.frac_f64), f2 = to_fixed(vd, a.frac_f64);
+        if (fl[c] & BF_MASKED) continue;
+        if (mtest(SM, j)) continue;
+        const Limbs f0 = to_limbs(vs, a.frac_f64), f1 = to_limbs(vi, a.frac_f64), f2 = to_limbs(vd, a.frac_f64);
         nid++;
-        add128(ilo, ihi, f1.lo, f1.hi);
-        add128(dlo, dhi, f2.lo, f2.hi);
-        emit_base(a, L, rg, ncodes, flags, neg, rs, rwx, rwy, hx, hy, bp, qv[c], f0, f1, f2, false);
+        i0 += f1.p0; i1 += f1.p1; i2 += f1.p2;
+        d0 += f2.p0; d1 += f2.p1; d2 += f2.p2;
+        emit_base(a, L, rg, ncodes, flags, neg, base, rwx, rwy, hx, hy, bp, qv[c], f0, f1, f2, false);
     }
     // the quality-table I and D keys are the same for every base of the read: one LDS update per read
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        nid += __shfl_xor(nid, off);
-        const uint64_t ol = __shfl_xor(ilo, off), oh = __shfl_xor(ihi, off);
-        add128(ilo, ihi, ol, oh);
-        const uint64_t pl = __shfl_xor(dlo, off), ph = __shfl_xor(dhi, off);
-        add128(dlo, dhi, pl, ph);
-    }
+    nid = (uint32_t)wave_sum(nid);
+    const Limbs ti = {wave_sum(i0), wave_sum(i1), wave_sum(i2)};
+    const Limbs td = {wave_sum(d0), wave_sum(d1), wave_sum(d2)};
     if (lane == 0 && nid) {
-        lds_add_n(L, 64, nid, ilo, ihi);
-        lds_add_n(L, 65, nid, dlo, dhi);
+        lds_add_n(L, 64, nid, ti);
+        lds_add_n(L, 65, nid, td);
     }
+    // the stage is reused by the wave's next read
+    __builtin_amdgcn_wave_barrier();
 }
 
-// one workgroup per (read group, tile of 256 reads of that group)
-__global__ void __launch_bounds__(256) k_cov(CovArgs a)
+// one workgroup per (read group, tile of TILE reads of that group)
+__global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
 {
     extern __shared__ unsigned long long lds_raw[];
     const uint32_t ncodes = a.ncodes;
     const uint32_t E = E_YID + 2 * ncodes;
+    const uint32_t MW = (a.mcodes * 64 + 1) / 2;    // words of packed cycle-M counts
     LdsAcc L;
-    L.lo = lds_raw;
-    L.hi = lds_raw + E;
-    L.cnt = (uint32_t *)(lds_raw + 2 * E);
+    L.p0 = lds_raw;
+    L.p1 = lds_raw + E;
+    L.p2 = lds_raw + 2 * E;
+    L.cnt = (uint32_t *)(lds_raw + 3 * E);
+    L.mcnt = L.cnt + E;
+    L.mcodes = a.mcodes;
+    uint8_t *stage = (uint8_t *)(L.mcnt + MW) + (threadIdx.x >> 6) * 256;
 
     // which read group / tile this block covers
     const uint32_t nrg = a.t.num_rg;
@@ -427,18 +503,19 @@ __global__ void __launch_bounds__(256) k_cov(CovArgs a)
     }
     const uint32_t rg = lo_rg;
     const uint32_t tile = blk - a.tile_offset[rg];
-    const uint32_t first = a.rg_offset[rg] + tile * 256;
-    const uint32_t last = min(a.rg_offset[rg + 1], first + 256);
+    const uint32_t first = a.rg_offset[rg] + tile * TILE;
+    const uint32_t last = min(a.rg_offset[rg + 1], first + TILE);
 
-    for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) { L.lo[e] = 0; L.hi[e] = 0; L.cnt[e] = 0; }
+    for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) { L.p0[e] = 0; L.p1[e] = 0; L.p2[e] = 0; L.cnt[e] = 0; }
+    for (uint32_t w = threadIdx.x; w < MW; w += blockDim.x) L.mcnt[w] = 0;
     __syncthreads();
 
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (uint32_t li = first + wave; li < last; li += 4) {
-        const uint32_t r = a.rg_list[li];
+    for (uint32_t li = first + wave; li < last; li += COV_WAVES) {
+        const uint32_t r = __builtin_amdgcn_readfirstlane(a.rg_list[li]);
         const uint32_t rw = a.s.rwc[r];
         const int len = (int)hi16(rw) - (int)lo16(rw) + 1;
-        if (len <= 256) cov_read_wave(a, L, r, rg, ncodes);
+        if (len <= 256) cov_read_wave(a, L, stage, r, rg, ncodes);
         else if (lane == 0) cov_read_serial(a, L, r, rg, ncodes);
     }
     __syncthreads();
@@ -461,7 +538,15 @@ __global__ void __launch_bounds__(256) k_cov(CovArgs a)
             const uint32_t ev = 1 + (e - E_YID) / ncodes, code = (e - E_YID) % ncodes;
             tab = 1; key = ev | (code << 2) | (45u << 12) | (rg << 20);
         }
-        global_add(a.t, tab, key, c, L.lo[e], L.hi[e]);
+        const Limbs v = {L.p0[e], L.p1[e], L.p2[e]};
+        global_add(a.t, tab, key, c, v);
+    }
+    for (uint32_t i = threadIdx.x; i < a.mcodes * 64; i += blockDim.x) {
+        const uint32_t c = (L.mcnt[i >> 1] >> (16 * (i & 1))) & 0xffff;
+        if (!c) continue;
+        const uint32_t q = i / a.mcodes, code = i % a.mcodes;
+        atomicAdd((unsigned long long *)&a.t.count[1][0u | (code << 2) | (q << 12) | (rg << 20)],
+                  (unsigned long long)c);
     }
 }
 
@@ -507,7 +592,7 @@ __global__ void __launch_bounds__(256) k_rg_offsets(uint32_t num_rg, const uint3
         tile_offset[g] = t;
         rg_cursor[g] = 0;
         o += rg_count[g];
-        t += (rg_count[g] + 255) / 256;
+        t += (rg_count[g] + TILE - 1) / TILE;
     }
     rg_offset[num_rg] = o;
     tile_offset[num_rg] = t;
@@ -527,9 +612,33 @@ __global__ void __launch_bounds__(256) k_rg_scatter(uint32_t n, const uint8_t *s
 
 } // namespace fpd
 
+// LDS layout of one covariate workgroup for reads up to max_read_len within the device's
+// per-workgroup limit: cycle codes beyond what fits go to the resident tables directly
+static void cov_layout(uint32_t max_read_len, uint32_t *ncodes, uint32_t *mcodes, size_t *bytes)
+{
+    static int limit = -1;
+    if (limit < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        limit = v;
+    }
+    const uint32_t budget = (uint32_t)limit - fpd::STAGE_BYTES;
+    uint32_t nc = 2 * max_read_len + 2, mc = 2 * max_read_len + 2;
+    auto need = [&](uint32_t n, uint32_t m) {
+        return (fpd::E_YID + 2 * n) * fpd::LDS_ENTRY_BYTES + 4 * ((m * 64 + 1) / 2);
+    };
+    while (need(nc, mc) > budget && mc > 0) mc = mc > 16 ? mc - 16 : 0;
+    while (need(nc, mc) > budget && nc > 0) nc = nc > 16 ? nc - 16 : 0;
+    *ncodes = nc;
+    *mcodes = mc;
+    *bytes = need(nc, mc) + fpd::STAGE_BYTES;
+}
+
 extern "C" hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
-                                     uint32_t frac_f64, uint32_t *counters, uint32_t *rg_buf, uint32_t max_read_len,
-                                     hipStream_t st)
+                                     uint32_t frac_f64, uint32_t keep_frac, uint32_t *counters, uint32_t *rg_buf,
+                                     uint32_t max_read_len, hipStream_t st)
 {
     const uint32_t n = b->num_reads;
     if (n == 0) return hipSuccess;
@@ -545,12 +654,12 @@ extern "C" hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStat
     hipLaunchKernelGGL(fpd::k_rg_scatter, dim3((n + 255) / 256), dim3(256), 0, st, n, s->status, b->read_group, nrg,
                        off, cur, list);
     fpd::CovArgs a;
-    a.b = *b; a.s = *s; a.t = *t; a.frac_f64 = frac_f64;
-    a.ncodes = 2 * max_read_len + 2;
+    a.b = *b; a.s = *s; a.t = *t; a.frac_f64 = frac_f64; a.keep_frac = keep_frac;
+    size_t lds = 0;
+    cov_layout(max_read_len, &a.ncodes, &a.mcodes, &lds);
     a.rg_list = list; a.rg_offset = off; a.tile_offset = toff;
-    const uint32_t E = fpd::E_YID + 2 * a.ncodes;
-    const size_t lds = (size_t)E * 20;
-    const uint32_t blocks = (n + 255) / 256 + nrg;
-    hipLaunchKernelGGL(fpd::k_cov, dim3(blocks), dim3(256), lds, st, a);
+    const uint32_t blocks = (n + fpd::TILE - 1) / fpd::TILE + nrg;
+    (void)hipFuncSetAttribute((const void *)fpd::k_cov, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(fpd::k_cov, dim3(blocks), dim3(64 * fpd::COV_WAVES), lds, st, a);
     return hipGetLastError();
 }

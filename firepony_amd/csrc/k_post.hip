@@ -84,8 +84,7 @@ __global__ void __launch_bounds__(256) k_empirical(EmpArgs a)
 struct RgArgs {
     uint32_t num_rg;
     const uint64_t *count;   // dense quality table
-    const uint64_t *lo;
-    const uint64_t *hi;
+    const uint64_t *p0, *p1, *p2;   // mismatch sum limbs
     uint32_t *valid;
     uint64_t *obs;
     double *mism, *expected, *estimated, *empirical;
@@ -103,7 +102,8 @@ __global__ void __launch_bounds__(256) k_rg_table(RgArgs a)
         const uint32_t key = ev | (q << 2) | (rg << 10);
         const uint64_t c = a.count[key];
         if (c == 0) continue;
-        const double mm = fixed_to_double(a.lo[key], a.hi[key]);
+        const u128 v = fixed_combine(a.p0[key], a.p1[key], a.p2[key]);
+        const double mm = fixed_to_double(v.lo, v.hi);
         const double ee = c * c_qual_err[q];
         if (!any) { obs = c; m = mm; ex = ee; any = true; }
         else { obs += c; m = m + mm; ex = ex + ee; }
@@ -145,13 +145,14 @@ extern "C" hipError_t fpd_launch_empirical(uint64_t n, const uint32_t *key, cons
     return hipGetLastError();
 }
 
-extern "C" hipError_t fpd_launch_rg_table(uint32_t num_rg, const uint64_t *count, const uint64_t *lo, const uint64_t *hi,
+extern "C" hipError_t fpd_launch_rg_table(uint32_t num_rg, const uint64_t *count, const uint64_t *const *parts,
                                           uint32_t *valid, uint64_t *obs, double *mism, double *expected,
                                           double *estimated, double *empirical, hipStream_t st)
 {
     if (num_rg == 0) return hipSuccess;
     fpd::RgArgs a;
-    a.num_rg = num_rg; a.count = count; a.lo = lo; a.hi = hi; a.valid = valid; a.obs = obs;
+    a.num_rg = num_rg; a.count = count; a.p0 = parts[0]; a.p1 = parts[1]; a.p2 = parts[2];
+    a.valid = valid; a.obs = obs;
     a.mism = mism; a.expected = expected; a.estimated = estimated; a.empirical = empirical;
     hipLaunchKernelGGL(fpd::k_rg_table, dim3((num_rg * 4 + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError();

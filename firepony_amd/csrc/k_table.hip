@@ -20,8 +20,8 @@ __global__ void __launch_bounds__(256) k_compact_count(const uint64_t *count, ui
     if (threadIdx.x == 0) block_counts[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
-__global__ void __launch_bounds__(256) k_compact_scatter(const uint64_t *count, const uint64_t *lo, const uint64_t *hi,
-                                                         uint64_t n, const uint64_t *block_offsets,
+__global__ void __launch_bounds__(256) k_compact_scatter(const uint64_t *count, const uint64_t *p0, const uint64_t *p1,
+                                                         const uint64_t *p2, uint64_t n, const uint64_t *block_offsets,
                                                          uint32_t *okey, uint64_t *ocount, uint64_t *olo,
                                                          uint64_t *ohi)
 {
@@ -39,22 +39,23 @@ __global__ void __launch_bounds__(256) k_compact_scatter(const uint64_t *count, 
         const uint64_t o = block_offsets[blockIdx.x] + rank;
         okey[o] = (uint32_t)i;
         ocount[o] = count[i];
-        olo[o] = lo[i];
-        ohi[o] = hi[i];
+        const u128 v = fixed_combine(p0[i], p1[i], p2[i]);
+        olo[o] = v.lo;
+        ohi[o] = v.hi;
     }
 }
 
-__global__ void __launch_bounds__(256) k_table_add(uint64_t *dc, uint64_t *dlo, uint64_t *dhi, const uint64_t *sc,
-                                                   const uint64_t *slo, const uint64_t *shi, uint64_t n)
+__global__ void __launch_bounds__(256) k_table_add(uint64_t *dc, uint64_t *d0, uint64_t *d1, uint64_t *d2,
+                                                   const uint64_t *sc, const uint64_t *s0, const uint64_t *s1,
+                                                   const uint64_t *s2, uint64_t n)
 {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     if (sc[i] == 0) return;
     dc[i] += sc[i];
-    const uint64_t a = dlo[i], b = slo[i];
-    const uint64_t s = a + b;
-    dlo[i] = s;
-    dhi[i] += shi[i] + (s < a ? 1 : 0);
+    d0[i] += s0[i];
+    d1[i] += s1[i];
+    d2[i] += s2[i];
 }
 
 } // namespace fpd
@@ -66,20 +67,21 @@ extern "C" hipError_t fpd_compact_count(const uint64_t *count, uint64_t n, uint3
     return hipGetLastError();
 }
 
-extern "C" hipError_t fpd_compact_scatter(const uint64_t *count, const uint64_t *lo, const uint64_t *hi, uint64_t n,
+extern "C" hipError_t fpd_compact_scatter(const uint64_t *count, const uint64_t *const *parts, uint64_t n,
                                           const uint64_t *block_offsets, uint32_t *okey, uint64_t *ocount,
                                           uint64_t *olo, uint64_t *ohi, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(fpd::k_compact_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, count, lo, hi, n,
-                       block_offsets, okey, ocount, olo, ohi);
+    hipLaunchKernelGGL(fpd::k_compact_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, count, parts[0],
+                       parts[1], parts[2], n, block_offsets, okey, ocount, olo, ohi);
     return hipGetLastError();
 }
 
-extern "C" hipError_t fpd_table_add(uint64_t *dc, uint64_t *dlo, uint64_t *dhi, const uint64_t *sc, const uint64_t *slo,
-                                    const uint64_t *shi, uint64_t n, hipStream_t st)
+extern "C" hipError_t fpd_table_add(uint64_t *dc, uint64_t *const *dparts, const uint64_t *sc,
+                                    const uint64_t *const *sparts, uint64_t n, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(fpd::k_table_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dc, dlo, dhi, sc, slo, shi, n);
+    hipLaunchKernelGGL(fpd::k_table_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dc, dparts[0], dparts[1],
+                       dparts[2], sc, sparts[0], sparts[1], sparts[2], n);
     return hipGetLastError();
 }

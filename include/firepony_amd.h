@@ -66,7 +66,10 @@ typedef struct fp_options {
     uint32_t max_read_length;        /* upper bound of read lengths; lets batches run with
                                         no host round trip; 0 = measured per batch      */
     uint32_t aggregation;            /* fp_aggregation                                 */
-    uint32_t reserved[6];
+    uint32_t debug_arrays;           /* 1: also keep the per-base fractional errors for
+                                        fp_engine_debug_fetch (the tables never need
+                                        them); 0: leave them out                       */
+    uint32_t reserved[5];
 } fp_options;
 
 /* how covariate observations reach the resident tables (identical results) */
