@@ -1,17 +1,23 @@
 """Benchmark: reads/s of the BQSR+BAQ device pipeline on one MI355X.
 
-Workload: 30 Mbp genome in 3 contigs, 10M single-end 101 bp reads over 4 read
-groups (substitution rates 0.5/1.0/1.5/2.0 %), 3 % indel reads, 5 % soft
-clips, 30k known sites, processed in batches of 20,000 reads (the reference's
-GPU batch size, firepony.cu:148).  All batches are uploaded to HBM before the
-timed region; the timed region covers every batch plus postprocess, which is
-how the reference defines reads/s (firepony.cu:348-376, 222).
+Workload (the reference's GPU-at-scale case): 30 Mbp genome in 3 contigs,
+10M single-end 101 bp reads over 4 read groups (substitution rates
+0.5/1.0/1.5/2.0 %), 3 % indel reads, 5 % soft clips, 30k known sites, BAQ
+band 7, seeded synthetic data.  All input batches are uploaded to HBM before
+the timed regions; a timed region covers every batch plus postprocess, which
+is how the reference defines reads/s (firepony.cu:348-376, 222).
+
+Two passes over the same 10M reads:
+  plain    batches of 20,000 reads (the reference's GPU batch size,
+           firepony.cu:148), one launch per stage per batch;
+  batched  batches of 500,000 reads (one step = one batch).
+`value` is the batched pass.  The roofline object describes the kernel with
+the most device time in that pass (HIP events on the engine's stream); its
+`traffic` comes from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.
+The CPU oracle (a restatement of the reference's host path) is timed on a
+bounded sample of the same reads.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-
-A step is one batch.  The JSON line reports the engine's whole-job reads/s,
-the dominant kernel's roofline fraction (HIP events on the engine's stream),
-and the CPU oracle timed on a bounded sample of the same workload.
 """
 import argparse
 import ctypes as C
@@ -73,8 +79,23 @@ class DeviceBatch:
         self.num_bases = int(arrs["read_len"].sum())
 
 
+def load_traffic():
+    """Per-dispatch HBM bytes by kernel from the committed counter summary
+    (lines 'traffic <kernel> <fetch KiB> <write KiB> <fetch scale> <write scale>')."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_config1_batch500k.txt")
+    out = {}
+    if not os.path.exists(path):
+        return out
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 6 and p[0] == "traffic":
+                out[p[1]] = (float(p[2]) * float(p[4]) + float(p[3]) * float(p[5])) * 1024.0
+    return out
+
+
 def roofline(stats, total_reads, bases):
-    """Roofline of the kernel with the most device time."""
+    """Roofline of the kernel with the most device time (per launch)."""
     k = {
         "baq_hmm": (stats.baq_hmm_kernel_ms, stats.baq_hmm_launches),
         "cov": (stats.cov_kernel_ms, stats.cov_launches),
@@ -83,17 +104,20 @@ def roofline(stats, total_reads, bases):
     name = max(k, key=lambda x: k[x][0])
     ms, launches = k[name]
     avg_s = ms / 1e3 / max(launches, 1)
+    traffic = load_traffic().get(name)
     if name == "baq_hmm":
+        # FP64 work of the banded HMM: 43 flops per band cell (forward 17 + rescale 3,
+        # backward 16 + rescale 3, MAP 4; baq.cu:374-681)
         flops = BAQ_FLOPS_PER_CELL * stats.baq_cells / max(launches, 1)
         ach = flops / avg_s / 1e12
         return name, {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                      "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
-    # HBM-bound kernels: compulsory bytes of the inputs they read and outputs they write
+                      "frac": ach / FP64_PEAK_TFLOPS, "traffic": traffic, "avg_launch_ms": avg_s * 1e3}
+    # scan stages: compulsory bytes per read (inputs read + per-read outputs written)
     per_read = {"prep": 49 + 13 * 4 + 101 + 4 * 1.2 + 54, "cov": 101 + 13 * 4 + 101 + 101 + 8}[name]
     byts = per_read * total_reads / max(launches, 1)
     ach = byts / avg_s / 1e9
     return name, {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": ach / HBM_PEAK_GBS, "traffic": None}
+                  "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_ms": avg_s * 1e3}
 
 
 def cpu_baseline(genome, rs, sample_reads):
@@ -113,12 +137,45 @@ def cpu_baseline(genome, rs, sample_reads):
                       f"{dt:.1f} s"}
 
 
+def upload(genome, rs, total, batch, count, torch):
+    dev = []
+    nthreads = min(16, os.cpu_count() or 4)
+    for i in range(count):
+        hb = synth.Batch(genome, rs, i * batch, min(batch, total - i * batch), threads=nthreads)
+        dev.append(DeviceBatch(hb, torch))
+        del hb
+    torch.cuda.synchronize()
+    return dev
+
+
+def timed_pass(genome, dev, warmup, torch, **opts):
+    """Process every batch plus postprocess; returns (seconds, engine stats)."""
+    e = Engine(genome, device=0, debug_arrays=0, **opts)
+    for i in range(warmup):
+        e.process_device(dev[i % len(dev)].view)
+    e.synchronize()
+    e.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for d in dev:
+        e.process_device(d.view)
+    e.postprocess()
+    e.synchronize()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = e.stats()
+    e.close()
+    return dt, st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch-reads", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=0, help="batched steps to time (0 = all 20 batches of 500k reads)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch-reads", type=int, default=500000)
+    ap.add_argument("--plain-batch-reads", type=int, default=20000)
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--wpe", type=int, default=0, help="BAQ waves per SIMD (0 = engine default)")
@@ -136,48 +193,42 @@ def main():
         print(json.dumps({"copy_bytes": n, "launches": 3}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the path would shard by batch, "
-              f"see the sharding paragraph of the design notes)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not run here (the path shards by batch with one "
+              f"table merge at the end; see the sharding paragraph of the design notes)", file=sys.stderr)
         sys.exit(2)
 
     import torch
     gs, rs, total = synth.named_spec(1)
     genome = synth.Genome(gs, rs.num_read_groups)
     B = args.batch_reads
-    K = min(args.steps, (total + B - 1) // B)
-    if B != 20000 and args.steps == 500:
-        K = (total + B - 1) // B
-    log(f"generating {K} batches of {B} reads")
+    K = (total + B - 1) // B
+    if args.steps:
+        K = min(K, args.steps)
+    reads_b = min(total, K * B)
+
+    out_plain = None
+    if not args.no_plain:
+        PB = args.plain_batch_reads
+        log(f"plain: generating {(reads_b + PB - 1) // PB} batches of {PB} reads")
+        devp = upload(genome, rs, reads_b, PB, (reads_b + PB - 1) // PB, torch)
+        dt, st = timed_pass(genome, devp, args.warmup, torch, baq_waves_per_simd=args.wpe)
+        out_plain = {"value": reads_b / dt, "batch_reads": PB, "batches": len(devp), "seconds": dt,
+                     "kernel_ms": {"prep": st.prep_kernel_ms, "baq_hmm": st.baq_hmm_kernel_ms,
+                                   "cov": st.cov_kernel_ms}}
+        log(f"plain: {reads_b} reads in {dt:.3f} s -> {reads_b / dt:.4g} reads/s")
+        del devp
+        torch.cuda.empty_cache()
+
+    log(f"batched: generating {K} batches of {B} reads")
     t0 = time.perf_counter()
-    dev = []
-    nthreads = min(16, os.cpu_count() or 4)
-    for i in range(K):
-        hb = synth.Batch(genome, rs, i * B, min(B, total - i * B), threads=nthreads)
-        dev.append(DeviceBatch(hb, torch))
-        del hb
-    torch.cuda.synchronize()
+    dev = upload(genome, rs, reads_b, B, K, torch)
     log(f"generated + uploaded in {time.perf_counter() - t0:.1f} s")
-
-    e = Engine(genome, device=0, baq_waves_per_simd=args.wpe, debug_arrays=0)
-    for i in range(args.warmup):
-        e.process_device(dev[i % K].view)
-    e.synchronize()
-    e.reset()
-    torch.cuda.synchronize()
-
-    t0 = time.perf_counter()
-    for d in dev:
-        e.process_device(d.view)
-    e.postprocess()
-    e.synchronize()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    dt, st = timed_pass(genome, dev, args.warmup, torch, baq_waves_per_simd=args.wpe)
     reads = sum(d.num_reads for d in dev)
     bases = sum(d.num_bases for d in dev)
-    st = e.stats()
     value = reads / dt
     kname, roof = roofline(st, reads, bases)
-    log(f"plain engine: {reads} reads in {dt:.3f} s -> {value:.4g} reads/s; kernels: prep {st.prep_kernel_ms:.1f} ms, "
+    log(f"batched: {reads} reads in {dt:.3f} s -> {value:.4g} reads/s; kernels: prep {st.prep_kernel_ms:.1f} ms, "
         f"baq_hmm {st.baq_hmm_kernel_ms:.1f} ms ({st.baq_hmm_launches} launches), cov {st.cov_kernel_ms:.1f} ms, "
         f"postprocess {st.postprocessing_ms:.1f} ms; baq reads {st.baq_reads}, cells {st.baq_cells}")
 
@@ -193,12 +244,16 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded generator, configs[1] shape)",
+        "data": "synthetic (seeded generator, 30 Mbp genome, 10M x 101 bp reads)",
         "config": {"workload": "30 Mbp / 3 contigs, 10M SE 101 bp reads, 4 read groups, 30k known sites, "
-                               f"batches of {B} reads, BAQ band 7", "reads": reads, "batch_reads": B,
-                   "engine": "plain"},
+                               "BAQ band 7", "reads": reads, "batch_reads": B, "engine": "batched"},
+        "kernel_ms": {"prep": st.prep_kernel_ms, "baq_hmm": st.baq_hmm_kernel_ms, "cov": st.cov_kernel_ms,
+                      "postprocess": st.postprocessing_ms},
         "roofline": dict(roof, kernel=kname),
     }
+    if out_plain:
+        out["plain"] = out_plain
+        out["batched_over_plain"] = value / out_plain["value"]
     if not args.no_cpu_baseline:
         log("timing the CPU oracle baseline")
         out["cpu_baseline"] = cpu_baseline(genome, rs, min(args.cpu_sample, total))
