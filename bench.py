@@ -3,19 +3,24 @@
 Workload (the reference's GPU-at-scale case): 30 Mbp genome in 3 contigs,
 10M single-end 101 bp reads over 4 read groups (substitution rates
 0.5/1.0/1.5/2.0 %), 3 % indel reads, 5 % soft clips, 30k known sites, BAQ
-band 7, seeded synthetic data.  All input batches are uploaded to HBM before
-the timed regions; a timed region covers every batch plus postprocess, which
-is how the reference defines reads/s (firepony.cu:348-376, 222).
+band 7, seeded synthetic data, delivered as batches of 20,000 reads (the
+reference's GPU batch size, firepony.cu:148).  All input batches are uploaded
+to HBM before the timed regions; a timed region covers every batch plus
+postprocess, which is how the reference defines reads/s
+(firepony.cu:348-376, 222).  A step is one input batch.
 
-Two passes over the same 10M reads:
-  plain    batches of 20,000 reads (the reference's GPU batch size,
-           firepony.cu:148), one launch per stage per batch;
-  batched  batches of 500,000 reads (one step = one batch).
-`value` is the batched pass.  The roofline object describes the kernel with
-the most device time in that pass (HIP events on the engine's stream); its
-`traffic` comes from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.
-The CPU oracle (a restatement of the reference's host path) is timed on a
-bounded sample of the same reads.
+Passes over the same input batches:
+  plain     each batch runs as it arrives, one launch per stage;
+  batched   batches are staged into 1M-read device buffers and each buffer
+            runs as captured hipGraph replays; the BAQ variant is autotuned
+            during one untimed warm-up pass over the input.  This is `value`.
+  profile   the batched engine with an event between kernels (no graphs):
+            per-kernel times for the roofline object.
+A bounded sample of the mixed-length workload (reads of 50-250 bp, 8 read
+groups, 5 % indels of 1-10 bp, clips on every read) is timed plain and
+batched as well.  The roofline's `traffic` comes from the committed
+rocprofv3 FETCH_SIZE/WRITE_SIZE summary.  The CPU oracle (a restatement of the
+reference's host path) is timed on a bounded sample of the same reads.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -137,11 +142,11 @@ def cpu_baseline(genome, rs, sample_reads):
                       f"{dt:.1f} s"}
 
 
-def upload(genome, rs, total, batch, count, torch):
+def upload(genome, rs, total, batch, torch):
     dev = []
     nthreads = min(16, os.cpu_count() or 4)
-    for i in range(count):
-        hb = synth.Batch(genome, rs, i * batch, min(batch, total - i * batch), threads=nthreads)
+    for first in range(0, total, batch):
+        hb = synth.Batch(genome, rs, first, min(batch, total - first), threads=nthreads)
         dev.append(DeviceBatch(hb, torch))
         del hb
     torch.cuda.synchronize()
@@ -149,12 +154,14 @@ def upload(genome, rs, total, batch, count, torch):
 
 
 def timed_pass(genome, dev, warmup, torch, **opts):
-    """Process every batch plus postprocess; returns (seconds, engine stats)."""
+    """warmup untimed passes over the input (reset after), then every batch plus postprocess timed;
+    returns (seconds, engine stats)."""
     e = Engine(genome, device=0, debug_arrays=0, **opts)
-    for i in range(warmup):
-        e.process_device(dev[i % len(dev)].view)
-    e.synchronize()
-    e.reset()
+    for _ in range(warmup):
+        for d in dev:
+            e.process_device(d.view)
+        e.synchronize()
+        e.reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for d in dev:
@@ -168,17 +175,23 @@ def timed_pass(genome, dev, warmup, torch, **opts):
     return dt, st
 
 
+def kernel_ms(st):
+    return {"prep": st.prep_kernel_ms, "baq_hmm": st.baq_hmm_kernel_ms, "cov": st.cov_kernel_ms,
+            "postprocess": st.postprocessing_ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=0, help="batched steps to time (0 = all 20 batches of 500k reads)")
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch-reads", type=int, default=500000)
-    ap.add_argument("--plain-batch-reads", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=0, help="input batches to time (0 = all 500)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed passes over the input before each timed pass")
+    ap.add_argument("--batch-reads", type=int, default=20000, help="reads per input batch")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--mixed-reads", type=int, default=2000000)
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--wpe", type=int, default=0, help="BAQ waves per SIMD (0 = engine default)")
+    ap.add_argument("--wpe", type=int, default=0, help="BAQ waves per SIMD (0 = autotuned)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -201,36 +214,36 @@ def main():
     gs, rs, total = synth.named_spec(1)
     genome = synth.Genome(gs, rs.num_read_groups)
     B = args.batch_reads
-    K = (total + B - 1) // B
     if args.steps:
-        K = min(K, args.steps)
-    reads_b = min(total, K * B)
+        total = min(total, args.steps * B)
+    log(f"generating {(total + B - 1) // B} input batches of {B} reads")
+    t0 = time.perf_counter()
+    dev = upload(genome, rs, total, B, torch)
+    log(f"generated + uploaded in {time.perf_counter() - t0:.1f} s")
+    K = len(dev)
+    reads = sum(d.num_reads for d in dev)
+    bases = sum(d.num_bases for d in dev)
+    batched_opts = dict(engine_kind=abi.FP_ENGINE_BATCHED, max_read_length=101)
 
     out_plain = None
     if not args.no_plain:
-        PB = args.plain_batch_reads
-        log(f"plain: generating {(reads_b + PB - 1) // PB} batches of {PB} reads")
-        devp = upload(genome, rs, reads_b, PB, (reads_b + PB - 1) // PB, torch)
-        dt, st = timed_pass(genome, devp, args.warmup, torch, baq_waves_per_simd=args.wpe)
-        out_plain = {"value": reads_b / dt, "batch_reads": PB, "batches": len(devp), "seconds": dt,
-                     "kernel_ms": {"prep": st.prep_kernel_ms, "baq_hmm": st.baq_hmm_kernel_ms,
-                                   "cov": st.cov_kernel_ms}}
-        log(f"plain: {reads_b} reads in {dt:.3f} s -> {reads_b / dt:.4g} reads/s")
-        del devp
-        torch.cuda.empty_cache()
+        dt, st = timed_pass(genome, dev, args.warmup, torch, baq_waves_per_simd=args.wpe or 1)
+        out_plain = {"value": reads / dt, "seconds": dt, "kernel_ms": kernel_ms(st)}
+        log(f"plain: {reads} reads in {dt:.3f} s -> {reads / dt:.4g} reads/s")
 
-    log(f"batched: generating {K} batches of {B} reads")
-    t0 = time.perf_counter()
-    dev = upload(genome, rs, reads_b, B, K, torch)
-    log(f"generated + uploaded in {time.perf_counter() - t0:.1f} s")
-    dt, st = timed_pass(genome, dev, args.warmup, torch, baq_waves_per_simd=args.wpe)
-    reads = sum(d.num_reads for d in dev)
-    bases = sum(d.num_bases for d in dev)
+    dt, st = timed_pass(genome, dev, args.warmup, torch, baq_waves_per_simd=args.wpe, **batched_opts)
     value = reads / dt
-    kname, roof = roofline(st, reads, bases)
-    log(f"batched: {reads} reads in {dt:.3f} s -> {value:.4g} reads/s; kernels: prep {st.prep_kernel_ms:.1f} ms, "
-        f"baq_hmm {st.baq_hmm_kernel_ms:.1f} ms ({st.baq_hmm_launches} launches), cov {st.cov_kernel_ms:.1f} ms, "
-        f"postprocess {st.postprocessing_ms:.1f} ms; baq reads {st.baq_reads}, cells {st.baq_cells}")
+    wpe = int(st.baq_waves_per_simd)
+    log(f"batched: {reads} reads in {dt:.3f} s -> {value:.4g} reads/s (BAQ waves/SIMD {wpe}); "
+        f"stage 1 {st.prep_kernel_ms:.1f} ms, stage 2 {st.baq_ms:.1f} ms, postprocess {st.postprocessing_ms:.1f} ms")
+
+    pdt, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=wpe, profile_stages=1, **batched_opts)
+    kname, roof = roofline(pst, reads, bases)
+    log(f"profile: prep {pst.prep_kernel_ms:.1f} ms, baq_hmm {pst.baq_hmm_kernel_ms:.1f} ms "
+        f"({pst.baq_hmm_launches} launches), cov {pst.cov_kernel_ms:.1f} ms; baq reads {pst.baq_reads}, "
+        f"cells {pst.baq_cells}")
+    del dev
+    torch.cuda.empty_cache()
 
     out = {
         "metric": "reads/s BQSR+BAQ at 1 GPU, batched vs plain; p50/p99 job latency",
@@ -246,14 +259,33 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded generator, 30 Mbp genome, 10M x 101 bp reads)",
         "config": {"workload": "30 Mbp / 3 contigs, 10M SE 101 bp reads, 4 read groups, 30k known sites, "
-                               "BAQ band 7", "reads": reads, "batch_reads": B, "engine": "batched"},
-        "kernel_ms": {"prep": st.prep_kernel_ms, "baq_hmm": st.baq_hmm_kernel_ms, "cov": st.cov_kernel_ms,
-                      "postprocess": st.postprocessing_ms},
+                               "BAQ band 7, input batches of 20,000 reads", "reads": reads, "batch_reads": B,
+                   "engine": "batched (1M-read staging buffers, hipGraph replays)", "baq_waves_per_simd": wpe},
+        "kernel_ms": kernel_ms(pst),
         "roofline": dict(roof, kernel=kname),
     }
     if out_plain:
         out["plain"] = out_plain
         out["batched_over_plain"] = value / out_plain["value"]
+
+    if not args.no_mixed:
+        gs2, rs2, total2 = synth.named_spec(2)
+        genome2 = synth.Genome(gs2, rs2.num_read_groups)
+        n2 = min(total2, args.mixed_reads)
+        log(f"mixed lengths: generating {n2} reads in batches of {B}")
+        dev2 = upload(genome2, rs2, n2, B, torch)
+        mopts = dict(engine_kind=abi.FP_ENGINE_BATCHED, max_read_length=250, frac_error_f64=1)
+        pdt2, _ = timed_pass(genome2, dev2, args.warmup, torch, baq_waves_per_simd=1, frac_error_f64=1)
+        bdt2, bst2 = timed_pass(genome2, dev2, args.warmup, torch, **mopts)
+        out["mixed_length"] = {"reads": n2, "workload": "50-250 bp reads, 8 read groups, 5 % indel reads of "
+                                                        "1-10 bp, clips of 0-30 bp on every read, fp64 "
+                                                        "fractional errors; first reads of the 20M-read set",
+                               "plain": n2 / pdt2, "batched": n2 / bdt2, "batched_over_plain": pdt2 / bdt2,
+                               "baq_waves_per_simd": int(bst2.baq_waves_per_simd)}
+        log(f"mixed lengths: plain {n2 / pdt2:.4g} reads/s, batched {n2 / bdt2:.4g} reads/s")
+        del dev2
+        torch.cuda.empty_cache()
+
     if not args.no_cpu_baseline:
         log("timing the CPU oracle baseline")
         out["cpu_baseline"] = cpu_baseline(genome, rs, min(args.cpu_sample, total))

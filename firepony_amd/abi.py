@@ -53,7 +53,8 @@ class FpOptions(C.Structure):
         ("max_read_length", C.c_uint32),
         ("aggregation", C.c_uint32),
         ("debug_arrays", C.c_uint32),
-        ("reserved", C.c_uint32 * 5),
+        ("profile_stages", C.c_uint32),
+        ("reserved", C.c_uint32 * 4),
     ]
 
 
@@ -152,7 +153,8 @@ class FpStats(C.Structure):
         ("baq_hmm_launches", C.c_uint64),
         ("cov_launches", C.c_uint64),
         ("baq_cells", C.c_uint64),
-        ("reserved", C.c_uint64 * 5),
+        ("baq_waves_per_simd", C.c_uint64),
+        ("reserved", C.c_uint64 * 4),
     ]
 
 

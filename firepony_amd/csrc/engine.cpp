@@ -28,15 +28,19 @@ extern "C" {
 hipError_t fpd_launch_prep(const fpd::BatchDev *b, const fpd::GenomeDev *g, const fpd::ReadStateDev *s,
                            uint32_t *counters, hipStream_t st);
 hipError_t fpd_baq_set_constants(const double *qual2prob, const double *thr, double gap_open);
-hipError_t fpd_launch_baq_classify(uint32_t num_reads, const fpd::ReadStateDev *s, uint32_t *lists,
-                                   uint32_t *counts, uint32_t capacity, hipStream_t st);
+hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t *live, const fpd::ReadStateDev *s,
+                                   uint32_t *lists, uint32_t *counts, uint32_t capacity, hipStream_t st);
 hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const fpd::GenomeDev *g, const fpd::ReadStateDev *s,
                               const uint32_t *list, const uint32_t *count, double *scratch, uint64_t slot_doubles,
                               uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves, unsigned long long *cells,
                               int wpe, hipStream_t st);
 hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *s, uint32_t band_done,
                                hipStream_t st);
-hipError_t fpd_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st);
+hipError_t fpd_fill_u8(uint8_t *p, uint64_t n, const uint32_t *live, uint8_t v, hipStream_t st);
+hipError_t fpd_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+hipError_t fpd_rebase(uint32_t n, uint32_t *cigar_start, uint32_t *read_start, uint32_t *qual_start,
+                      uint32_t cig_off, uint32_t base_off, uint32_t qual_off, hipStream_t st);
+hipError_t fpd_cov_prepare();
 hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
                           uint32_t frac_f64, uint32_t keep_frac, uint32_t *counters, uint32_t *rg_buf,
                           uint32_t max_read_len, hipStream_t st);
@@ -185,7 +189,8 @@ struct fp_engine {
     DevBuf in[18];
     // per-batch state
     DevBuf st_status, st_rwc, st_noins, st_refw, st_nerr, st_hmmw, st_bw, st_hqw, st_bflags, st_baq, st_state, st_frac;
-    DevBuf counters, baq_lists, baq_counts, scratch, rg_buf;
+    DevBuf counters, baq_lists, baq_counts, scratch, scratch_gen, rg_buf;
+    uint32_t *host_counts = nullptr;   // pinned readback area (64 u32)
     uint32_t last_num_reads = 0;
     uint64_t last_num_bases = 0, last_num_quals = 0;
     fpd::ReadStateDev rs{};
@@ -201,6 +206,37 @@ struct fp_engine {
 
     fp_stats stats;
     hipEvent_t ev[16];
+
+    // ---- batched engine: input batches are appended to one of two staging buffers; a full buffer
+    // runs as two graph replays (stage 1: prep + classification; stage 2: BAQ bands, cap/recode,
+    // covariates) with one small host readback between them (band classes, longest read)
+    struct Stage {
+        DevBuf arr[18];                 // the fp_batch_view arrays, capacity-sized
+        DevBuf live;                    // {reads, qualities lo, qualities hi, 0}
+        uint32_t *live_host = nullptr;  // pinned
+        uint32_t n = 0;
+        uint64_t ncig = 0, nbases = 0, nquals = 0;
+        hipEvent_t filled = nullptr, consumed = nullptr;
+        hipGraphExec_t gfront = nullptr, gback = nullptr;
+        int g_wpe = 0;
+        uint64_t g_epoch = 0;
+    } stg[2];
+    bool batched = false;
+    int cur = 0;
+    hipStream_t copy_stream = nullptr;
+    uint32_t cap_reads = 0, rows_cap = 0;
+    uint64_t cap_cig = 0, cap_bases = 0, cap_quals = 0;
+    DevBuf bcounters;                   // prep / BAQ counters accumulated over staged batches
+    uint64_t epoch = 1;                 // changes when a buffer the graphs refer to is reallocated
+    bool staged_ready = false;
+    // autotuner: waves per SIMD of the 15-cell BAQ band kernel, timed on the first staged batches
+    bool tuned = false;
+    int tune_pending = -1;              // candidate whose stage-2 timing is in flight
+    uint32_t tune_pending_reads = 0;
+    double tune_ms_per_read[4] = {0, 0, 0, 0};
+    hipEvent_t bev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool back_pending = false;          // stage-2 time of the last staged batch not yet collected
+    uint64_t eager_filtered = 0, eager_baq = 0, eager_cells = 0;   // oversized batches run eagerly
 };
 
 namespace {
@@ -223,7 +259,9 @@ fp_status alloc_tables(fp_engine *e)
     return FP_OK;
 }
 
-fp_status ensure_state(fp_engine *e, uint32_t n, uint64_t nbases, uint64_t nquals)
+// per-read and per-base state for n reads; the per-quality fractional-error array is needed for
+// inspection (debug_arrays) and as working memory of windows longer than 256 bases
+fp_status ensure_state(fp_engine *e, uint32_t n, uint64_t nbases, uint64_t nquals, bool need_frac = true)
 {
     HIP_TRY(e->st_status.ensure(n));
     HIP_TRY(e->st_rwc.ensure(4ull * n));
@@ -236,7 +274,7 @@ fp_status ensure_state(fp_engine *e, uint32_t n, uint64_t nbases, uint64_t nqual
     HIP_TRY(e->st_bflags.ensure(nbases + 8));
     HIP_TRY(e->st_baq.ensure(nquals + 8));
     HIP_TRY(e->st_state.ensure(4 * (nquals + 8)));
-    HIP_TRY(e->st_frac.ensure(24 * (nquals + 8)));
+    if (need_frac) HIP_TRY(e->st_frac.ensure(24 * (nquals + 8)));
     HIP_TRY(e->counters.ensure(64));
     HIP_TRY(e->baq_lists.ensure(4ull * 4 * std::max<uint32_t>(n, 1)));
     HIP_TRY(e->rg_buf.ensure(4ull * (4ull * std::max<uint32_t>(e->num_rg, 1) + 2 + n)));
@@ -259,8 +297,9 @@ fp_status ensure_state(fp_engine *e, uint32_t n, uint64_t nbases, uint64_t nqual
 
 fpd::BatchDev to_dev(const fp_batch_view *v)
 {
-    fpd::BatchDev b;
+    fpd::BatchDev b{};
     b.num_reads = v->num_reads;
+    b.live = nullptr;          // sizes are the view's (a staged batch sets its device descriptor)
     b.chromosome = v->chromosome;
     b.alignment_start = v->alignment_start;
     b.alignment_stop = v->alignment_stop;
@@ -291,6 +330,82 @@ float elapsed(hipEvent_t a, hipEvent_t b)
 
 static const uint32_t BAQ_CLASS_W[4] = {15, 23, 33, 0};
 
+// what the launch sequence needs to know about a batch
+struct Plan {
+    uint32_t n;              // reads (the capacity for a staged batch)
+    uint64_t nquals;         // qualities (the capacity for a staged batch)
+    const uint32_t *live;    // device live sizes of a staged batch, or null
+    uint32_t rows_cap;       // >= the longest read
+};
+
+// resident waves of a band class; its per-wave scratch slot in doubles
+static uint32_t band_waves(uint32_t wpe_class0, int c, uint32_t n)
+{
+    const uint32_t wpe = c == 0 ? wpe_class0 : 1u;
+    const uint32_t w = std::min<uint32_t>((n + 63) / 64, 1024u * wpe);
+    return (w + 3) / 4 * 4;
+}
+static uint64_t band_slot(int c, uint32_t rows_cap, uint32_t wgen)
+{
+    const uint64_t W = c < 3 ? BAQ_CLASS_W[c] : wgen;
+    // forward rows (M, I) | scale factors | staged cap codes and tags (band kernels) or band state (generic)
+    return ((uint64_t)(rows_cap + 1) * W * 2 + (rows_cap + 2) + (c == 3 ? 3ull * W : rows_cap + 1ull)) * 64;
+}
+static uint64_t band_scratch_doubles(int wpe, uint32_t n, uint32_t rows_cap)
+{
+    uint64_t need = 0;
+    for (int c = 0; c < 3; c++) need = std::max(need, band_slot(c, rows_cap, 0) * band_waves(wpe, c, n));
+    return need;
+}
+
+// stage 1 (graph-safe): per-read prep, BAQ tag reset, band classification
+fp_status launch_front(fp_engine *e, const fpd::BatchDev &b, const Plan &p, uint32_t *counters, hipStream_t st)
+{
+    HIP_TRY(fpd_fill_u32(e->baq_counts.as<uint32_t>(), 16, 0u, st));
+    HIP_TRY(fpd_launch_prep(&b, &e->genome, &e->rs, counters, st));
+    HIP_TRY(fpd_fill_u8(e->rs.baq, p.nquals, p.live, 0xff, st));
+    HIP_TRY(fpd_launch_baq_classify(p.n, p.live, &e->rs, e->baq_lists.as<uint32_t>(), e->baq_counts.as<uint32_t>(),
+                                    std::max<uint32_t>(p.n, 1), st));
+    return FP_OK;
+}
+
+// stage 2 (graph-safe): BAQ band kernels, cap/recode, covariate tables.  The class of bands wider
+// than 33 cells runs separately (launch_generic) before it, when present.
+fp_status launch_back(fp_engine *e, const fpd::BatchDev &b, const Plan &p, int wpe, uint32_t *counters,
+                      hipStream_t st, hipEvent_t *marks, const uint32_t *known_counts)
+{
+    const uint32_t cap = std::max<uint32_t>(p.n, 1);
+    for (int c = 0; c < 3; c++) {
+        if (known_counts && known_counts[c] == 0) continue;   // eager launches skip empty classes
+        HIP_TRY(fpd_launch_baq_hmm(c, &b, &e->genome, &e->rs, e->baq_lists.as<uint32_t>() + (uint64_t)c * cap,
+                                   e->baq_counts.as<uint32_t>() + c, e->scratch.as<double>(),
+                                   band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c],
+                                   band_waves(wpe, c, p.n), (unsigned long long *)(counters + 8), wpe, st));
+    }
+    if (marks) HIP_TRY(hipEventRecord(marks[0], st));
+    HIP_TRY(fpd_launch_baq_post(&b, &e->rs, 1u, st));
+    if (marks) HIP_TRY(hipEventRecord(marks[1], st));
+    HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, e->opts.debug_arrays ? 1u : 0u, counters,
+                           e->rg_buf.as<uint32_t>(), std::min<uint32_t>(p.rows_cap, 499), st));
+    return FP_OK;
+}
+
+// bands wider than 33 cells (rare: large indels), sized from the widest such band
+fp_status launch_generic(fp_engine *e, const fpd::BatchDev &b, const Plan &p, uint32_t count, uint32_t wgen,
+                         uint32_t *counters, hipStream_t st)
+{
+    if (count == 0) return FP_OK;
+    const uint32_t waves = std::min<uint32_t>((count + 63) / 64, 256u);
+    const uint64_t slot = band_slot(3, p.rows_cap, wgen);
+    HIP_TRY(e->scratch_gen.ensure(slot * waves * 8 + 8));
+    HIP_TRY(fpd_launch_baq_hmm(3, &b, &e->genome, &e->rs,
+                               e->baq_lists.as<uint32_t>() + 3ull * std::max<uint32_t>(p.n, 1),
+                               e->baq_counts.as<uint32_t>() + 3, e->scratch_gen.as<double>(), slot, p.rows_cap, wgen,
+                               waves, (unsigned long long *)(counters + 8), e->wpe, st));
+    return FP_OK;
+}
+
+// one batch, eagerly, with the host reading the class counts between the stages (the plain engine)
 fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_hint)
 {
     const uint32_t n = v->num_reads;
@@ -301,72 +416,31 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     e->last_num_quals = v->num_qualities;
     hipStream_t st = e->stream;
     fpd::BatchDev b = to_dev(v);
+    uint32_t *counters = e->counters.as<uint32_t>();
 
     HIP_TRY(hipEventRecord(e->ev[0], st));
-    HIP_TRY(hipMemsetAsync(e->counters.p, 0, 64, st));
-    HIP_TRY(hipMemsetAsync(e->baq_counts.p, 0, 64, st));
+    HIP_TRY(fpd_fill_u32(counters, 16, 0u, st));
     HIP_TRY(hipEventRecord(e->ev[6], st));
-    HIP_TRY(fpd_launch_prep(&b, &e->genome, &e->rs, e->counters.as<uint32_t>(), st));
+    Plan p{n, v->num_qualities, nullptr, 1};
+    rc = launch_front(e, b, p, counters, st);
+    if (rc != FP_OK) return rc;
     HIP_TRY(hipEventRecord(e->ev[7], st));
-    HIP_TRY(hipEventRecord(e->ev[1], st));
-    HIP_TRY(fpd_fill_u8(e->rs.baq, v->num_qualities, 0xff, st));
-    HIP_TRY(hipMemsetAsync(e->rs.frac, 0, 24 * v->num_qualities, st));
-    HIP_TRY(fpd_launch_baq_classify(n, &e->rs, e->baq_lists.as<uint32_t>(), e->baq_counts.as<uint32_t>(),
-                                    std::max<uint32_t>(n, 1), st));
-    uint32_t cnt[16];
-    HIP_TRY(hipMemcpyAsync(cnt, e->baq_counts.p, 16, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(cnt + 4, e->counters.p, 16, hipMemcpyDeviceToHost, st));
+    uint32_t *hc = e->host_counts;
+    HIP_TRY(hipMemcpyAsync(hc, e->baq_counts.p, 32, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hc + 8, counters, 32, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    const uint32_t rows_cap = std::max<uint32_t>(std::max(max_read_len_hint, cnt[4 + 3]), 1);
+    p.rows_cap = std::max<uint32_t>(std::max(max_read_len_hint, hc[8 + 3]), 1);
+    HIP_TRY(e->scratch.ensure(band_scratch_doubles(e->wpe, n, p.rows_cap) * 8 + 8));
 
-    // band-class kernels; generic class sized by the widest band
-    uint32_t wmax_generic = 0;
-    if (cnt[3]) {
-        std::vector<uint32_t> ids(cnt[3]);
-        std::vector<uint16_t> bwv(n);
-        HIP_TRY(hipMemcpy(ids.data(), e->baq_lists.as<uint32_t>() + 3ull * std::max<uint32_t>(n, 1), 4ull * cnt[3],
-                          hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(bwv.data(), e->rs.bandwidth, 2ull * n, hipMemcpyDeviceToHost));
-        for (uint32_t id : ids) wmax_generic = std::max<uint32_t>(wmax_generic, 2u * bwv[id] + 1);
-    }
-    uint64_t need = 0;
-    uint32_t waves[4];
-    uint64_t slot[4];
-    for (int c = 0; c < 4; c++) {
-        const uint32_t W = c < 3 ? BAQ_CLASS_W[c] : wmax_generic;
-        if (c < 3) {
-            // band kernels: one resident set of 4-wave workgroups (256 CUs x 4 SIMDs x waves per SIMD)
-            const uint32_t wpe = c == 0 ? (uint32_t)e->wpe : 1u;
-            waves[c] = std::min<uint32_t>((cnt[c] + 63) / 64, 1024u * wpe);
-            waves[c] = (waves[c] + 3) / 4 * 4;
-        } else {
-            waves[c] = std::min<uint32_t>((cnt[c] + 63) / 64, 2048u);
-        }
-        // forward rows (M, I) | scale factors | staged MAP results (band kernels) or band state (generic)
-        slot[c] = ((uint64_t)(rows_cap + 1) * W * 2 + (rows_cap + 2) + (c == 3 ? 3ull * W : rows_cap + 1ull)) * 64;
-        if (cnt[c]) need = std::max(need, slot[c] * waves[c]);
-    }
-    HIP_TRY(e->scratch.ensure(need * 8 + 8));
-    uint32_t hmm_launches = 0;
     HIP_TRY(hipEventRecord(e->ev[8], st));
-    for (int c = 0; c < 4; c++) {
-        if (!cnt[c]) continue;
-        HIP_TRY(fpd_launch_baq_hmm(c, &b, &e->genome, &e->rs,
-                                   e->baq_lists.as<uint32_t>() + (uint64_t)c * std::max<uint32_t>(n, 1),
-                                   e->baq_counts.as<uint32_t>() + c, e->scratch.as<double>(), slot[c], rows_cap,
-                                   c == 3 ? wmax_generic : BAQ_CLASS_W[c], waves[c],
-                                   (unsigned long long *)(e->counters.as<uint32_t>() + 8), e->wpe, st));
-        hmm_launches++;
-    }
-    HIP_TRY(hipEventRecord(e->ev[9], st));
-    HIP_TRY(fpd_launch_baq_post(&b, &e->rs, 1u, st));
-    HIP_TRY(hipEventRecord(e->ev[2], st));
-    HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, e->opts.debug_arrays ? 1u : 0u,
-                           e->counters.as<uint32_t>(), e->rg_buf.as<uint32_t>(), std::min<uint32_t>(rows_cap, 499),
-                           st));
+    rc = launch_generic(e, b, p, hc[3], hc[4], counters, st);
+    if (rc != FP_OK) return rc;
+    hipEvent_t marks[2] = {e->ev[9], e->ev[2]};
+    rc = launch_back(e, b, p, e->wpe, counters, st, marks, hc);
+    if (rc != FP_OK) return rc;
     HIP_TRY(hipEventRecord(e->ev[3], st));
-    uint32_t post[16];
-    HIP_TRY(hipMemcpyAsync(post, e->counters.p, 64, hipMemcpyDeviceToHost, st));
+    uint32_t *post = e->host_counts + 16;
+    HIP_TRY(hipMemcpyAsync(post, counters, 64, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (post[4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
 
@@ -377,17 +451,350 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     uint64_t cells;
     memcpy(&cells, post + 8, 8);
     e->stats.baq_cells += cells;
+    const uint32_t hmm_launches = (hc[0] ? 1 : 0) + (hc[1] ? 1 : 0) + (hc[2] ? 1 : 0) + (hc[3] ? 1 : 0);
     e->stats.prep_kernel_ms += elapsed(e->ev[6], e->ev[7]);
     e->stats.prep_launches += 1;
     if (hmm_launches) e->stats.baq_hmm_kernel_ms += elapsed(e->ev[8], e->ev[9]);
     e->stats.baq_hmm_launches += hmm_launches;
     e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[3]);
     e->stats.cov_launches += 1;
-    e->stats.read_filter_ms += elapsed(e->ev[0], e->ev[1]);
-    e->stats.baq_ms += elapsed(e->ev[1], e->ev[2]);
+    e->stats.read_filter_ms += elapsed(e->ev[0], e->ev[7]);
+    e->stats.baq_ms += elapsed(e->ev[7], e->ev[2]);
     e->stats.covariates_ms += elapsed(e->ev[2], e->ev[3]);
     e->postprocessed = false;
     return FP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched engine
+
+static const int TUNE_CANDIDATES[3] = {1, 2, 3};
+
+fpd::BatchDev stage_dev(fp_engine *e, fp_engine::Stage &s)
+{
+    fp_batch_view v{};
+    v.num_reads = e->cap_reads;
+    v.chromosome = s.arr[0].as<uint16_t>();
+    v.alignment_start = s.arr[1].as<uint32_t>();
+    v.alignment_stop = s.arr[2].as<uint32_t>();
+    v.mate_chromosome = s.arr[3].as<uint32_t>();
+    v.mate_alignment_start = s.arr[4].as<uint32_t>();
+    v.inferred_insert_size = s.arr[5].as<int32_t>();
+    v.cigars = s.arr[6].as<uint32_t>();
+    v.cigar_start = s.arr[7].as<uint32_t>();
+    v.cigar_len = s.arr[8].as<uint32_t>();
+    v.reads = s.arr[9].as<uint32_t>();
+    v.read_start = s.arr[10].as<uint32_t>();
+    v.read_len = s.arr[11].as<uint32_t>();
+    v.qualities = s.arr[12].as<uint8_t>();
+    v.qual_start = s.arr[13].as<uint32_t>();
+    v.qual_len = s.arr[14].as<uint32_t>();
+    v.flags = s.arr[15].as<uint16_t>();
+    v.mapq = s.arr[16].as<uint8_t>();
+    v.read_group = s.arr[17].as<uint32_t>();
+    fpd::BatchDev b = to_dev(&v);
+    b.live = s.live.as<uint32_t>();
+    return b;
+}
+
+// capacity-sized buffers of the batched engine, allocated once (the graphs refer to them)
+fp_status batched_alloc(fp_engine *e)
+{
+    if (e->staged_ready) return FP_OK;
+    const uint64_t C = e->cap_reads;
+    const uint64_t sz[18] = {2 * C, 4 * C, 4 * C, 4 * C, 4 * C, 4 * C, 4 * e->cap_cig, 4 * C, 4 * C,
+                             e->cap_bases / 2, 4 * C, 4 * C, e->cap_quals, 4 * C, 4 * C, 2 * C, C, 4 * C};
+    for (auto &s : e->stg) {
+        for (int i = 0; i < 18; i++) HIP_TRY(s.arr[i].ensure(sz[i] + 64));
+        HIP_TRY(s.live.ensure(16));
+        if (!s.live_host) HIP_TRY(hipHostMalloc((void **)&s.live_host, 16, hipHostMallocDefault));
+        if (!s.filled) HIP_TRY(hipEventCreateWithFlags(&s.filled, hipEventDisableTiming));
+        if (!s.consumed) HIP_TRY(hipEventCreateWithFlags(&s.consumed, hipEventDisableTiming));
+    }
+    fp_status rc = ensure_state(e, e->cap_reads, e->cap_bases, e->cap_quals,
+                                e->opts.debug_arrays != 0 || e->rows_cap > 256);
+    if (rc != FP_OK) return rc;
+    HIP_TRY(e->scratch.ensure(band_scratch_doubles(std::max(3, e->wpe), e->cap_reads, e->rows_cap) * 8 + 8));
+    HIP_TRY(e->bcounters.ensure(64));
+    HIP_TRY(fpd_fill_u32(e->bcounters.as<uint32_t>(), 16, 0u, e->stream));
+    for (auto &ev : e->bev) if (!ev) HIP_TRY(hipEventCreate(&ev));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    e->staged_ready = true;
+    e->epoch++;
+    return FP_OK;
+}
+
+template <class F>
+fp_status capture(fp_engine *e, hipGraphExec_t *out, F body)
+{
+    if (*out) { (void)hipGraphExecDestroy(*out); *out = nullptr; }
+    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const fp_status rc = body();
+    hipGraph_t g = nullptr;
+    const hipError_t he = hipStreamEndCapture(e->stream, &g);
+    if (rc != FP_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+    HIP_TRY(he);
+    hipGraphExec_t ex = nullptr;
+    const hipError_t hi = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_TRY(hi);
+    *out = ex;
+    return FP_OK;
+}
+
+// finish the timing of the last staged batch (its stream work has completed)
+void tune_collect(fp_engine *e)
+{
+    if (e->back_pending) {
+        e->stats.baq_ms += elapsed(e->bev[0], e->bev[1]);
+        e->back_pending = false;
+    }
+    if (e->tune_pending < 0) return;
+    const double ms = elapsed(e->bev[0], e->bev[1]);
+    e->tune_ms_per_read[e->tune_pending] = ms / std::max<uint32_t>(e->tune_pending_reads, 1);
+    e->tune_pending = -1;
+    if (e->tune_ms_per_read[0] > 0 && e->tune_ms_per_read[1] > 0 && e->tune_ms_per_read[2] > 0) {
+        int best = 0;
+        for (int i = 1; i < 3; i++) if (e->tune_ms_per_read[i] < e->tune_ms_per_read[best]) best = i;
+        e->wpe = TUNE_CANDIDATES[best];
+        e->tuned = true;
+    }
+}
+
+// run the current staging buffer (if it holds reads) and switch to the other one
+fp_status batched_flush(fp_engine *e)
+{
+    fp_engine::Stage &s = e->stg[e->cur];
+    if (s.n == 0) return FP_OK;
+    fp_status rc = batched_alloc(e);
+    if (rc != FP_OK) return rc;
+    hipStream_t st = e->stream;
+    s.live_host[0] = s.n;
+    s.live_host[1] = (uint32_t)s.nquals;
+    s.live_host[2] = (uint32_t)(s.nquals >> 32);
+    s.live_host[3] = 0;
+    HIP_TRY(hipMemcpyAsync(s.live.p, s.live_host, 16, hipMemcpyHostToDevice, e->copy_stream));
+    HIP_TRY(hipEventRecord(s.filled, e->copy_stream));
+    HIP_TRY(hipStreamWaitEvent(st, s.filled, 0));
+
+    const fpd::BatchDev b = stage_dev(e, s);
+    const Plan p{e->cap_reads, e->cap_quals, s.live.as<uint32_t>(), e->rows_cap};
+    uint32_t *counters = e->bcounters.as<uint32_t>();
+    if (s.g_epoch != e->epoch) {
+        rc = capture(e, &s.gfront, [&] { return launch_front(e, b, p, counters, st); });
+        if (rc != FP_OK) return rc;
+        if (s.gback) { (void)hipGraphExecDestroy(s.gback); s.gback = nullptr; }
+        s.g_epoch = e->epoch;
+    }
+    HIP_TRY(hipEventRecord(e->bev[2], st));
+    if (e->opts.profile_stages) {
+        rc = launch_front(e, b, p, counters, st);
+        if (rc != FP_OK) return rc;
+    } else {
+        HIP_TRY(hipGraphLaunch(s.gfront, st));
+    }
+    HIP_TRY(hipEventRecord(e->bev[3], st));
+    // the band classes and the longest read: one small readback per staged batch
+    uint32_t *hc = e->host_counts;
+    HIP_TRY(hipMemcpyAsync(hc, e->baq_counts.p, 32, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hc + 8, counters, 64, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    tune_collect(e);
+    e->stats.read_filter_ms += elapsed(e->bev[2], e->bev[3]);
+    e->stats.prep_kernel_ms += elapsed(e->bev[2], e->bev[3]);
+    e->stats.prep_launches += 1;
+    if (hc[8 + 4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
+    if (hc[8 + 3] > e->rows_cap)
+        return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(e->rows_cap) + ")");
+    e->stats.filtered_reads = e->eager_filtered + hc[8 + 1];
+    e->stats.baq_reads = e->eager_baq + hc[8 + 2];
+
+    int wpe = e->wpe;
+    int cand = -1;
+    if (!e->tuned) {
+        for (int i = 0; i < 3; i++) if (e->tune_ms_per_read[i] == 0) { cand = i; break; }
+        if (cand >= 0) wpe = TUNE_CANDIDATES[cand];
+    }
+    if (e->opts.profile_stages) {
+        // eager stage 2 with an event after the BAQ kernels and after cap/recode
+        HIP_TRY(hipEventRecord(e->ev[8], st));
+        rc = launch_generic(e, b, p, hc[3], hc[4], counters, st);
+        if (rc != FP_OK) return rc;
+        hipEvent_t marks[2] = {e->ev[9], e->ev[2]};
+        rc = launch_back(e, b, p, wpe, counters, st, marks, hc);
+        if (rc != FP_OK) return rc;
+        HIP_TRY(hipEventRecord(e->ev[3], st));
+        HIP_TRY(hipStreamSynchronize(st));
+        const uint32_t launches = (hc[0] ? 1 : 0) + (hc[1] ? 1 : 0) + (hc[2] ? 1 : 0) + (hc[3] ? 1 : 0);
+        e->stats.baq_hmm_kernel_ms += elapsed(e->ev[8], e->ev[9]);
+        e->stats.baq_hmm_launches += launches;
+        e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[3]);
+        e->stats.cov_launches += 1;
+        e->stats.baq_ms += elapsed(e->ev[8], e->ev[2]);
+        e->stats.covariates_ms += elapsed(e->ev[2], e->ev[3]);
+        HIP_TRY(hipEventRecord(s.consumed, st));
+        e->last_num_reads = s.n;
+        e->last_num_bases = s.nbases;
+        e->last_num_quals = s.nquals;
+        s.n = 0;
+        s.ncig = s.nbases = s.nquals = 0;
+        e->cur ^= 1;
+        e->postprocessed = false;
+        return FP_OK;
+    }
+    rc = launch_generic(e, b, p, hc[3], hc[4], counters, st);
+    if (rc != FP_OK) return rc;
+    HIP_TRY(hipEventRecord(e->bev[0], st));
+    if (cand < 0) {
+        if (!s.gback || s.g_wpe != wpe) {
+            rc = capture(e, &s.gback, [&] { return launch_back(e, b, p, wpe, counters, st, nullptr, nullptr); });
+            if (rc != FP_OK) return rc;
+            s.g_wpe = wpe;
+        }
+        HIP_TRY(hipGraphLaunch(s.gback, st));
+    } else {
+        rc = launch_back(e, b, p, wpe, counters, st, nullptr, nullptr);   // timed candidate, eager
+        if (rc != FP_OK) return rc;
+        e->tune_pending = cand;
+        e->tune_pending_reads = s.n;
+    }
+    HIP_TRY(hipEventRecord(e->bev[1], st));
+    e->back_pending = true;
+    HIP_TRY(hipEventRecord(s.consumed, st));
+    e->last_num_reads = s.n;
+    e->last_num_bases = s.nbases;
+    e->last_num_quals = s.nquals;
+    s.n = 0;
+    s.ncig = s.nbases = s.nquals = 0;
+    e->cur ^= 1;
+    e->postprocessed = false;
+    return FP_OK;
+}
+
+// wait for all staged work; collect the stage-2 time and the BAQ cell count
+fp_status batched_sync(fp_engine *e)
+{
+    fp_status rc = batched_flush(e);
+    if (rc != FP_OK) return rc;
+    HIP_TRY(hipStreamSynchronize(e->copy_stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    if (!e->staged_ready) return FP_OK;
+    const bool had_timing = e->tune_pending >= 0 || e->last_num_reads;
+    tune_collect(e);
+    uint32_t *hc = e->host_counts + 32;
+    HIP_TRY(hipMemcpy(hc, e->bcounters.p, 64, hipMemcpyDeviceToHost));
+    if (hc[4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
+    e->stats.filtered_reads = e->eager_filtered + hc[1];
+    e->stats.baq_reads = e->eager_baq + hc[2];
+    uint64_t cells;
+    memcpy(&cells, hc + 8, 8);
+    e->stats.baq_cells = e->eager_cells + cells;
+    (void)had_timing;
+    return FP_OK;
+}
+
+fp_status plain_host_batch(fp_engine *e, const fp_batch_view *hb);
+
+// append one batch (host or device arrays) to the current staging buffer
+fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device)
+{
+    const uint32_t k = v->num_reads;
+    if (k == 0) { e->stats.num_batches += 1; return FP_OK; }
+    if (k > e->cap_reads || v->num_cigar_ops > e->cap_cig || v->num_read_bases > e->cap_bases ||
+        v->num_qualities > e->cap_quals) {
+        // larger than a staging buffer: runs on its own, eagerly (it may grow shared buffers, so the
+        // graphs are captured again afterwards)
+        fp_status rc = batched_sync(e);
+        if (rc != FP_OK) return rc;
+        e->epoch++;
+        const fp_stats before = e->stats;
+        rc = on_device ? run_batch(e, v, 0) : plain_host_batch(e, v);
+        e->eager_filtered += e->stats.filtered_reads - before.filtered_reads;
+        e->eager_baq += e->stats.baq_reads - before.baq_reads;
+        e->eager_cells += e->stats.baq_cells - before.baq_cells;
+        return rc;
+    }
+    e->stats.num_batches += 1;
+    fp_engine::Stage *s = &e->stg[e->cur];
+    if (s->n + k > e->cap_reads || s->ncig + v->num_cigar_ops > e->cap_cig ||
+        s->nbases + v->num_read_bases > e->cap_bases || s->nquals + v->num_qualities > e->cap_quals) {
+        fp_status rc = batched_flush(e);
+        if (rc != FP_OK) return rc;
+        s = &e->stg[e->cur];
+    }
+    fp_status rc = batched_alloc(e);
+    if (rc != FP_OK) return rc;
+    hipStream_t cs = e->copy_stream;
+    if (s->n == 0) HIP_TRY(hipStreamWaitEvent(cs, s->consumed, 0));   // the buffer's last batch is done
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const void *src[18] = {v->chromosome, v->alignment_start, v->alignment_stop, v->mate_chromosome,
+                           v->mate_alignment_start, v->inferred_insert_size, v->cigars, v->cigar_start,
+                           v->cigar_len, v->reads, v->read_start, v->read_len, v->qualities, v->qual_start,
+                           v->qual_len, v->flags, v->mapq, v->read_group};
+    const uint64_t elem[18] = {2, 4, 4, 4, 4, 4, 0, 4, 4, 0, 4, 4, 0, 4, 4, 2, 1, 4};
+    for (int i = 0; i < 18; i++) {
+        uint64_t off, bytes;
+        if (i == 6) { off = 4 * s->ncig; bytes = 4 * v->num_cigar_ops; }
+        else if (i == 9) { off = s->nbases / 2; bytes = v->num_read_bases / 2; }
+        else if (i == 12) { off = s->nquals; bytes = v->num_qualities; }
+        else { off = elem[i] * s->n; bytes = elem[i] * k; }
+        if (bytes) HIP_TRY(hipMemcpyAsync((char *)s->arr[i].p + off, src[i], bytes, kind, cs));
+    }
+    HIP_TRY(fpd_rebase(k, s->arr[7].as<uint32_t>() + s->n, s->arr[10].as<uint32_t>() + s->n,
+                       s->arr[13].as<uint32_t>() + s->n, (uint32_t)s->ncig, (uint32_t)s->nbases,
+                       (uint32_t)s->nquals, cs));
+    s->n += k;
+    s->ncig += v->num_cigar_ops;
+    s->nbases += v->num_read_bases;
+    s->nquals += v->num_qualities;
+    e->stats.total_reads += k;
+    if (!on_device) HIP_TRY(hipStreamSynchronize(cs));   // host arrays are borrowed for the call only
+    return FP_OK;
+}
+
+// host batch: copy to the device and run it eagerly (the plain engine)
+fp_status plain_host_batch(fp_engine *e, const fp_batch_view *hb)
+{
+    const uint32_t n = hb->num_reads;
+    uint32_t maxlen = 0;
+    for (uint32_t i = 0; i < n; i++) maxlen = std::max(maxlen, std::max(hb->read_len[i], hb->qual_len[i]));
+    hipStream_t st = e->stream;
+    const size_t sz[18] = {2ull * n, 4ull * n, 4ull * n, 4ull * n, 4ull * n, 4ull * n,
+                           4 * hb->num_cigar_ops + 4, 4ull * n, 4ull * n,
+                           hb->num_read_bases / 2 + 4, 4ull * n, 4ull * n,
+                           hb->num_qualities + 4, 4ull * n, 4ull * n, 2ull * n, 1ull * n, 4ull * n};
+    const void *src[18] = {hb->chromosome, hb->alignment_start, hb->alignment_stop, hb->mate_chromosome,
+                           hb->mate_alignment_start, hb->inferred_insert_size, hb->cigars, hb->cigar_start,
+                           hb->cigar_len, hb->reads, hb->read_start, hb->read_len, hb->qualities, hb->qual_start,
+                           hb->qual_len, hb->flags, hb->mapq, hb->read_group};
+    const size_t copy_sz[18] = {2ull * n, 4ull * n, 4ull * n, 4ull * n, 4ull * n, 4ull * n,
+                                4 * hb->num_cigar_ops, 4ull * n, 4ull * n,
+                                hb->num_read_bases / 2, 4ull * n, 4ull * n,
+                                hb->num_qualities, 4ull * n, 4ull * n, 2ull * n, 1ull * n, 4ull * n};
+    for (int i = 0; i < 18; i++) {
+        HIP_TRY(e->in[i].ensure(sz[i]));
+        if (copy_sz[i]) HIP_TRY(hipMemcpyAsync(e->in[i].p, src[i], copy_sz[i], hipMemcpyHostToDevice, st));
+    }
+    fp_batch_view dv = *hb;
+    dv.chromosome = e->in[0].as<uint16_t>();
+    dv.alignment_start = e->in[1].as<uint32_t>();
+    dv.alignment_stop = e->in[2].as<uint32_t>();
+    dv.mate_chromosome = e->in[3].as<uint32_t>();
+    dv.mate_alignment_start = e->in[4].as<uint32_t>();
+    dv.inferred_insert_size = e->in[5].as<int32_t>();
+    dv.cigars = e->in[6].as<uint32_t>();
+    dv.cigar_start = e->in[7].as<uint32_t>();
+    dv.cigar_len = e->in[8].as<uint32_t>();
+    dv.reads = e->in[9].as<uint32_t>();
+    dv.read_start = e->in[10].as<uint32_t>();
+    dv.read_len = e->in[11].as<uint32_t>();
+    dv.qualities = e->in[12].as<uint8_t>();
+    dv.qual_start = e->in[13].as<uint32_t>();
+    dv.qual_len = e->in[14].as<uint32_t>();
+    dv.flags = e->in[15].as<uint16_t>();
+    dv.mapq = e->in[16].as<uint8_t>();
+    dv.read_group = e->in[17].as<uint32_t>();
+    return run_batch(e, &dv, maxlen);
 }
 
 // compaction of one dense table into key-ordered rows on the device
@@ -513,7 +920,7 @@ void fp_options_init(fp_options *o)
     memset(o, 0, sizeof(*o));
     o->abi_version = FP_ABI_VERSION;
     o->engine_kind = FP_ENGINE_PLAIN;
-    o->max_batch_reads = 20000;
+    o->max_batch_reads = 0;
     o->debug_arrays = 1;
 }
 
@@ -558,6 +965,20 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (he != hipSuccess) { delete e; return fail(FP_ERR_HIP, hipGetErrorString(he)); }
     for (int i = 0; i < 16; i++) (void)hipEventCreate(&e->ev[i]);
+    he = hipHostMalloc((void **)&e->host_counts, 64 * sizeof(uint32_t), hipHostMallocDefault);
+    if (he == hipSuccess) he = fpd_cov_prepare();
+    if (he != hipSuccess) { fp_engine_destroy(e); return fail(FP_ERR_HIP, hipGetErrorString(he)); }
+    if (o.engine_kind == FP_ENGINE_BATCHED) {
+        e->batched = true;
+        e->cap_reads = o.max_batch_reads ? std::max<uint32_t>(o.max_batch_reads, 64) : (1u << 20);
+        e->rows_cap = o.max_read_length ? o.max_read_length : 500;
+        e->cap_bases = (uint64_t)e->cap_reads * ((e->rows_cap + 7) & ~7u);
+        e->cap_quals = (uint64_t)e->cap_reads * e->rows_cap;
+        e->cap_cig = (uint64_t)e->cap_reads * 8;
+        e->tuned = o.baq_waves_per_simd != 0;
+        he = hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking);
+        if (he != hipSuccess) { fp_engine_destroy(e); return fail(FP_ERR_HIP, hipGetErrorString(he)); }
+    }
     compute_constants(e->consts);
     he = fpd_baq_set_constants(e->consts.qual2prob, e->consts.map_thr, e->consts.gap_open);
     if (he == hipSuccess)
@@ -647,58 +1068,16 @@ fp_status fp_engine_process_batch(fp_engine *e, const fp_batch_view *hb)
     rc = validate_view(hb);
     if (rc != FP_OK) return rc;
     const uint32_t n = hb->num_reads;
-    if (n == 0) { e->stats.num_batches++; return FP_OK; }
     // bounds of the host arrays (host-side validation: reads must lie inside the arrays)
-    uint32_t maxlen = 0;
     for (uint32_t i = 0; i < n; i++) {
         if ((uint64_t)hb->cigar_start[i] + hb->cigar_len[i] > hb->num_cigar_ops ||
             (uint64_t)hb->read_start[i] + ((hb->read_len[i] + 7) & ~7u) > hb->num_read_bases ||
             (uint64_t)hb->qual_start[i] + hb->qual_len[i] > hb->num_qualities || (hb->read_start[i] & 7))
             return fail(FP_ERR_INVALID_ARG, "read " + std::to_string(i) + " lies outside the batch arrays");
-        maxlen = std::max(maxlen, std::max(hb->read_len[i], hb->qual_len[i]));
     }
-    auto t0 = std::chrono::steady_clock::now();
-    hipStream_t st = e->stream;
-    const size_t sz[18] = {2ull * n, 4ull * n, 4ull * n, 4ull * n, 4ull * n, 4ull * n,
-                           4 * hb->num_cigar_ops + 4, 4ull * n, 4ull * n,
-                           hb->num_read_bases / 2 + 4, 4ull * n, 4ull * n,
-                           hb->num_qualities + 4, 4ull * n, 4ull * n, 2ull * n, 1ull * n, 4ull * n};
-    const void *src[18] = {hb->chromosome, hb->alignment_start, hb->alignment_stop, hb->mate_chromosome,
-                           hb->mate_alignment_start, hb->inferred_insert_size, hb->cigars, hb->cigar_start,
-                           hb->cigar_len, hb->reads, hb->read_start, hb->read_len, hb->qualities, hb->qual_start,
-                           hb->qual_len, hb->flags, hb->mapq, hb->read_group};
-    const size_t copy_sz[18] = {2ull * n, 4ull * n, 4ull * n, 4ull * n, 4ull * n, 4ull * n,
-                                4 * hb->num_cigar_ops, 4ull * n, 4ull * n,
-                                hb->num_read_bases / 2, 4ull * n, 4ull * n,
-                                hb->num_qualities, 4ull * n, 4ull * n, 2ull * n, 1ull * n, 4ull * n};
-    for (int i = 0; i < 18; i++) {
-        HIP_TRY(e->in[i].ensure(sz[i]));
-        if (copy_sz[i]) HIP_TRY(hipMemcpyAsync(e->in[i].p, src[i], copy_sz[i], hipMemcpyHostToDevice, st));
-    }
-    fp_batch_view dv = *hb;
-    dv.chromosome = e->in[0].as<uint16_t>();
-    dv.alignment_start = e->in[1].as<uint32_t>();
-    dv.alignment_stop = e->in[2].as<uint32_t>();
-    dv.mate_chromosome = e->in[3].as<uint32_t>();
-    dv.mate_alignment_start = e->in[4].as<uint32_t>();
-    dv.inferred_insert_size = e->in[5].as<int32_t>();
-    dv.cigars = e->in[6].as<uint32_t>();
-    dv.cigar_start = e->in[7].as<uint32_t>();
-    dv.cigar_len = e->in[8].as<uint32_t>();
-    dv.reads = e->in[9].as<uint32_t>();
-    dv.read_start = e->in[10].as<uint32_t>();
-    dv.read_len = e->in[11].as<uint32_t>();
-    dv.qualities = e->in[12].as<uint8_t>();
-    dv.qual_start = e->in[13].as<uint32_t>();
-    dv.qual_len = e->in[14].as<uint32_t>();
-    dv.flags = e->in[15].as<uint16_t>();
-    dv.mapq = e->in[16].as<uint8_t>();
-    dv.read_group = e->in[17].as<uint32_t>();
-    rc = run_batch(e, &dv, maxlen);
-    auto t1 = std::chrono::steady_clock::now();
-    e->stats.io_ms += 0;   // the copies are part of the batch time here
-    (void)t0; (void)t1;
-    return rc;
+    if (e->batched) return batched_append(e, hb, false);
+    if (n == 0) { e->stats.num_batches++; return FP_OK; }
+    return plain_host_batch(e, hb);
 }
 
 fp_status fp_engine_process_batch_device(fp_engine *e, const fp_batch_view *db)
@@ -707,6 +1086,7 @@ fp_status fp_engine_process_batch_device(fp_engine *e, const fp_batch_view *db)
     if (rc != FP_OK) return rc;
     rc = validate_view(db);
     if (rc != FP_OK) return rc;
+    if (e->batched) return batched_append(e, db, true);
     if (db->num_reads == 0) { e->stats.num_batches++; return FP_OK; }
     return run_batch(e, db, 0);
 }
@@ -715,6 +1095,7 @@ fp_status fp_engine_synchronize(fp_engine *e)
 {
     fp_status rc = check_engine(e, false);
     if (rc != FP_OK) return rc;
+    if (e->batched && e->is_setup) return batched_sync(e);
     HIP_TRY(hipStreamSynchronize(e->stream));
     return FP_OK;
 }
@@ -724,6 +1105,17 @@ fp_status fp_engine_gather(fp_engine *dst, fp_engine *src)
     fp_status rc = check_engine(dst, true);
     if (rc != FP_OK) return rc;
     if (!src || !src->is_setup) return fail(FP_ERR_STATE, "source engine is not set up");
+    if (src->batched) {
+        rc = check_engine(src, true);
+        if (rc == FP_OK) rc = batched_sync(src);
+        if (rc != FP_OK) return rc;
+        rc = check_engine(dst, true);
+        if (rc != FP_OK) return rc;
+    }
+    if (dst->batched) {
+        rc = batched_sync(dst);
+        if (rc != FP_OK) return rc;
+    }
     if (src->num_rg != dst->num_rg) return fail(FP_ERR_INVALID_ARG, "engines disagree on read groups");
     HIP_TRY(hipStreamSynchronize(src->stream));
     for (int t = 0; t < 3; t++) {
@@ -755,6 +1147,10 @@ fp_status fp_engine_postprocess(fp_engine *e)
 {
     fp_status rc = check_engine(e, true);
     if (rc != FP_OK) return rc;
+    if (e->batched) {
+        rc = batched_sync(e);
+        if (rc != FP_OK) return rc;
+    }
     HIP_TRY(hipEventRecord(e->ev[4], e->stream));
     rc = postprocess(e);
     if (rc != FP_OK) return rc;
@@ -801,6 +1197,10 @@ fp_status fp_engine_get_table(fp_engine *e, int table_id, fp_table_out *out)
         R = &e->rows[table_id];
     } else {
         if (table_id == 0) { out->count = 0; return FP_OK; }
+        if (e->batched) {
+            rc = batched_sync(e);
+            if (rc != FP_OK) return rc;
+        }
         DevBuf k, c, lo, hi;
         uint64_t nr = 0;
         rc = compact(e, table_id - 1, k, c, lo, hi, &nr);
@@ -835,7 +1235,13 @@ fp_status fp_engine_get_table(fp_engine *e, int table_id, fp_table_out *out)
 fp_status fp_engine_stats(fp_engine *e, fp_stats *out)
 {
     if (!e || !out) return fail(FP_ERR_INVALID_ARG, "null argument");
+    if (e->batched && e->is_setup) {
+        fp_status rc = check_engine(e, true);
+        if (rc == FP_OK) rc = batched_sync(e);
+        if (rc != FP_OK) return rc;
+    }
     *out = e->stats;
+    out->baq_waves_per_simd = (uint64_t)e->wpe;
     return FP_OK;
 }
 
@@ -938,10 +1344,16 @@ fp_status fp_engine_reset(fp_engine *e)
 {
     fp_status rc = check_engine(e, true);
     if (rc != FP_OK) return rc;
+    if (e->batched) {
+        rc = batched_sync(e);
+        if (rc != FP_OK) return rc;
+        if (e->staged_ready) HIP_TRY(fpd_fill_u32(e->bcounters.as<uint32_t>(), 16, 0u, e->stream));
+    }
     for (int t = 0; t < 3; t++)
         for (int k = 0; k < 4; k++) HIP_TRY(hipMemsetAsync(e->tab[t][k].p, 0, e->tab_size[t] * 8, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     memset(&e->stats, 0, sizeof(e->stats));
+    e->eager_filtered = e->eager_baq = e->eager_cells = 0;
     e->postprocessed = false;
     for (auto &r : e->rows) r = TableRows();
     return FP_OK;
@@ -957,6 +1369,20 @@ void fp_engine_destroy(fp_engine *e)
                       &e->st_bw, &e->st_hqw, &e->st_bflags, &e->st_baq, &e->st_state, &e->st_frac, &e->counters,
                       &e->baq_lists, &e->baq_counts, &e->scratch, &e->rg_buf};
     for (DevBuf *b : bufs) b->release();
+    e->scratch_gen.release();
+    e->bcounters.release();
+    for (auto &s : e->stg) {
+        for (auto &b : s.arr) b.release();
+        s.live.release();
+        if (s.live_host) (void)hipHostFree(s.live_host);
+        if (s.filled) (void)hipEventDestroy(s.filled);
+        if (s.consumed) (void)hipEventDestroy(s.consumed);
+        if (s.gfront) (void)hipGraphExecDestroy(s.gfront);
+        if (s.gback) (void)hipGraphExecDestroy(s.gback);
+    }
+    for (auto &ev : e->bev) if (ev) (void)hipEventDestroy(ev);
+    if (e->copy_stream) { (void)hipStreamSynchronize(e->copy_stream); (void)hipStreamDestroy(e->copy_stream); }
+    if (e->host_counts) (void)hipHostFree(e->host_counts);
     for (auto &b : e->in) b.release();
     for (auto &t : e->tab) for (auto &b : t) b.release();
     for (int i = 0; i < 16; i++) if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);

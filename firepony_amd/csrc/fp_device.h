@@ -43,7 +43,8 @@ static constexpr uint16_t NONE16 = 0xffff;
 static constexpr int TABLE_BITS[3] = {18, 28, 29};
 
 struct BatchDev {
-    uint32_t num_reads;
+    uint32_t num_reads;            // reads, or the capacity when `live` is set
+    const uint32_t *live;          // device {reads, qualities lo, qualities hi} of a staged batch; null: num_reads
     const uint16_t *chromosome;
     const uint32_t *alignment_start;
     const uint32_t *alignment_stop;
@@ -63,6 +64,8 @@ struct BatchDev {
     const uint8_t *mapq;
     const uint32_t *read_group;
 };
+
+__device__ inline uint32_t live_reads(const BatchDev &b) { return b.live ? b.live[0] : b.num_reads; }
 
 struct GenomeDev {
     uint32_t num_contigs;

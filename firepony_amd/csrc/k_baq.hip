@@ -19,6 +19,8 @@
  */
 #include "fp_device.h"
 
+#include <algorithm>
+
 namespace fpd {
 
 __constant__ double c_qual2prob[256];
@@ -786,7 +788,7 @@ __global__ void __launch_bounds__(256) k_baq_post(PostArgs a)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     const BatchDev &b = a.b;
-    if (r >= b.num_reads) return;
+    if (r >= live_reads(b)) return;
     if (a.s.status[r] != RS_ACTIVE) return;
     const uint32_t rw = a.s.rwc[r];
     const uint16_t rwx = lo16(rw), rwy = hi16(rw);
@@ -810,23 +812,26 @@ __global__ void __launch_bounds__(256) k_baq_post(PostArgs a)
 // BAQ read lists per band class
 struct ClassifyArgs {
     uint32_t num_reads;
+    const uint32_t *live;  // staged batch: live read count (else num_reads)
     const uint8_t *status;
     const uint16_t *num_errors;
     const uint16_t *bandwidth;
     const uint32_t *rwc;
     uint32_t *lists;       // 4 lists of capacity num_reads
-    uint32_t *counts;      // 4 counters
+    uint32_t *counts;      // 4 class counters, then the widest band of class 3
     uint32_t capacity;
 };
 
 __global__ void __launch_bounds__(256) k_baq_classify(ClassifyArgs a)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = r < a.num_reads && a.status[r] == RS_ACTIVE && a.num_errors[r] != 0;
+    const uint32_t n = a.live ? a.live[0] : a.num_reads;
+    const bool act = r < n && a.status[r] == RS_ACTIVE && a.num_errors[r] != 0;
     uint32_t cls = 0;
     if (act) {
         const uint32_t w = 2u * a.bandwidth[r] + 1;
         cls = w <= 15 ? 0 : w <= 23 ? 1 : w <= 33 ? 2 : 3;
+        if (cls == 3) atomicMax(&a.counts[4], w);
     }
     // wave-aggregated append: one counter atomic per band class present in the wavefront
     const uint32_t lane = threadIdx.x & 63;
@@ -843,10 +848,17 @@ __global__ void __launch_bounds__(256) k_baq_classify(ClassifyArgs a)
     if (act) a.lists[(uint64_t)cls * a.capacity + pos] = r;
 }
 
-__global__ void __launch_bounds__(256) k_fill_u8(uint8_t *p, uint64_t n, uint8_t v)
+__global__ void __launch_bounds__(256) k_fill_u8(uint8_t *p, uint64_t n, const uint32_t *live, uint8_t v)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    const uint64_t m = live ? ((uint64_t)live[1] | ((uint64_t)live[2] << 32)) : n;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 } // namespace fpd
@@ -860,12 +872,12 @@ extern "C" hipError_t fpd_baq_set_constants(const double *qual2prob, const doubl
     return hipMemcpyToSymbol(HIP_SYMBOL(fpd::c_gap_open), &gap_open, sizeof(double));
 }
 
-extern "C" hipError_t fpd_launch_baq_classify(uint32_t num_reads, const fpd::ReadStateDev *s, uint32_t *lists,
-                                              uint32_t *counts, uint32_t capacity, hipStream_t st)
+extern "C" hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t *live, const fpd::ReadStateDev *s,
+                                              uint32_t *lists, uint32_t *counts, uint32_t capacity, hipStream_t st)
 {
     if (num_reads == 0) return hipSuccess;
-    fpd::ClassifyArgs a;
-    a.num_reads = num_reads; a.status = s->status; a.num_errors = s->num_errors; a.bandwidth = s->bandwidth;
+    fpd::ClassifyArgs a{};
+    a.num_reads = num_reads; a.live = live; a.status = s->status; a.num_errors = s->num_errors; a.bandwidth = s->bandwidth;
     a.rwc = s->rwc; a.lists = lists; a.counts = counts; a.capacity = capacity;
     hipLaunchKernelGGL(fpd::k_baq_classify, dim3((num_reads + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError();
@@ -878,7 +890,7 @@ extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const 
                                          uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves,
                                          unsigned long long *cells, int wpe, hipStream_t st)
 {
-    fpd::BaqArgs a;
+    fpd::BaqArgs a{};
     a.b = *b; a.g = *g; a.s = *s; a.list = list; a.count = count; a.scratch = scratch;
     a.slot_doubles = slot_doubles; a.rows_cap = rows_cap; a.wmax_runtime = wmax_runtime; a.cells = cells;
     if (waves == 0) return hipSuccess;
@@ -904,15 +916,26 @@ extern "C" hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::Rea
                                           hipStream_t st)
 {
     if (b->num_reads == 0) return hipSuccess;
-    fpd::PostArgs a;
+    fpd::PostArgs a{};
     a.b = *b; a.s = *s; a.band_done = band_done;
     hipLaunchKernelGGL(fpd::k_baq_post, dim3((b->num_reads + 255) / 256), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
-extern "C" hipError_t fpd_fill_u8(uint8_t *p, uint64_t n, uint8_t v, hipStream_t st)
+// fill n bytes, or the staged batch's quality count when live is set (n is then the capacity)
+extern "C" hipError_t fpd_fill_u8(uint8_t *p, uint64_t n, const uint32_t *live, uint8_t v, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(fpd::k_fill_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(fpd::k_fill_u8, dim3((unsigned)blocks), dim3(256), 0, st, p, n, live, v);
+    return hipGetLastError();
+}
+
+// device-side fill (graph-safe replacement for memset)
+extern "C" hipError_t fpd_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(fpd::k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
     return hipGetLastError();
 }

@@ -570,11 +570,19 @@ __device__ inline uint32_t wave_append(uint32_t *counters, uint32_t key, bool ac
     return slot;
 }
 
+// graph-safe clear of a small counter array
+__global__ void __launch_bounds__(256) k_fill_u32c(uint32_t *p, uint32_t n, uint32_t v)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
 // per-read-group read lists: counts -> offsets -> scatter
-__global__ void __launch_bounds__(256) k_rg_count(uint32_t n, const uint8_t *status, const uint32_t *read_group,
-                                                  uint32_t num_rg, uint32_t *rg_count, uint32_t *errors)
+__global__ void __launch_bounds__(256) k_rg_count(uint32_t n, const uint32_t *live, const uint8_t *status,
+                                                  const uint32_t *read_group, uint32_t num_rg, uint32_t *rg_count,
+                                                  uint32_t *errors)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (live) n = live[0];
     const bool act = r < n && status[r] == RS_ACTIVE;
     const uint32_t rg = act ? read_group[r] : 0;
     const bool bad = act && rg >= num_rg;
@@ -598,11 +606,12 @@ __global__ void __launch_bounds__(256) k_rg_offsets(uint32_t num_rg, const uint3
     tile_offset[num_rg] = t;
 }
 
-__global__ void __launch_bounds__(256) k_rg_scatter(uint32_t n, const uint8_t *status, const uint32_t *read_group,
-                                                    uint32_t num_rg, const uint32_t *rg_offset, uint32_t *rg_cursor,
-                                                    uint32_t *rg_list)
+__global__ void __launch_bounds__(256) k_rg_scatter(uint32_t n, const uint32_t *live, const uint8_t *status,
+                                                    const uint32_t *read_group, uint32_t num_rg,
+                                                    const uint32_t *rg_offset, uint32_t *rg_cursor, uint32_t *rg_list)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (live) n = live[0];
     const bool act = r < n && status[r] == RS_ACTIVE;
     const uint32_t rg = act ? read_group[r] : 0;
     const bool ok = act && rg < num_rg;
@@ -614,16 +623,11 @@ __global__ void __launch_bounds__(256) k_rg_scatter(uint32_t n, const uint8_t *s
 
 // LDS layout of one covariate workgroup for reads up to max_read_len within the device's
 // per-workgroup limit: cycle codes beyond what fits go to the resident tables directly
+static int g_cov_lds_limit = -1;
+
 static void cov_layout(uint32_t max_read_len, uint32_t *ncodes, uint32_t *mcodes, size_t *bytes)
 {
-    static int limit = -1;
-    if (limit < 0) {
-        int dev = 0, v = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
-            v = 64 * 1024;
-        limit = v;
-    }
+    const int limit = g_cov_lds_limit > 0 ? g_cov_lds_limit : 64 * 1024;
     const uint32_t budget = (uint32_t)limit - fpd::STAGE_BYTES;
     uint32_t nc = 2 * max_read_len + 2, mc = 2 * max_read_len + 2;
     auto need = [&](uint32_t n, uint32_t m) {
@@ -636,6 +640,20 @@ static void cov_layout(uint32_t max_read_len, uint32_t *ncodes, uint32_t *mcodes
     *bytes = need(nc, mc) + fpd::STAGE_BYTES;
 }
 
+// once per device, before any launch (and outside stream capture): the per-workgroup LDS limit and the
+// matching dynamic-LDS attribute of the covariate kernel
+extern "C" hipError_t fpd_cov_prepare()
+{
+    int dev = 0, v = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    g_cov_lds_limit = v > 0 ? v : 64 * 1024;
+    (void)hipFuncSetAttribute((const void *)fpd::k_cov, hipFuncAttributeMaxDynamicSharedMemorySize, g_cov_lds_limit);
+    return hipSuccess;
+}
+
 extern "C" hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
                                      uint32_t frac_f64, uint32_t keep_frac, uint32_t *counters, uint32_t *rg_buf,
                                      uint32_t max_read_len, hipStream_t st)
@@ -646,20 +664,18 @@ extern "C" hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStat
     // rg_buf layout: count[nrg] | cursor[nrg] | offset[nrg+1] | tile_offset[nrg+1] | list[n]
     uint32_t *cnt = rg_buf, *cur = rg_buf + nrg, *off = rg_buf + 2 * nrg, *toff = rg_buf + 3 * nrg + 1;
     uint32_t *list = rg_buf + 4 * nrg + 2;
-    hipError_t e = hipMemsetAsync(cnt, 0, 4ull * nrg, st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(fpd::k_rg_count, dim3((n + 255) / 256), dim3(256), 0, st, n, s->status, b->read_group, nrg, cnt,
-                       counters + 4);
+    hipLaunchKernelGGL(fpd::k_fill_u32c, dim3(1), dim3(256), 0, st, cnt, nrg, 0u);
+    hipLaunchKernelGGL(fpd::k_rg_count, dim3((n + 255) / 256), dim3(256), 0, st, n, b->live, s->status, b->read_group,
+                       nrg, cnt, counters + 4);
     hipLaunchKernelGGL(fpd::k_rg_offsets, dim3(1), dim3(64), 0, st, nrg, cnt, off, toff, cur);
-    hipLaunchKernelGGL(fpd::k_rg_scatter, dim3((n + 255) / 256), dim3(256), 0, st, n, s->status, b->read_group, nrg,
-                       off, cur, list);
-    fpd::CovArgs a;
+    hipLaunchKernelGGL(fpd::k_rg_scatter, dim3((n + 255) / 256), dim3(256), 0, st, n, b->live, s->status,
+                       b->read_group, nrg, off, cur, list);
+    fpd::CovArgs a{};
     a.b = *b; a.s = *s; a.t = *t; a.frac_f64 = frac_f64; a.keep_frac = keep_frac;
     size_t lds = 0;
     cov_layout(max_read_len, &a.ncodes, &a.mcodes, &lds);
     a.rg_list = list; a.rg_offset = off; a.tile_offset = toff;
     const uint32_t blocks = (n + fpd::TILE - 1) / fpd::TILE + nrg;
-    (void)hipFuncSetAttribute((const void *)fpd::k_cov, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(fpd::k_cov, dim3(blocks), dim3(64 * fpd::COV_WAVES), lds, st, a);
     return hipGetLastError();
 }

@@ -138,7 +138,7 @@ extern "C" hipError_t fpd_launch_empirical(uint64_t n, const uint32_t *key, cons
                                            double *estimated, double *empirical, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
-    fpd::EmpArgs a;
+    fpd::EmpArgs a{};
     a.n = n; a.key = key; a.count = count; a.lo = lo; a.hi = hi; a.qual_shift = qual_shift;
     a.mism = mism; a.expected = expected; a.estimated = estimated; a.empirical = empirical;
     hipLaunchKernelGGL(fpd::k_empirical, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a);
@@ -150,7 +150,7 @@ extern "C" hipError_t fpd_launch_rg_table(uint32_t num_rg, const uint64_t *count
                                           double *estimated, double *empirical, hipStream_t st)
 {
     if (num_rg == 0) return hipSuccess;
-    fpd::RgArgs a;
+    fpd::RgArgs a{};
     a.num_rg = num_rg; a.count = count; a.p0 = parts[0]; a.p1 = parts[1]; a.p2 = parts[2];
     a.valid = valid; a.obs = obs;
     a.mism = mism; a.expected = expected; a.estimated = estimated; a.empirical = empirical;

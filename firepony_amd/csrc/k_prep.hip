@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_prep(PrepArgs a)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     const BatchDev &b = a.b;
-    if (r >= b.num_reads) return;
+    if (r >= live_reads(b)) return;
     ReadStateDev &s = a.s;
 
     const uint32_t L = b.read_len[r];
@@ -526,10 +526,33 @@ __global__ void __launch_bounds__(256) k_prep(PrepArgs a)
 extern "C" hipError_t fpd_launch_prep(const fpd::BatchDev *b, const fpd::GenomeDev *g,
                                       const fpd::ReadStateDev *s, uint32_t *counters, hipStream_t st)
 {
-    fpd::PrepArgs a;
+    fpd::PrepArgs a{};
     a.b = *b; a.g = *g; a.s = *s; a.counters = counters;
     const uint32_t n = b->num_reads;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(fpd::k_prep, dim3((n + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+namespace fpd {
+// append of a batch into a staging buffer: its start offsets become offsets into the staged arrays
+__global__ void __launch_bounds__(256) k_rebase(uint32_t n, uint32_t *cigar_start, uint32_t *read_start,
+                                               uint32_t *qual_start, uint32_t cig_off, uint32_t base_off,
+                                               uint32_t qual_off)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    cigar_start[r] += cig_off;
+    read_start[r] += base_off;
+    qual_start[r] += qual_off;
+}
+} // namespace fpd
+
+extern "C" hipError_t fpd_rebase(uint32_t n, uint32_t *cigar_start, uint32_t *read_start, uint32_t *qual_start,
+                                 uint32_t cig_off, uint32_t base_off, uint32_t qual_off, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fpd::k_rebase, dim3((n + 255) / 256), dim3(256), 0, st, n, cigar_start, read_start, qual_start,
+                       cig_off, base_off, qual_off);
     return hipGetLastError();
 }

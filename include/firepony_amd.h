@@ -47,14 +47,21 @@ typedef enum fp_status {
 
 /* engine flavours (both produce bit-identical tables) */
 typedef enum fp_engine_kind {
-    FP_ENGINE_PLAIN = 0,    /* one kernel launch per stage, stream-ordered          */
-    FP_ENGINE_BATCHED = 1   /* fused persistent stages replayed as a hipGraph       */
+    FP_ENGINE_PLAIN = 0,    /* each batch runs as it arrives: one launch per stage,
+                               with host readbacks between stages                   */
+    FP_ENGINE_BATCHED = 1   /* batches are appended to large device staging buffers
+                               (copy stream, double-buffered); a full buffer runs as
+                               two captured hipGraph replays with persistent BAQ
+                               kernels; the BAQ variant is autotuned               */
 } fp_engine_kind;
 
 typedef struct fp_options {
     uint32_t abi_version;            /* must be FP_ABI_VERSION                        */
     uint32_t engine_kind;            /* fp_engine_kind                                */
-    uint32_t max_batch_reads;        /* capacity hint; 0 = 20000 (firepony.cu:148)    */
+    uint32_t max_batch_reads;        /* batched engine: reads per staging buffer
+                                        (0 = 1,048,576); the plain engine sizes from
+                                        each batch (the reference uses 20,000,
+                                        firepony.cu:148)                              */
     uint32_t max_batch_bases;        /* capacity hint for padded bases; 0 = derived   */
     uint32_t frac_error_f64;         /* 0: per-base fractional errors narrowed to float
                                         before accumulation, as the reference does
@@ -69,7 +76,10 @@ typedef struct fp_options {
     uint32_t debug_arrays;           /* 1: also keep the per-base fractional errors for
                                         fp_engine_debug_fetch (the tables never need
                                         them); 0: leave them out                       */
-    uint32_t reserved[5];
+    uint32_t profile_stages;         /* batched engine: run the stages eagerly with an
+                                        event between kernels, so fp_stats carries
+                                        per-kernel times (no graph replay)             */
+    uint32_t reserved[4];
 } fp_options;
 
 /* how covariate observations reach the resident tables (identical results) */
@@ -199,7 +209,8 @@ typedef struct fp_stats {
     uint64_t baq_hmm_launches;
     uint64_t cov_launches;
     uint64_t baq_cells;        /* band cells evaluated by the BAQ HMM (algorithmic work) */
-    uint64_t reserved[5];
+    uint64_t baq_waves_per_simd; /* BAQ band-kernel variant in use (autotuned or set)    */
+    uint64_t reserved[4];
 } fp_stats;
 
 /* Per-batch intermediates of the last processed batch, for stage-by-stage
