@@ -54,7 +54,8 @@ class FpOptions(C.Structure):
         ("aggregation", C.c_uint32),
         ("debug_arrays", C.c_uint32),
         ("profile_stages", C.c_uint32),
-        ("reserved", C.c_uint32 * 4),
+        ("serve_rg_slots", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 

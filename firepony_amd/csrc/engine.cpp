@@ -39,7 +39,8 @@ hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *
 hipError_t fpd_fill_u8(uint8_t *p, uint64_t n, const uint32_t *live, uint8_t v, hipStream_t st);
 hipError_t fpd_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
 hipError_t fpd_rebase(uint32_t n, uint32_t *cigar_start, uint32_t *read_start, uint32_t *qual_start,
-                      uint32_t cig_off, uint32_t base_off, uint32_t qual_off, hipStream_t st);
+                      uint32_t *read_group, uint32_t cig_off, uint32_t base_off, uint32_t qual_off, uint32_t rg_off,
+                      uint32_t rg_limit, hipStream_t st);
 hipError_t fpd_cov_prepare();
 hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
                           uint32_t frac_f64, uint32_t keep_frac, uint32_t *counters, uint32_t *rg_buf,
@@ -237,6 +238,22 @@ struct fp_engine {
     hipEvent_t bev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool back_pending = false;          // stage-2 time of the last staged batch not yet collected
     uint64_t eager_filtered = 0, eager_baq = 0, eager_cells = 0;   // oversized batches run eagerly
+
+    // serving: owner of each read-group table slot (-1 = free)
+    std::vector<int64_t> slot_owner;
+    int64_t next_job_id = 1;
+};
+
+// a serving job: its read groups occupy table slots [base, base + nrg)
+struct fp_job {
+    fp_engine *e = nullptr;
+    int64_t id = 0;
+    uint32_t base = 0, nrg = 0;
+    std::vector<std::string> names;
+    bool closed = false;
+    uint64_t reads = 0, batches = 0;
+    double post_ms = 0;
+    TableRows rows[4];
 };
 
 namespace {
@@ -695,13 +712,17 @@ fp_status batched_sync(fp_engine *e)
 
 fp_status plain_host_batch(fp_engine *e, const fp_batch_view *hb);
 
-// append one batch (host or device arrays) to the current staging buffer
-fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device)
+// append one batch (host or device arrays) to the current staging buffer; a serving job's reads move
+// to its read-group slots [rg_off, rg_off + rg_limit)
+fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, uint32_t rg_off = 0,
+                         uint32_t rg_limit = 0xffffffffu)
 {
     const uint32_t k = v->num_reads;
     if (k == 0) { e->stats.num_batches += 1; return FP_OK; }
     if (k > e->cap_reads || v->num_cigar_ops > e->cap_cig || v->num_read_bases > e->cap_bases ||
         v->num_qualities > e->cap_quals) {
+        if (rg_off || rg_limit != 0xffffffffu)
+            return fail(FP_ERR_INVALID_ARG, "a job's batch must fit one staging buffer (max_batch_reads)");
         // larger than a staging buffer: runs on its own, eagerly (it may grow shared buffers, so the
         // graphs are captured again afterwards)
         fp_status rc = batched_sync(e);
@@ -741,8 +762,8 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device)
         if (bytes) HIP_TRY(hipMemcpyAsync((char *)s->arr[i].p + off, src[i], bytes, kind, cs));
     }
     HIP_TRY(fpd_rebase(k, s->arr[7].as<uint32_t>() + s->n, s->arr[10].as<uint32_t>() + s->n,
-                       s->arr[13].as<uint32_t>() + s->n, (uint32_t)s->ncig, (uint32_t)s->nbases,
-                       (uint32_t)s->nquals, cs));
+                       s->arr[13].as<uint32_t>() + s->n, s->arr[17].as<uint32_t>() + s->n, (uint32_t)s->ncig,
+                       (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off, rg_limit, cs));
     s->n += k;
     s->ncig += v->num_cigar_ops;
     s->nbases += v->num_read_bases;
@@ -798,14 +819,20 @@ fp_status plain_host_batch(fp_engine *e, const fp_batch_view *hb)
 }
 
 // compaction of one dense table into key-ordered rows on the device
-fp_status compact(fp_engine *e, int t, DevBuf &okey, DevBuf &ocount, DevBuf &olo, DevBuf &ohi, uint64_t *nrows)
+// read groups [rg_base, rg_base + nrg) of table t: rows keyed relative to rg_base
+fp_status compact(fp_engine *e, int t, uint32_t rg_base, uint32_t nrg, DevBuf &okey, DevBuf &ocount, DevBuf &olo,
+                  DevBuf &ohi, uint64_t *nrows)
 {
-    const uint64_t n = e->tab_size[t];
+    const uint64_t off0 = (uint64_t)rg_base << (fpd::TABLE_BITS[t] - 8);
+    const uint64_t n = (uint64_t)nrg << (fpd::TABLE_BITS[t] - 8);
+    const uint64_t *count = e->tables.count[t] + off0;
+    const uint64_t *parts[3] = {e->tables.part[t][0] + off0, e->tables.part[t][1] + off0,
+                                e->tables.part[t][2] + off0};
     const uint64_t nb = (n + 255) / 256;
     DevBuf bc, bo;
     HIP_TRY(bc.ensure(4 * nb + 4));
     HIP_TRY(bo.ensure(8 * nb + 8));
-    HIP_TRY(fpd_compact_count(e->tables.count[t], n, bc.as<uint32_t>(), e->stream));
+    HIP_TRY(fpd_compact_count(count, n, bc.as<uint32_t>(), e->stream));
     std::vector<uint32_t> h(nb);
     std::vector<uint64_t> off(nb);
     HIP_TRY(hipMemcpyAsync(h.data(), bc.p, 4 * nb, hipMemcpyDeviceToHost, e->stream));
@@ -817,7 +844,7 @@ fp_status compact(fp_engine *e, int t, DevBuf &okey, DevBuf &ocount, DevBuf &olo
     HIP_TRY(ocount.ensure(8 * total + 8));
     HIP_TRY(olo.ensure(8 * total + 8));
     HIP_TRY(ohi.ensure(8 * total + 8));
-    HIP_TRY(fpd_compact_scatter(e->tables.count[t], e->tables.part[t], n, bo.as<uint64_t>(),
+    HIP_TRY(fpd_compact_scatter(count, parts, n, bo.as<uint64_t>(),
                                 okey.as<uint32_t>(), ocount.as<uint64_t>(), olo.as<uint64_t>(), ohi.as<uint64_t>(),
                                 e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -827,14 +854,15 @@ fp_status compact(fp_engine *e, int t, DevBuf &okey, DevBuf &ocount, DevBuf &olo
     return FP_OK;
 }
 
-fp_status postprocess(fp_engine *e)
+// postprocess of read groups [rg_base, rg_base + nrg) into rows (keys relative to rg_base)
+fp_status postprocess_range(fp_engine *e, uint32_t rg_base, uint32_t nrg, TableRows *rows)
 {
     hipStream_t st = e->stream;
     static const int qshift[3] = {2, 12, 13};
     for (int t = 0; t < 3; t++) {
         DevBuf k, c, lo, hi, m, ex, es, em;
         uint64_t nr = 0;
-        fp_status rc = compact(e, t, k, c, lo, hi, &nr);
+        fp_status rc = compact(e, t, rg_base, nrg, k, c, lo, hi, &nr);
         if (rc != FP_OK) return rc;
         HIP_TRY(m.ensure(8 * nr + 8));
         HIP_TRY(ex.ensure(8 * nr + 8));
@@ -842,7 +870,7 @@ fp_status postprocess(fp_engine *e)
         HIP_TRY(em.ensure(8 * nr + 8));
         HIP_TRY(fpd_launch_empirical(nr, k.as<uint32_t>(), c.as<uint64_t>(), lo.as<uint64_t>(), hi.as<uint64_t>(),
                                      qshift[t], m.as<double>(), ex.as<double>(), es.as<double>(), em.as<double>(), st));
-        TableRows &R = e->rows[t + 1];
+        TableRows &R = rows[t + 1];
         R.key.resize(nr); R.obs.resize(nr); R.lo.resize(nr); R.hi.resize(nr);
         R.mism.resize(nr); R.expected.resize(nr); R.estimated.resize(nr); R.empirical.resize(nr);
         if (nr) {
@@ -860,7 +888,7 @@ fp_status postprocess(fp_engine *e)
     }
     // read group table (RecalTable0)
     {
-        const uint32_t ns = e->num_rg * 4;
+        const uint32_t ns = nrg * 4;
         DevBuf valid, obs, m, ex, es, em;
         HIP_TRY(valid.ensure(4 * ns + 4));
         HIP_TRY(obs.ensure(8 * ns + 8));
@@ -868,7 +896,9 @@ fp_status postprocess(fp_engine *e)
         HIP_TRY(ex.ensure(8 * ns + 8));
         HIP_TRY(es.ensure(8 * ns + 8));
         HIP_TRY(em.ensure(8 * ns + 8));
-        HIP_TRY(fpd_launch_rg_table(e->num_rg, e->tables.count[0], e->tables.part[0],
+        const uint64_t q0 = (uint64_t)rg_base << 10;
+        const uint64_t *qparts[3] = {e->tables.part[0][0] + q0, e->tables.part[0][1] + q0, e->tables.part[0][2] + q0};
+        HIP_TRY(fpd_launch_rg_table(nrg, e->tables.count[0] + q0, qparts,
                                     valid.as<uint32_t>(), obs.as<uint64_t>(), m.as<double>(), ex.as<double>(),
                                     es.as<double>(), em.as<double>(), st));
         std::vector<uint32_t> hv(ns);
@@ -883,7 +913,7 @@ fp_status postprocess(fp_engine *e)
             HIP_TRY(hipMemcpyAsync(he.data(), em.p, 8 * ns, hipMemcpyDeviceToHost, st));
         }
         HIP_TRY(hipStreamSynchronize(st));
-        TableRows &R = e->rows[0];
+        TableRows &R = rows[0];
         R = TableRows();
         for (uint32_t s = 0; s < ns; s++) {
             if (!hv[s]) continue;
@@ -897,6 +927,13 @@ fp_status postprocess(fp_engine *e)
             R.empirical.push_back(he[s]);
         }
     }
+    return FP_OK;
+}
+
+fp_status postprocess(fp_engine *e)
+{
+    fp_status rc = postprocess_range(e, 0, e->num_rg, e->rows);
+    if (rc != FP_OK) return rc;
     e->postprocessed = true;
     return FP_OK;
 }
@@ -994,12 +1031,21 @@ fp_status fp_engine_setup(fp_engine *e, const fp_header_view *h, const fp_refere
     fp_status rc = check_engine(e, false);
     if (rc != FP_OK) return rc;
     if (!h || !r || !v) return fail(FP_ERR_INVALID_ARG, "null view");
-    if (h->num_read_groups == 0 || h->num_read_groups > 256)
-        return fail(FP_ERR_UNSUPPORTED, "read group count must be 1..256");
     e->rg_names.clear();
-    for (uint32_t i = 0; i < h->num_read_groups; i++)
-        e->rg_names.push_back(h->read_group_names && h->read_group_names[i] ? h->read_group_names[i] : "");
-    e->num_rg = h->num_read_groups;
+    if (e->opts.serve_rg_slots) {
+        // serving: the tables hold the jobs' read-group slots; jobs bring their own names
+        if (!e->batched || e->opts.serve_rg_slots > 256)
+            return fail(FP_ERR_UNSUPPORTED, "serving needs the batched engine and 1..256 read-group slots");
+        e->num_rg = e->opts.serve_rg_slots;
+        for (uint32_t i = 0; i < e->num_rg; i++) e->rg_names.push_back("slot" + std::to_string(i));
+        e->slot_owner.assign(e->num_rg, -1);
+    } else {
+        if (h->num_read_groups == 0 || h->num_read_groups > 256)
+            return fail(FP_ERR_UNSUPPORTED, "read group count must be 1..256");
+        for (uint32_t i = 0; i < h->num_read_groups; i++)
+            e->rg_names.push_back(h->read_group_names && h->read_group_names[i] ? h->read_group_names[i] : "");
+        e->num_rg = h->num_read_groups;
+    }
 
     // reference
     HIP_TRY(e->g_word_offset.ensure(8ull * r->num_sequences + 8));
@@ -1203,7 +1249,7 @@ fp_status fp_engine_get_table(fp_engine *e, int table_id, fp_table_out *out)
         }
         DevBuf k, c, lo, hi;
         uint64_t nr = 0;
-        rc = compact(e, table_id - 1, k, c, lo, hi, &nr);
+        rc = compact(e, table_id - 1, 0, e->num_rg, k, c, lo, hi, &nr);
         if (rc != FP_OK) return rc;
         tmp.key.resize(nr); tmp.obs.resize(nr); tmp.lo.resize(nr); tmp.hi.resize(nr);
         if (nr) {
@@ -1388,6 +1434,152 @@ void fp_engine_destroy(fp_engine *e)
     for (int i = 0; i < 16; i++) if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
+}
+
+// --- serving jobs -----------------------------------------------------------
+
+fp_status fp_job_open(fp_engine *e, uint32_t num_read_groups, const char *const *read_group_names, fp_job **out)
+{
+    if (!out) return fail(FP_ERR_INVALID_ARG, "null output pointer");
+    *out = nullptr;
+    fp_status rc = check_engine(e, true);
+    if (rc != FP_OK) return rc;
+    if (e->slot_owner.empty()) return fail(FP_ERR_STATE, "not a serving engine (serve_rg_slots = 0)");
+    const uint32_t n = num_read_groups;
+    if (n == 0 || n > e->num_rg) return fail(FP_ERR_INVALID_ARG, "a job needs 1..serve_rg_slots read groups");
+    // first fit of n contiguous free slots: the job's keys form one range per table
+    uint32_t run = 0, base = 0;
+    bool found = false;
+    for (uint32_t i = 0; i < e->num_rg; i++) {
+        if (e->slot_owner[i] >= 0) { run = 0; continue; }
+        if (run == 0) base = i;
+        if (++run == n) { found = true; break; }
+    }
+    if (!found) return fail(FP_ERR_BUDGET, "no room for the job's tables now (read-group slots in use)");
+    fp_job *j = new fp_job();
+    j->e = e;
+    j->id = e->next_job_id++;
+    j->base = base;
+    j->nrg = n;
+    for (uint32_t i = 0; i < n; i++)
+        j->names.push_back(read_group_names && read_group_names[i] ? read_group_names[i] : "");
+    for (uint32_t i = 0; i < n; i++) e->slot_owner[base + i] = j->id;
+    *out = j;
+    return FP_OK;
+}
+
+static fp_status job_submit(fp_job *j, const fp_batch_view *b, bool on_device)
+{
+    if (!j || !j->e) return fail(FP_ERR_INVALID_ARG, "null job");
+    if (j->closed) return fail(FP_ERR_STATE, "the job is closed");
+    fp_engine *e = j->e;
+    fp_status rc = check_engine(e, true);
+    if (rc != FP_OK) return rc;
+    rc = validate_view(b);
+    if (rc != FP_OK) return rc;
+    if (!on_device) {
+        for (uint32_t i = 0; i < b->num_reads; i++) {
+            if ((uint64_t)b->cigar_start[i] + b->cigar_len[i] > b->num_cigar_ops ||
+                (uint64_t)b->read_start[i] + ((b->read_len[i] + 7) & ~7u) > b->num_read_bases ||
+                (uint64_t)b->qual_start[i] + b->qual_len[i] > b->num_qualities || (b->read_start[i] & 7))
+                return fail(FP_ERR_INVALID_ARG, "read " + std::to_string(i) + " lies outside the batch arrays");
+            if (b->read_group[i] >= j->nrg)
+                return fail(FP_ERR_INVALID_ARG, "read group id beyond the job's read groups");
+        }
+    }
+    rc = batched_append(e, b, on_device, j->base, j->nrg);
+    if (rc != FP_OK) return rc;
+    j->reads += b->num_reads;
+    j->batches += 1;
+    return FP_OK;
+}
+
+fp_status fp_job_submit(fp_job *j, const fp_batch_view *b) { return job_submit(j, b, false); }
+fp_status fp_job_submit_device(fp_job *j, const fp_batch_view *b) { return job_submit(j, b, true); }
+
+fp_status fp_job_close(fp_job *j)
+{
+    if (!j || !j->e) return fail(FP_ERR_INVALID_ARG, "null job");
+    if (j->closed) return FP_OK;
+    fp_engine *e = j->e;
+    fp_status rc = check_engine(e, true);
+    if (rc != FP_OK) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    rc = batched_sync(e);   // runs whatever is staged, this job's reads included
+    if (rc != FP_OK) return rc;
+    rc = postprocess_range(e, j->base, j->nrg, j->rows);
+    if (rc != FP_OK) return rc;
+    j->post_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    j->closed = true;
+    return FP_OK;
+}
+
+fp_status fp_job_write_report(fp_job *j, const char *path)
+{
+    if (!j || !j->e || !path) return fail(FP_ERR_INVALID_ARG, "null argument");
+    if (!j->closed) return fail(FP_ERR_STATE, "close the job first");
+    fpr::ReportTables rt;
+    for (int t = 0; t < 4; t++) {
+        const TableRows &R = j->rows[t];
+        fpr::ReportRows &o = rt.table[t];
+        o.key = R.key; o.obs = R.obs; o.mism = R.mism; o.estimated = R.estimated; o.empirical = R.empirical;
+    }
+    std::string text = fpr::format_report(rt, j->names, j->e->opts.disable_output_rounding != 0);
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail(FP_ERR_IO, std::string("cannot open ") + path);
+    const size_t w = fwrite(text.data(), 1, text.size(), fp);
+    fclose(fp);
+    if (w != text.size()) return fail(FP_ERR_IO, "short write");
+    return FP_OK;
+}
+
+fp_status fp_job_get_table(fp_job *j, int table_id, fp_table_out *out)
+{
+    if (!j || !j->e || !out || table_id < 0 || table_id > 3) return fail(FP_ERR_INVALID_ARG, "bad table request");
+    if (!j->closed) return fail(FP_ERR_STATE, "close the job first");
+    const TableRows &R = j->rows[table_id];
+    const size_t n = R.size();
+    out->count = n;
+    for (size_t i = 0; i < n; i++) {
+        if (out->keys) out->keys[i] = R.key[i];
+        if (out->observations) out->observations[i] = R.obs[i];
+        if (out->mismatches) out->mismatches[i] = R.mism[i];
+        if (out->mismatches_fixed_lo) out->mismatches_fixed_lo[i] = R.lo[i];
+        if (out->mismatches_fixed_hi) out->mismatches_fixed_hi[i] = R.hi[i];
+        if (out->expected_errors) out->expected_errors[i] = R.expected[i];
+        if (out->estimated_quality) out->estimated_quality[i] = R.estimated[i];
+        if (out->empirical_quality) out->empirical_quality[i] = R.empirical[i];
+    }
+    return FP_OK;
+}
+
+fp_status fp_job_stats(fp_job *j, fp_stats *out)
+{
+    if (!j || !out) return fail(FP_ERR_INVALID_ARG, "null argument");
+    memset(out, 0, sizeof(*out));
+    out->total_reads = j->reads;
+    out->num_batches = j->batches;
+    out->postprocessing_ms = j->post_ms;
+    return FP_OK;
+}
+
+void fp_job_release(fp_job *j)
+{
+    if (!j) return;
+    fp_engine *e = j->e;
+    if (e && check_engine(e, true) == FP_OK) {
+        if (!j->closed) (void)batched_sync(e);   // nothing of the job may still be in flight
+        for (int t = 0; t < 3; t++) {
+            const uint64_t off0 = (uint64_t)j->base << (fpd::TABLE_BITS[t] - 8);
+            const uint64_t n = (uint64_t)j->nrg << (fpd::TABLE_BITS[t] - 8);
+            for (int k = 0; k < 4; k++)
+                (void)hipMemsetAsync(e->tab[t][k].as<uint64_t>() + off0, 0, 8 * n, e->stream);
+        }
+        (void)hipStreamSynchronize(e->stream);
+        for (uint32_t i = 0; i < j->nrg; i++)
+            if (e->slot_owner[j->base + i] == j->id) e->slot_owner[j->base + i] = -1;
+    }
+    delete j;
 }
 
 // host-side self checks usable without a GPU ------------------------------

@@ -535,24 +535,32 @@ extern "C" hipError_t fpd_launch_prep(const fpd::BatchDev *b, const fpd::GenomeD
 }
 
 namespace fpd {
-// append of a batch into a staging buffer: its start offsets become offsets into the staged arrays
+// append of a batch into a staging buffer: its start offsets become offsets into the staged arrays,
+// and its read groups move to the owning job's table slots (ids at or beyond rg_limit become invalid,
+// which the read-group check reports)
 __global__ void __launch_bounds__(256) k_rebase(uint32_t n, uint32_t *cigar_start, uint32_t *read_start,
-                                               uint32_t *qual_start, uint32_t cig_off, uint32_t base_off,
-                                               uint32_t qual_off)
+                                               uint32_t *qual_start, uint32_t *read_group, uint32_t cig_off,
+                                               uint32_t base_off, uint32_t qual_off, uint32_t rg_off,
+                                               uint32_t rg_limit)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     cigar_start[r] += cig_off;
     read_start[r] += base_off;
     qual_start[r] += qual_off;
+    if (rg_off || rg_limit != 0xffffffffu) {
+        const uint32_t g = read_group[r];
+        read_group[r] = g < rg_limit ? g + rg_off : 0xffffffffu;
+    }
 }
 } // namespace fpd
 
 extern "C" hipError_t fpd_rebase(uint32_t n, uint32_t *cigar_start, uint32_t *read_start, uint32_t *qual_start,
-                                 uint32_t cig_off, uint32_t base_off, uint32_t qual_off, hipStream_t st)
+                                 uint32_t *read_group, uint32_t cig_off, uint32_t base_off, uint32_t qual_off,
+                                 uint32_t rg_off, uint32_t rg_limit, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(fpd::k_rebase, dim3((n + 255) / 256), dim3(256), 0, st, n, cigar_start, read_start, qual_start,
-                       cig_off, base_off, qual_off);
+                       read_group, cig_off, base_off, qual_off, rg_off, rg_limit);
     return hipGetLastError();
 }

@@ -22,7 +22,8 @@ EXPORTED = [
     "fp_engine_postprocess", "fp_engine_write_report", "fp_engine_get_table", "fp_engine_stats",
     "fp_engine_get_name", "fp_engine_get_total_memory", "fp_engine_debug_size",
     "fp_engine_debug_fetch", "fp_engine_reset", "fp_engine_destroy", "fp_last_error",
-    "fp_status_string",
+    "fp_status_string", "fp_job_open", "fp_job_submit", "fp_job_submit_device", "fp_job_close",
+    "fp_job_write_report", "fp_job_get_table", "fp_job_stats", "fp_job_release",
 ]
 
 
@@ -62,6 +63,15 @@ def lib():
         L.fp_last_error.restype = C.c_char_p
         L.fp_status_string.restype = C.c_char_p
         L.fp_status_string.argtypes = [C.c_int]
+        L.fp_job_open.argtypes = [vp, C.c_uint32, C.POINTER(C.c_char_p), C.POINTER(vp)]
+        L.fp_job_submit.argtypes = [vp, C.POINTER(abi.FpBatchView)]
+        L.fp_job_submit_device.argtypes = [vp, C.POINTER(abi.FpBatchView)]
+        L.fp_job_close.argtypes = [vp]
+        L.fp_job_write_report.argtypes = [vp, C.c_char_p]
+        L.fp_job_get_table.argtypes = [vp, C.c_int, C.POINTER(abi.FpTableOut)]
+        L.fp_job_stats.argtypes = [vp, C.POINTER(abi.FpStats)]
+        L.fp_job_release.argtypes = [vp]
+        L.fp_job_release.restype = None
         for name in EXPORTED:
             f = getattr(L, name)
             if f.restype is C.c_int and name not in ("fp_options_init",):
@@ -142,6 +152,17 @@ class Engine:
         with open(path) as f:
             return f.read()
 
+    def open_job(self, read_group_names):
+        """Serving engines: admit a job with its own read groups.  Returns None when the
+        job's tables do not fit the read-group slots now (retry after a release)."""
+        names = (C.c_char_p * len(read_group_names))(*[n.encode() for n in read_group_names])
+        h = C.c_void_p()
+        rc = lib().fp_job_open(self._h, len(read_group_names), names, C.byref(h))
+        if rc == abi.FP_ERR_BUDGET:
+            return None
+        _check(rc)
+        return Job(self, h)
+
     def stats(self):
         s = abi.FpStats()
         _check(lib().fp_engine_stats(self._h, C.byref(s)))
@@ -167,3 +188,61 @@ class Engine:
 
     def __del__(self):
         self.close()
+
+
+def _read_table(getter, table_id):
+    out = abi.FpTableOut()
+    _check(getter(table_id, C.byref(out)))
+    n = int(out.count)
+    t = {
+        "keys": np.zeros(n, np.uint32),
+        "observations": np.zeros(n, np.uint64),
+        "mismatches": np.zeros(n, np.float64),
+        "mismatches_fixed_lo": np.zeros(n, np.uint64),
+        "mismatches_fixed_hi": np.zeros(n, np.uint64),
+        "expected_errors": np.zeros(n, np.float64),
+        "estimated_quality": np.zeros(n, np.float64),
+        "empirical_quality": np.zeros(n, np.float64),
+    }
+    out = abi.FpTableOut()
+    out.count = n
+    for k, v in t.items():
+        ctype = {np.uint32: abi.u32p, np.uint64: abi.u64p, np.float64: abi.f64p}[v.dtype.type]
+        setattr(out, k, v.ctypes.data_as(ctype))
+    _check(getter(table_id, C.byref(out)))
+    return t
+
+
+class Job:
+    """A recalibration job on a serving engine (its tables live in the engine's read-group slots)."""
+
+    def __init__(self, engine, handle):
+        self.engine = engine
+        self._h = handle
+
+    def submit(self, batch):
+        _check(lib().fp_job_submit(self._h, C.byref(batch.view)))
+
+    def submit_device(self, view):
+        _check(lib().fp_job_submit_device(self._h, C.byref(view)))
+
+    def close(self):
+        _check(lib().fp_job_close(self._h))
+
+    def table(self, table_id):
+        return _read_table(lambda t, o: lib().fp_job_get_table(self._h, t, o), table_id)
+
+    def report(self, path):
+        _check(lib().fp_job_write_report(self._h, path.encode()))
+        with open(path) as f:
+            return f.read()
+
+    def stats(self):
+        s = abi.FpStats()
+        _check(lib().fp_job_stats(self._h, C.byref(s)))
+        return s
+
+    def release(self):
+        if self._h:
+            lib().fp_job_release(self._h)
+            self._h = None

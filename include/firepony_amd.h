@@ -79,7 +79,10 @@ typedef struct fp_options {
     uint32_t profile_stages;         /* batched engine: run the stages eagerly with an
                                         event between kernels, so fp_stats carries
                                         per-kernel times (no graph replay)             */
-    uint32_t reserved[4];
+    uint32_t serve_rg_slots;         /* serving: read-group table slots shared by the
+                                        open jobs (HBM budget = slots x ~100 MB of
+                                        resident tables); 0 = not a serving engine    */
+    uint32_t reserved[3];
 } fp_options;
 
 /* how covariate observations reach the resident tables (identical results) */
@@ -269,6 +272,32 @@ fp_status fp_engine_debug_fetch(fp_engine *e, int item, void *dst, uint64_t byte
 /* clears the accumulated tables and statistics; keeps setup state */
 fp_status fp_engine_reset(fp_engine *e);
 void fp_engine_destroy(fp_engine *e);
+
+/* --- serving: recalibration jobs sharing one batched engine --------------
+ * No counterpart in the reference, which runs one job per process
+ * (firepony.cu:225-423).  A serving engine (engine_kind FP_ENGINE_BATCHED,
+ * serve_rg_slots > 0) keeps each open job's covariate tables resident in its
+ * own read-group slots.  Reads of all open jobs share the staging buffers and
+ * graph replays; a job's tables are exact and its own, so its result is the
+ * same alone or in any mix.  fp_engine_setup's header gives the contigs only. */
+typedef struct fp_job fp_job;
+
+/* admits a job with its read groups (names as in fp_header_view) if its tables
+ * fit the slot budget now; FP_ERR_BUDGET otherwise (retry after a release) */
+fp_status fp_job_open(fp_engine *e, uint32_t num_read_groups, const char *const *read_group_names,
+                      fp_job **out);
+/* one batch of the job (read_group ids local to the job); host or device arrays */
+fp_status fp_job_submit(fp_job *j, const fp_batch_view *host_batch);
+fp_status fp_job_submit_device(fp_job *j, const fp_batch_view *device_batch);
+/* all of the job's reads are in: runs what is staged, then postprocesses the
+ * job's tables (firepony_postprocess for this job only) */
+fp_status fp_job_close(fp_job *j);
+fp_status fp_job_write_report(fp_job *j, const char *path);
+fp_status fp_job_get_table(fp_job *j, int table_id, fp_table_out *out);
+/* reads submitted and postprocess time of the job */
+fp_status fp_job_stats(fp_job *j, fp_stats *out);
+/* clears the job's tables and returns its slots to the budget */
+void fp_job_release(fp_job *j);
 
 const char *fp_last_error(void);
 const char *fp_status_string(fp_status s);
