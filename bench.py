@@ -18,7 +18,10 @@ Passes over the same input batches:
             per-kernel times for the roofline object.
 A bounded sample of the mixed-length workload (reads of 50-250 bp, 8 read
 groups, 5 % indels of 1-10 bp, clips on every read) is timed plain and
-batched as well.  The roofline's `traffic` comes from the committed
+batched as well, and the serving workload (a Poisson stream of jobs of
+50k-5M reads, each with its own read groups, submitted in 64k-read batches
+to one serving engine in real time) reports throughput and p50/p99 job
+latency under `served`.  The roofline's `traffic` comes from the committed
 rocprofv3 FETCH_SIZE/WRITE_SIZE summary.  The CPU oracle (a restatement of the
 reference's host path) is timed on a bounded sample of the same reads.
 
@@ -27,6 +30,7 @@ reference's host path) is timed on a bounded sample of the same reads.
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -175,6 +179,84 @@ def timed_pass(genome, dev, warmup, torch, **opts):
     return dt, st
 
 
+def served(args, wpe, torch):
+    """The serving workload: a Poisson stream (mean 20 jobs/s) of jobs of 50k-5M reads (log-uniform),
+    each with 1-4 read groups and 100 or 150 bp reads, submitted in batches of 64k reads to one serving
+    engine in real time.  Jobs are admitted by the read-group slot budget; a job's latency runs from its
+    arrival to its postprocessed tables."""
+    import random
+    gs, rs0, _ = synth.named_spec(3)
+    genome = synth.Genome(gs, 1)
+    rnd = random.Random(0xF1E90003)
+    jobs, t = [], 0.0
+    for k in range(args.served_jobs):
+        t += rnd.expovariate(20.0)
+        n = int(math.exp(rnd.uniform(math.log(50000), math.log(5000000))))
+        nrg = rnd.randint(1, 4)
+        L = rnd.choice((100, 150))
+        rs = abi.FpsynReadSpec.from_buffer_copy(rs0)
+        rs.seed = 0xF1E90003 * 1000 + k
+        rs.num_lengths, rs.lengths[0] = 1, L
+        rs.num_read_groups = nrg
+        jobs.append({"arrival": t, "reads": n, "nrg": nrg, "rs": rs})
+    total = sum(j["reads"] for j in jobs)
+    log(f"served: generating {len(jobs)} jobs, {total} reads in batches of 65536")
+    t0 = time.perf_counter()
+    for j in jobs:
+        j["batches"] = upload(genome, j["rs"], j["reads"], 65536, torch)
+    log(f"generated + uploaded in {time.perf_counter() - t0:.1f} s")
+
+    e = Engine(genome, device=0, debug_arrays=0, engine_kind=abi.FP_ENGINE_BATCHED, max_read_length=150,
+               serve_rg_slots=64, baq_waves_per_simd=wpe)
+    for k in range(2):   # warm-up: capture each staging buffer's graphs
+        w = e.open_job(["warm"] * jobs[k]["nrg"])
+        w.submit_device(jobs[k]["batches"][0].view)
+        w.close()
+        w.release()
+    torch.cuda.synchronize()
+
+    pending = list(range(len(jobs)))
+    waiting, active = [], []
+    done = {}
+    t0 = time.perf_counter()
+    while pending or waiting or active:
+        now = time.perf_counter() - t0
+        while pending and jobs[pending[0]]["arrival"] <= now:
+            waiting.append(pending.pop(0))
+        while waiting:
+            j = jobs[waiting[0]]
+            h = e.open_job([f"j{waiting[0]}_rg{i}" for i in range(j["nrg"])])
+            if h is None:
+                break
+            j["job"], j["next"] = h, 0
+            active.append(waiting.pop(0))
+        if not active:
+            if pending:
+                time.sleep(max(0.0, jobs[pending[0]]["arrival"] - (time.perf_counter() - t0)))
+            continue
+        # continuous batching: one batch of every active job per round
+        for k in list(active):
+            j = jobs[k]
+            j["job"].submit_device(j["batches"][j["next"]].view)
+            j["next"] += 1
+            if j["next"] == len(j["batches"]):
+                j["job"].close()
+                done[k] = time.perf_counter() - t0
+                j["job"].release()
+                active.remove(k)
+    e.close()
+    lat = sorted(done[k] - jobs[k]["arrival"] for k in done)
+    makespan = max(done.values()) - jobs[0]["arrival"]
+    pct = lambda q: lat[min(len(lat) - 1, int(math.ceil(q * len(lat))) - 1)]
+    res = {"jobs": len(jobs), "reads": total, "value": total / makespan, "unit": "reads/s",
+           "p50_latency_s": pct(0.50), "p99_latency_s": pct(0.99), "makespan_s": makespan,
+           "arrival_rate_jobs_per_s": 20, "batch_reads": 65536, "read_group_slots": 64}
+    for j in jobs:
+        del j["batches"]
+    torch.cuda.empty_cache()
+    return res
+
+
 def kernel_ms(st):
     return {"prep": st.prep_kernel_ms, "baq_hmm": st.baq_hmm_kernel_ms, "cov": st.cov_kernel_ms,
             "postprocess": st.postprocessing_ms}
@@ -187,8 +269,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed passes over the input before each timed pass")
     ap.add_argument("--batch-reads", type=int, default=20000, help="reads per input batch")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the per-kernel pass of the batched engine (eager stages, no graphs): the run "
+                         "the committed rocprofv3 summaries profile")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-reads", type=int, default=2000000)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-jobs", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--wpe", type=int, default=0, help="BAQ waves per SIMD (0 = autotuned)")
@@ -224,6 +311,14 @@ def main():
     reads = sum(d.num_reads for d in dev)
     bases = sum(d.num_bases for d in dev)
     batched_opts = dict(engine_kind=abi.FP_ENGINE_BATCHED, max_read_length=101)
+    if args.profile_only:
+        pdt, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=args.wpe or 1, profile_stages=1,
+                              **batched_opts)
+        log(f"profile: {reads} reads in {pdt:.3f} s; prep {pst.prep_kernel_ms:.1f} ms, baq_hmm "
+            f"{pst.baq_hmm_kernel_ms:.1f} ms ({pst.baq_hmm_launches} launches), cov {pst.cov_kernel_ms:.1f} ms")
+        print(json.dumps({"profile_only": True, "reads": reads, "seconds": pdt, "kernel_ms": kernel_ms(pst)}),
+              flush=True)
+        return
 
     out_plain = None
     if not args.no_plain:
@@ -285,6 +380,11 @@ def main():
         log(f"mixed lengths: plain {n2 / pdt2:.4g} reads/s, batched {n2 / bdt2:.4g} reads/s")
         del dev2
         torch.cuda.empty_cache()
+
+    if not args.no_served:
+        out["served"] = served(args, wpe, torch)
+        log(f"served: {out['served']['value']:.4g} reads/s, p50 {out['served']['p50_latency_s']:.3f} s, "
+            f"p99 {out['served']['p99_latency_s']:.3f} s")
 
     if not args.no_cpu_baseline:
         log("timing the CPU oracle baseline")
