@@ -55,7 +55,8 @@ class FpOptions(C.Structure):
         ("debug_arrays", C.c_uint32),
         ("profile_stages", C.c_uint32),
         ("serve_rg_slots", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("disable_baq_staging", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
     ]
 
 

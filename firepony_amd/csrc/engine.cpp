@@ -33,7 +33,8 @@ hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t *live, con
 hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const fpd::GenomeDev *g, const fpd::ReadStateDev *s,
                               const uint32_t *list, const uint32_t *count, double *scratch, uint64_t slot_doubles,
                               uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves, unsigned long long *cells,
-                              int wpe, hipStream_t st);
+                              int wpe, int allow_stage, hipStream_t st);
+hipError_t fpd_baq_prepare();
 hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *s, uint32_t band_done,
                                hipStream_t st);
 hipError_t fpd_fill_u8(uint8_t *p, uint64_t n, const uint32_t *live, uint8_t v, hipStream_t st);
@@ -397,7 +398,8 @@ fp_status launch_back(fp_engine *e, const fpd::BatchDev &b, const Plan &p, int w
         HIP_TRY(fpd_launch_baq_hmm(c, &b, &e->genome, &e->rs, e->baq_lists.as<uint32_t>() + (uint64_t)c * cap,
                                    e->baq_counts.as<uint32_t>() + c, e->scratch.as<double>(),
                                    band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c],
-                                   band_waves(wpe, c, p.n), (unsigned long long *)(counters + 8), wpe, st));
+                                   band_waves(wpe, c, p.n), (unsigned long long *)(counters + 8), wpe,
+                                   e->opts.disable_baq_staging ? 0 : 1, st));
     }
     if (marks) HIP_TRY(hipEventRecord(marks[0], st));
     HIP_TRY(fpd_launch_baq_post(&b, &e->rs, 1u, st));
@@ -418,7 +420,7 @@ fp_status launch_generic(fp_engine *e, const fpd::BatchDev &b, const Plan &p, ui
     HIP_TRY(fpd_launch_baq_hmm(3, &b, &e->genome, &e->rs,
                                e->baq_lists.as<uint32_t>() + 3ull * std::max<uint32_t>(p.n, 1),
                                e->baq_counts.as<uint32_t>() + 3, e->scratch_gen.as<double>(), slot, p.rows_cap, wgen,
-                               waves, (unsigned long long *)(counters + 8), e->wpe, st));
+                               waves, (unsigned long long *)(counters + 8), e->wpe, 0, st));
     return FP_OK;
 }
 
@@ -1004,6 +1006,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     for (int i = 0; i < 16; i++) (void)hipEventCreate(&e->ev[i]);
     he = hipHostMalloc((void **)&e->host_counts, 64 * sizeof(uint32_t), hipHostMallocDefault);
     if (he == hipSuccess) he = fpd_cov_prepare();
+    if (he == hipSuccess) he = fpd_baq_prepare();
     if (he != hipSuccess) { fp_engine_destroy(e); return fail(FP_ERR_HIP, hipGetErrorString(he)); }
     if (o.engine_kind == FP_ENGINE_BATCHED) {
         e->batched = true;

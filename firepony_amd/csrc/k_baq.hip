@@ -406,7 +406,12 @@ __device__ inline uint8_t cap_tag(uint32_t kind, uint32_t expectedPos, uint8_t o
 // in the backward pass, the forward row that the MAP step consumes are all
 // loaded one row ahead so their latency overlaps the current row's FP64 work.
 
-template <int W, int WPE>
+// STAGE: each lane first copies its read's query bases and qualities (one u16 per window position)
+// and its reference window (one byte per cell) into LDS, laid out [index][lane]; the row loops then
+// read nothing from global memory but the spilled forward rows.  Per-lane global reads of a few
+// bytes per row are otherwise fetched again from HBM on every row, because the forward-row stream
+// evicts them from the caches in between.
+template <int W, int WPE, bool STAGE>
 __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
 {
     constexpr int NWIN = (W + 15) / 16;       // u64 words of band reference nibbles
@@ -430,6 +435,10 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
     uint64_t *Ost = (uint64_t *)(Sst + (uint64_t)(a.rows_cap + 2) * 64);  // [row][lane] MAP state | quality << 32
     const BatchDev &b = a.b;
     uint64_t ncell = 0;
+    // per-wave staging area (STAGE): rows_cap query positions, then rows_cap + 2W reference cells
+    extern __shared__ unsigned char baq_stage[];
+    uint16_t *rowd = (uint16_t *)(baq_stage + (threadIdx.x >> 6) * (a.rows_cap * 128 + (a.rows_cap + 2 * W) * 64));
+    uint8_t *refd = (uint8_t *)(rowd + (uint64_t)a.rows_cap * 64);
 
     auto qprob = [&](uint8_t q) -> double { return s_q2p[q < 4 ? 4 : q]; };
     // Compiler fence: makes a value materialise here (emits no instruction).  The row loops decode
@@ -512,6 +521,17 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         uint64_t win[NWIN];
         auto rb_of = [&](int r) -> uint8_t { return (uint8_t)((win[r >> 4] >> (4 * (r & 15))) & 15); };
         auto win_fill = [&](int idx0) {
+            if constexpr (STAGE) {
+#pragma unroll
+                for (int w = 0; w < NWIN; w++) {
+                    uint64_t v = 0;
+#pragma unroll
+                    for (int j = 0; j < 16; j++)
+                        if (w * 16 + j < W) v |= (uint64_t)refd[(idx0 + w * 16 + j + W) * 64 + lane] << (4 * j);
+                    win[w] = v;
+                }
+                return;
+            }
             const int64_t p0 = hoff + idx0;
             const int64_t w0 = p0 >> 3;               // floor
             const int sh = (int)(p0 & 7) * 4;
@@ -561,12 +581,32 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             Ost[(uint64_t)((int)p - qStart + 1) * 64 + lane] = (uint64_t)ep | ((uint64_t)kind << 32);
         });
 
+        if constexpr (STAGE) {
+            // this read's inputs into the wave's LDS area (each lane writes and later reads only its own column)
+#pragma unroll 4
+            for (int q = 0; q < qLen; q++) {
+                const uint32_t w = b.reads[(qbase + q) >> 3];
+                const uint32_t base = (w >> (((qbase + q) & 7) * 4)) & 15;
+                rowd[q * 64 + lane] = (uint16_t)(b.quals[qqual + q] | (base << 8));
+            }
+#pragma unroll 4
+            for (int k = -bw; k <= qLen + W - 1 - bw; k++) refd[(k + W) * 64 + lane] = nibble(ref_word(hoff + k), k);
+        }
+
         double M[W], I[W], D[W];
 
         // ---- forward row 1 (baq.cu:396-442)
         {
-            const uint8_t qb = read_base(b.reads, qbase);
-            const double p = qprob(b.quals[qqual]);
+            uint8_t qb;
+            double p;
+            if constexpr (STAGE) {
+                const uint32_t v = rowd[lane];
+                qb = (uint8_t)(v >> 8);
+                p = qprob((uint8_t)(v & 255));
+            } else {
+                qb = read_base(b.reads, qbase);
+                p = qprob(b.quals[qqual]);
+            }
             const double em_eq = 1 - p, em_ne = p * EM;
             const int khi = refLen < bw + 1 ? refLen : bw + 1;
             ncell += khi >= 1 ? (uint64_t)khi : 0;
@@ -597,15 +637,31 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         // retire in order, so a decode placed behind the stores would wait for them.
         const int qlast = qLen - 1;
         const int q1 = 1 < qlast ? 1 : qlast;
-        uint32_t cqb = (read_word(b.reads, qbase + q1) >> (((qbase + q1) & 7) * 4)) & 15;
-        double cp = qprob(b.quals[qqual + q1]);
-        uint32_t crb = nibble(ref_word(hoff + (2 - bw - 1 + (W - 1))), 2 - bw - 1 + (W - 1));
+        uint32_t cqb, crb;
+        double cp;
+        if constexpr (STAGE) {
+            const uint32_t v = rowd[q1 * 64 + lane];
+            cqb = v >> 8;
+            cp = qprob((uint8_t)(v & 255));
+            crb = refd[(2 - bw - 1 + (W - 1) + W) * 64 + lane];
+        } else {
+            cqb = (read_word(b.reads, qbase + q1) >> (((qbase + q1) & 7) * 4)) & 15;
+            cp = qprob(b.quals[qqual + q1]);
+            crb = nibble(ref_word(hoff + (2 - bw - 1 + (W - 1))), 2 - bw - 1 + (W - 1));
+        }
         for (int i = 2; i <= qLen; i++) {
             const int in_ = i < qlast ? i : qlast;
-            const uint32_t nqw = read_word(b.reads, qbase + in_);
-            const uint8_t nqq = b.quals[qqual + in_];
             const int nidx = i + 1 - bw - 1 + (W - 1);
-            const uint32_t nrw = ref_word(hoff + nidx);
+            uint32_t nqw = 0, nrw = 0, nv = 0;
+            uint8_t nqq = 0;
+            if constexpr (STAGE) {
+                nv = rowd[in_ * 64 + lane];
+                nrw = refd[(nidx + W) * 64 + lane];
+            } else {
+                nqw = read_word(b.reads, qbase + in_);
+                nqq = b.quals[qqual + in_];
+                nrw = ref_word(hoff + nidx);
+            }
             const uint8_t qb = (uint8_t)cqb;
             win_shr((uint8_t)crb);
             const double em_eq = 1 - cp, em_ne = cp * EM;
@@ -629,9 +685,15 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
                 M[r] = mv; I[r] = iv; D[r] = dv;
                 cM = mv; cD = dv;
             }
-            cqb = (nqw >> (((qbase + in_) & 7) * 4)) & 15;
-            cp = qprob(nqq);
-            crb = nibble(nrw, nidx);
+            if constexpr (STAGE) {
+                cqb = nv >> 8;
+                cp = qprob((uint8_t)(nv & 255));
+                crb = nrw;
+            } else {
+                cqb = (nqw >> (((qbase + in_) & 7) * 4)) & 15;
+                cp = qprob(nqq);
+                crb = nibble(nrw, nidx);
+            }
             pin(cqb); pin(cp); pin(crb);
             Sst[(uint64_t)i * 64 + lane] = sum;
             const double inv = 1. / sum;
@@ -694,7 +756,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         };
         load_f(qLen);
         // backward row i emits query index i (the row below it); map_row(i) needs quality index i - 1
-        const uint8_t lastq = b.quals[qqual + qlast];
+        const uint8_t lastq = STAGE ? (uint8_t)(rowd[qlast * 64 + lane] & 255) : b.quals[qqual + qlast];
         {
             const double sq = Sst[(uint64_t)qLen * 64 + lane];
             const int klo = 1 > qLen - bw ? 1 : qLen - bw;
@@ -711,16 +773,29 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         if (qLen >= 2) load_f(qLen - 1);
         win_fill(qLen - bw);                     // backward cell r of row i: window index i - bw + r
         // inputs of backward row qLen - 1, then loaded a row ahead and decoded as in the forward pass
-        cqb = (read_word(b.reads, qbase + qlast) >> (((qbase + qlast) & 7) * 4)) & 15;
+        if constexpr (STAGE) {
+            cqb = rowd[qlast * 64 + lane] >> 8;
+            crb = refd[(qLen - 1 - bw + W) * 64 + lane];
+        } else {
+            cqb = (read_word(b.reads, qbase + qlast) >> (((qbase + qlast) & 7) * 4)) & 15;
+            crb = nibble(ref_word(hoff + (qLen - 1 - bw)), qLen - 1 - bw);
+        }
         cp = qprob(lastq);
-        crb = nibble(ref_word(hoff + (qLen - 1 - bw)), qLen - 1 - bw);
         double cyy = 1.0 / Sst[(uint64_t)(qLen >= 2 ? qLen - 1 : qLen) * 64 + lane];   // 1 / s[i]
         for (int i = qLen - 1; i >= 1; i--) {
             const int in_ = i - 1;
-            const uint32_t nqw = read_word(b.reads, qbase + in_);
-            const uint8_t nqq = b.quals[qqual + in_];
             const int nidx = i - 1 - bw;
-            const uint32_t nrw = ref_word(hoff + nidx);
+            uint32_t nqw = 0, nrw = 0, nv = 0;
+            uint8_t nqq = 0;
+            if constexpr (STAGE) {
+                nv = rowd[in_ * 64 + lane];
+                nrw = refd[(nidx + W) * 64 + lane];
+                nqq = (uint8_t)(nv & 255);
+            } else {
+                nqw = read_word(b.reads, qbase + in_);
+                nqq = b.quals[qqual + in_];
+                nrw = ref_word(hoff + nidx);
+            }
             const double nS = Sst[(uint64_t)(i >= 2 ? i - 1 : i) * 64 + lane];
             win_shl((uint8_t)crb);
             const double y = (i > 1) ? 1. : 0.;
@@ -747,9 +822,14 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             const double yy = cyy;
 #pragma unroll
             for (int r = 0; r < W; r++) { M[r] = M[r] * yy; I[r] = I[r] * yy; }
-            cqb = (nqw >> (((qbase + in_) & 7) * 4)) & 15;
+            if constexpr (STAGE) {
+                cqb = nv >> 8;
+                crb = nrw;
+            } else {
+                cqb = (nqw >> (((qbase + in_) & 7) * 4)) & 15;
+                crb = nibble(nrw, nidx);
+            }
             cp = qprob(nqq);
-            crb = nibble(nrw, nidx);
             cyy = 1.0 / nS;
             pin(cqb); pin(cp); pin(crb); pin(cyy);
             map_row(i, nqq);
@@ -883,12 +963,45 @@ extern "C" hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t
     return hipGetLastError();
 }
 
+static int g_baq_lds_limit = 64 * 1024;
+
+// LDS bytes of a STAGE band workgroup (4 waves): per wave rows_cap u16 query cells and
+// rows_cap + 2W reference cells
+static size_t baq_stage_bytes(int W, uint32_t rows_cap)
+{
+    return 4 * ((size_t)rows_cap * 128 + ((size_t)rows_cap + 2 * W) * 64);
+}
+
+extern "C" hipError_t fpd_baq_prepare()
+{
+    int dev = 0, v = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    // the static tables (qualities and thresholds, 3 KiB) share the limit
+    g_baq_lds_limit = (v > 0 ? v : 64 * 1024) - 4096;
+    const int lim = g_baq_lds_limit;
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<15, 1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<23, 1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<33, 1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    return hipSuccess;
+}
+
+// whether a band class runs its STAGE kernel for this row capacity (class 0 only at one wave per SIMD)
+extern "C" int fpd_baq_staged(int cls, uint32_t rows_cap, int wpe, int allow)
+{
+    if (!allow || cls > 2 || (cls == 0 && wpe != 1)) return 0;
+    const int W = cls == 0 ? 15 : (cls == 1 ? 23 : 33);
+    return baq_stage_bytes(W, rows_cap) <= (size_t)g_baq_lds_limit;
+}
+
 // class: 0 -> 15 cells, 1 -> 23, 2 -> 33, 3 -> generic (wmax_runtime)
 extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const fpd::GenomeDev *g,
                                          const fpd::ReadStateDev *s, const uint32_t *list,
                                          const uint32_t *count, double *scratch, uint64_t slot_doubles,
                                          uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves,
-                                         unsigned long long *cells, int wpe, hipStream_t st)
+                                         unsigned long long *cells, int wpe, int allow_stage, hipStream_t st)
 {
     fpd::BaqArgs a{};
     a.b = *b; a.g = *g; a.s = *s; a.list = list; a.count = count; a.scratch = scratch;
@@ -896,17 +1009,26 @@ extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const 
     if (waves == 0) return hipSuccess;
     // band kernels: 4 waves per workgroup (the caller sizes waves as a multiple of 4)
     const dim3 g4((waves + 3) / 4), b4(256);
+    const bool stage = fpd_baq_staged(cls, rows_cap, wpe, allow_stage) != 0;
+    const size_t lds = stage ? baq_stage_bytes(cls == 0 ? 15 : (cls == 1 ? 23 : 33), rows_cap) : 0;
     switch (cls) {
     case 0:
+        if (stage) { hipLaunchKernelGGL((fpd::k_baq_band<15, 1, true>), g4, b4, lds, st, a); break; }
         switch (wpe) {
-        case 1: hipLaunchKernelGGL((fpd::k_baq_band<15, 1>), g4, b4, 0, st, a); break;
-        case 3: hipLaunchKernelGGL((fpd::k_baq_band<15, 3>), g4, b4, 0, st, a); break;
-        case 4: hipLaunchKernelGGL((fpd::k_baq_band<15, 4>), g4, b4, 0, st, a); break;
-        default: hipLaunchKernelGGL((fpd::k_baq_band<15, 2>), g4, b4, 0, st, a); break;
+        case 1: hipLaunchKernelGGL((fpd::k_baq_band<15, 1, false>), g4, b4, 0, st, a); break;
+        case 3: hipLaunchKernelGGL((fpd::k_baq_band<15, 3, false>), g4, b4, 0, st, a); break;
+        case 4: hipLaunchKernelGGL((fpd::k_baq_band<15, 4, false>), g4, b4, 0, st, a); break;
+        default: hipLaunchKernelGGL((fpd::k_baq_band<15, 2, false>), g4, b4, 0, st, a); break;
         }
         break;
-    case 1: hipLaunchKernelGGL((fpd::k_baq_band<23, 1>), g4, b4, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((fpd::k_baq_band<33, 1>), g4, b4, 0, st, a); break;
+    case 1:
+        if (stage) hipLaunchKernelGGL((fpd::k_baq_band<23, 1, true>), g4, b4, lds, st, a);
+        else hipLaunchKernelGGL((fpd::k_baq_band<23, 1, false>), g4, b4, 0, st, a);
+        break;
+    case 2:
+        if (stage) hipLaunchKernelGGL((fpd::k_baq_band<33, 1, true>), g4, b4, lds, st, a);
+        else hipLaunchKernelGGL((fpd::k_baq_band<33, 1, false>), g4, b4, 0, st, a);
+        break;
     default: hipLaunchKernelGGL((fpd::k_baq_hmm<0, 2>), dim3(waves), dim3(64), 0, st, a); break;
     }
     return hipGetLastError();

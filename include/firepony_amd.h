@@ -82,7 +82,11 @@ typedef struct fp_options {
     uint32_t serve_rg_slots;         /* serving: read-group table slots shared by the
                                         open jobs (HBM budget = slots x ~100 MB of
                                         resident tables); 0 = not a serving engine    */
-    uint32_t reserved[3];
+    uint32_t disable_baq_staging;    /* 1: BAQ band kernels read their per-read inputs
+                                        from HBM on every row instead of staging them
+                                        in LDS once per read (0 = stage when the batch's
+                                        longest read fits the LDS budget)              */
+    uint32_t reserved[2];
 } fp_options;
 
 /* how covariate observations reach the resident tables (identical results) */

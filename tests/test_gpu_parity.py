@@ -117,12 +117,12 @@ def run_parity(tmp_path, genome, rs, n, batch_size, per_batch=True, **engine_opt
     e.close()
 
 
-@pytest.mark.parametrize("wpe", [1, 2, 3, 4])
-def test_config0_slice(tmp_path, wpe):
-    say(f"parity: config 0 first 4000 reads, BAQ register budget for {wpe} waves/SIMD")
+@pytest.mark.parametrize("wpe,nostage", [(1, 0), (1, 1), (2, 0), (3, 0), (4, 0)])
+def test_config0_slice(tmp_path, wpe, nostage):
+    say(f"parity: config 0 first 4000 reads, BAQ register budget for {wpe} waves/SIMD, LDS staging {1 - nostage}")
     gs, rs, n = synth.named_spec(0)
     g = synth.Genome(gs, rs.num_read_groups)
-    run_parity(tmp_path, g, rs, 4000, 1000, baq_waves_per_simd=wpe)
+    run_parity(tmp_path, g, rs, 4000, 1000, baq_waves_per_simd=wpe, disable_baq_staging=nostage)
 
 
 def test_edge_cases(tmp_path):
@@ -133,11 +133,12 @@ def test_edge_cases(tmp_path):
     run_parity(tmp_path, g, rs, n, 1000)
 
 
-def test_mixed_lengths_f64(tmp_path):
-    say("parity: mixed lengths, large indels, fp64 fractional errors")
+@pytest.mark.parametrize("nostage", [0, 1])
+def test_mixed_lengths_f64(tmp_path, nostage):
+    say(f"parity: mixed lengths, large indels, fp64 fractional errors, LDS staging {1 - nostage}")
     g, rs, n = synth.small_workload(seed=11, num_reads=3000, lengths=(50, 76, 101, 151, 250), read_groups=8,
                                     contigs=(300000,), sites=600, clip=1.0, indel=0.3)
-    run_parity(tmp_path, g, rs, n, 1500, frac_error_f64=1)
+    run_parity(tmp_path, g, rs, n, 1500, frac_error_f64=1, disable_baq_staging=nostage)
 
 
 def test_batch_size_invariance(tmp_path):
