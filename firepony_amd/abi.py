@@ -156,7 +156,8 @@ class FpStats(C.Structure):
         ("cov_launches", C.c_uint64),
         ("baq_cells", C.c_uint64),
         ("baq_waves_per_simd", C.c_uint64),
-        ("reserved", C.c_uint64 * 4),
+        ("sort_digit_bits", C.c_uint64),
+        ("reserved", C.c_uint64 * 3),
     ]
 
 

@@ -27,6 +27,7 @@ ENGINE_SOURCES = [
     "k_prep.hip",
     "k_baq.hip",
     "k_cov.hip",
+    "k_sort.hip",
     "k_table.hip",
     "k_post.hip",
 ]

@@ -26,7 +26,7 @@
 // kernels (k_*.hip)
 extern "C" {
 hipError_t fpd_launch_prep(const fpd::BatchDev *b, const fpd::GenomeDev *g, const fpd::ReadStateDev *s,
-                           uint32_t *counters, hipStream_t st);
+                           uint32_t *counters, uint32_t max_read_len, hipStream_t st);
 hipError_t fpd_baq_set_constants(const double *qual2prob, const double *thr, double gap_open);
 hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t *live, const fpd::ReadStateDev *s,
                                    uint32_t *lists, uint32_t *counts, uint32_t capacity, hipStream_t st);
@@ -45,7 +45,12 @@ hipError_t fpd_rebase(uint32_t n, uint32_t *cigar_start, uint32_t *read_start, u
 hipError_t fpd_cov_prepare();
 hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
                           uint32_t frac_f64, uint32_t keep_frac, uint32_t *counters, uint32_t *rg_buf,
-                          uint32_t max_read_len, hipStream_t st);
+                          uint32_t max_read_len, uint32_t *const *pkey, double *const *pval, uint32_t *pcursor,
+                          uint64_t pcap, hipStream_t st);
+hipError_t fpd_sort_reduce(uint32_t *keys, double *vals, uint32_t *tkeys, double *tvals, uint64_t cap,
+                           const uint32_t *n_emit, uint32_t *n_kept, uint32_t *hist, uint32_t *totals,
+                           uint32_t key_bits, uint32_t digit_bits, uint32_t table_entries,
+                           unsigned long long *count, unsigned long long *const *parts, hipStream_t st);
 hipError_t fpd_compact_count(const uint64_t *count, uint64_t n, uint32_t *block_counts, hipStream_t st);
 hipError_t fpd_compact_scatter(const uint64_t *count, const uint64_t *const *parts, uint64_t n,
                                const uint64_t *block_offsets, uint32_t *okey, uint64_t *ocount, uint64_t *olo,
@@ -197,6 +202,14 @@ struct fp_engine {
     uint64_t last_num_bases = 0, last_num_quals = 0;
     fpd::ReadStateDev rs{};
 
+    // sort aggregation: emitted pairs per table, one ping-pong pair buffer, radix histograms,
+    // counters {append cursor x3, overflow, kept x3}
+    DevBuf pk[3], pv[3], tk, tv, shist, stot, sctr;
+    uint64_t pair_cap = 0;
+    int sort_bits = 8;                  // radix digit width in use (4-8)
+    bool sort_tuned = false;
+    double sort_tune_ms[5] = {0, 0, 0, 0, 0};
+
     // resident tables
     DevBuf tab[3][4];     // per table: count, then the three limb sums
     uint64_t tab_size[3] = {0, 0, 0};
@@ -220,7 +233,7 @@ struct fp_engine {
         uint64_t ncig = 0, nbases = 0, nquals = 0;
         hipEvent_t filled = nullptr, consumed = nullptr;
         hipGraphExec_t gfront = nullptr, gback = nullptr;
-        int g_wpe = 0;
+        int g_wpe = 0, g_bits = 0;
         uint64_t g_epoch = 0;
     } stg[2];
     bool batched = false;
@@ -234,6 +247,7 @@ struct fp_engine {
     // autotuner: waves per SIMD of the 15-cell BAQ band kernel, timed on the first staged batches
     bool tuned = false;
     int tune_pending = -1;              // candidate whose stage-2 timing is in flight
+    int tune_kind = 0;                  // 0: BAQ waves per SIMD, 1: sort digit width
     uint32_t tune_pending_reads = 0;
     double tune_ms_per_read[4] = {0, 0, 0, 0};
     hipEvent_t bev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -380,7 +394,7 @@ static uint64_t band_scratch_doubles(int wpe, uint32_t n, uint32_t rows_cap)
 fp_status launch_front(fp_engine *e, const fpd::BatchDev &b, const Plan &p, uint32_t *counters, hipStream_t st)
 {
     HIP_TRY(fpd_fill_u32(e->baq_counts.as<uint32_t>(), 16, 0u, st));
-    HIP_TRY(fpd_launch_prep(&b, &e->genome, &e->rs, counters, st));
+    HIP_TRY(fpd_launch_prep(&b, &e->genome, &e->rs, counters, p.rows_cap, st));
     HIP_TRY(fpd_fill_u8(e->rs.baq, p.nquals, p.live, 0xff, st));
     HIP_TRY(fpd_launch_baq_classify(p.n, p.live, &e->rs, e->baq_lists.as<uint32_t>(), e->baq_counts.as<uint32_t>(),
                                     std::max<uint32_t>(p.n, 1), st));
@@ -389,8 +403,66 @@ fp_status launch_front(fp_engine *e, const fpd::BatchDev &b, const Plan &p, uint
 
 // stage 2 (graph-safe): BAQ band kernels, cap/recode, covariate tables.  The class of bands wider
 // than 33 cells runs separately (launch_generic) before it, when present.
-fp_status launch_back(fp_engine *e, const fpd::BatchDev &b, const Plan &p, int wpe, uint32_t *counters,
-                      hipStream_t st, hipEvent_t *marks, const uint32_t *known_counts)
+// key bits a table's sort has to order: the fields below the read group plus the bits of the
+// largest read-group id
+static uint32_t sort_key_bits(uint32_t num_rg, int t)
+{
+    const uint32_t rg_bits = num_rg <= 1 ? 0u : 32u - (uint32_t)__builtin_clz(num_rg - 1);
+    return (uint32_t)fpd::TABLE_BITS[t] - 8u + rg_bits;
+}
+
+// pair buffers of the sort aggregation for up to `cap` observations per table
+fp_status ensure_pairs(fp_engine *e, uint64_t cap)
+{
+    if (cap > 0xf0000000ull)
+        return fail(FP_ERR_INVALID_ARG, "sort aggregation: over 4e9 observations per table in one batch");
+    cap = std::max<uint64_t>(cap, 4096);
+    if (cap <= e->pair_cap) return FP_OK;
+    for (int t = 0; t < 3; t++) {
+        HIP_TRY(e->pk[t].ensure(4 * cap + 64));
+        HIP_TRY(e->pv[t].ensure(8 * cap + 64));
+    }
+    HIP_TRY(e->tk.ensure(4 * cap + 64));
+    HIP_TRY(e->tv.ensure(8 * cap + 64));
+    HIP_TRY(e->shist.ensure(4ull * 256 * ((cap + 4095) / 4096) + 64));
+    HIP_TRY(e->stot.ensure(4 * 256));
+    HIP_TRY(e->sctr.ensure(64));
+    e->pair_cap = cap;
+    return FP_OK;
+}
+
+// the covariate stage: direct LDS-privatised table updates, or pairs -> radix sort -> reduce_by_key
+fp_status launch_cov_stage(fp_engine *e, const fpd::BatchDev &b, const Plan &p, int sort_bits, uint32_t *counters,
+                           hipStream_t st, hipEvent_t sort_mark)
+{
+    const uint32_t keep = e->opts.debug_arrays ? 1u : 0u;
+    const uint32_t maxlen = std::min<uint32_t>(p.rows_cap, 499);
+    if (e->opts.aggregation != FP_AGG_SORT) {
+        HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, keep, counters,
+                               e->rg_buf.as<uint32_t>(), maxlen, nullptr, nullptr, nullptr, 0, st));
+        return FP_OK;
+    }
+    uint32_t *sc = e->sctr.as<uint32_t>();
+    HIP_TRY(fpd_fill_u32(sc, 8, 0u, st));
+    uint32_t *pkeys[3];
+    double *pvals[3];
+    for (int t = 0; t < 3; t++) { pkeys[t] = e->pk[t].as<uint32_t>(); pvals[t] = e->pv[t].as<double>(); }
+    HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, keep, counters, e->rg_buf.as<uint32_t>(),
+                           maxlen, pkeys, pvals, sc, e->pair_cap, st));
+    if (sort_mark) HIP_TRY(hipEventRecord(sort_mark, st));
+    for (int t = 0; t < 3; t++) {
+        unsigned long long *parts[3];
+        for (int k = 0; k < 3; k++) parts[k] = (unsigned long long *)e->tables.part[t][k];
+        HIP_TRY(fpd_sort_reduce(pkeys[t], pvals[t], e->tk.as<uint32_t>(), e->tv.as<double>(), e->pair_cap, sc + t,
+                                sc + 4 + t, e->shist.as<uint32_t>(), e->stot.as<uint32_t>(),
+                                sort_key_bits(e->num_rg, t), (uint32_t)sort_bits, (uint32_t)e->tab_size[t],
+                                (unsigned long long *)e->tables.count[t], parts, st));
+    }
+    return FP_OK;
+}
+
+fp_status launch_back(fp_engine *e, const fpd::BatchDev &b, const Plan &p, int wpe, int sort_bits,
+                      uint32_t *counters, hipStream_t st, hipEvent_t *marks, const uint32_t *known_counts)
 {
     const uint32_t cap = std::max<uint32_t>(p.n, 1);
     for (int c = 0; c < 3; c++) {
@@ -404,9 +476,7 @@ fp_status launch_back(fp_engine *e, const fpd::BatchDev &b, const Plan &p, int w
     if (marks) HIP_TRY(hipEventRecord(marks[0], st));
     HIP_TRY(fpd_launch_baq_post(&b, &e->rs, 1u, st));
     if (marks) HIP_TRY(hipEventRecord(marks[1], st));
-    HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, e->opts.debug_arrays ? 1u : 0u, counters,
-                           e->rg_buf.as<uint32_t>(), std::min<uint32_t>(p.rows_cap, 499), st));
-    return FP_OK;
+    return launch_cov_stage(e, b, p, sort_bits, counters, st, marks ? marks[2] : nullptr);
 }
 
 // bands wider than 33 cells (rare: large indels), sized from the widest such band
@@ -440,7 +510,7 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     HIP_TRY(hipEventRecord(e->ev[0], st));
     HIP_TRY(fpd_fill_u32(counters, 16, 0u, st));
     HIP_TRY(hipEventRecord(e->ev[6], st));
-    Plan p{n, v->num_qualities, nullptr, 1};
+    Plan p{n, v->num_qualities, nullptr, max_read_len_hint};   // rows known after the front stage
     rc = launch_front(e, b, p, counters, st);
     if (rc != FP_OK) return rc;
     HIP_TRY(hipEventRecord(e->ev[7], st));
@@ -450,18 +520,24 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     HIP_TRY(hipStreamSynchronize(st));
     p.rows_cap = std::max<uint32_t>(std::max(max_read_len_hint, hc[8 + 3]), 1);
     HIP_TRY(e->scratch.ensure(band_scratch_doubles(e->wpe, n, p.rows_cap) * 8 + 8));
+    const bool sorting = e->opts.aggregation == FP_AGG_SORT;
+    if (sorting) {
+        rc = ensure_pairs(e, 3ull * n * p.rows_cap);
+        if (rc != FP_OK) return rc;
+    }
 
     HIP_TRY(hipEventRecord(e->ev[8], st));
     rc = launch_generic(e, b, p, hc[3], hc[4], counters, st);
     if (rc != FP_OK) return rc;
-    hipEvent_t marks[2] = {e->ev[9], e->ev[2]};
-    rc = launch_back(e, b, p, e->wpe, counters, st, marks, hc);
+    hipEvent_t marks[3] = {e->ev[9], e->ev[2], e->ev[10]};
+    rc = launch_back(e, b, p, e->wpe, e->sort_bits, counters, st, marks, hc);
     if (rc != FP_OK) return rc;
     HIP_TRY(hipEventRecord(e->ev[3], st));
     uint32_t *post = e->host_counts + 16;
     HIP_TRY(hipMemcpyAsync(post, counters, 64, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (post[4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
+    if (post[6]) return fail(FP_ERR_INTERNAL, "sort aggregation: observation pairs overflowed their buffer");
 
     e->stats.total_reads += n;
     e->stats.num_batches += 1;
@@ -475,7 +551,12 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     e->stats.prep_launches += 1;
     if (hmm_launches) e->stats.baq_hmm_kernel_ms += elapsed(e->ev[8], e->ev[9]);
     e->stats.baq_hmm_launches += hmm_launches;
-    e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[3]);
+    if (sorting) {
+        e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[10]);
+        e->stats.covariates_sort_ms += elapsed(e->ev[10], e->ev[3]);
+    } else {
+        e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[3]);
+    }
     e->stats.cov_launches += 1;
     e->stats.read_filter_ms += elapsed(e->ev[0], e->ev[7]);
     e->stats.baq_ms += elapsed(e->ev[7], e->ev[2]);
@@ -488,6 +569,7 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
 // batched engine
 
 static const int TUNE_CANDIDATES[3] = {1, 2, 3};
+static const int SORT_CANDIDATES[5] = {4, 5, 6, 7, 8};
 
 fpd::BatchDev stage_dev(fp_engine *e, fp_engine::Stage &s)
 {
@@ -534,6 +616,10 @@ fp_status batched_alloc(fp_engine *e)
                                 e->opts.debug_arrays != 0 || e->rows_cap > 256);
     if (rc != FP_OK) return rc;
     HIP_TRY(e->scratch.ensure(band_scratch_doubles(std::max(3, e->wpe), e->cap_reads, e->rows_cap) * 8 + 8));
+    if (e->opts.aggregation == FP_AGG_SORT) {
+        rc = ensure_pairs(e, 3 * e->cap_quals);
+        if (rc != FP_OK) return rc;
+    }
     HIP_TRY(e->bcounters.ensure(64));
     HIP_TRY(fpd_fill_u32(e->bcounters.as<uint32_t>(), 16, 0u, e->stream));
     for (auto &ev : e->bev) if (!ev) HIP_TRY(hipEventCreate(&ev));
@@ -569,15 +655,27 @@ void tune_collect(fp_engine *e)
         e->back_pending = false;
     }
     if (e->tune_pending < 0) return;
-    const double ms = elapsed(e->bev[0], e->bev[1]);
-    e->tune_ms_per_read[e->tune_pending] = ms / std::max<uint32_t>(e->tune_pending_reads, 1);
-    e->tune_pending = -1;
-    if (e->tune_ms_per_read[0] > 0 && e->tune_ms_per_read[1] > 0 && e->tune_ms_per_read[2] > 0) {
-        int best = 0;
-        for (int i = 1; i < 3; i++) if (e->tune_ms_per_read[i] < e->tune_ms_per_read[best]) best = i;
-        e->wpe = TUNE_CANDIDATES[best];
-        e->tuned = true;
+    const double per_read = elapsed(e->bev[0], e->bev[1]) / std::max<uint32_t>(e->tune_pending_reads, 1);
+    if (e->tune_kind == 0) {
+        e->tune_ms_per_read[e->tune_pending] = per_read;
+        if (e->tune_ms_per_read[0] > 0 && e->tune_ms_per_read[1] > 0 && e->tune_ms_per_read[2] > 0) {
+            int best = 0;
+            for (int i = 1; i < 3; i++) if (e->tune_ms_per_read[i] < e->tune_ms_per_read[best]) best = i;
+            e->wpe = TUNE_CANDIDATES[best];
+            e->tuned = true;
+        }
+    } else {
+        e->sort_tune_ms[e->tune_pending] = per_read;
+        bool all = true;
+        for (int i = 0; i < 5; i++) all = all && e->sort_tune_ms[i] > 0;
+        if (all) {
+            int best = 0;
+            for (int i = 1; i < 5; i++) if (e->sort_tune_ms[i] < e->sort_tune_ms[best]) best = i;
+            e->sort_bits = SORT_CANDIDATES[best];
+            e->sort_tuned = true;
+        }
     }
+    e->tune_pending = -1;
 }
 
 // run the current staging buffer (if it holds reads) and switch to the other one
@@ -623,31 +721,42 @@ fp_status batched_flush(fp_engine *e)
     e->stats.prep_kernel_ms += elapsed(e->bev[2], e->bev[3]);
     e->stats.prep_launches += 1;
     if (hc[8 + 4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
+    if (hc[8 + 6]) return fail(FP_ERR_INTERNAL, "sort aggregation: observation pairs overflowed their buffer");
     if (hc[8 + 3] > e->rows_cap)
         return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(e->rows_cap) + ")");
     e->stats.filtered_reads = e->eager_filtered + hc[8 + 1];
     e->stats.baq_reads = e->eager_baq + hc[8 + 2];
 
-    int wpe = e->wpe;
-    int cand = -1;
+    // autotuner: BAQ waves per SIMD first, then the radix digit width of the sort aggregation; each
+    // candidate runs one staged batch eagerly and is timed per read
+    int wpe = e->wpe, bits = e->sort_bits;
+    int cand = -1, kind = 0;
     if (!e->tuned) {
         for (int i = 0; i < 3; i++) if (e->tune_ms_per_read[i] == 0) { cand = i; break; }
         if (cand >= 0) wpe = TUNE_CANDIDATES[cand];
+    } else if (!e->sort_tuned) {
+        for (int i = 0; i < 5; i++) if (e->sort_tune_ms[i] == 0) { cand = i; break; }
+        if (cand >= 0) { bits = SORT_CANDIDATES[cand]; kind = 1; }
     }
     if (e->opts.profile_stages) {
         // eager stage 2 with an event after the BAQ kernels and after cap/recode
         HIP_TRY(hipEventRecord(e->ev[8], st));
         rc = launch_generic(e, b, p, hc[3], hc[4], counters, st);
         if (rc != FP_OK) return rc;
-        hipEvent_t marks[2] = {e->ev[9], e->ev[2]};
-        rc = launch_back(e, b, p, wpe, counters, st, marks, hc);
+        hipEvent_t marks[3] = {e->ev[9], e->ev[2], e->ev[10]};
+        rc = launch_back(e, b, p, wpe, bits, counters, st, marks, hc);
         if (rc != FP_OK) return rc;
         HIP_TRY(hipEventRecord(e->ev[3], st));
         HIP_TRY(hipStreamSynchronize(st));
         const uint32_t launches = (hc[0] ? 1 : 0) + (hc[1] ? 1 : 0) + (hc[2] ? 1 : 0) + (hc[3] ? 1 : 0);
         e->stats.baq_hmm_kernel_ms += elapsed(e->ev[8], e->ev[9]);
         e->stats.baq_hmm_launches += launches;
-        e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[3]);
+        if (e->opts.aggregation == FP_AGG_SORT) {
+            e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[10]);
+            e->stats.covariates_sort_ms += elapsed(e->ev[10], e->ev[3]);
+        } else {
+            e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[3]);
+        }
         e->stats.cov_launches += 1;
         e->stats.baq_ms += elapsed(e->ev[8], e->ev[2]);
         e->stats.covariates_ms += elapsed(e->ev[2], e->ev[3]);
@@ -665,16 +774,19 @@ fp_status batched_flush(fp_engine *e)
     if (rc != FP_OK) return rc;
     HIP_TRY(hipEventRecord(e->bev[0], st));
     if (cand < 0) {
-        if (!s.gback || s.g_wpe != wpe) {
-            rc = capture(e, &s.gback, [&] { return launch_back(e, b, p, wpe, counters, st, nullptr, nullptr); });
+        if (!s.gback || s.g_wpe != wpe || s.g_bits != bits) {
+            rc = capture(e, &s.gback,
+                         [&] { return launch_back(e, b, p, wpe, bits, counters, st, nullptr, nullptr); });
             if (rc != FP_OK) return rc;
             s.g_wpe = wpe;
+            s.g_bits = bits;
         }
         HIP_TRY(hipGraphLaunch(s.gback, st));
     } else {
-        rc = launch_back(e, b, p, wpe, counters, st, nullptr, nullptr);   // timed candidate, eager
+        rc = launch_back(e, b, p, wpe, bits, counters, st, nullptr, nullptr);   // timed candidate, eager
         if (rc != FP_OK) return rc;
         e->tune_pending = cand;
+        e->tune_kind = kind;
         e->tune_pending_reads = s.n;
     }
     HIP_TRY(hipEventRecord(e->bev[1], st));
@@ -703,6 +815,7 @@ fp_status batched_sync(fp_engine *e)
     uint32_t *hc = e->host_counts + 32;
     HIP_TRY(hipMemcpy(hc, e->bcounters.p, 64, hipMemcpyDeviceToHost));
     if (hc[4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
+    if (hc[6]) return fail(FP_ERR_INTERNAL, "sort aggregation: observation pairs overflowed their buffer");
     e->stats.filtered_reads = e->eager_filtered + hc[1];
     e->stats.baq_reads = e->eager_baq + hc[2];
     uint64_t cells;
@@ -990,6 +1103,10 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
         if (opts->abi_version != FP_ABI_VERSION) return fail(FP_ERR_UNSUPPORTED, "ABI version mismatch");
         o = *opts;
     }
+    if (o.aggregation != FP_AGG_DIRECT && o.aggregation != FP_AGG_SORT)
+        return fail(FP_ERR_INVALID_ARG, "aggregation must be FP_AGG_DIRECT or FP_AGG_SORT");
+    if (o.sort_digit_bits != 0 && (o.sort_digit_bits < 4 || o.sort_digit_bits > 8))
+        return fail(FP_ERR_INVALID_ARG, "sort_digit_bits must be 0 (autotune) or 4..8");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (hip_device < 0 || hip_device >= ndev) return fail(FP_ERR_INVALID_ARG, "no such HIP device");
@@ -998,6 +1115,9 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     e->device = hip_device;
     e->opts = o;
     e->wpe = o.baq_waves_per_simd >= 1 && o.baq_waves_per_simd <= 4 ? (int)o.baq_waves_per_simd : 1;
+    e->sort_bits = o.sort_digit_bits ? (int)o.sort_digit_bits : 8;
+    // the digit width is tuned on the batched engine's first staged batches unless it is given
+    e->sort_tuned = o.sort_digit_bits != 0 || o.aggregation != FP_AGG_SORT || o.engine_kind != FP_ENGINE_BATCHED;
     memset(&e->stats, 0, sizeof(e->stats));
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess) e->name = prop.name;
@@ -1291,6 +1411,7 @@ fp_status fp_engine_stats(fp_engine *e, fp_stats *out)
     }
     *out = e->stats;
     out->baq_waves_per_simd = (uint64_t)e->wpe;
+    out->sort_digit_bits = e->opts.aggregation == FP_AGG_SORT ? (uint64_t)e->sort_bits : 0;
     return FP_OK;
 }
 

@@ -45,6 +45,12 @@ struct CovArgs {
     const uint32_t *rg_list;     // active reads grouped by read group
     const uint32_t *rg_offset;   // num_rg + 1
     const uint32_t *tile_offset; // num_rg + 1 (tiles of TILE reads)
+    // pair mode (sort aggregation): one (key, value) pair per observation and table
+    uint32_t *pkey[3];
+    double *pval[3];
+    uint32_t *pcursor;           // [3] append cursors (elements)
+    uint32_t *perr;              // set when the pairs of a batch would overflow pcap
+    uint64_t pcap;               // pair capacity per table
 };
 
 // calculateFractionalErrorArray for one error class over the clipped window (serial form)
@@ -243,8 +249,81 @@ __device__ inline void emit_base(const CovArgs &a, const LdsAcc &L, uint32_t rg,
     }
 }
 
+// ---- pair mode -------------------------------------------------------------
+
+static constexpr uint32_t PAIR_SENTINEL = 0xffffffffu;
+
+// a workgroup's slice of the pair arrays: base[t] and the slots used so far (LDS)
+struct PairRegion {
+    uint32_t *used;
+    const uint64_t *base;
+};
+
+// append n (0..3) pairs per lane to table t; a collective of the active lanes of the wavefront
+__device__ inline void append_pairs(const CovArgs &a, const PairRegion &pr, int t, uint32_t n, uint32_t k0,
+                                    uint32_t k1, uint32_t k2, double v0, double v1, double v2)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt = (1ull << lane) - 1;
+    const uint64_t b0 = __ballot(n & 1), b1 = __ballot((n >> 1) & 1);
+    const uint32_t total = (uint32_t)__popcll(b0) + 2 * (uint32_t)__popcll(b1);
+    if (total == 0) return;
+    const int leader = __builtin_ctzll(__ballot(1));
+    uint32_t off = 0;
+    if ((int)lane == leader) off = atomicAdd(&pr.used[t], total);
+    off = (uint32_t)__shfl((int)off, leader);
+    const uint64_t pos = pr.base[t] + off + (uint32_t)__popcll(b0 & lt) + 2 * (uint32_t)__popcll(b1 & lt);
+    uint32_t *kk = a.pkey[t] + pos;
+    double *vv = a.pval[t] + pos;
+    if (n > 0) { kk[0] = k0; vv[0] = v0; }
+    if (n > 1) { kk[1] = k1; vv[1] = v1; }
+    if (n > 2) { kk[2] = k2; vv[2] = v2; }
+}
+
+// the covariate keys of one base as pairs (the same keys emit_base accumulates); emit false:
+// the lane takes part in the collective appends with no pairs
+template <class BASE>
+__device__ inline void emit_pairs(const CovArgs &a, const PairRegion &pr, uint32_t rg, uint16_t flags, bool neg,
+                                  BASE base, uint16_t rwx, uint16_t rwy, uint16_t hx, uint16_t hy, uint16_t bp,
+                                  uint32_t q, double vs, double vi, double vd, bool emit)
+{
+    if (!a.frac_f64) { vs = (double)(float)vs; vi = (double)(float)vi; vd = (double)(float)vd; }
+    uint32_t nq = 0, ny = 0, nx = 0;
+    uint32_t y0 = 0, y1 = 0, y2 = 0, x0 = 0, x1 = 0, x2 = 0;
+    double xv0 = 0.0, xv1 = 0.0, xv2 = 0.0;
+    if (emit) {
+        nq = 3;
+        uint32_t cm, cid;
+        cycle_keys(flags, rwx, rwy, bp, cm, cid);
+        y0 = 0u | (cm << 2) | (q << 12) | (rg << 20);
+        ny = 1;
+        if (cid != 1023u) {
+            y1 = 1u | (cid << 2) | (45u << 12) | (rg << 20);
+            y2 = 2u | (cid << 2) | (45u << 12) | (rg << 20);
+            ny = 3;
+        }
+        uint32_t xm, xid;
+        context_keys(base, hx, hy, neg, bp, xm, xid);
+        const uint32_t km = 0u | (xm << 2) | (q << 13) | (rg << 21);
+        const uint32_t ki = 1u | (xid << 2) | (45u << 13) | (rg << 21);
+        const uint32_t kd = 2u | (xid << 2) | (45u << 13) | (rg << 21);
+        if (xm != 2047u && xid != 2047u) { x0 = km; xv0 = vs; x1 = ki; xv1 = vi; x2 = kd; xv2 = vd; nx = 3; }
+        else if (xm != 2047u) { x0 = km; xv0 = vs; nx = 1; }
+        else if (xid != 2047u) { x0 = ki; xv0 = vi; x1 = kd; xv1 = vd; nx = 2; }
+    }
+    // RecalTable1: EventTracker(2) | QualityScore(8) | ReadGroup(8)
+    append_pairs(a, pr, 0, nq, 0u | (q << 2) | (rg << 10), 1u | (45u << 2) | (rg << 10), 2u | (45u << 2) | (rg << 10),
+                 vs, vi, vd);
+    // cycle: EventTracker(2) | Cycle(10) | QualityScore(8) | ReadGroup(8)
+    append_pairs(a, pr, 1, ny, y0, y1, y2, vs, vi, vd);
+    // context: EventTracker(2) | Context(11) | QualityScore(8) | ReadGroup(8)
+    append_pairs(a, pr, 2, nx, x0, x1, x2, xv0, xv1, xv2);
+}
+
 // one read on one lane (windows longer than 256 bases)
-__device__ void cov_read_serial(const CovArgs &a, const LdsAcc &L, uint32_t r, uint32_t rg, uint32_t ncodes)
+template <bool PAIRS>
+__device__ void cov_read_serial(const CovArgs &a, const LdsAcc &L, const PairRegion &pr, uint32_t r, uint32_t rg,
+                                uint32_t ncodes)
 {
     const BatchDev &b = a.b;
     const uint32_t rw = a.s.rwc[r];
@@ -282,9 +361,13 @@ __device__ void cov_read_serial(const CovArgs &a, const LdsAcc &L, uint32_t r, u
                 if (bp < rwx || bp > rwy) continue;
                 if (a.s.base_flags[rs + bp] & BF_MASKED) continue;
                 const double *fe = a.s.frac + 3ull * (qs + bp);
-                emit_base(a, L, rg, ncodes, flags, neg, base, rwx, rwy, hx, hy, bp, b.quals[qs + bp],
-                          to_limbs(fe[0], a.frac_f64), to_limbs(fe[1], a.frac_f64), to_limbs(fe[2], a.frac_f64),
-                          true);
+                if constexpr (PAIRS)
+                    emit_pairs(a, pr, rg, flags, neg, base, rwx, rwy, hx, hy, bp, b.quals[qs + bp], fe[0], fe[1],
+                               fe[2], true);
+                else
+                    emit_base(a, L, rg, ncodes, flags, neg, base, rwx, rwy, hx, hy, bp, b.quals[qs + bp],
+                              to_limbs(fe[0], a.frac_f64), to_limbs(fe[1], a.frac_f64),
+                              to_limbs(fe[2], a.frac_f64), true);
             }
         }
         ro = (uint16_t)(ro + l);
@@ -356,8 +439,9 @@ __device__ inline uint64_t wave_sum(uint64_t v)
 }
 
 // one read on one wavefront (clipped window of at most 256 bases); r is wave-uniform
-__device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, uint8_t *stage, uint32_t r, uint32_t rg,
-                              uint32_t ncodes)
+template <bool PAIRS>
+__device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegion &pr, uint8_t *stage, uint32_t r,
+                              uint32_t rg, uint32_t ncodes)
 {
     const BatchDev &b = a.b;
     const int lane = threadIdx.x & 63;
@@ -418,14 +502,8 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, uint8_t *stage,
     __builtin_amdgcn_wave_barrier();
     auto base = [&](int i) -> uint8_t { return stage[i - (int)rwx]; };
 
-    uint32_t nid = 0;
-    uint64_t i0 = 0, i1 = 0, i2 = 0, d0 = 0, d1 = 0, d2 = 0;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int j = lane + 64 * c;
-        if (j >= len) continue;
-        // the block (if any) whose average this base carries
-        double vs, vi, vd;
+    // GATK block-averaged fractional errors of window position j (calculateFractionalErrorArray)
+    auto frac_at = [&](int j, double &vs, double &vi, double &vd) {
         int s0 = -1, e = -1;
         if (mtest(IB, j)) {
             const int bs = prev_zero(IB, j) + 1;
@@ -451,11 +529,36 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, uint8_t *stage,
             vi = mtest(EI_, j) ? 1.0 : 0.0;
             vd = mtest(ED, j) ? 1.0 : 0.0;
         }
-        const uint16_t bp = (uint16_t)(rwx + j);
         if (a.keep_frac) {
-            double *fo = a.s.frac + 3ull * (qs + bp);
+            double *fo = a.s.frac + 3ull * (qs + rwx + j);
             fo[0] = vs; fo[1] = vi; fo[2] = vd;
         }
+    };
+
+    if constexpr (PAIRS) {
+        // every lane stays in the loop: the appends are wavefront collectives
+        for (int c = 0; c < 4 && 64 * c < len; c++) {
+            const int j = lane + 64 * c;
+            double vs = 0.0, vi = 0.0, vd = 0.0;
+            const bool in = j < len;
+            if (in) frac_at(j, vs, vi, vd);
+            const bool emit = in && !(fl[c] & BF_MASKED) && !mtest(SM, j);
+            emit_pairs(a, pr, rg, flags, neg, base, rwx, rwy, hx, hy, (uint16_t)(rwx + (in ? j : 0)), qv[c], vs, vi,
+                       vd, emit);
+        }
+        __builtin_amdgcn_wave_barrier();
+        return;
+    }
+
+    uint32_t nid = 0;
+    uint64_t i0 = 0, i1 = 0, i2 = 0, d0 = 0, d1 = 0, d2 = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int j = lane + 64 * c;
+        if (j >= len) continue;
+        double vs, vi, vd;
+        frac_at(j, vs, vi, vd);
+        const uint16_t bp = (uint16_t)(rwx + j);
         if (fl[c] & BF_MASKED) continue;
         if (mtest(SM, j)) continue;
         const Limbs f0 = to_limbs(vs, a.frac_f64), f1 = to_limbs(vi, a.frac_f64), f2 = to_limbs(vd, a.frac_f64);
@@ -477,9 +580,12 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, uint8_t *stage,
 }
 
 // one workgroup per (read group, tile of TILE reads of that group)
+template <bool PAIRS>
 __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
 {
     extern __shared__ unsigned long long lds_raw[];
+    __shared__ uint32_t pr_used[3], pr_bound, pr_over;
+    __shared__ uint64_t pr_base[3];
     const uint32_t ncodes = a.ncodes;
     const uint32_t E = E_YID + 2 * ncodes;
     const uint32_t MW = (a.mcodes * 64 + 1) / 2;    // words of packed cycle-M counts
@@ -490,7 +596,9 @@ __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
     L.cnt = (uint32_t *)(lds_raw + 3 * E);
     L.mcnt = L.cnt + E;
     L.mcodes = a.mcodes;
-    uint8_t *stage = (uint8_t *)(L.mcnt + MW) + (threadIdx.x >> 6) * 256;
+    uint8_t *stage = PAIRS ? (uint8_t *)lds_raw + (threadIdx.x >> 6) * 256
+                           : (uint8_t *)(L.mcnt + MW) + (threadIdx.x >> 6) * 256;
+    const PairRegion pr{pr_used, pr_base};
 
     // which read group / tile this block covers
     const uint32_t nrg = a.t.num_rg;
@@ -506,19 +614,53 @@ __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
     const uint32_t first = a.rg_offset[rg] + tile * TILE;
     const uint32_t last = min(a.rg_offset[rg + 1], first + TILE);
 
-    for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) { L.p0[e] = 0; L.p1[e] = 0; L.p2[e] = 0; L.cnt[e] = 0; }
-    for (uint32_t w = threadIdx.x; w < MW; w += blockDim.x) L.mcnt[w] = 0;
-    __syncthreads();
-
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if constexpr (PAIRS) {
+        // this tile's slice of each table's pair array: 3 slots per window base bound its pairs
+        if (threadIdx.x == 0) { pr_bound = 0; pr_over = 0; }
+        if (threadIdx.x < 3) pr_used[threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t part = 0;
+        for (uint32_t li = first + threadIdx.x; li < last; li += blockDim.x) {
+            const uint32_t rw = a.s.rwc[a.rg_list[li]];
+            part += (uint32_t)((int)hi16(rw) - (int)lo16(rw) + 1);
+        }
+        part = (uint32_t)wave_sum(part);
+        if (lane == 0 && part) atomicAdd(&pr_bound, part);
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            const uint64_t want = 3ull * pr_bound;
+            const uint32_t b = atomicAdd(&a.pcursor[threadIdx.x], (uint32_t)want);
+            pr_base[threadIdx.x] = b;
+            if ((uint64_t)b + want > a.pcap) {
+                atomicOr(a.perr, 1u);
+                pr_over = 1;
+            }
+        }
+        __syncthreads();
+        // a slice past the capacity is never written; the engine reports the overflow
+        if (pr_over) return;
+    } else {
+        for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) { L.p0[e] = 0; L.p1[e] = 0; L.p2[e] = 0; L.cnt[e] = 0; }
+        for (uint32_t w = threadIdx.x; w < MW; w += blockDim.x) L.mcnt[w] = 0;
+        __syncthreads();
+    }
+
     for (uint32_t li = first + wave; li < last; li += COV_WAVES) {
         const uint32_t r = __builtin_amdgcn_readfirstlane(a.rg_list[li]);
         const uint32_t rw = a.s.rwc[r];
         const int len = (int)hi16(rw) - (int)lo16(rw) + 1;
-        if (len <= 256) cov_read_wave(a, L, stage, r, rg, ncodes);
-        else if (lane == 0) cov_read_serial(a, L, r, rg, ncodes);
+        if (len <= 256) cov_read_wave<PAIRS>(a, L, pr, stage, r, rg, ncodes);
+        else if (lane == 0) cov_read_serial<PAIRS>(a, L, pr, r, rg, ncodes);
     }
     __syncthreads();
+    if constexpr (PAIRS) {
+        // unused slots of the slice sort last and are dropped by the first radix pass
+        const uint64_t want = 3ull * pr_bound;
+        for (int t = 0; t < 3; t++)
+            for (uint64_t i = pr_used[t] + threadIdx.x; i < want; i += blockDim.x) a.pkey[t][pr_base[t] + i] = PAIR_SENTINEL;
+        return;
+    }
 
     // flush the tile's entries into the resident tables
     for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) {
@@ -576,18 +718,33 @@ __global__ void __launch_bounds__(256) k_fill_u32c(uint32_t *p, uint32_t n, uint
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
 }
 
-// per-read-group read lists: counts -> offsets -> scatter
+// per-read-group read lists: counts -> offsets -> scatter.  A fixed grid of RG_BLOCKS workgroups
+// strides over the reads; each keeps its per-read-group counts in LDS and touches a global
+// counter once per read group.
+static constexpr uint32_t RG_BLOCKS = 512;
+
 __global__ void __launch_bounds__(256) k_rg_count(uint32_t n, const uint32_t *live, const uint8_t *status,
                                                   const uint32_t *read_group, uint32_t num_rg, uint32_t *rg_count,
                                                   uint32_t *errors)
 {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t nbad;
+    for (uint32_t g = threadIdx.x; g < 256; g += blockDim.x) cnt[g] = 0;
+    if (threadIdx.x == 0) nbad = 0;
+    __syncthreads();
     if (live) n = live[0];
-    const bool act = r < n && status[r] == RS_ACTIVE;
-    const uint32_t rg = act ? read_group[r] : 0;
-    const bool bad = act && rg >= num_rg;
-    if (__ballot(bad) && bad) atomicAdd(errors, 1u);
-    wave_append(rg_count, rg, act && !bad);
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t r = base + threadIdx.x;
+        const bool act = r < n && status[r] == RS_ACTIVE;
+        const uint32_t rg = act ? read_group[r] : 0;
+        const bool bad = act && rg >= num_rg;
+        if (bad) atomicAdd(&nbad, 1u);
+        wave_append(cnt, rg, act && !bad);
+    }
+    __syncthreads();
+    for (uint32_t g = threadIdx.x; g < num_rg; g += blockDim.x)
+        if (cnt[g]) atomicAdd(&rg_count[g], cnt[g]);
+    if (threadIdx.x == 0 && nbad) atomicAdd(errors, nbad);
 }
 
 __global__ void __launch_bounds__(256) k_rg_offsets(uint32_t num_rg, const uint32_t *rg_count, uint32_t *rg_offset,
@@ -610,13 +767,32 @@ __global__ void __launch_bounds__(256) k_rg_scatter(uint32_t n, const uint32_t *
                                                     const uint32_t *read_group, uint32_t num_rg,
                                                     const uint32_t *rg_offset, uint32_t *rg_cursor, uint32_t *rg_list)
 {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t cnt[256], first[256];
+    for (uint32_t g = threadIdx.x; g < 256; g += blockDim.x) cnt[g] = 0;
+    __syncthreads();
     if (live) n = live[0];
-    const bool act = r < n && status[r] == RS_ACTIVE;
-    const uint32_t rg = act ? read_group[r] : 0;
-    const bool ok = act && rg < num_rg;
-    const uint32_t slot = wave_append(rg_cursor, rg, ok);
-    if (ok) rg_list[rg_offset[rg] + slot] = r;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // this workgroup's reads per read group, then one reservation per read group
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t r = base + threadIdx.x;
+        const bool act = r < n && status[r] == RS_ACTIVE;
+        const uint32_t rg = act ? read_group[r] : 0;
+        wave_append(cnt, rg, act && rg < num_rg);
+    }
+    __syncthreads();
+    for (uint32_t g = threadIdx.x; g < num_rg; g += blockDim.x) {
+        first[g] = cnt[g] ? rg_offset[g] + atomicAdd(&rg_cursor[g], cnt[g]) : 0;
+        cnt[g] = 0;
+    }
+    __syncthreads();
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t r = base + threadIdx.x;
+        const bool act = r < n && status[r] == RS_ACTIVE;
+        const uint32_t rg = act ? read_group[r] : 0;
+        const bool ok = act && rg < num_rg;
+        const uint32_t slot = wave_append(cnt, rg, ok);
+        if (ok) rg_list[first[rg] + slot] = r;
+    }
 }
 
 } // namespace fpd
@@ -628,7 +804,7 @@ static int g_cov_lds_limit = -1;
 static void cov_layout(uint32_t max_read_len, uint32_t *ncodes, uint32_t *mcodes, size_t *bytes)
 {
     const int limit = g_cov_lds_limit > 0 ? g_cov_lds_limit : 64 * 1024;
-    const uint32_t budget = (uint32_t)limit - fpd::STAGE_BYTES;
+    const uint32_t budget = (uint32_t)limit - fpd::STAGE_BYTES - 256;   // 256: the kernel's static LDS
     uint32_t nc = 2 * max_read_len + 2, mc = 2 * max_read_len + 2;
     auto need = [&](uint32_t n, uint32_t m) {
         return (fpd::E_YID + 2 * n) * fpd::LDS_ENTRY_BYTES + 4 * ((m * 64 + 1) / 2);
@@ -650,13 +826,15 @@ extern "C" hipError_t fpd_cov_prepare()
     e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess) return e;
     g_cov_lds_limit = v > 0 ? v : 64 * 1024;
-    (void)hipFuncSetAttribute((const void *)fpd::k_cov, hipFuncAttributeMaxDynamicSharedMemorySize, g_cov_lds_limit);
+    (void)hipFuncSetAttribute((const void *)fpd::k_cov<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              g_cov_lds_limit);
     return hipSuccess;
 }
 
 extern "C" hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
                                      uint32_t frac_f64, uint32_t keep_frac, uint32_t *counters, uint32_t *rg_buf,
-                                     uint32_t max_read_len, hipStream_t st)
+                                     uint32_t max_read_len, uint32_t *const *pkey, double *const *pval,
+                                     uint32_t *pcursor, uint64_t pcap, hipStream_t st)
 {
     const uint32_t n = b->num_reads;
     if (n == 0) return hipSuccess;
@@ -665,17 +843,27 @@ extern "C" hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStat
     uint32_t *cnt = rg_buf, *cur = rg_buf + nrg, *off = rg_buf + 2 * nrg, *toff = rg_buf + 3 * nrg + 1;
     uint32_t *list = rg_buf + 4 * nrg + 2;
     hipLaunchKernelGGL(fpd::k_fill_u32c, dim3(1), dim3(256), 0, st, cnt, nrg, 0u);
-    hipLaunchKernelGGL(fpd::k_rg_count, dim3((n + 255) / 256), dim3(256), 0, st, n, b->live, s->status, b->read_group,
+    const uint32_t rg_blocks = std::min<uint32_t>((n + 255) / 256, fpd::RG_BLOCKS);
+    hipLaunchKernelGGL(fpd::k_rg_count, dim3(rg_blocks), dim3(256), 0, st, n, b->live, s->status, b->read_group,
                        nrg, cnt, counters + 4);
     hipLaunchKernelGGL(fpd::k_rg_offsets, dim3(1), dim3(64), 0, st, nrg, cnt, off, toff, cur);
-    hipLaunchKernelGGL(fpd::k_rg_scatter, dim3((n + 255) / 256), dim3(256), 0, st, n, b->live, s->status,
+    hipLaunchKernelGGL(fpd::k_rg_scatter, dim3(rg_blocks), dim3(256), 0, st, n, b->live, s->status,
                        b->read_group, nrg, off, cur, list);
     fpd::CovArgs a{};
     a.b = *b; a.s = *s; a.t = *t; a.frac_f64 = frac_f64; a.keep_frac = keep_frac;
-    size_t lds = 0;
-    cov_layout(max_read_len, &a.ncodes, &a.mcodes, &lds);
     a.rg_list = list; a.rg_offset = off; a.tile_offset = toff;
     const uint32_t blocks = (n + fpd::TILE - 1) / fpd::TILE + nrg;
-    hipLaunchKernelGGL(fpd::k_cov, dim3(blocks), dim3(64 * fpd::COV_WAVES), lds, st, a);
+    if (pkey) {
+        // pair mode (sort aggregation): pairs out instead of table updates; LDS holds the base stages
+        for (int i = 0; i < 3; i++) { a.pkey[i] = pkey[i]; a.pval[i] = pval[i]; }
+        a.pcursor = pcursor;
+        a.perr = counters + 6;
+        a.pcap = pcap;
+        hipLaunchKernelGGL(fpd::k_cov<true>, dim3(blocks), dim3(64 * fpd::COV_WAVES), fpd::STAGE_BYTES, st, a);
+        return hipGetLastError();
+    }
+    size_t lds = 0;
+    cov_layout(max_read_len, &a.ncodes, &a.mcodes, &lds);
+    hipLaunchKernelGGL(fpd::k_cov<false>, dim3(blocks), dim3(64 * fpd::COV_WAVES), lds, st, a);
     return hipGetLastError();
 }

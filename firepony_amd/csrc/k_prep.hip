@@ -1,5 +1,5 @@
 /*
- * K_prep: per-read preparation, one lane per read.
+ * K_prep: per-read preparation, one wave per 64 reads.
  *
  * Covers, for each read, what the reference spreads over ~40 launches
  * (device/read_filters.cu, snp_filter.cu, cigar.cu, baq.cu:89-204):
@@ -11,10 +11,13 @@
  * CIGAR events are never materialised: an EventCursor walks (op, offset)
  * pairs and yields each event's class, read coordinate and reference
  * coordinate (the values cigar_coordinates_expand would have stored,
- * cigar.cu:538-609).  All per-base output of a read is written by the lane
- * that owns the read, so there are no read-modify-write hazards between reads.
+ * cigar.cu:538-609).  The CIGAR walks run one lane per read; the per-base
+ * flags of the wave's reads are built in LDS (lanes over one read's bases at a
+ * time, coalesced loads) and written to HBM once, as whole words.
  */
 #include "fp_device.h"
+
+#include <algorithm>
 
 namespace fpd {
 
@@ -111,6 +114,7 @@ struct PrepArgs {
     GenomeDev g;
     ReadStateDev s;
     uint32_t *counters;   // [0] active reads, [1] filtered, [2] baq reads, [3] max read length
+    uint32_t lds_len;     // LDS flag row per read (multiple of 8); longer reads take the lane path
 };
 
 // filter_mapq + filter_if_any_set + filter_if_cigar_malformed (read_filters.cu:43-258),
@@ -145,29 +149,22 @@ __device__ bool passes_invalid_filters(const BatchDev &b, uint32_t r)
     return qlen == b.read_len[r];
 }
 
-__global__ void __launch_bounds__(256) k_prep(PrepArgs a)
-{
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    const BatchDev &b = a.b;
-    if (r >= live_reads(b)) return;
-    ReadStateDev &s = a.s;
+// per-read windows (cigar.cu:98-535); meaningful for reads that pass the filters
+struct Windows {
+    uint16_t rwx, rwy;   // read window, clipped
+    uint16_t nix, niy;   // read window without leading / trailing insertions
+    uint16_t fwx, fwy;   // reference window
+};
 
+// filter_invalid_reads + filter_malformed_reads (read_filters.cu:43-154); false: the read is filtered
+__device__ bool prep_filter(const PrepArgs &a, uint32_t r)
+{
+    const BatchDev &b = a.b;
     const uint32_t L = b.read_len[r];
-    const uint32_t rs = b.read_start[r];
-    const uint32_t qs = b.qual_start[r];
     const uint32_t *ops = b.cigars + b.cigar_start[r];
     const uint32_t nops = b.cigar_len[r];
     const uint16_t flags = b.flags[r];
     const uint32_t astart = b.alignment_start[r];
-    const uint32_t padded = (L + 7) & ~7u;
-
-    // defined initial state (per read)
-    s.rwc[r] = 0; s.noins[r] = 0; s.refw[r] = 0; s.num_errors[r] = 0;
-    s.hmmw[r] = 0; s.bandwidth[r] = 0; s.hqw[r] = 0;
-    for (uint32_t i = 0; i < padded; i++) s.base_flags[rs + i] = 0;
-    if (L > 0) atomicMax(&a.counters[3], L);
-
-    // ---- filter_invalid_reads
     bool ok = passes_invalid_filters(b, r);
 
     // ---- alignment window (snp_filter.cu:44-147): offset of the last aligned base
@@ -204,11 +201,16 @@ __global__ void __launch_bounds__(256) k_prep(PrepArgs a)
         if (b.qual_len[r] != L) ok = false;
         if (L == 0) ok = false;
     }
-    if (!ok) {
-        s.status[r] = RS_FILTERED;
-        atomicAdd(&a.counters[1], 1u);
-        return;
-    }
+    return ok;
+}
+
+__device__ Windows prep_windows(const BatchDev &b, uint32_t r)
+{
+    const uint32_t L = b.read_len[r];
+    const uint32_t *ops = b.cigars + b.cigar_start[r];
+    const uint32_t nops = b.cigar_len[r];
+    const uint16_t flags = b.flags[r];
+    const uint32_t astart = b.alignment_start[r];
 
     // ---- read window + adapter clipping (cigar.cu:98-326)
     uint16_t rwx = 0, rwy = (uint16_t)(L - 1);
@@ -342,13 +344,23 @@ __global__ void __launch_bounds__(256) k_prep(PrepArgs a)
             else fwy = cur.fc();
         }
     }
-    s.rwc[r] = pack2(rwx, rwy);
-    s.noins[r] = pack2(nix, niy);
-    s.refw[r] = pack2(fwx, fwy);
+    return Windows{rwx, rwy, nix, niy, fwx, fwy};
+}
 
-    // ---- error vectors (cigar.cu:612-733)
+// error vectors (cigar.cu:612-733) of one read by one lane: orf(read offset, flag) ORs a flag in;
+// returns num_errors
+template <class OrFlag>
+__device__ uint16_t error_walk_lane(const PrepArgs &a, uint32_t r, const Windows &w, OrFlag orf)
+{
+    const BatchDev &b = a.b;
+    const uint32_t L = b.read_len[r];
+    const uint32_t rs = b.read_start[r];
+    const uint32_t *ops = b.cigars + b.cigar_start[r];
+    const uint32_t nops = b.cigar_len[r];
+    const uint32_t astart = b.alignment_start[r];
     const uint32_t chrom = b.chromosome[r];
-    const bool neg = (flags & F_REVERSE) != 0;
+    const bool neg = (b.flags[r] & F_REVERSE) != 0;
+    const uint16_t rwx = w.rwx, rwy = w.rwy, fwx = w.fwx, fwy = w.fwy;
     uint16_t nerr = 0;
     {
         uint16_t bp = 0;      // current_bp_idx
@@ -361,7 +373,7 @@ __global__ void __launch_bounds__(256) k_prep(PrepArgs a)
                     const uint8_t rb = read_base(b.reads, (uint64_t)rs + bp);
                     const uint8_t fb = contig_base(a.g, chrom, astart + (uint16_t)(fo + k));
                     if (fb != rb) {
-                        s.base_flags[rs + bp] |= BF_SNP;
+                        orf(bp, BF_SNP);
                         if (bp >= rwx && bp <= rwy) nerr++;
                     }
                 }
@@ -370,17 +382,17 @@ __global__ void __launch_bounds__(256) k_prep(PrepArgs a)
                     bp = ro;
                     if (bp >= rwx && bp <= rwy) {
                         const int off = neg ? (int)bp + (int)l : (int)bp - 1;
-                        if (off >= 0 && off <= rwy) { s.base_flags[rs + off] |= BF_INS; nerr++; }
+                        if (off >= 0 && off <= rwy) { orf((uint32_t)off, BF_INS); nerr++; }
                     }
                 }
             } else if (op == OP_D) {
                 if (l) {
                     const uint16_t cri = fo;
                     if (cri >= fwx && cri <= fwy) {
-                        if (!neg) { s.base_flags[rs + bp] |= BF_DEL; nerr++; }
+                        if (!neg) { orf(bp, BF_DEL); nerr++; }
                         else {
                             const uint16_t off = (uint16_t)(bp + 1);
-                            if (off < L) { s.base_flags[rs + off] |= BF_DEL; nerr++; }
+                            if (off < L) { orf(off, BF_DEL); nerr++; }
                         }
                     }
                 }
@@ -389,15 +401,20 @@ __global__ void __launch_bounds__(256) k_prep(PrepArgs a)
             if (op_refs(op)) fo = (uint16_t)(fo + l);
         }
     }
-    s.num_errors[r] = nerr;
+    return nerr;
+}
 
-    // ---- per-base filters (read_filters.cu:376-438)
-    for (uint32_t i = 0; i < L; i++) {
-        const uint8_t bpv = read_base(b.reads, (uint64_t)rs + i);
-        const bool regular = bpv == B_A || bpv == B_C || bpv == B_G || bpv == B_T;
-        if (!regular || b.quals[qs + i] < 6) s.base_flags[rs + i] |= BF_MASKED;
-    }
-
+// known-site masking (snp_filter.cu:162-395): mask(d0, d1) masks read offsets d0..d1 (inside the
+// read window)
+template <class Mask>
+__device__ void known_sites(const PrepArgs &a, uint32_t r, const Windows &w, Mask mask)
+{
+    const BatchDev &b = a.b;
+    const uint32_t *ops = b.cigars + b.cigar_start[r];
+    const uint32_t nops = b.cigar_len[r];
+    const uint32_t astart = b.alignment_start[r];
+    const uint32_t chrom = b.chromosome[r];
+    const uint16_t rwx = w.rwx, rwy = w.rwy, fwx = w.fwx, fwy = w.fwy;
     // ---- known sites (snp_filter.cu:162-395)
     if (chrom < a.g.num_variant_contigs) {
         const uint64_t f0 = a.g.feature_offset[chrom];
@@ -481,12 +498,24 @@ __global__ void __launch_bounds__(256) k_prep(PrepArgs a)
                     if ((r0 < rwx && r1 < rwx) || (r0 > rwy && r1 > rwy)) continue;
                     r0 = r0 > rwx ? r0 : rwx; r0 = r0 < rwy ? r0 : rwy;
                     r1 = r1 > rwx ? r1 : rwx; r1 = r1 < rwy ? r1 : rwy;
-                    for (uint32_t d = r0; d <= r1; d++) s.base_flags[rs + d] |= BF_MASKED;
+                    mask(r0, r1);
                 }
             }
         }
     }
 
+}
+
+// HMM reference window + band width (baq.cu:89-187); RS_ACTIVE, or RS_BAQ_DROPPED when the window
+// runs past the contig
+__device__ uint8_t hmm_window(const PrepArgs &a, uint32_t r, const Windows &w, uint16_t nerr)
+{
+    const BatchDev &b = a.b;
+    const ReadStateDev &s = a.s;
+    const uint32_t L = b.read_len[r];
+    const uint32_t astart = b.alignment_start[r];
+    const uint32_t chrom = b.chromosome[r];
+    const uint16_t rwx = w.rwx, rwy = w.rwy, nix = w.nix, niy = w.niy, fwx = w.fwx, fwy = w.fwy;
     // ---- HMM window + band width (baq.cu:89-187)
     {
         const bool needs = nerr != 0;
@@ -501,8 +530,8 @@ __global__ void __launch_bounds__(256) k_prep(PrepArgs a)
         if (hx < 0 && astart < ahx) lshift = -1;
         if (needs && (uint32_t)(astart + (int)hy + lshift) >= a.g.length[chrom]) {
             s.hmmw[r] = pack2((uint16_t)-1, (uint16_t)-1);
-            s.status[r] = RS_BAQ_DROPPED;
-            return;
+            s.bandwidth[r] = 0;
+            return RS_BAQ_DROPPED;
         }
         s.hmmw[r] = pack2((uint16_t)hx, (uint16_t)hy);
         const int refLen = (int)hy - (int)hx + 1;
@@ -515,22 +544,217 @@ __global__ void __launch_bounds__(256) k_prep(PrepArgs a)
         if (bw < ad) bw = (uint16_t)ad;
         s.bandwidth[r] = bw;
     }
-    s.status[r] = RS_ACTIVE;
-    atomicAdd(&a.counters[0], 1u);
-    if (nerr != 0) atomicAdd(&a.counters[2], 1u);
+    return RS_ACTIVE;
+}
+
+// One read by one lane, every per-base flag read-modify-written in HBM: the path of waves that hold
+// a read longer than the LDS rows.  Returns the read's status.
+__device__ uint8_t prep_read_lane(const PrepArgs &a, uint32_t r, uint16_t &nerr)
+{
+    const BatchDev &b = a.b;
+    const ReadStateDev &s = a.s;
+    const uint32_t L = b.read_len[r];
+    const uint32_t rs = b.read_start[r];
+    const uint32_t qs = b.qual_start[r];
+    const uint32_t padded = (L + 7) & ~7u;
+    uint8_t *fl = s.base_flags + rs;
+    s.rwc[r] = 0; s.noins[r] = 0; s.refw[r] = 0; s.num_errors[r] = 0;
+    s.hmmw[r] = 0; s.bandwidth[r] = 0; s.hqw[r] = 0;
+    for (uint32_t i = 0; i < padded; i++) fl[i] = 0;
+    nerr = 0;
+    if (!prep_filter(a, r)) { s.status[r] = RS_FILTERED; return RS_FILTERED; }
+    const Windows w = prep_windows(b, r);
+    s.rwc[r] = pack2(w.rwx, w.rwy);
+    s.noins[r] = pack2(w.nix, w.niy);
+    s.refw[r] = pack2(w.fwx, w.fwy);
+    nerr = error_walk_lane(a, r, w, [&](uint32_t i, uint8_t v) { fl[i] |= v; });
+    s.num_errors[r] = nerr;
+    // per-base filters (read_filters.cu:376-438)
+    for (uint32_t i = 0; i < L; i++) {
+        const uint8_t bpv = read_base(b.reads, (uint64_t)rs + i);
+        const bool regular = bpv == B_A || bpv == B_C || bpv == B_G || bpv == B_T;
+        if (!regular || b.quals[qs + i] < 6) fl[i] |= BF_MASKED;
+    }
+    known_sites(a, r, w, [&](uint32_t d0, uint32_t d1) { for (uint32_t d = d0; d <= d1; d++) fl[d] |= BF_MASKED; });
+    const uint8_t st = hmm_window(a, r, w, nerr);
+    s.status[r] = st;
+    return st;
+}
+
+__device__ inline uint32_t rl(uint32_t v, int j) { return (uint32_t)__builtin_amdgcn_readlane((int)v, j); }
+
+// One wave per 64 reads.  The CIGAR walks (filters, windows, known sites, HMM window) run one lane
+// per read; the per-base work (base masks, SNP / insertion / deletion flags) runs one read at a time
+// across the wave, lanes over bases, so its loads are coalesced.  Flags are built in LDS rows of
+// lds_len bytes per read and leave as whole 8-byte words; nothing is read-modify-written in HBM.
+// A wave that holds a read longer than lds_len takes prep_read_lane for all its reads.
+__global__ void __launch_bounds__(64) k_prep(PrepArgs a)
+{
+    extern __shared__ __align__(16) uint8_t prep_rows[];
+    const BatchDev &b = a.b;
+    const ReadStateDev &s = a.s;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t n = live_reads(b);
+    if (blockIdx.x * 64u >= n) return;
+    const uint32_t r = blockIdx.x * 64u + lane;
+    const bool valid = r < n;
+    const uint32_t L = valid ? b.read_len[r] : 0u;
+    const uint32_t rs = valid ? b.read_start[r] : 0u;
+    const uint32_t RL = a.lds_len;
+
+    uint8_t st = RS_FILTERED;
+    uint16_t nerr = 0;
+    if (!__all(L <= RL)) {
+        if (valid) st = prep_read_lane(a, r, nerr);
+    } else {
+        const bool ok = valid && prep_filter(a, r);
+        Windows w{};
+        if (ok) w = prep_windows(b, r);
+        const uint32_t qs = valid ? b.qual_start[r] : 0u;
+        const uint32_t cs = valid ? b.cigar_start[r] : 0u;
+        const uint32_t nops = valid ? b.cigar_len[r] : 0u;
+        const uint32_t astart = valid ? b.alignment_start[r] : 0u;
+        const uint32_t chrom = valid ? b.chromosome[r] : 0u;
+        const uint32_t neg = valid ? ((b.flags[r] & F_REVERSE) != 0) : 0u;
+        const uint64_t vmask = __ballot(valid), okmask = __ballot(ok);
+
+        // per-base pass, one read at a time (read_filters.cu:376-438, cigar.cu:612-733)
+        for (uint64_t m = vmask; m; m &= m - 1) {
+            const int j = __builtin_ctzll(m);
+            const uint32_t Lj = rl(L, j), pj = (Lj + 7) & ~7u;
+            uint8_t *row = prep_rows + (uint32_t)j * RL;
+            if (!((okmask >> j) & 1)) {
+                for (uint32_t i = lane; i < pj; i += 64) row[i] = 0;
+                continue;
+            }
+            const uint32_t rsj = rl(rs, j), qsj = rl(qs, j), astj = rl(astart, j), chj = rl(chrom, j);
+            const uint32_t nopsj = rl(nops, j);
+            const bool negj = rl(neg, j) != 0;
+            const uint32_t *opsj = b.cigars + rl(cs, j);
+            const uint32_t rwxj = rl(w.rwx, j), rwyj = rl(w.rwy, j), fwxj = rl(w.fwx, j), fwyj = rl(w.fwy, j);
+            uint32_t cnt = 0;
+            for (uint32_t i0 = 0; i0 < pj; i0 += 64) {
+                const uint32_t i = i0 + lane;
+                uint8_t f = 0;
+                uint32_t rb = 0;
+                if (i < Lj) {
+                    rb = (b.reads[(rsj + i) >> 3] >> (((rsj + i) & 7) * 4)) & 15;
+                    const bool regular = rb == B_A || rb == B_C || rb == B_G || rb == B_T;
+                    if (!regular || b.quals[qsj + i] < 6) f |= BF_MASKED;
+                }
+                // the error walk with the lanes' bases side by side: current_bp_idx (bp) and the
+                // read / reference offsets are the same in every lane
+                uint16_t ro = 0, fo = 0, bp = 0;
+                bool snp_in = false;
+                for (uint32_t oi = 0; oi < nopsj; oi++) {
+                    const uint32_t wop = opsj[oi];
+                    const uint32_t op = op_code(wop), l = op_len(wop);
+                    if (op == OP_M) {
+                        if (i >= ro && i < (uint32_t)ro + l) {
+                            const uint8_t fb = contig_base(a.g, chj, astj + (uint16_t)(fo + (i - ro)));
+                            if (fb != rb) {
+                                f |= BF_SNP;
+                                snp_in = i >= rwxj && i <= rwyj;
+                            }
+                        }
+                        if (l) bp = (uint16_t)(ro + l - 1);
+                    } else if (op == OP_I) {
+                        if (l) {
+                            bp = ro;
+                            if (bp >= rwxj && bp <= rwyj) {
+                                const int off = negj ? (int)bp + (int)l : (int)bp - 1;
+                                if (off >= 0 && off <= (int)rwyj) {
+                                    if ((int)i == off) f |= BF_INS;
+                                    cnt += i0 == 0;
+                                }
+                            }
+                        }
+                    } else if (op == OP_D) {
+                        if (l) {
+                            const uint16_t cri = fo;
+                            if (cri >= fwxj && cri <= fwyj) {
+                                if (!negj) {
+                                    if (i == bp) f |= BF_DEL;
+                                    cnt += i0 == 0;
+                                } else {
+                                    const uint16_t off = (uint16_t)(bp + 1);
+                                    if (off < Lj) {
+                                        if (i == off) f |= BF_DEL;
+                                        cnt += i0 == 0;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    if (op_reads(op)) ro = (uint16_t)(ro + l);
+                    if (op_refs(op)) fo = (uint16_t)(fo + l);
+                }
+                cnt += (uint32_t)__popcll(__ballot(snp_in));
+                if (i < pj) row[i] = f;
+            }
+            if ((int)lane == j) nerr = (uint16_t)cnt;
+        }
+        __syncthreads();
+
+        if (ok) {
+            uint8_t *own = prep_rows + lane * RL;
+            known_sites(a, r, w, [&](uint32_t d0, uint32_t d1) { for (uint32_t d = d0; d <= d1; d++) own[d] |= BF_MASKED; });
+            st = hmm_window(a, r, w, nerr);
+        }
+        if (valid) {
+            s.rwc[r] = ok ? pack2(w.rwx, w.rwy) : 0u;
+            s.noins[r] = ok ? pack2(w.nix, w.niy) : 0u;
+            s.refw[r] = ok ? pack2(w.fwx, w.fwy) : 0u;
+            s.num_errors[r] = nerr;
+            s.hqw[r] = 0;
+            s.status[r] = st;
+            if (!ok) { s.hmmw[r] = 0; s.bandwidth[r] = 0; }
+        }
+        __syncthreads();
+
+        // flags out: two reads per step, 32 lanes x 8 bytes each
+        for (uint32_t jj = 0; jj < 64; jj += 2) {
+            const uint32_t j = jj + (lane >> 5), k = (lane & 31) * 8;
+            const uint32_t pj = ((uint32_t)__shfl((int)L, (int)j) + 7) & ~7u;
+            const uint32_t rsj = (uint32_t)__shfl((int)rs, (int)j);
+            if (k < pj) {
+                const uint64_t v = *(const uint64_t *)(prep_rows + j * RL + k);
+                uint8_t *dst = s.base_flags + rsj + k;
+                if ((rsj & 7) == 0) *(uint64_t *)dst = v;
+                else for (int t = 0; t < 8; t++) dst[t] = (uint8_t)(v >> (8 * t));
+            }
+        }
+    }
+
+    // counters: wave totals, one atomic each
+    uint32_t mx = L;
+    for (int o = 32; o > 0; o >>= 1) { const uint32_t v = (uint32_t)__shfl_xor((int)mx, o); mx = v > mx ? v : mx; }
+    const uint64_t m_act = __ballot(valid && st == RS_ACTIVE);
+    const uint64_t m_filt = __ballot(valid && st == RS_FILTERED);
+    const uint64_t m_baq = __ballot(valid && st == RS_ACTIVE && nerr != 0);
+    if (lane == 0) {
+        if (m_act) atomicAdd(&a.counters[0], (uint32_t)__popcll(m_act));
+        if (m_filt) atomicAdd(&a.counters[1], (uint32_t)__popcll(m_filt));
+        if (m_baq) atomicAdd(&a.counters[2], (uint32_t)__popcll(m_baq));
+        if (mx) atomicMax(&a.counters[3], mx);
+    }
 }
 
 } // namespace fpd
 
 // host launcher
 extern "C" hipError_t fpd_launch_prep(const fpd::BatchDev *b, const fpd::GenomeDev *g,
-                                      const fpd::ReadStateDev *s, uint32_t *counters, hipStream_t st)
+                                      const fpd::ReadStateDev *s, uint32_t *counters, uint32_t max_read_len,
+                                      hipStream_t st)
 {
     fpd::PrepArgs a{};
     a.b = *b; a.g = *g; a.s = *s; a.counters = counters;
     const uint32_t n = b->num_reads;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(fpd::k_prep, dim3((n + 255) / 256), dim3(256), 0, st, a);
+    // flag rows sized for the longest expected read (256 when unknown)
+    uint32_t rows = max_read_len ? std::min<uint32_t>(max_read_len, 256u) : 256u;
+    a.lds_len = (rows + 7) & ~7u;
+    hipLaunchKernelGGL(fpd::k_prep, dim3((n + 63) / 64), dim3(64), (size_t)a.lds_len * 64, st, a);
     return hipGetLastError();
 }
 

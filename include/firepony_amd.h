@@ -69,7 +69,8 @@ typedef struct fp_options {
     uint32_t disable_output_rounding;/* report floats as %.64f (table_formatter.cu:102) */
     uint32_t baq_waves_per_simd;     /* BAQ HMM register budget: 1 to 4 waves per SIMD;
                                         0 = chosen by the engine's autotuner            */
-    uint32_t sort_digit_bits;        /* FP_AGG_SORT radix digit width 4..8; 0 = autotune */
+    uint32_t sort_digit_bits;        /* FP_AGG_SORT radix digit width 4..8; 0 = tuned by
+                                        the batched engine (8 on the plain engine)      */
     uint32_t max_read_length;        /* upper bound of read lengths; lets batches run with
                                         no host round trip; 0 = measured per batch      */
     uint32_t aggregation;            /* fp_aggregation                                 */
@@ -217,7 +218,8 @@ typedef struct fp_stats {
     uint64_t cov_launches;
     uint64_t baq_cells;        /* band cells evaluated by the BAQ HMM (algorithmic work) */
     uint64_t baq_waves_per_simd; /* BAQ band-kernel variant in use (autotuned or set)    */
-    uint64_t reserved[4];
+    uint64_t sort_digit_bits;    /* FP_AGG_SORT radix digit width in use; 0 = direct    */
+    uint64_t reserved[3];
 } fp_stats;
 
 /* Per-batch intermediates of the last processed batch, for stage-by-stage

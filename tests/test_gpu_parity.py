@@ -141,6 +141,13 @@ def test_mixed_lengths_f64(tmp_path, nostage):
     run_parity(tmp_path, g, rs, n, 1500, frac_error_f64=1, disable_baq_staging=nostage)
 
 
+def test_long_reads(tmp_path):
+    say("parity: reads longer than 256 bases next to short ones (per-lane flag path of the prep kernel)")
+    g, rs, n = synth.small_workload(seed=13, num_reads=1500, edge=0.1, lengths=(90, 300, 420), read_groups=2,
+                                    contigs=(200000,), sites=400, clip=0.3, indel=0.2)
+    run_parity(tmp_path, g, rs, n, 750)
+
+
 def test_batch_size_invariance(tmp_path):
     say("invariance: one batch vs many batches give identical tables")
     g, rs, n = synth.small_workload(seed=5, num_reads=3000, edge=0.1)
