@@ -91,7 +91,7 @@ class DeviceBatch:
 def load_traffic():
     """Per-dispatch HBM bytes by kernel from the committed counter summary
     (lines 'traffic <kernel> <fetch KiB> <write KiB> <fetch scale> <write scale>')."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_config1_batch500k.txt")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_config1_batched.txt")
     out = {}
     if not os.path.exists(path):
         return out
@@ -257,6 +257,38 @@ def served(args, wpe, torch):
     return res
 
 
+def sort_pass(args, genome, dev, wpe, batched_opts, reads, torch):
+    """The batched engine with the sort aggregation (covariate pairs -> LDS-tiled radix sort ->
+    segmented reduce_by_key): reads/s with the digit width autotuned during the warm-up pass, then a
+    per-stage pass at that width for the sort/reduce time and its HBM rate."""
+    sopts = dict(batched_opts, aggregation=abi.FP_AGG_SORT)
+    dt, st = timed_pass(genome, dev, max(args.warmup, 1), torch, baq_waves_per_simd=wpe, **sopts)
+    bits = int(st.sort_digit_bits)
+    _, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=wpe, profile_stages=1,
+                        **dict(sopts, sort_digit_bits=bits))
+    batches = max(int(pst.cov_launches), 1)
+    sort_s = pst.covariates_sort_ms / 1e3
+    # algorithmic bytes: first radix pass reads every emitted slot's key twice and value once and
+    # writes the real pairs; each further pass reads key (twice) and value and writes both; the
+    # reduce reads the sorted pairs once.  Passes are averaged over the three tables.
+    slots, pairs = float(pst.sort_slots), float(pst.sort_pairs)
+    passes = float(pst.sort_passes) / (3.0 * batches)
+    alg = slots * 16 + pairs * 12 + pairs * 28 * max(passes - 1, 0) + pairs * 12
+    traffic = load_traffic().get("sort")
+    res = {"value": reads / dt, "unit": "reads/s", "digit_bits": bits, "sort_reduce_ms": pst.covariates_sort_ms,
+           "emit_ms": pst.cov_kernel_ms, "pairs": int(pairs), "slots": int(slots), "avg_passes": passes,
+           "hbm": {"achieved": alg / sort_s / 1e9 if sort_s > 0 else None, "peak": HBM_PEAK_GBS,
+                   "unit": "GB/s", "frac": (alg / sort_s / 1e9 / HBM_PEAK_GBS) if sort_s > 0 else None,
+                   "bytes": "algorithmic (pair reads and writes of the radix passes and the reduce)"}}
+    if traffic and sort_s > 0:
+        meas = traffic * batches / sort_s / 1e9
+        res["hbm"]["measured"] = meas
+        res["hbm"]["measured_frac"] = meas / HBM_PEAK_GBS
+    log(f"sort aggregation: {reads / dt:.4g} reads/s, digits {bits}, sort+reduce {pst.covariates_sort_ms:.1f} ms, "
+        f"{res['hbm']['achieved']} GB/s algorithmic")
+    return res
+
+
 def kernel_ms(st):
     return {"prep": st.prep_kernel_ms, "baq_hmm": st.baq_hmm_kernel_ms, "cov": st.cov_kernel_ms,
             "postprocess": st.postprocessing_ms}
@@ -272,6 +304,9 @@ def main():
     ap.add_argument("--profile-only", action="store_true",
                     help="only the per-kernel pass of the batched engine (eager stages, no graphs): the run "
                          "the committed rocprofv3 summaries profile")
+    ap.add_argument("--aggregation", choices=("direct", "sort"), default="direct",
+                    help="covariate aggregation of the profile-only pass")
+    ap.add_argument("--no-sort", action="store_true", help="skip the sort-aggregation pass")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-reads", type=int, default=2000000)
     ap.add_argument("--no-served", action="store_true")
@@ -312,12 +347,14 @@ def main():
     bases = sum(d.num_bases for d in dev)
     batched_opts = dict(engine_kind=abi.FP_ENGINE_BATCHED, max_read_length=101)
     if args.profile_only:
-        pdt, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=args.wpe or 1, profile_stages=1,
-                              **batched_opts)
+        popts = dict(batched_opts)
+        if args.aggregation == "sort":
+            popts.update(aggregation=abi.FP_AGG_SORT, sort_digit_bits=8)
+        pdt, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=args.wpe or 1, profile_stages=1, **popts)
         log(f"profile: {reads} reads in {pdt:.3f} s; prep {pst.prep_kernel_ms:.1f} ms, baq_hmm "
             f"{pst.baq_hmm_kernel_ms:.1f} ms ({pst.baq_hmm_launches} launches), cov {pst.cov_kernel_ms:.1f} ms")
-        print(json.dumps({"profile_only": True, "reads": reads, "seconds": pdt, "kernel_ms": kernel_ms(pst)}),
-              flush=True)
+        print(json.dumps({"profile_only": True, "reads": reads, "seconds": pdt, "kernel_ms": kernel_ms(pst),
+                          "sort_ms": pst.covariates_sort_ms, "sort_pairs": pst.sort_pairs}), flush=True)
         return
 
     out_plain = None
@@ -337,9 +374,6 @@ def main():
     log(f"profile: prep {pst.prep_kernel_ms:.1f} ms, baq_hmm {pst.baq_hmm_kernel_ms:.1f} ms "
         f"({pst.baq_hmm_launches} launches), cov {pst.cov_kernel_ms:.1f} ms; baq reads {pst.baq_reads}, "
         f"cells {pst.baq_cells}")
-    del dev
-    torch.cuda.empty_cache()
-
     out = {
         "metric": "reads/s BQSR+BAQ at 1 GPU, batched vs plain; p50/p99 job latency",
         "value": value,
@@ -362,6 +396,10 @@ def main():
     if out_plain:
         out["plain"] = out_plain
         out["batched_over_plain"] = value / out_plain["value"]
+    if not args.no_sort:
+        out["sort_aggregation"] = sort_pass(args, genome, dev, wpe, batched_opts, reads, torch)
+    del dev
+    torch.cuda.empty_cache()
 
     if not args.no_mixed:
         gs2, rs2, total2 = synth.named_spec(2)

@@ -157,7 +157,9 @@ class FpStats(C.Structure):
         ("baq_cells", C.c_uint64),
         ("baq_waves_per_simd", C.c_uint64),
         ("sort_digit_bits", C.c_uint64),
-        ("reserved", C.c_uint64 * 3),
+        ("sort_slots", C.c_uint64),
+        ("sort_pairs", C.c_uint64),
+        ("sort_passes", C.c_uint64),
     ]
 
 

@@ -424,7 +424,7 @@ fp_status ensure_pairs(fp_engine *e, uint64_t cap)
     }
     HIP_TRY(e->tk.ensure(4 * cap + 64));
     HIP_TRY(e->tv.ensure(8 * cap + 64));
-    HIP_TRY(e->shist.ensure(4ull * 256 * ((cap + 4095) / 4096) + 64));
+    HIP_TRY(e->shist.ensure(4ull * 256 * 2048 + 64));   // radix x the sort kernels' grid (at most 2048)
     HIP_TRY(e->stot.ensure(4 * 256));
     HIP_TRY(e->sctr.ensure(64));
     e->pair_cap = cap;
@@ -457,6 +457,19 @@ fp_status launch_cov_stage(fp_engine *e, const fpd::BatchDev &b, const Plan &p, 
                                 sc + 4 + t, e->shist.as<uint32_t>(), e->stot.as<uint32_t>(),
                                 sort_key_bits(e->num_rg, t), (uint32_t)sort_bits, (uint32_t)e->tab_size[t],
                                 (unsigned long long *)e->tables.count[t], parts, st));
+    }
+    return FP_OK;
+}
+
+// pair counts of the last sort-aggregated batch (its stream work has completed)
+fp_status collect_sort_counts(fp_engine *e, int sort_bits)
+{
+    uint32_t c[8];
+    HIP_TRY(hipMemcpy(c, e->sctr.p, sizeof c, hipMemcpyDeviceToHost));
+    for (int t = 0; t < 3; t++) {
+        e->stats.sort_slots += c[t];
+        e->stats.sort_pairs += c[4 + t];
+        e->stats.sort_passes += (sort_key_bits(e->num_rg, t) + sort_bits - 1) / sort_bits;
     }
     return FP_OK;
 }
@@ -554,6 +567,8 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     if (sorting) {
         e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[10]);
         e->stats.covariates_sort_ms += elapsed(e->ev[10], e->ev[3]);
+        rc = collect_sort_counts(e, e->sort_bits);
+        if (rc != FP_OK) return rc;
     } else {
         e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[3]);
     }
@@ -754,6 +769,8 @@ fp_status batched_flush(fp_engine *e)
         if (e->opts.aggregation == FP_AGG_SORT) {
             e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[10]);
             e->stats.covariates_sort_ms += elapsed(e->ev[10], e->ev[3]);
+            rc = collect_sort_counts(e, bits);
+            if (rc != FP_OK) return rc;
         } else {
             e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[3]);
         }

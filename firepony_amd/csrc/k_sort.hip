@@ -7,20 +7,23 @@
  *   per table an LSD radix sort of the pairs with D-bit digits (D = 4..8) ->
  *   a segmented reduce_by_key of the sorted runs into the resident direct-indexed tables.
  *
- * One radix pass is k_sort_hist (per-tile digit counts, digit-major) -> k_sort_totals +
- * k_sort_offsets (exclusive scan = each tile's global output offset per digit) -> k_sort_scatter.
- * The scatter ranks a tile's elements stably in LDS: equal digits inside a wavefront are matched with
- * D ballots and counted by per-wave running digit counters, the tile is reordered by digit in LDS,
- * and each digit's run is written to its global offset, so consecutive lanes write consecutive
- * addresses.  The first pass drops the sentinel keys of unused pair slots.  Element counts stay on
- * the device and every kernel strides a fixed grid over the tiles that the count implies, so the
- * whole chain can be captured in a graph.
+ * Each workgroup of a fixed grid owns one contiguous range of the elements (a whole number of
+ * 2048-element tiles).  One radix pass is k_sort_hist (the range's digit counts, digit-major, one
+ * row of the grid per digit) -> k_sort_totals + k_sort_offsets (exclusive scan = each range's
+ * global output offset per digit) -> k_sort_scatter.  The scatter walks its range tile by tile and
+ * ranks a tile's elements stably in LDS: equal digits inside a wavefront are matched with D ballots
+ * and counted by per-wave running digit counters, the tile is reordered by digit in LDS, each
+ * digit's run is written at the range's running offset for that digit (consecutive lanes write
+ * consecutive addresses), and the offsets move on by the tile's counts.  The first pass drops the
+ * sentinel keys of unused pair slots.  Element counts stay on the device and the kernels derive
+ * their ranges from them, so the whole chain can be captured in a graph.
  *
- * The reduce walks the sorted pairs in wavefront chunks of 2048: a segmented scan across the lanes
- * (head = key change) sums each run's count and exact fixed-point value (three 27-bit limbs, as the
- * direct path keeps them), and the last element of each run adds the run total to the table with
- * non-returning atomics; a run that spans chunks adds once per chunk.  Sums are exact, so the tables
- * equal the direct path's bit for bit whatever the digit width.
+ * The reduce gives each lane 32 consecutive sorted pairs: the lane sums its runs sequentially
+ * (count and exact fixed-point value as three 27-bit limbs, as the direct path keeps them) and adds
+ * the runs that start and end inside its slice to the table at once; the runs crossing lane
+ * boundaries are joined by one segmented scan across the wavefront per 2048 pairs, and each joined
+ * run is added where it ends (non-returning atomics; a run that spans chunks adds once per chunk).
+ * Sums are exact, so the tables equal the direct path's bit for bit whatever the digit width.
  */
 #include "fp_device.h"
 
@@ -29,11 +32,11 @@
 namespace fpd {
 
 static constexpr uint32_t SORT_THREADS = 256;
-static constexpr uint32_t SORT_ITEMS = 16;
-static constexpr uint32_t SORT_TILE = SORT_THREADS * SORT_ITEMS;   // 4096 elements
+static constexpr uint32_t SORT_ITEMS = 8;
+static constexpr uint32_t SORT_TILE = SORT_THREADS * SORT_ITEMS;   // 2048 elements
 static constexpr uint32_t SORT_SENTINEL = 0xffffffffu;
 static constexpr uint32_t SORT_GRID = 2048;                          // workgroups of hist / scatter
-static constexpr uint32_t RED_STEPS = 32;                            // 64 x 32 elements per reduce chunk
+static constexpr uint32_t RED_LANE = 32;                             // consecutive pairs per reduce lane
 
 struct SortPass {
     const uint32_t *kin;
@@ -42,10 +45,9 @@ struct SortPass {
     double *vout;
     const uint32_t *n_in;     // input elements (device)
     uint32_t *n_out;          // kept elements, written by the offsets kernel (first pass), or null
-    uint32_t *hist;           // [radix][tiles_cap]: tile counts, then tile offsets
+    uint32_t *hist;           // [radix][grid]: range counts, then range offsets
     uint32_t *totals;         // [radix]
     uint64_t n_cap;           // capacity of the pair arrays (elements)
-    uint32_t tiles_cap;
     uint32_t shift, bits;
     uint32_t drop;            // drop sentinel keys (first pass)
 };
@@ -54,6 +56,16 @@ __device__ inline uint32_t pass_n(const SortPass &p)
 {
     const uint64_t n = *p.n_in;
     return (uint32_t)(n < p.n_cap ? n : p.n_cap);
+}
+
+// the contiguous element range [lo, hi) of workgroup b (whole tiles)
+__device__ inline void block_range(uint32_t n, uint32_t b, uint32_t grid, uint32_t &lo, uint32_t &hi)
+{
+    const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    const uint32_t per = (tiles + grid - 1) / grid;
+    const uint64_t l = (uint64_t)b * per * SORT_TILE, h = l + (uint64_t)per * SORT_TILE;
+    lo = (uint32_t)(l < n ? l : n);
+    hi = (uint32_t)(h < n ? h : n);
 }
 
 // lanes of the wavefront holding the same digit (among valid lanes)
@@ -72,33 +84,36 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(SortPass p)
 {
     __shared__ uint32_t h[4][256];
     const uint32_t n = pass_n(p);
-    const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
     const uint32_t R = 1u << p.bits, mask = R - 1;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1;
-    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        for (uint32_t i = threadIdx.x; i < 4 * 256; i += SORT_THREADS) (&h[0][0])[i] = 0;
-        __syncthreads();
-        const uint32_t t0 = tile * SORT_TILE + wave * (SORT_ITEMS * 64);
-        uint32_t key[SORT_ITEMS];
+    uint32_t lo, hi;
+    block_range(n, blockIdx.x, gridDim.x, lo, hi);
+    for (uint32_t i = threadIdx.x; i < 4 * 256; i += SORT_THREADS) (&h[0][0])[i] = 0;
+    __syncthreads();
+    // four tiles per step (the order of counting does not matter here): 32 loads in flight per lane;
+    // item k of a wave is element k & 7 of the wave's part of tile k >> 3
+    constexpr uint32_t HI = 4 * SORT_ITEMS;
+    for (uint32_t base = lo; base < hi; base += 4 * SORT_TILE) {
+        const uint32_t t0 = base + wave * (SORT_ITEMS * 64);
+        uint32_t key[HI];
 #pragma unroll
-        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
-            const uint32_t i = t0 + k * 64 + lane;
-            key[k] = i < n ? p.kin[i] : SORT_SENTINEL;
+        for (uint32_t k = 0; k < HI; k++) {
+            const uint32_t i = t0 + (k >> 3) * SORT_TILE + (k & 7) * 64 + lane;
+            key[k] = i < hi ? p.kin[i] : SORT_SENTINEL;
         }
 #pragma unroll
-        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
-            const uint32_t i = t0 + k * 64 + lane;
-            const bool valid = i < n && !(p.drop && key[k] == SORT_SENTINEL);
+        for (uint32_t k = 0; k < HI; k++) {
+            const uint32_t i = t0 + (k >> 3) * SORT_TILE + (k & 7) * 64 + lane;
+            const bool valid = i < hi && !(p.drop && key[k] == SORT_SENTINEL);
             const uint32_t d = (key[k] >> p.shift) & mask;
             const uint64_t peers = match_digit(d, p.bits, valid);
             if (valid && (peers & lt) == 0) h[wave][d] += (uint32_t)__popcll(peers);
         }
-        __syncthreads();
-        for (uint32_t d = threadIdx.x; d < R; d += SORT_THREADS)
-            p.hist[(uint64_t)d * p.tiles_cap + tile] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
-        __syncthreads();
     }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < R; d += SORT_THREADS)
+        p.hist[(uint64_t)d * gridDim.x + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
 }
 
 __device__ inline uint32_t wave_incl_scan(uint32_t v)
@@ -129,28 +144,26 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *wtot, uint32_t 
     return before + inc - v;
 }
 
-// totals[d] = sum over tiles of hist[d][tile]; one workgroup per digit
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_totals(SortPass p)
+// totals[d] = sum over the ranges of hist[d][range]; one workgroup per digit
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_totals(SortPass p, uint32_t grid)
 {
     __shared__ uint32_t wtot[4];
     const uint32_t d = blockIdx.x;
-    const uint32_t tiles = (pass_n(p) + SORT_TILE - 1) / SORT_TILE;
-    const uint32_t *h = p.hist + (uint64_t)d * p.tiles_cap;
+    const uint32_t *h = p.hist + (uint64_t)d * grid;
     uint32_t s = 0;
-    for (uint32_t t = threadIdx.x; t < tiles; t += SORT_THREADS) s += h[t];
+    for (uint32_t t = threadIdx.x; t < grid; t += SORT_THREADS) s += h[t];
     uint32_t total;
     (void)block_excl_scan(s, wtot, &total);
     if (threadIdx.x == 0) p.totals[d] = total;
 }
 
-// hist[d][t] <- sum of totals below d + sum of hist[d][t' < t]; one workgroup per digit, eight
-// consecutive tiles per thread and step
-__global__ void __launch_bounds__(SORT_THREADS) k_sort_offsets(SortPass p)
+// hist[d][r] <- sum of totals below d + sum of hist[d][r' < r]; one workgroup per digit, eight
+// consecutive ranges per thread and step
+__global__ void __launch_bounds__(SORT_THREADS) k_sort_offsets(SortPass p, uint32_t grid)
 {
     __shared__ uint32_t wtot[4];
     const uint32_t d = blockIdx.x;
     const uint32_t R = 1u << p.bits;
-    const uint32_t tiles = (pass_n(p) + SORT_TILE - 1) / SORT_TILE;
     uint32_t below = 0, all = 0;
     for (uint32_t x = 0; x < R; x++) {
         const uint32_t t = p.totals[x];
@@ -158,21 +171,21 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_offsets(SortPass p)
         all += t;
     }
     if (d == 0 && threadIdx.x == 0 && p.n_out) *p.n_out = all;
-    uint32_t *h = p.hist + (uint64_t)d * p.tiles_cap;
+    uint32_t *h = p.hist + (uint64_t)d * grid;
     uint32_t running = below;
-    for (uint32_t t0 = 0; t0 < tiles; t0 += SORT_THREADS * 8) {
+    for (uint32_t t0 = 0; t0 < grid; t0 += SORT_THREADS * 8) {
         const uint32_t tb = t0 + threadIdx.x * 8;
         uint32_t v[8], s = 0;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            v[j] = tb + j < tiles ? h[tb + j] : 0u;
+            v[j] = tb + j < grid ? h[tb + j] : 0u;
             s += v[j];
         }
         uint32_t step;
         uint32_t ex = block_excl_scan(s, wtot, &step) + running;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            if (tb + j < tiles) h[tb + j] = ex;
+            if (tb + j < grid) h[tb + j] = ex;
             ex += v[j];
         }
         running += step;
@@ -182,40 +195,52 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_offsets(SortPass p)
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(SortPass p)
 {
     __shared__ uint32_t cnt[4][256];    // per-wave running digit counters, then per-wave digit bases
-    __shared__ uint32_t gofs[256];      // the tile's global offset per digit
+    __shared__ uint32_t gofs[256];      // the range's running global offset per digit
     __shared__ uint32_t lofs[256];      // start of each digit in the tile's digit order
     __shared__ uint32_t wtot[4];
     __shared__ uint32_t kept_s;
     __shared__ uint32_t lk[SORT_TILE];
     __shared__ double lv[SORT_TILE];
     const uint32_t n = pass_n(p);
-    const uint32_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
     const uint32_t R = 1u << p.bits, mask = R - 1;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1;
-    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    uint32_t lo, hi;
+    block_range(n, blockIdx.x, gridDim.x, lo, hi);
+    if (lo >= hi) return;
+    for (uint32_t d = threadIdx.x; d < R; d += SORT_THREADS) gofs[d] = p.hist[(uint64_t)d * gridDim.x + blockIdx.x];
+    // the next tile's elements are loaded while this one is ranked and written
+    uint32_t nkey[SORT_ITEMS];
+    double nval[SORT_ITEMS];
+    auto load_tile = [&](uint32_t base) {
+        const uint32_t t0 = base + wave * (SORT_ITEMS * 64);
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
+            const uint32_t i = t0 + k * 64 + lane;
+            nkey[k] = i < hi ? p.kin[i] : SORT_SENTINEL;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
+            const uint32_t i = t0 + k * 64 + lane;
+            nval[k] = i < hi ? p.vin[i] : 0.0;
+        }
+    };
+    load_tile(lo);
+    for (uint32_t base = lo; base < hi; base += SORT_TILE) {
         for (uint32_t i = threadIdx.x; i < 4 * 256; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
-        for (uint32_t d = threadIdx.x; d < R; d += SORT_THREADS) gofs[d] = p.hist[(uint64_t)d * p.tiles_cap + tile];
-        __syncthreads();
         // stable ranks: element order inside the tile is (wave, item, lane)
-        const uint32_t t0 = tile * SORT_TILE + wave * (SORT_ITEMS * 64);
+        const uint32_t t0 = base + wave * (SORT_ITEMS * 64);
         uint32_t key[SORT_ITEMS], rank[SORT_ITEMS];
         double val[SORT_ITEMS];
         uint32_t vmask = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
-            const uint32_t i = t0 + k * 64 + lane;
-            key[k] = i < n ? p.kin[i] : SORT_SENTINEL;
-        }
+        for (uint32_t k = 0; k < SORT_ITEMS; k++) { key[k] = nkey[k]; val[k] = nval[k]; }
+        if (base + SORT_TILE < hi) load_tile(base + SORT_TILE);
+        __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < SORT_ITEMS; k++) {
             const uint32_t i = t0 + k * 64 + lane;
-            val[k] = i < n ? p.vin[i] : 0.0;
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
-            const uint32_t i = t0 + k * 64 + lane;
-            const bool valid = i < n && !(p.drop && key[k] == SORT_SENTINEL);
+            const bool valid = i < hi && !(p.drop && key[k] == SORT_SENTINEL);
             vmask |= (uint32_t)valid << k;
             const uint32_t d = (key[k] >> p.shift) & mask;
             const uint64_t peers = match_digit(d, p.bits, valid);
@@ -228,9 +253,10 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(SortPass p)
         }
         __syncthreads();
         // digit bases per wave, and the start of each digit in the tile
+        uint32_t tot = 0;
         {
             const uint32_t d = threadIdx.x;
-            uint32_t t[4] = {0, 0, 0, 0}, tot = 0;
+            uint32_t t[4] = {0, 0, 0, 0};
             if (d < R) {
                 for (int w = 0; w < 4; w++) { t[w] = cnt[w][d]; tot += t[w]; }
             }
@@ -263,6 +289,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(SortPass p)
             p.vout[g] = lv[j];
         }
         __syncthreads();
+        if (threadIdx.x < R) gofs[threadIdx.x] += tot;
     }
 }
 
@@ -279,61 +306,99 @@ template <typename T> __device__ inline T shfl_up_t(T v, int off) { return __shf
 
 __device__ inline void table_add(const ReduceArgs &a, uint32_t key, uint64_t c, uint64_t v0, uint64_t v1, uint64_t v2)
 {
-    if (key >= a.key_limit) return;
+    if (key >= a.key_limit || c == 0) return;
     atomicAdd(&a.count[key], (unsigned long long)c);
     if (v0) atomicAdd(&a.p0[key], (unsigned long long)v0);
     if (v1) atomicAdd(&a.p1[key], (unsigned long long)v1);
     if (v2) atomicAdd(&a.p2[key], (unsigned long long)v2);
 }
 
+struct RunSum {
+    uint64_t c, v0, v1, v2;
+};
+
 __global__ void __launch_bounds__(256) k_seg_reduce(ReduceArgs a)
 {
     const uint64_t nn = *a.n;
     const uint32_t n = (uint32_t)(nn < a.n_cap ? nn : a.n_cap);
-    const uint32_t chunk_elems = 64 * RED_STEPS;
+    const uint32_t chunk_elems = 64 * RED_LANE;
     const uint32_t chunks = (n + chunk_elems - 1) / chunk_elems;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t chunk = gw; chunk < chunks; chunk += nw) {
-        uint32_t ck = SORT_SENTINEL;                     // the run carried from the previous step
-        uint64_t cc = 0, c0 = 0, c1 = 0, c2 = 0;
-        for (uint32_t s = 0; s < RED_STEPS; s++) {
-            const uint32_t i = chunk * chunk_elems + s * 64 + lane;
-            if (chunk * chunk_elems + s * 64 >= n) break;    // wave-uniform
-            const bool valid = i < n;
-            const uint32_t k = valid ? a.keys[i] : SORT_SENTINEL;
-            uint64_t l0 = 0, l1 = 0, l2 = 0;
-            if (valid) fixed_split(fixed_from_double(a.vals[i]), l0, l1, l2);
-            uint64_t vc = valid ? 1u : 0u;
-            // the carried run ends where this step starts another key
-            if ((uint32_t)__builtin_amdgcn_readlane((int)k, 0) != ck) {
-                if (lane == 0 && ck != SORT_SENTINEL) table_add(a, ck, cc, c0, c1, c2);
-                ck = SORT_SENTINEL;
-                cc = c0 = c1 = c2 = 0;
-            }
-            uint32_t prevk = (uint32_t)__shfl_up((int)k, 1);
-            if (lane == 0) prevk = ck;
-            bool f = k != prevk;        // a run starts at this lane
-            // segmented inclusive scan: sums from the start of the lane's run (inside this step)
+        // this lane's slice, summed run by run
+        const uint32_t e0 = chunk * chunk_elems + lane * RED_LANE;
+        uint32_t hk = SORT_SENTINEL, cur = SORT_SENTINEL;
+        RunSum hs = {0, 0, 0, 0}, cs = {0, 0, 0, 0};
+        bool single = true;     // the slice holds one run so far
+        for (uint32_t q = 0; q < RED_LANE; q += 8) {
+            uint32_t kb[8];
+            double vb[8];
+            if (e0 + q + 8 <= n) {
+                const uint4 k0 = *(const uint4 *)(a.keys + e0 + q), k1 = *(const uint4 *)(a.keys + e0 + q + 4);
+                kb[0] = k0.x; kb[1] = k0.y; kb[2] = k0.z; kb[3] = k0.w;
+                kb[4] = k1.x; kb[5] = k1.y; kb[6] = k1.z; kb[7] = k1.w;
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const uint64_t uc = shfl_up_t(vc, off), u0 = shfl_up_t(l0, off), u1 = shfl_up_t(l1, off),
-                               u2 = shfl_up_t(l2, off);
-                const bool uf = __shfl_up((int)f, off) != 0;
-                if (lane >= (uint32_t)off && !f) {
-                    vc += uc; l0 += u0; l1 += u1; l2 += u2;
-                    f = uf;
+                for (int j = 0; j < 4; j++) {
+                    const double2 v = *(const double2 *)(a.vals + e0 + q + 2 * j);
+                    vb[2 * j] = v.x;
+                    vb[2 * j + 1] = v.y;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const bool in = e0 + q + j < n;
+                    kb[j] = in ? a.keys[e0 + q + j] : SORT_SENTINEL;
+                    vb[j] = in ? a.vals[e0 + q + j] : 0.0;
                 }
             }
-            // a run with no start in this step continues the carried run
-            if (!f) { vc += cc; l0 += c0; l1 += c1; l2 += c2; }
-            const uint32_t nextk = (uint32_t)__shfl_down((int)k, 1);
-            if (lane < 63 && k != nextk && k != SORT_SENTINEL) table_add(a, k, vc, l0, l1, l2);
-            ck = (uint32_t)__builtin_amdgcn_readlane((int)k, 63);
-            cc = __shfl(vc, 63); c0 = __shfl(l0, 63); c1 = __shfl(l1, 63); c2 = __shfl(l2, 63);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const uint32_t k = kb[j];
+                if (k == SORT_SENTINEL) continue;     // past the end
+                uint64_t l0, l1, l2;
+                fixed_split(fixed_from_double(vb[j]), l0, l1, l2);
+                if (k != cur) {
+                    if (cur != SORT_SENTINEL) {
+                        if (single) { hs = cs; single = false; }            // the slice's first run ended
+                        else table_add(a, cur, cs.c, cs.v0, cs.v1, cs.v2);  // a run inside the slice
+                    } else {
+                        hk = k;
+                    }
+                    cur = k;
+                    cs = {0, 0, 0, 0};
+                }
+                cs.c += 1; cs.v0 += l0; cs.v1 += l1; cs.v2 += l2;
+            }
         }
-        if (lane == 0 && ck != SORT_SENTINEL) table_add(a, ck, cc, c0, c1, c2);
+        // lane summary: head run (hk, hs) unless single; tail run (cur, cs) in every lane
+        if (single) hk = cur;
+        const uint32_t tk = cur;
+        // join runs across lanes: the carry out of lane l is its tail run plus, when the slice is one
+        // run continuing the left neighbour's tail, that neighbour's carry
+        const uint32_t prev_tk = (uint32_t)__shfl_up((int)tk, 1);
+        const bool joins_left = lane > 0 && prev_tk == hk && hk != SORT_SENTINEL;
+        bool f = !(single && joins_left);
+        RunSum x = cs;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t uc = shfl_up_t(x.c, off), u0 = shfl_up_t(x.v0, off), u1 = shfl_up_t(x.v1, off),
+                           u2 = shfl_up_t(x.v2, off);
+            const bool uf = __shfl_up((int)f, off) != 0;
+            if (lane >= (uint32_t)off && !f) {
+                x.c += uc; x.v0 += u0; x.v1 += u1; x.v2 += u2;
+                f = uf;
+            }
+        }
+        // the left neighbour's carry, for a head run that continues it
+        RunSum in = {shfl_up_t(x.c, 1), shfl_up_t(x.v0, 1), shfl_up_t(x.v1, 1), shfl_up_t(x.v2, 1)};
+        if (!single) {
+            if (joins_left) { hs.c += in.c; hs.v0 += in.v0; hs.v1 += in.v1; hs.v2 += in.v2; }
+            table_add(a, hk, hs.c, hs.v0, hs.v1, hs.v2);
+        }
+        const uint32_t next_hk = (uint32_t)__shfl_down((int)hk, 1);
+        if (tk != SORT_SENTINEL && (lane == 63 || next_hk != tk)) table_add(a, tk, x.c, x.v0, x.v1, x.v2);
     }
 }
 
@@ -341,7 +406,7 @@ __global__ void __launch_bounds__(256) k_seg_reduce(ReduceArgs a)
 
 // one table: radix passes over its key bits, then the reduce into the resident table.  keys/vals
 // hold the emitted pairs (n_emit of them, sentinels included); tkeys/tvals are the ping-pong
-// buffers.  n_kept receives the number of real pairs.
+// buffers.  n_kept receives the number of real pairs.  hist holds 256 x SORT_GRID counters.
 extern "C" hipError_t fpd_sort_reduce(uint32_t *keys, double *vals, uint32_t *tkeys, double *tvals, uint64_t cap,
                                       const uint32_t *n_emit, uint32_t *n_kept, uint32_t *hist, uint32_t *totals,
                                       uint32_t key_bits, uint32_t digit_bits, uint32_t table_entries,
@@ -358,14 +423,14 @@ extern "C" hipError_t fpd_sort_reduce(uint32_t *keys, double *vals, uint32_t *tk
         p.kin = ka; p.vin = va; p.kout = kb; p.vout = vb;
         p.n_in = pass == 0 ? n_emit : n_kept;
         p.n_out = pass == 0 ? n_kept : nullptr;
-        p.hist = hist; p.totals = totals; p.n_cap = cap; p.tiles_cap = tiles_cap;
+        p.hist = hist; p.totals = totals; p.n_cap = cap;
         p.shift = shift;
         p.bits = std::min<uint32_t>(digit_bits, key_bits - shift);
         p.drop = pass == 0 ? 1u : 0u;
         const uint32_t R = 1u << p.bits;
         hipLaunchKernelGGL(fpd::k_sort_hist, dim3(grid), dim3(fpd::SORT_THREADS), 0, st, p);
-        hipLaunchKernelGGL(fpd::k_sort_totals, dim3(R), dim3(fpd::SORT_THREADS), 0, st, p);
-        hipLaunchKernelGGL(fpd::k_sort_offsets, dim3(R), dim3(fpd::SORT_THREADS), 0, st, p);
+        hipLaunchKernelGGL(fpd::k_sort_totals, dim3(R), dim3(fpd::SORT_THREADS), 0, st, p, grid);
+        hipLaunchKernelGGL(fpd::k_sort_offsets, dim3(R), dim3(fpd::SORT_THREADS), 0, st, p, grid);
         hipLaunchKernelGGL(fpd::k_sort_scatter, dim3(grid), dim3(fpd::SORT_THREADS), 0, st, p);
         std::swap(ka, kb);
         std::swap(va, vb);
@@ -373,8 +438,8 @@ extern "C" hipError_t fpd_sort_reduce(uint32_t *keys, double *vals, uint32_t *tk
     fpd::ReduceArgs r{};
     r.keys = ka; r.vals = va; r.n = n_kept; r.n_cap = cap; r.key_limit = table_entries;
     r.count = count; r.p0 = parts[0]; r.p1 = parts[1]; r.p2 = parts[2];
-    const uint32_t chunks = (uint32_t)((cap + 64 * fpd::RED_STEPS - 1) / (64 * fpd::RED_STEPS));
-    const uint32_t rgrid = std::max<uint32_t>(1, std::min<uint32_t>((chunks + 3) / 4, 4096));
+    const uint32_t chunks = (uint32_t)((cap + 64 * fpd::RED_LANE - 1) / (64 * fpd::RED_LANE));
+    const uint32_t rgrid = std::max<uint32_t>(1, std::min<uint32_t>((chunks + 3) / 4, 2048));
     hipLaunchKernelGGL(fpd::k_seg_reduce, dim3(rgrid), dim3(256), 0, st, r);
     return hipGetLastError();
 }

@@ -219,7 +219,9 @@ typedef struct fp_stats {
     uint64_t baq_cells;        /* band cells evaluated by the BAQ HMM (algorithmic work) */
     uint64_t baq_waves_per_simd; /* BAQ band-kernel variant in use (autotuned or set)    */
     uint64_t sort_digit_bits;    /* FP_AGG_SORT radix digit width in use; 0 = direct    */
-    uint64_t reserved[3];
+    uint64_t sort_slots;         /* FP_AGG_SORT, profile_stages or plain engine: pair    */
+    uint64_t sort_pairs;         /*   slots emitted, real pairs sorted, and radix       */
+    uint64_t sort_passes;        /*   passes run (summed over tables and batches)       */
 } fp_stats;
 
 /* Per-batch intermediates of the last processed batch, for stage-by-stage
