@@ -310,7 +310,7 @@ fp_status ensure_state(fp_engine *e, uint32_t n, uint64_t nbases, uint64_t nqual
     HIP_TRY(e->counters.ensure(64));
     HIP_TRY(e->baq_lists.ensure(4ull * 4 * std::max<uint32_t>(n, 1)));
     HIP_TRY(e->rg_buf.ensure(4ull * (4ull * std::max<uint32_t>(e->num_rg, 1) + 2 + n)));
-    HIP_TRY(e->baq_counts.ensure(64));
+    HIP_TRY(e->baq_counts.ensure(4 * 400));   // class sizes, (class, length bucket) sizes, offsets, cursors
     fpd::ReadStateDev &s = e->rs;
     s.status = e->st_status.as<uint8_t>();
     s.rwc = e->st_rwc.as<uint32_t>();
@@ -393,7 +393,7 @@ static uint64_t band_scratch_doubles(int wpe, uint32_t n, uint32_t rows_cap)
 // stage 1 (graph-safe): per-read prep, BAQ tag reset, band classification
 fp_status launch_front(fp_engine *e, const fpd::BatchDev &b, const Plan &p, uint32_t *counters, hipStream_t st)
 {
-    HIP_TRY(fpd_fill_u32(e->baq_counts.as<uint32_t>(), 16, 0u, st));
+    HIP_TRY(fpd_fill_u32(e->baq_counts.as<uint32_t>(), 144, 0u, st));
     HIP_TRY(fpd_launch_prep(&b, &e->genome, &e->rs, counters, p.rows_cap, st));
     HIP_TRY(fpd_fill_u8(e->rs.baq, p.nquals, p.live, 0xff, st));
     HIP_TRY(fpd_launch_baq_classify(p.n, p.live, &e->rs, e->baq_lists.as<uint32_t>(), e->baq_counts.as<uint32_t>(),

@@ -902,30 +902,104 @@ struct ClassifyArgs {
     uint32_t capacity;
 };
 
-__global__ void __launch_bounds__(256) k_baq_classify(ClassifyArgs a)
+// BAQ reads grouped by band class (which band kernel runs them) and, inside a class, by window
+// length in 16-base buckets, so the 64 reads of a wavefront have similar row counts.  counts layout:
+// [0..3] class sizes, [4] widest class-3 band, [16 + 32c + b] (class, bucket) sizes,
+// [144 + 32c + b] their offsets in the class list, [272 + 32c + b] scatter cursors.  A fixed grid
+// strides over the reads; counts are kept in LDS and reach global memory once per workgroup.
+static constexpr uint32_t CLS_BLOCKS = 512, CLS_BUCKETS = 32;
+static constexpr uint32_t CB_COUNT = 16, CB_OFFSET = 144, CB_CURSOR = 272;
+
+__device__ inline bool classify_read(const ClassifyArgs &a, uint32_t r, uint32_t n, uint32_t &key, uint32_t &w)
 {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = a.live ? a.live[0] : a.num_reads;
-    const bool act = r < n && a.status[r] == RS_ACTIVE && a.num_errors[r] != 0;
-    uint32_t cls = 0;
-    if (act) {
-        const uint32_t w = 2u * a.bandwidth[r] + 1;
-        cls = w <= 15 ? 0 : w <= 23 ? 1 : w <= 33 ? 2 : 3;
-        if (cls == 3) atomicMax(&a.counts[4], w);
-    }
-    // wave-aggregated append: one counter atomic per band class present in the wavefront
+    if (r >= n || a.status[r] != RS_ACTIVE || a.num_errors[r] == 0) return false;
+    w = 2u * a.bandwidth[r] + 1;
+    const uint32_t cls = w <= 15 ? 0 : w <= 23 ? 1 : w <= 33 ? 2 : 3;
+    const uint32_t rw = a.rwc[r];
+    const uint32_t qlen = (uint32_t)((int)hi16(rw) - (int)lo16(rw) + 1);
+    const uint32_t bucket = qlen / 16 < CLS_BUCKETS ? qlen / 16 : CLS_BUCKETS - 1;
+    key = cls * CLS_BUCKETS + bucket;
+    return true;
+}
+
+// one LDS increment per distinct key in the wavefront; returns this lane's slot
+__device__ inline uint32_t lds_append(uint32_t *cnt, uint32_t key, bool active)
+{
     const uint32_t lane = threadIdx.x & 63;
-    uint32_t pos = 0;
-    for (uint32_t c = 0; c < 4; c++) {
-        const uint64_t m = __ballot(act && cls == c);
-        if (!m) continue;
-        const int leader = __builtin_ctzll(m);
+    uint32_t slot = 0;
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const uint32_t k = (uint32_t)__shfl((int)key, leader);
+        const uint64_t same = __ballot(active && key == k);
         uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&a.counts[c], (uint32_t)__popcll(m));
-        base = __shfl(base, leader);
-        if (act && cls == c) pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+        if ((int)lane == leader) base = atomicAdd(&cnt[k], (uint32_t)__popcll(same));
+        base = (uint32_t)__shfl((int)base, leader);
+        if (active && key == k) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1));
+        pending &= ~same;
     }
-    if (act) a.lists[(uint64_t)cls * a.capacity + pos] = r;
+    return slot;
+}
+
+__global__ void __launch_bounds__(256) k_baq_class_count(ClassifyArgs a)
+{
+    __shared__ uint32_t cnt[4 * CLS_BUCKETS];
+    __shared__ uint32_t wmax;
+    for (uint32_t i = threadIdx.x; i < 4 * CLS_BUCKETS; i += blockDim.x) cnt[i] = 0;
+    if (threadIdx.x == 0) wmax = 0;
+    __syncthreads();
+    const uint32_t n = a.live ? a.live[0] : a.num_reads;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        uint32_t key = 0, w = 0;
+        const bool act = classify_read(a, base + threadIdx.x, n, key, w);
+        if (act && key >= 3 * CLS_BUCKETS) atomicMax(&wmax, w);
+        lds_append(cnt, key, act);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 4 * CLS_BUCKETS; i += blockDim.x)
+        if (cnt[i]) atomicAdd(&a.counts[CB_COUNT + i], cnt[i]);
+    if (threadIdx.x == 0 && wmax) atomicMax(&a.counts[4], wmax);
+}
+
+// class sizes and bucket offsets: one thread per class
+__global__ void __launch_bounds__(64) k_baq_class_offsets(ClassifyArgs a)
+{
+    const uint32_t c = threadIdx.x;
+    if (c >= 4) return;
+    uint32_t o = 0;
+    for (uint32_t b = 0; b < CLS_BUCKETS; b++) {
+        a.counts[CB_OFFSET + c * CLS_BUCKETS + b] = o;
+        a.counts[CB_CURSOR + c * CLS_BUCKETS + b] = 0;
+        o += a.counts[CB_COUNT + c * CLS_BUCKETS + b];
+    }
+    a.counts[c] = o;
+}
+
+__global__ void __launch_bounds__(256) k_baq_class_scatter(ClassifyArgs a)
+{
+    __shared__ uint32_t cnt[4 * CLS_BUCKETS], first[4 * CLS_BUCKETS];
+    for (uint32_t i = threadIdx.x; i < 4 * CLS_BUCKETS; i += blockDim.x) cnt[i] = 0;
+    __syncthreads();
+    const uint32_t n = a.live ? a.live[0] : a.num_reads;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        uint32_t key = 0, w = 0;
+        const bool act = classify_read(a, base + threadIdx.x, n, key, w);
+        lds_append(cnt, key, act);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 4 * CLS_BUCKETS; i += blockDim.x) {
+        first[i] = cnt[i] ? a.counts[CB_OFFSET + i] + atomicAdd(&a.counts[CB_CURSOR + i], cnt[i]) : 0;
+        cnt[i] = 0;
+    }
+    __syncthreads();
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t r = base + threadIdx.x;
+        uint32_t key = 0, w = 0;
+        const bool act = classify_read(a, r, n, key, w);
+        const uint32_t slot = lds_append(cnt, key, act);
+        if (act) a.lists[(uint64_t)(key / CLS_BUCKETS) * a.capacity + first[key] + slot] = r;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_fill_u8(uint8_t *p, uint64_t n, const uint32_t *live, uint8_t v)
@@ -959,7 +1033,10 @@ extern "C" hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t
     fpd::ClassifyArgs a{};
     a.num_reads = num_reads; a.live = live; a.status = s->status; a.num_errors = s->num_errors; a.bandwidth = s->bandwidth;
     a.rwc = s->rwc; a.lists = lists; a.counts = counts; a.capacity = capacity;
-    hipLaunchKernelGGL(fpd::k_baq_classify, dim3((num_reads + 255) / 256), dim3(256), 0, st, a);
+    const uint32_t blocks = std::min<uint32_t>((num_reads + 255) / 256, fpd::CLS_BLOCKS);
+    hipLaunchKernelGGL(fpd::k_baq_class_count, dim3(blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(fpd::k_baq_class_offsets, dim3(1), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(fpd::k_baq_class_scatter, dim3(blocks), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

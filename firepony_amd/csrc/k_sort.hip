@@ -68,11 +68,13 @@ __device__ inline void block_range(uint32_t n, uint32_t b, uint32_t grid, uint32
     hi = (uint32_t)(h < n ? h : n);
 }
 
-// lanes of the wavefront holding the same digit (among valid lanes)
-__device__ inline uint64_t match_digit(uint32_t d, uint32_t bits, bool valid)
+// lanes of the wavefront holding the same BITS-bit digit (among valid lanes)
+template <uint32_t BITS>
+__device__ inline uint64_t match_digit(uint32_t d, bool valid)
 {
     uint64_t m = __ballot(valid);
-    for (uint32_t b = 0; b < bits; b++) {
+#pragma unroll
+    for (uint32_t b = 0; b < BITS; b++) {
         const bool one = (d >> b) & 1;
         const uint64_t x = __ballot(one);
         m &= one ? x : ~x;
@@ -80,40 +82,45 @@ __device__ inline uint64_t match_digit(uint32_t d, uint32_t bits, bool valid)
     return m;
 }
 
+// the range's digit counts: LDS atomics into 16 KB of sub-histograms (per wave and lane group:
+// 4 groups at 8-bit digits, up to 16 at narrower ones), which keeps same-address collisions of
+// skewed digits (the read-group and quality fields) low
+template <uint32_t BITS>
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(SortPass p)
 {
-    __shared__ uint32_t h[4][256];
+    constexpr uint32_t R = 1u << BITS, mask = R - 1;
+    constexpr uint32_t G = (1024 / R) < 4 ? 4 : ((1024 / R) > 16 ? 16 : (1024 / R));   // groups per wave
+    constexpr uint32_t C = 4 * G;                                                      // sub-histograms
+    __shared__ uint32_t h[C][R + 1];   // +1: the copies of one digit fall in different banks
     const uint32_t n = pass_n(p);
-    const uint32_t R = 1u << p.bits, mask = R - 1;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint64_t lt = (1ull << lane) - 1;
+    uint32_t *mine = h[wave * G + (lane & (G - 1))];
     uint32_t lo, hi;
     block_range(n, blockIdx.x, gridDim.x, lo, hi);
-    for (uint32_t i = threadIdx.x; i < 4 * 256; i += SORT_THREADS) (&h[0][0])[i] = 0;
+    for (uint32_t i = threadIdx.x; i < C * (R + 1); i += SORT_THREADS) (&h[0][0])[i] = 0;
     __syncthreads();
-    // four tiles per step (the order of counting does not matter here): 32 loads in flight per lane;
-    // item k of a wave is element k & 7 of the wave's part of tile k >> 3
+    // four tiles per step: 32 loads in flight per lane (the counting order does not matter)
     constexpr uint32_t HI = 4 * SORT_ITEMS;
     for (uint32_t base = lo; base < hi; base += 4 * SORT_TILE) {
-        const uint32_t t0 = base + wave * (SORT_ITEMS * 64);
         uint32_t key[HI];
 #pragma unroll
         for (uint32_t k = 0; k < HI; k++) {
-            const uint32_t i = t0 + (k >> 3) * SORT_TILE + (k & 7) * 64 + lane;
+            const uint32_t i = base + k * SORT_THREADS + threadIdx.x;
             key[k] = i < hi ? p.kin[i] : SORT_SENTINEL;
         }
 #pragma unroll
         for (uint32_t k = 0; k < HI; k++) {
-            const uint32_t i = t0 + (k >> 3) * SORT_TILE + (k & 7) * 64 + lane;
-            const bool valid = i < hi && !(p.drop && key[k] == SORT_SENTINEL);
-            const uint32_t d = (key[k] >> p.shift) & mask;
-            const uint64_t peers = match_digit(d, p.bits, valid);
-            if (valid && (peers & lt) == 0) h[wave][d] += (uint32_t)__popcll(peers);
+            const uint32_t i = base + k * SORT_THREADS + threadIdx.x;
+            if (i < hi && !(p.drop && key[k] == SORT_SENTINEL)) atomicAdd(&mine[(key[k] >> p.shift) & mask], 1u);
         }
     }
     __syncthreads();
-    for (uint32_t d = threadIdx.x; d < R; d += SORT_THREADS)
-        p.hist[(uint64_t)d * gridDim.x + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+    for (uint32_t d = threadIdx.x; d < R; d += SORT_THREADS) {
+        uint32_t t = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < C; c++) t += h[c][d];
+        p.hist[(uint64_t)d * gridDim.x + blockIdx.x] = t;
+    }
 }
 
 __device__ inline uint32_t wave_incl_scan(uint32_t v)
@@ -163,7 +170,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_offsets(SortPass p, uint3
 {
     __shared__ uint32_t wtot[4];
     const uint32_t d = blockIdx.x;
-    const uint32_t R = 1u << p.bits;
+    const uint32_t R = gridDim.x;       // one workgroup per digit
     uint32_t below = 0, all = 0;
     for (uint32_t x = 0; x < R; x++) {
         const uint32_t t = p.totals[x];
@@ -192,6 +199,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_offsets(SortPass p, uint3
     }
 }
 
+template <uint32_t BITS>
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(SortPass p)
 {
     __shared__ uint32_t cnt[4][256];    // per-wave running digit counters, then per-wave digit bases
@@ -202,48 +210,38 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(SortPass p)
     __shared__ uint32_t lk[SORT_TILE];
     __shared__ double lv[SORT_TILE];
     const uint32_t n = pass_n(p);
-    const uint32_t R = 1u << p.bits, mask = R - 1;
+    constexpr uint32_t R = 1u << BITS, mask = R - 1;
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t lt = (1ull << lane) - 1;
     uint32_t lo, hi;
     block_range(n, blockIdx.x, gridDim.x, lo, hi);
     if (lo >= hi) return;
     for (uint32_t d = threadIdx.x; d < R; d += SORT_THREADS) gofs[d] = p.hist[(uint64_t)d * gridDim.x + blockIdx.x];
-    // the next tile's elements are loaded while this one is ranked and written
-    uint32_t nkey[SORT_ITEMS];
-    double nval[SORT_ITEMS];
-    auto load_tile = [&](uint32_t base) {
-        const uint32_t t0 = base + wave * (SORT_ITEMS * 64);
-#pragma unroll
-        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
-            const uint32_t i = t0 + k * 64 + lane;
-            nkey[k] = i < hi ? p.kin[i] : SORT_SENTINEL;
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
-            const uint32_t i = t0 + k * 64 + lane;
-            nval[k] = i < hi ? p.vin[i] : 0.0;
-        }
-    };
-    load_tile(lo);
     for (uint32_t base = lo; base < hi; base += SORT_TILE) {
         for (uint32_t i = threadIdx.x; i < 4 * 256; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
+        __syncthreads();
         // stable ranks: element order inside the tile is (wave, item, lane)
         const uint32_t t0 = base + wave * (SORT_ITEMS * 64);
         uint32_t key[SORT_ITEMS], rank[SORT_ITEMS];
         double val[SORT_ITEMS];
         uint32_t vmask = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < SORT_ITEMS; k++) { key[k] = nkey[k]; val[k] = nval[k]; }
-        if (base + SORT_TILE < hi) load_tile(base + SORT_TILE);
-        __syncthreads();
+        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
+            const uint32_t i = t0 + k * 64 + lane;
+            key[k] = i < hi ? p.kin[i] : SORT_SENTINEL;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
+            const uint32_t i = t0 + k * 64 + lane;
+            val[k] = i < hi ? p.vin[i] : 0.0;
+        }
 #pragma unroll
         for (uint32_t k = 0; k < SORT_ITEMS; k++) {
             const uint32_t i = t0 + k * 64 + lane;
             const bool valid = i < hi && !(p.drop && key[k] == SORT_SENTINEL);
             vmask |= (uint32_t)valid << k;
             const uint32_t d = (key[k] >> p.shift) & mask;
-            const uint64_t peers = match_digit(d, p.bits, valid);
+            const uint64_t peers = match_digit<BITS>(d, valid);
             const uint32_t c = valid ? cnt[wave][d] : 0u;
             // every lane has read the counter before the leader of each digit moves it on
             __builtin_amdgcn_wave_barrier();
@@ -418,22 +416,36 @@ extern "C" hipError_t fpd_sort_reduce(uint32_t *keys, double *vals, uint32_t *tk
     const uint32_t grid = std::min<uint32_t>(tiles_cap, fpd::SORT_GRID);
     uint32_t *ka = keys, *kb = tkeys;
     double *va = vals, *vb = tvals;
-    for (uint32_t shift = 0, pass = 0; shift < key_bits; shift += digit_bits, pass++) {
+    // ceil(key_bits / digit_bits) passes with the key bits spread evenly over them (a narrower digit
+    // ranks and writes faster, so 12 bits sort as 6 + 6 rather than 8 + 4)
+    const uint32_t npass = (key_bits + digit_bits - 1) / digit_bits;
+    for (uint32_t shift = 0, pass = 0; pass < npass; pass++) {
+        const uint32_t bits = key_bits / npass + (pass < key_bits % npass ? 1u : 0u);
         fpd::SortPass p{};
         p.kin = ka; p.vin = va; p.kout = kb; p.vout = vb;
         p.n_in = pass == 0 ? n_emit : n_kept;
         p.n_out = pass == 0 ? n_kept : nullptr;
         p.hist = hist; p.totals = totals; p.n_cap = cap;
         p.shift = shift;
-        p.bits = std::min<uint32_t>(digit_bits, key_bits - shift);
+        p.bits = bits;
         p.drop = pass == 0 ? 1u : 0u;
         const uint32_t R = 1u << p.bits;
-        hipLaunchKernelGGL(fpd::k_sort_hist, dim3(grid), dim3(fpd::SORT_THREADS), 0, st, p);
-        hipLaunchKernelGGL(fpd::k_sort_totals, dim3(R), dim3(fpd::SORT_THREADS), 0, st, p, grid);
-        hipLaunchKernelGGL(fpd::k_sort_offsets, dim3(R), dim3(fpd::SORT_THREADS), 0, st, p, grid);
-        hipLaunchKernelGGL(fpd::k_sort_scatter, dim3(grid), dim3(fpd::SORT_THREADS), 0, st, p);
+        switch (p.bits) {
+#define FP_SORT_BITS(B)                                                                                      \
+    case B:                                                                                                  \
+        hipLaunchKernelGGL(fpd::k_sort_hist<B>, dim3(grid), dim3(fpd::SORT_THREADS), 0, st, p);              \
+        hipLaunchKernelGGL(fpd::k_sort_totals, dim3(R), dim3(fpd::SORT_THREADS), 0, st, p, grid);            \
+        hipLaunchKernelGGL(fpd::k_sort_offsets, dim3(R), dim3(fpd::SORT_THREADS), 0, st, p, grid);           \
+        hipLaunchKernelGGL(fpd::k_sort_scatter<B>, dim3(grid), dim3(fpd::SORT_THREADS), 0, st, p);           \
+        break;
+            FP_SORT_BITS(1) FP_SORT_BITS(2) FP_SORT_BITS(3) FP_SORT_BITS(4)
+            FP_SORT_BITS(5) FP_SORT_BITS(6) FP_SORT_BITS(7) FP_SORT_BITS(8)
+#undef FP_SORT_BITS
+        default: return hipErrorInvalidValue;
+        }
         std::swap(ka, kb);
         std::swap(va, vb);
+        shift += bits;
     }
     fpd::ReduceArgs r{};
     r.keys = ka; r.vals = va; r.n = n_kept; r.n_cap = cap; r.key_limit = table_entries;
