@@ -139,6 +139,9 @@ struct Limbs {
 __device__ inline Limbs to_limbs(double v, uint32_t f64)
 {
     Limbs l;
+    // most per-base errors are exactly 0 or 1 (outside BAQ blocks); 2^80 has limbs (0, 0, 2^26)
+    if (v == 0.0) { l.p0 = 0; l.p1 = 0; l.p2 = 0; return l; }
+    if (v == 1.0) { l.p0 = 0; l.p1 = 0; l.p2 = 1ull << 26; return l; }
     fixed_split(fixed_from_double(f64 ? v : (double)(float)v), l.p0, l.p1, l.p2);
     return l;
 }
@@ -376,42 +379,48 @@ __device__ void cov_read_serial(const CovArgs &a, const LdsAcc &L, const PairReg
 
 // ---- 256-bit masks (4 x u64, bit j = window position j) -------------------
 
-__device__ inline bool mtest(const uint64_t *m, int j) { return (m[j >> 6] >> (j & 63)) & 1; }
+// The helpers below visit the four words with compile-time indices, so the masks stay in
+// registers (a word picked by a per-lane index would put the arrays in scratch memory).
 
 // highest index < j whose bit is clear, or -1
 __device__ inline int prev_zero(const uint64_t *m, int j)
 {
-    for (int w = (j - 1) >> 6; w >= 0; w--) {
-        uint64_t z = ~m[w];
-        if (w == ((j - 1) >> 6)) {
-            const int top = (j - 1) & 63;          // keep bits 0..top
-            z &= (top == 63) ? ~0ull : ((1ull << (top + 1)) - 1);
-        }
-        if (z) return w * 64 + 63 - __builtin_clzll(z);
+    int res = -1;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const int top = j - 64 * w;           // bits of word w below j
+        const uint64_t z0 = ~m[w];
+        const uint64_t z = top <= 0 ? 0ull : (top >= 64 ? z0 : (z0 & ((1ull << (top & 63)) - 1)));
+        if (z) res = w * 64 + 63 - __builtin_clzll(z);
     }
-    return -1;
+    return res;
 }
 
 // lowest index >= j whose bit is clear, or 256
 __device__ inline int next_zero(const uint64_t *m, int j)
 {
-    for (int w = j >> 6; w < 4; w++) {
-        uint64_t z = ~m[w];
-        if (w == (j >> 6)) z &= ~0ull << (j & 63);
-        if (z) return w * 64 + __builtin_ctzll(z);
+    int res = 256;
+#pragma unroll
+    for (int w = 3; w >= 0; w--) {
+        const int lo = j - 64 * w;            // first bit of word w at or above j
+        const uint64_t z0 = ~m[w];
+        const uint64_t z = lo >= 64 ? 0ull : (lo <= 0 ? z0 : (z0 & (~0ull << (lo & 63))));
+        if (z) res = w * 64 + __builtin_ctzll(z);
     }
-    return 256;
+    return res;
 }
 
 // number of set bits in [a, b]
 __device__ inline int pop_range(const uint64_t *m, int a, int b)
 {
     int n = 0;
-    for (int w = a >> 6; w <= (b >> 6); w++) {
-        uint64_t v = m[w];
-        if (w == (a >> 6)) v &= ~0ull << (a & 63);
-        if (w == (b >> 6)) v &= ((b & 63) == 63) ? ~0ull : ((1ull << ((b & 63) + 1)) - 1);
-        n += __popcll(v);
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const int lo = a - 64 * w, hi = b - 64 * w;
+        if (hi < 0 || lo > 63) continue;
+        const uint64_t lm = lo <= 0 ? ~0ull : (~0ull << (lo & 63));
+        const uint64_t hm = hi >= 63 ? ~0ull : ((1ull << ((hi + 1) & 63)) - 1);
+        n += __popcll(m[w] & lm & hm);
     }
     return n;
 }
@@ -502,19 +511,32 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
     __builtin_amdgcn_wave_barrier();
     auto base = [&](int i) -> uint8_t { return stage[i - (int)rwx]; };
 
-    // GATK block-averaged fractional errors of window position j (calculateFractionalErrorArray)
-    auto frac_at = [&](int j, double &vs, double &vi, double &vd) {
+    // bits of window position j = lane + 64c and of its neighbours, with c known at compile time
+    // after unrolling (one shift of a wave-uniform word instead of a lookup over four words)
+    auto bit_at = [&](const uint64_t *M, int c) -> bool { return (M[c] >> lane) & 1; };
+    auto bit_next = [&](const uint64_t *M, int c) -> bool {
+        const uint64_t up = c < 3 ? M[c < 3 ? c + 1 : 3] : 0ull;
+        return lane < 63 ? ((M[c] >> ((lane + 1) & 63)) & 1) : (up & 1);
+    };
+    auto bit_prev = [&](const uint64_t *M, int c) -> bool {
+        const uint64_t down = c > 0 ? M[c > 0 ? c - 1 : 0] : 0ull;
+        return lane > 0 ? ((M[c] >> ((lane - 1) & 63)) & 1) : (down >> 63);
+    };
+
+    // GATK block-averaged fractional errors of window position j = lane + 64c
+    // (calculateFractionalErrorArray)
+    auto frac_at = [&](int j, int c, double &vs, double &vi, double &vd) {
         int s0 = -1, e = -1;
-        if (mtest(IB, j)) {
+        if (bit_at(IB, c)) {
             const int bs = prev_zero(IB, j) + 1;
             const int be = next_zero(IB, j) - 1;
             e = be + 1 < len ? be + 1 : be;
             s0 = bs - 1 > 0 ? bs - 1 : 0;
-        } else if (j + 1 < len && mtest(IB, j + 1)) {
+        } else if (j + 1 < len && bit_next(IB, c)) {
             const int be = next_zero(IB, j + 1) - 1;
             e = be + 1 < len ? be + 1 : be;
             s0 = j;
-        } else if (j > 0 && mtest(IB, j - 1)) {
+        } else if (j > 0 && bit_prev(IB, c)) {
             const int bs = prev_zero(IB, j - 1) + 1;
             s0 = bs - 1 > 0 ? bs - 1 : 0;
             e = j;
@@ -525,9 +547,9 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
             vi = (double)pop_range(EI_, s0, e) / n;
             vd = (double)pop_range(ED, s0, e) / n;
         } else {
-            vs = mtest(ES, j) ? 1.0 : 0.0;
-            vi = mtest(EI_, j) ? 1.0 : 0.0;
-            vd = mtest(ED, j) ? 1.0 : 0.0;
+            vs = bit_at(ES, c) ? 1.0 : 0.0;
+            vi = bit_at(EI_, c) ? 1.0 : 0.0;
+            vd = bit_at(ED, c) ? 1.0 : 0.0;
         }
         if (a.keep_frac) {
             double *fo = a.s.frac + 3ull * (qs + rwx + j);
@@ -537,12 +559,14 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
 
     if constexpr (PAIRS) {
         // every lane stays in the loop: the appends are wavefront collectives
-        for (int c = 0; c < 4 && 64 * c < len; c++) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            if (64 * c >= len) break;
             const int j = lane + 64 * c;
             double vs = 0.0, vi = 0.0, vd = 0.0;
             const bool in = j < len;
-            if (in) frac_at(j, vs, vi, vd);
-            const bool emit = in && !(fl[c] & BF_MASKED) && !mtest(SM, j);
+            if (in) frac_at(j, c, vs, vi, vd);
+            const bool emit = in && !(fl[c] & BF_MASKED) && !bit_at(SM, c);
             emit_pairs(a, pr, rg, flags, neg, base, rwx, rwy, hx, hy, (uint16_t)(rwx + (in ? j : 0)), qv[c], vs, vi,
                        vd, emit);
         }
@@ -557,10 +581,10 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
         const int j = lane + 64 * c;
         if (j >= len) continue;
         double vs, vi, vd;
-        frac_at(j, vs, vi, vd);
+        frac_at(j, c, vs, vi, vd);
         const uint16_t bp = (uint16_t)(rwx + j);
         if (fl[c] & BF_MASKED) continue;
-        if (mtest(SM, j)) continue;
+        if (bit_at(SM, c)) continue;
         const Limbs f0 = to_limbs(vs, a.frac_f64), f1 = to_limbs(vi, a.frac_f64), f2 = to_limbs(vd, a.frac_f64);
         nid++;
         i0 += f1.p0; i1 += f1.p1; i2 += f1.p2;
