@@ -177,6 +177,20 @@ struct TableRows {
 
 } // namespace
 
+// per-batch device state: one set for the plain path (and batches run eagerly), one per staging
+// buffer of the batched engine, so two staged batches can be in flight on two streams
+struct BatchState {
+    DevBuf st_status, st_rwc, st_noins, st_refw, st_nerr, st_hmmw, st_bw, st_hqw, st_bflags, st_baq, st_state, st_frac;
+    DevBuf baq_lists, baq_counts, rg_buf;
+    fpd::ReadStateDev rs{};
+    void release()
+    {
+        DevBuf *all[] = {&st_status, &st_rwc, &st_noins, &st_refw, &st_nerr, &st_hmmw, &st_bw, &st_hqw,
+                         &st_bflags, &st_baq, &st_state, &st_frac, &baq_lists, &baq_counts, &rg_buf};
+        for (DevBuf *b : all) b->release();
+    }
+};
+
 struct fp_engine {
     int device = 0;
     fp_options opts;
@@ -195,12 +209,12 @@ struct fp_engine {
     // batch buffers (host batches are copied here)
     DevBuf in[18];
     // per-batch state
-    DevBuf st_status, st_rwc, st_noins, st_refw, st_nerr, st_hmmw, st_bw, st_hqw, st_bflags, st_baq, st_state, st_frac;
-    DevBuf counters, baq_lists, baq_counts, scratch, scratch_gen, rg_buf;
+    BatchState ps;                      // the plain path's (and that of batches run eagerly)
+    BatchState *last = &ps;             // the state of the last processed batch (inspection)
+    DevBuf counters, scratch, scratch_gen;
     uint32_t *host_counts = nullptr;   // pinned readback area (64 u32)
     uint32_t last_num_reads = 0;
     uint64_t last_num_bases = 0, last_num_quals = 0;
-    fpd::ReadStateDev rs{};
 
     // sort aggregation: emitted pairs per table, one ping-pong pair buffer, radix histograms,
     // counters {append cursor x3, overflow, kept x3}
@@ -232,9 +246,15 @@ struct fp_engine {
         uint32_t n = 0;
         uint64_t ncig = 0, nbases = 0, nquals = 0;
         hipEvent_t filled = nullptr, consumed = nullptr;
-        hipGraphExec_t gfront = nullptr, gback = nullptr;
+        hipGraphExec_t gfront = nullptr, gbaq = nullptr, gcov = nullptr;
         int g_wpe = 0, g_bits = 0;
         uint64_t g_epoch = 0;
+        BatchState bs;                  // this buffer's per-batch state
+        hipStream_t stream = nullptr;   // this buffer's compute stream
+        hipEvent_t baq_done = nullptr;  // after its BAQ kernels (they share the band scratch)
+        hipEvent_t cov_done = nullptr;  // after its covariate stage (sort buffers are shared)
+        hipEvent_t t0 = nullptr, t1 = nullptr;   // around its stage 2 (timing)
+        bool back_pending = false;      // stage-2 time not yet collected
     } stg[2];
     bool batched = false;
     int cur = 0;
@@ -293,37 +313,38 @@ fp_status alloc_tables(fp_engine *e)
 
 // per-read and per-base state for n reads; the per-quality fractional-error array is needed for
 // inspection (debug_arrays) and as working memory of windows longer than 256 bases
-fp_status ensure_state(fp_engine *e, uint32_t n, uint64_t nbases, uint64_t nquals, bool need_frac = true)
+fp_status ensure_state(fp_engine *e, BatchState &S, uint32_t n, uint64_t nbases, uint64_t nquals,
+                       bool need_frac = true)
 {
-    HIP_TRY(e->st_status.ensure(n));
-    HIP_TRY(e->st_rwc.ensure(4ull * n));
-    HIP_TRY(e->st_noins.ensure(4ull * n));
-    HIP_TRY(e->st_refw.ensure(4ull * n));
-    HIP_TRY(e->st_nerr.ensure(2ull * n));
-    HIP_TRY(e->st_hmmw.ensure(4ull * n));
-    HIP_TRY(e->st_bw.ensure(2ull * n));
-    HIP_TRY(e->st_hqw.ensure(4ull * n));
-    HIP_TRY(e->st_bflags.ensure(nbases + 8));
-    HIP_TRY(e->st_baq.ensure(nquals + 8));
-    HIP_TRY(e->st_state.ensure(4 * (nquals + 8)));
-    if (need_frac) HIP_TRY(e->st_frac.ensure(24 * (nquals + 8)));
+    HIP_TRY(S.st_status.ensure(n));
+    HIP_TRY(S.st_rwc.ensure(4ull * n));
+    HIP_TRY(S.st_noins.ensure(4ull * n));
+    HIP_TRY(S.st_refw.ensure(4ull * n));
+    HIP_TRY(S.st_nerr.ensure(2ull * n));
+    HIP_TRY(S.st_hmmw.ensure(4ull * n));
+    HIP_TRY(S.st_bw.ensure(2ull * n));
+    HIP_TRY(S.st_hqw.ensure(4ull * n));
+    HIP_TRY(S.st_bflags.ensure(nbases + 8));
+    HIP_TRY(S.st_baq.ensure(nquals + 8));
+    HIP_TRY(S.st_state.ensure(4 * (nquals + 8)));
+    if (need_frac) HIP_TRY(S.st_frac.ensure(24 * (nquals + 8)));
     HIP_TRY(e->counters.ensure(64));
-    HIP_TRY(e->baq_lists.ensure(4ull * 4 * std::max<uint32_t>(n, 1)));
-    HIP_TRY(e->rg_buf.ensure(4ull * (4ull * std::max<uint32_t>(e->num_rg, 1) + 2 + n)));
-    HIP_TRY(e->baq_counts.ensure(4 * 400));   // class sizes, (class, length bucket) sizes, offsets, cursors
-    fpd::ReadStateDev &s = e->rs;
-    s.status = e->st_status.as<uint8_t>();
-    s.rwc = e->st_rwc.as<uint32_t>();
-    s.noins = e->st_noins.as<uint32_t>();
-    s.refw = e->st_refw.as<uint32_t>();
-    s.num_errors = e->st_nerr.as<uint16_t>();
-    s.hmmw = e->st_hmmw.as<uint32_t>();
-    s.bandwidth = e->st_bw.as<uint16_t>();
-    s.hqw = e->st_hqw.as<uint32_t>();
-    s.base_flags = e->st_bflags.as<uint8_t>();
-    s.baq = e->st_baq.as<uint8_t>();
-    s.baq_state = e->st_state.as<uint32_t>();
-    s.frac = e->st_frac.as<double>();
+    HIP_TRY(S.baq_lists.ensure(4ull * 4 * std::max<uint32_t>(n, 1)));
+    HIP_TRY(S.rg_buf.ensure(4ull * (4ull * std::max<uint32_t>(e->num_rg, 1) + 2 + n)));
+    HIP_TRY(S.baq_counts.ensure(4 * 400));   // class sizes, (class, length bucket) sizes, offsets, cursors
+    fpd::ReadStateDev &r = S.rs;
+    r.status = S.st_status.as<uint8_t>();
+    r.rwc = S.st_rwc.as<uint32_t>();
+    r.noins = S.st_noins.as<uint32_t>();
+    r.refw = S.st_refw.as<uint32_t>();
+    r.num_errors = S.st_nerr.as<uint16_t>();
+    r.hmmw = S.st_hmmw.as<uint32_t>();
+    r.bandwidth = S.st_bw.as<uint16_t>();
+    r.hqw = S.st_hqw.as<uint32_t>();
+    r.base_flags = S.st_bflags.as<uint8_t>();
+    r.baq = S.st_baq.as<uint8_t>();
+    r.baq_state = S.st_state.as<uint32_t>();
+    r.frac = S.st_frac.as<double>();
     return FP_OK;
 }
 
@@ -391,12 +412,12 @@ static uint64_t band_scratch_doubles(int wpe, uint32_t n, uint32_t rows_cap)
 }
 
 // stage 1 (graph-safe): per-read prep, BAQ tag reset, band classification
-fp_status launch_front(fp_engine *e, const fpd::BatchDev &b, const Plan &p, uint32_t *counters, hipStream_t st)
+fp_status launch_front(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, uint32_t *counters, hipStream_t st)
 {
-    HIP_TRY(fpd_fill_u32(e->baq_counts.as<uint32_t>(), 144, 0u, st));
-    HIP_TRY(fpd_launch_prep(&b, &e->genome, &e->rs, counters, p.rows_cap, st));
-    HIP_TRY(fpd_fill_u8(e->rs.baq, p.nquals, p.live, 0xff, st));
-    HIP_TRY(fpd_launch_baq_classify(p.n, p.live, &e->rs, e->baq_lists.as<uint32_t>(), e->baq_counts.as<uint32_t>(),
+    HIP_TRY(fpd_fill_u32(S.baq_counts.as<uint32_t>(), 144, 0u, st));
+    HIP_TRY(fpd_launch_prep(&b, &e->genome, &S.rs, counters, p.rows_cap, st));
+    HIP_TRY(fpd_fill_u8(S.rs.baq, p.nquals, p.live, 0xff, st));
+    HIP_TRY(fpd_launch_baq_classify(p.n, p.live, &S.rs, S.baq_lists.as<uint32_t>(), S.baq_counts.as<uint32_t>(),
                                     std::max<uint32_t>(p.n, 1), st));
     return FP_OK;
 }
@@ -432,14 +453,14 @@ fp_status ensure_pairs(fp_engine *e, uint64_t cap)
 }
 
 // the covariate stage: direct LDS-privatised table updates, or pairs -> radix sort -> reduce_by_key
-fp_status launch_cov_stage(fp_engine *e, const fpd::BatchDev &b, const Plan &p, int sort_bits, uint32_t *counters,
+fp_status launch_cov_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, int sort_bits, uint32_t *counters,
                            hipStream_t st, hipEvent_t sort_mark)
 {
     const uint32_t keep = e->opts.debug_arrays ? 1u : 0u;
     const uint32_t maxlen = std::min<uint32_t>(p.rows_cap, 499);
     if (e->opts.aggregation != FP_AGG_SORT) {
-        HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, keep, counters,
-                               e->rg_buf.as<uint32_t>(), maxlen, nullptr, nullptr, nullptr, 0, st));
+        HIP_TRY(fpd_launch_cov(&b, &S.rs, &e->tables, e->opts.frac_error_f64, keep, counters,
+                               S.rg_buf.as<uint32_t>(), maxlen, nullptr, nullptr, nullptr, 0, st));
         return FP_OK;
     }
     uint32_t *sc = e->sctr.as<uint32_t>();
@@ -447,7 +468,7 @@ fp_status launch_cov_stage(fp_engine *e, const fpd::BatchDev &b, const Plan &p, 
     uint32_t *pkeys[3];
     double *pvals[3];
     for (int t = 0; t < 3; t++) { pkeys[t] = e->pk[t].as<uint32_t>(); pvals[t] = e->pv[t].as<double>(); }
-    HIP_TRY(fpd_launch_cov(&b, &e->rs, &e->tables, e->opts.frac_error_f64, keep, counters, e->rg_buf.as<uint32_t>(),
+    HIP_TRY(fpd_launch_cov(&b, &S.rs, &e->tables, e->opts.frac_error_f64, keep, counters, S.rg_buf.as<uint32_t>(),
                            maxlen, pkeys, pvals, sc, e->pair_cap, st));
     if (sort_mark) HIP_TRY(hipEventRecord(sort_mark, st));
     for (int t = 0; t < 3; t++) {
@@ -474,35 +495,45 @@ fp_status collect_sort_counts(fp_engine *e, int sort_bits)
     return FP_OK;
 }
 
-fp_status launch_back(fp_engine *e, const fpd::BatchDev &b, const Plan &p, int wpe, int sort_bits,
-                      uint32_t *counters, hipStream_t st, hipEvent_t *marks, const uint32_t *known_counts)
+// stage 2a (graph-safe): the BAQ band classes and cap/recode
+fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, int wpe,
+                           uint32_t *counters, hipStream_t st, hipEvent_t mark, const uint32_t *known_counts)
 {
     const uint32_t cap = std::max<uint32_t>(p.n, 1);
     for (int c = 0; c < 3; c++) {
         if (known_counts && known_counts[c] == 0) continue;   // eager launches skip empty classes
-        HIP_TRY(fpd_launch_baq_hmm(c, &b, &e->genome, &e->rs, e->baq_lists.as<uint32_t>() + (uint64_t)c * cap,
-                                   e->baq_counts.as<uint32_t>() + c, e->scratch.as<double>(),
+        HIP_TRY(fpd_launch_baq_hmm(c, &b, &e->genome, &S.rs, S.baq_lists.as<uint32_t>() + (uint64_t)c * cap,
+                                   S.baq_counts.as<uint32_t>() + c, e->scratch.as<double>(),
                                    band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c],
                                    band_waves(wpe, c, p.n), (unsigned long long *)(counters + 8), wpe,
                                    e->opts.disable_baq_staging ? 0 : 1, st));
     }
-    if (marks) HIP_TRY(hipEventRecord(marks[0], st));
-    HIP_TRY(fpd_launch_baq_post(&b, &e->rs, 1u, st));
+    if (mark) HIP_TRY(hipEventRecord(mark, st));
+    HIP_TRY(fpd_launch_baq_post(&b, &S.rs, 1u, st));
+    return FP_OK;
+}
+
+// stage 2 = 2a + 2b (covariates); marks: after the BAQ kernels, after cap/recode, after pair emission
+fp_status launch_back(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, int wpe, int sort_bits,
+                      uint32_t *counters, hipStream_t st, hipEvent_t *marks, const uint32_t *known_counts)
+{
+    fp_status rc = launch_baq_stage(e, S, b, p, wpe, counters, st, marks ? marks[0] : nullptr, known_counts);
+    if (rc != FP_OK) return rc;
     if (marks) HIP_TRY(hipEventRecord(marks[1], st));
-    return launch_cov_stage(e, b, p, sort_bits, counters, st, marks ? marks[2] : nullptr);
+    return launch_cov_stage(e, S, b, p, sort_bits, counters, st, marks ? marks[2] : nullptr);
 }
 
 // bands wider than 33 cells (rare: large indels), sized from the widest such band
-fp_status launch_generic(fp_engine *e, const fpd::BatchDev &b, const Plan &p, uint32_t count, uint32_t wgen,
+fp_status launch_generic(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, uint32_t count, uint32_t wgen,
                          uint32_t *counters, hipStream_t st)
 {
     if (count == 0) return FP_OK;
     const uint32_t waves = std::min<uint32_t>((count + 63) / 64, 256u);
     const uint64_t slot = band_slot(3, p.rows_cap, wgen);
     HIP_TRY(e->scratch_gen.ensure(slot * waves * 8 + 8));
-    HIP_TRY(fpd_launch_baq_hmm(3, &b, &e->genome, &e->rs,
-                               e->baq_lists.as<uint32_t>() + 3ull * std::max<uint32_t>(p.n, 1),
-                               e->baq_counts.as<uint32_t>() + 3, e->scratch_gen.as<double>(), slot, p.rows_cap, wgen,
+    HIP_TRY(fpd_launch_baq_hmm(3, &b, &e->genome, &S.rs,
+                               S.baq_lists.as<uint32_t>() + 3ull * std::max<uint32_t>(p.n, 1),
+                               S.baq_counts.as<uint32_t>() + 3, e->scratch_gen.as<double>(), slot, p.rows_cap, wgen,
                                waves, (unsigned long long *)(counters + 8), e->wpe, 0, st));
     return FP_OK;
 }
@@ -511,9 +542,11 @@ fp_status launch_generic(fp_engine *e, const fpd::BatchDev &b, const Plan &p, ui
 fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_hint)
 {
     const uint32_t n = v->num_reads;
-    fp_status rc = ensure_state(e, n, v->num_read_bases, v->num_qualities);
+    BatchState &S = e->ps;
+    fp_status rc = ensure_state(e, S, n, v->num_read_bases, v->num_qualities);
     if (rc != FP_OK) return rc;
     e->last_num_reads = n;
+    e->last = &S;
     e->last_num_bases = v->num_read_bases;
     e->last_num_quals = v->num_qualities;
     hipStream_t st = e->stream;
@@ -524,11 +557,11 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     HIP_TRY(fpd_fill_u32(counters, 16, 0u, st));
     HIP_TRY(hipEventRecord(e->ev[6], st));
     Plan p{n, v->num_qualities, nullptr, max_read_len_hint};   // rows known after the front stage
-    rc = launch_front(e, b, p, counters, st);
+    rc = launch_front(e, S, b, p, counters, st);
     if (rc != FP_OK) return rc;
     HIP_TRY(hipEventRecord(e->ev[7], st));
     uint32_t *hc = e->host_counts;
-    HIP_TRY(hipMemcpyAsync(hc, e->baq_counts.p, 32, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hc, S.baq_counts.p, 32, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(hc + 8, counters, 32, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     p.rows_cap = std::max<uint32_t>(std::max(max_read_len_hint, hc[8 + 3]), 1);
@@ -540,10 +573,10 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     }
 
     HIP_TRY(hipEventRecord(e->ev[8], st));
-    rc = launch_generic(e, b, p, hc[3], hc[4], counters, st);
+    rc = launch_generic(e, S, b, p, hc[3], hc[4], counters, st);
     if (rc != FP_OK) return rc;
     hipEvent_t marks[3] = {e->ev[9], e->ev[2], e->ev[10]};
-    rc = launch_back(e, b, p, e->wpe, e->sort_bits, counters, st, marks, hc);
+    rc = launch_back(e, S, b, p, e->wpe, e->sort_bits, counters, st, marks, hc);
     if (rc != FP_OK) return rc;
     HIP_TRY(hipEventRecord(e->ev[3], st));
     uint32_t *post = e->host_counts + 16;
@@ -626,10 +659,23 @@ fp_status batched_alloc(fp_engine *e)
         if (!s.live_host) HIP_TRY(hipHostMalloc((void **)&s.live_host, 16, hipHostMallocDefault));
         if (!s.filled) HIP_TRY(hipEventCreateWithFlags(&s.filled, hipEventDisableTiming));
         if (!s.consumed) HIP_TRY(hipEventCreateWithFlags(&s.consumed, hipEventDisableTiming));
+        if (!s.baq_done) HIP_TRY(hipEventCreateWithFlags(&s.baq_done, hipEventDisableTiming));
+        if (!s.cov_done) HIP_TRY(hipEventCreateWithFlags(&s.cov_done, hipEventDisableTiming));
+        if (!s.t0) HIP_TRY(hipEventCreate(&s.t0));
+        if (!s.t1) HIP_TRY(hipEventCreate(&s.t1));
+        // The first buffer runs on the engine stream and the second on a stream of its own: a
+        // process has few hardware queues (streams beyond them share one and run in turn), so
+        // the engine adds one compute stream, not two.  profile_stages keeps every kernel on the
+        // engine stream (clean per-kernel times).
+        if (!s.stream) {
+            if (&s == &e->stg[0] || e->opts.profile_stages) s.stream = e->stream;
+            else HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+        }
+        fp_status rc = ensure_state(e, s.bs, e->cap_reads, e->cap_bases, e->cap_quals,
+                                    e->opts.debug_arrays != 0 || e->rows_cap > 256);
+        if (rc != FP_OK) return rc;
     }
-    fp_status rc = ensure_state(e, e->cap_reads, e->cap_bases, e->cap_quals,
-                                e->opts.debug_arrays != 0 || e->rows_cap > 256);
-    if (rc != FP_OK) return rc;
+    fp_status rc = FP_OK;
     HIP_TRY(e->scratch.ensure(band_scratch_doubles(std::max(3, e->wpe), e->cap_reads, e->rows_cap) * 8 + 8));
     if (e->opts.aggregation == FP_AGG_SORT) {
         rc = ensure_pairs(e, 3 * e->cap_quals);
@@ -645,13 +691,13 @@ fp_status batched_alloc(fp_engine *e)
 }
 
 template <class F>
-fp_status capture(fp_engine *e, hipGraphExec_t *out, F body)
+fp_status capture(hipStream_t st, hipGraphExec_t *out, F body)
 {
     if (*out) { (void)hipGraphExecDestroy(*out); *out = nullptr; }
-    HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const fp_status rc = body();
     hipGraph_t g = nullptr;
-    const hipError_t he = hipStreamEndCapture(e->stream, &g);
+    const hipError_t he = hipStreamEndCapture(st, &g);
     if (rc != FP_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
     HIP_TRY(he);
     hipGraphExec_t ex = nullptr;
@@ -662,17 +708,20 @@ fp_status capture(fp_engine *e, hipGraphExec_t *out, F body)
     return FP_OK;
 }
 
-// finish the timing of the last staged batch (its stream work has completed)
-void tune_collect(fp_engine *e)
+// the stage-2 time of a staging buffer's last batch, once its stream has passed it
+void collect_back_time(fp_engine *e, fp_engine::Stage &s)
 {
-    if (e->back_pending) {
-        e->stats.baq_ms += elapsed(e->bev[0], e->bev[1]);
-        e->back_pending = false;
-    }
-    if (e->tune_pending < 0) return;
-    const double per_read = elapsed(e->bev[0], e->bev[1]) / std::max<uint32_t>(e->tune_pending_reads, 1);
-    if (e->tune_kind == 0) {
-        e->tune_ms_per_read[e->tune_pending] = per_read;
+    if (!s.back_pending) return;
+    (void)hipEventSynchronize(s.t1);
+    e->stats.baq_ms += elapsed(s.t0, s.t1);
+    s.back_pending = false;
+}
+
+// record one autotuner candidate's stage-2 time per read
+void tune_record(fp_engine *e, int kind, int cand, double per_read)
+{
+    if (kind == 0) {
+        e->tune_ms_per_read[cand] = per_read;
         if (e->tune_ms_per_read[0] > 0 && e->tune_ms_per_read[1] > 0 && e->tune_ms_per_read[2] > 0) {
             int best = 0;
             for (int i = 1; i < 3; i++) if (e->tune_ms_per_read[i] < e->tune_ms_per_read[best]) best = i;
@@ -680,7 +729,7 @@ void tune_collect(fp_engine *e)
             e->tuned = true;
         }
     } else {
-        e->sort_tune_ms[e->tune_pending] = per_read;
+        e->sort_tune_ms[cand] = per_read;
         bool all = true;
         for (int i = 0; i < 5; i++) all = all && e->sort_tune_ms[i] > 0;
         if (all) {
@@ -690,17 +739,37 @@ void tune_collect(fp_engine *e)
             e->sort_tuned = true;
         }
     }
-    e->tune_pending = -1;
 }
 
-// run the current staging buffer (if it holds reads) and switch to the other one
+// finish a staging buffer's turn: its batch is consumed and the other buffer fills next
+static void stage_done(fp_engine *e, fp_engine::Stage &s)
+{
+    e->last_num_reads = s.n;
+    e->last_num_bases = s.nbases;
+    e->last_num_quals = s.nquals;
+    e->last = &s.bs;
+    s.n = 0;
+    s.ncig = s.nbases = s.nquals = 0;
+    e->cur ^= 1;
+    e->postprocessed = false;
+}
+
+// Run the current staging buffer (if it holds reads) and switch to the other one.  Each buffer has
+// its own compute stream and per-batch state, so one buffer's covariate stage and the other's prep
+// and BAQ run at the same time.  The BAQ kernels of the two buffers share the band scratch and take
+// turns (event baq_done); so do the covariate stages of the sort aggregation (its pair buffers are
+// shared, event cov_done).  The direct tables take both buffers' atomic adds in any order.
 fp_status batched_flush(fp_engine *e)
 {
     fp_engine::Stage &s = e->stg[e->cur];
+    fp_engine::Stage &o = e->stg[e->cur ^ 1];
     if (s.n == 0) return FP_OK;
     fp_status rc = batched_alloc(e);
     if (rc != FP_OK) return rc;
-    hipStream_t st = e->stream;
+    const bool prof = e->opts.profile_stages != 0;
+    hipStream_t st = prof ? e->stream : s.stream;
+    BatchState &S = s.bs;
+    collect_back_time(e, s);
     s.live_host[0] = s.n;
     s.live_host[1] = (uint32_t)s.nquals;
     s.live_host[2] = (uint32_t)(s.nquals >> 32);
@@ -713,14 +782,15 @@ fp_status batched_flush(fp_engine *e)
     const Plan p{e->cap_reads, e->cap_quals, s.live.as<uint32_t>(), e->rows_cap};
     uint32_t *counters = e->bcounters.as<uint32_t>();
     if (s.g_epoch != e->epoch) {
-        rc = capture(e, &s.gfront, [&] { return launch_front(e, b, p, counters, st); });
+        rc = capture(st, &s.gfront, [&] { return launch_front(e, S, b, p, counters, st); });
         if (rc != FP_OK) return rc;
-        if (s.gback) { (void)hipGraphExecDestroy(s.gback); s.gback = nullptr; }
+        if (s.gbaq) { (void)hipGraphExecDestroy(s.gbaq); s.gbaq = nullptr; }
+        if (s.gcov) { (void)hipGraphExecDestroy(s.gcov); s.gcov = nullptr; }
         s.g_epoch = e->epoch;
     }
     HIP_TRY(hipEventRecord(e->bev[2], st));
-    if (e->opts.profile_stages) {
-        rc = launch_front(e, b, p, counters, st);
+    if (prof) {
+        rc = launch_front(e, S, b, p, counters, st);
         if (rc != FP_OK) return rc;
     } else {
         HIP_TRY(hipGraphLaunch(s.gfront, st));
@@ -728,10 +798,9 @@ fp_status batched_flush(fp_engine *e)
     HIP_TRY(hipEventRecord(e->bev[3], st));
     // the band classes and the longest read: one small readback per staged batch
     uint32_t *hc = e->host_counts;
-    HIP_TRY(hipMemcpyAsync(hc, e->baq_counts.p, 32, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hc, S.baq_counts.p, 32, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(hc + 8, counters, 64, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    tune_collect(e);
     e->stats.read_filter_ms += elapsed(e->bev[2], e->bev[3]);
     e->stats.prep_kernel_ms += elapsed(e->bev[2], e->bev[3]);
     e->stats.prep_launches += 1;
@@ -743,7 +812,7 @@ fp_status batched_flush(fp_engine *e)
     e->stats.baq_reads = e->eager_baq + hc[8 + 2];
 
     // autotuner: BAQ waves per SIMD first, then the radix digit width of the sort aggregation; each
-    // candidate runs one staged batch eagerly and is timed per read
+    // candidate runs one staged batch eagerly, alone on the device, and is timed per read
     int wpe = e->wpe, bits = e->sort_bits;
     int cand = -1, kind = 0;
     if (!e->tuned) {
@@ -753,20 +822,21 @@ fp_status batched_flush(fp_engine *e)
         for (int i = 0; i < 5; i++) if (e->sort_tune_ms[i] == 0) { cand = i; break; }
         if (cand >= 0) { bits = SORT_CANDIDATES[cand]; kind = 1; }
     }
-    if (e->opts.profile_stages) {
+    const bool sorting = e->opts.aggregation == FP_AGG_SORT;
+    if (prof) {
         // eager stage 2 with an event after the BAQ kernels and after cap/recode
         HIP_TRY(hipEventRecord(e->ev[8], st));
-        rc = launch_generic(e, b, p, hc[3], hc[4], counters, st);
+        rc = launch_generic(e, S, b, p, hc[3], hc[4], counters, st);
         if (rc != FP_OK) return rc;
         hipEvent_t marks[3] = {e->ev[9], e->ev[2], e->ev[10]};
-        rc = launch_back(e, b, p, wpe, bits, counters, st, marks, hc);
+        rc = launch_back(e, S, b, p, wpe, bits, counters, st, marks, hc);
         if (rc != FP_OK) return rc;
         HIP_TRY(hipEventRecord(e->ev[3], st));
         HIP_TRY(hipStreamSynchronize(st));
         const uint32_t launches = (hc[0] ? 1 : 0) + (hc[1] ? 1 : 0) + (hc[2] ? 1 : 0) + (hc[3] ? 1 : 0);
         e->stats.baq_hmm_kernel_ms += elapsed(e->ev[8], e->ev[9]);
         e->stats.baq_hmm_launches += launches;
-        if (e->opts.aggregation == FP_AGG_SORT) {
+        if (sorting) {
             e->stats.cov_kernel_ms += elapsed(e->ev[2], e->ev[10]);
             e->stats.covariates_sort_ms += elapsed(e->ev[10], e->ev[3]);
             rc = collect_sort_counts(e, bits);
@@ -778,57 +848,62 @@ fp_status batched_flush(fp_engine *e)
         e->stats.baq_ms += elapsed(e->ev[8], e->ev[2]);
         e->stats.covariates_ms += elapsed(e->ev[2], e->ev[3]);
         HIP_TRY(hipEventRecord(s.consumed, st));
-        e->last_num_reads = s.n;
-        e->last_num_bases = s.nbases;
-        e->last_num_quals = s.nquals;
-        s.n = 0;
-        s.ncig = s.nbases = s.nquals = 0;
-        e->cur ^= 1;
-        e->postprocessed = false;
+        stage_done(e, s);
         return FP_OK;
     }
-    rc = launch_generic(e, b, p, hc[3], hc[4], counters, st);
+    if (cand >= 0) HIP_TRY(hipStreamSynchronize(o.stream));   // the candidate runs alone
+    // the band scratch is free once the other buffer's BAQ kernels are done
+    HIP_TRY(hipStreamWaitEvent(st, o.baq_done, 0));
+    HIP_TRY(hipEventRecord(s.t0, st));
+    rc = launch_generic(e, S, b, p, hc[3], hc[4], counters, st);
     if (rc != FP_OK) return rc;
-    HIP_TRY(hipEventRecord(e->bev[0], st));
     if (cand < 0) {
-        if (!s.gback || s.g_wpe != wpe || s.g_bits != bits) {
-            rc = capture(e, &s.gback,
-                         [&] { return launch_back(e, b, p, wpe, bits, counters, st, nullptr, nullptr); });
+        if (!s.gbaq || s.g_wpe != wpe || !s.gcov || s.g_bits != bits) {
+            rc = capture(st, &s.gbaq,
+                         [&] { return launch_baq_stage(e, S, b, p, wpe, counters, st, nullptr, nullptr); });
+            if (rc != FP_OK) return rc;
+            rc = capture(st, &s.gcov,
+                         [&] { return launch_cov_stage(e, S, b, p, bits, counters, st, nullptr); });
             if (rc != FP_OK) return rc;
             s.g_wpe = wpe;
             s.g_bits = bits;
         }
-        HIP_TRY(hipGraphLaunch(s.gback, st));
+        HIP_TRY(hipGraphLaunch(s.gbaq, st));
+        HIP_TRY(hipEventRecord(s.baq_done, st));
+        if (sorting) HIP_TRY(hipStreamWaitEvent(st, o.cov_done, 0));
+        HIP_TRY(hipGraphLaunch(s.gcov, st));
+        HIP_TRY(hipEventRecord(s.cov_done, st));
+        HIP_TRY(hipEventRecord(s.t1, st));
+        s.back_pending = true;
     } else {
-        rc = launch_back(e, b, p, wpe, bits, counters, st, nullptr, nullptr);   // timed candidate, eager
+        rc = launch_baq_stage(e, S, b, p, wpe, counters, st, nullptr, nullptr);   // timed candidate, eager
         if (rc != FP_OK) return rc;
-        e->tune_pending = cand;
-        e->tune_kind = kind;
-        e->tune_pending_reads = s.n;
+        HIP_TRY(hipEventRecord(s.baq_done, st));
+        rc = launch_cov_stage(e, S, b, p, bits, counters, st, nullptr);
+        if (rc != FP_OK) return rc;
+        HIP_TRY(hipEventRecord(s.cov_done, st));
+        HIP_TRY(hipEventRecord(s.t1, st));
+        HIP_TRY(hipEventSynchronize(s.t1));
+        const double ms = elapsed(s.t0, s.t1);
+        e->stats.baq_ms += ms;
+        tune_record(e, kind, cand, ms / std::max<uint32_t>(s.n, 1));
     }
-    HIP_TRY(hipEventRecord(e->bev[1], st));
-    e->back_pending = true;
     HIP_TRY(hipEventRecord(s.consumed, st));
-    e->last_num_reads = s.n;
-    e->last_num_bases = s.nbases;
-    e->last_num_quals = s.nquals;
-    s.n = 0;
-    s.ncig = s.nbases = s.nquals = 0;
-    e->cur ^= 1;
-    e->postprocessed = false;
+    stage_done(e, s);
     return FP_OK;
 }
 
-// wait for all staged work; collect the stage-2 time and the BAQ cell count
+// wait for all staged work; collect the stage-2 times and the BAQ cell count
 fp_status batched_sync(fp_engine *e)
 {
     fp_status rc = batched_flush(e);
     if (rc != FP_OK) return rc;
     HIP_TRY(hipStreamSynchronize(e->copy_stream));
+    for (auto &s : e->stg)
+        if (s.stream) HIP_TRY(hipStreamSynchronize(s.stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (!e->staged_ready) return FP_OK;
-    const bool had_timing = e->tune_pending >= 0 || e->last_num_reads;
-    tune_collect(e);
+    for (auto &s : e->stg) collect_back_time(e, s);
     uint32_t *hc = e->host_counts + 32;
     HIP_TRY(hipMemcpy(hc, e->bcounters.p, 64, hipMemcpyDeviceToHost));
     if (hc[4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
@@ -838,7 +913,6 @@ fp_status batched_sync(fp_engine *e)
     uint64_t cells;
     memcpy(&cells, hc + 8, 8);
     e->stats.baq_cells = e->eager_cells + cells;
-    (void)had_timing;
     return FP_OK;
 }
 
@@ -1457,7 +1531,7 @@ static fp_status debug_bytes(fp_engine *e, int item, uint64_t *bytes)
     switch (item) {
     case FP_DBG_ACTIVE_READS: {
         std::vector<uint8_t> st(n);
-        if (n) HIP_TRY(hipMemcpy(st.data(), e->rs.status, n, hipMemcpyDeviceToHost));
+        if (n) HIP_TRY(hipMemcpy(st.data(), e->last->rs.status, n, hipMemcpyDeviceToHost));
         uint64_t c = 0;
         for (uint8_t s : st) c += s == fpd::RS_ACTIVE;
         *bytes = 4 * c;
@@ -1495,7 +1569,7 @@ fp_status fp_engine_debug_fetch(fp_engine *e, int item, void *dst, uint64_t byte
     switch (item) {
     case FP_DBG_ACTIVE_READS: {
         std::vector<uint8_t> st(n);
-        if (n) HIP_TRY(hipMemcpy(st.data(), e->rs.status, n, hipMemcpyDeviceToHost));
+        if (n) HIP_TRY(hipMemcpy(st.data(), e->last->rs.status, n, hipMemcpyDeviceToHost));
         uint32_t *o = (uint32_t *)dst;
         uint64_t c = 0;
         for (uint64_t i = 0; i < n; i++) if (st[i] == fpd::RS_ACTIVE) o[c++] = (uint32_t)i;
@@ -1503,23 +1577,23 @@ fp_status fp_engine_debug_fetch(fp_engine *e, int item, void *dst, uint64_t byte
     }
     case FP_DBG_ERROR_FLAGS: case FP_DBG_ACTIVE_MASK: {
         std::vector<uint8_t> f(need);
-        if (need) HIP_TRY(hipMemcpy(f.data(), e->rs.base_flags, need, hipMemcpyDeviceToHost));
+        if (need) HIP_TRY(hipMemcpy(f.data(), e->last->rs.base_flags, need, hipMemcpyDeviceToHost));
         uint8_t *o = (uint8_t *)dst;
         for (uint64_t i = 0; i < need; i++)
             o[i] = item == FP_DBG_ERROR_FLAGS ? (f[i] & 7) : ((f[i] & fpd::BF_MASKED) ? 0 : 1);
         return FP_OK;
     }
-    case FP_DBG_READ_WINDOW: src = e->rs.rwc; break;
-    case FP_DBG_NOINS_WINDOW: src = e->rs.noins; break;
-    case FP_DBG_REF_WINDOW: src = e->rs.refw; break;
-    case FP_DBG_HMM_WINDOW: src = e->rs.hmmw; break;
-    case FP_DBG_HQ_WINDOW: src = e->rs.hqw; break;
-    case FP_DBG_NUM_ERRORS: src = e->rs.num_errors; break;
-    case FP_DBG_BANDWIDTH: src = e->rs.bandwidth; break;
-    case FP_DBG_BAQ: src = e->rs.baq; break;
+    case FP_DBG_READ_WINDOW: src = e->last->rs.rwc; break;
+    case FP_DBG_NOINS_WINDOW: src = e->last->rs.noins; break;
+    case FP_DBG_REF_WINDOW: src = e->last->rs.refw; break;
+    case FP_DBG_HMM_WINDOW: src = e->last->rs.hmmw; break;
+    case FP_DBG_HQ_WINDOW: src = e->last->rs.hqw; break;
+    case FP_DBG_NUM_ERRORS: src = e->last->rs.num_errors; break;
+    case FP_DBG_BANDWIDTH: src = e->last->rs.bandwidth; break;
+    case FP_DBG_BAQ: src = e->last->rs.baq; break;
     case FP_DBG_FRAC_ERRORS:
         if (!e->opts.debug_arrays) return fail(FP_ERR_STATE, "fractional errors are kept only with debug_arrays = 1");
-        src = e->rs.frac;
+        src = e->last->rs.frac;
         break;
     default: return fail(FP_ERR_INVALID_ARG, "unknown debug item");
     }
@@ -1550,12 +1624,14 @@ void fp_engine_destroy(fp_engine *e)
 {
     if (!e) return;
     (void)hipSetDevice(e->device);
+    for (auto &s : e->stg)
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     DevBuf *bufs[] = {&e->g_word_offset, &e->g_length, &e->g_words, &e->g_foff, &e->g_fstart, &e->g_fstop,
-                      &e->g_mel, &e->st_status, &e->st_rwc, &e->st_noins, &e->st_refw, &e->st_nerr, &e->st_hmmw,
-                      &e->st_bw, &e->st_hqw, &e->st_bflags, &e->st_baq, &e->st_state, &e->st_frac, &e->counters,
-                      &e->baq_lists, &e->baq_counts, &e->scratch, &e->rg_buf};
+                      &e->g_mel, &e->counters, &e->scratch, &e->tk, &e->tv, &e->shist, &e->stot, &e->sctr};
     for (DevBuf *b : bufs) b->release();
+    for (int t = 0; t < 3; t++) { e->pk[t].release(); e->pv[t].release(); }
+    e->ps.release();
     e->scratch_gen.release();
     e->bcounters.release();
     for (auto &s : e->stg) {
@@ -1565,7 +1641,12 @@ void fp_engine_destroy(fp_engine *e)
         if (s.filled) (void)hipEventDestroy(s.filled);
         if (s.consumed) (void)hipEventDestroy(s.consumed);
         if (s.gfront) (void)hipGraphExecDestroy(s.gfront);
-        if (s.gback) (void)hipGraphExecDestroy(s.gback);
+        if (s.gbaq) (void)hipGraphExecDestroy(s.gbaq);
+        if (s.gcov) (void)hipGraphExecDestroy(s.gcov);
+        hipEvent_t evs[] = {s.baq_done, s.cov_done, s.t0, s.t1};
+        for (hipEvent_t ev : evs) if (ev) (void)hipEventDestroy(ev);
+        if (s.stream && s.stream != e->stream) (void)hipStreamDestroy(s.stream);
+        s.bs.release();
     }
     for (auto &ev : e->bev) if (ev) (void)hipEventDestroy(ev);
     if (e->copy_stream) { (void)hipStreamSynchronize(e->copy_stream); (void)hipStreamDestroy(e->copy_stream); }

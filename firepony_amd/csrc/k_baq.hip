@@ -435,10 +435,13 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
     uint64_t *Ost = (uint64_t *)(Sst + (uint64_t)(a.rows_cap + 2) * 64);  // [row][lane] MAP state | quality << 32
     const BatchDev &b = a.b;
     uint64_t ncell = 0;
-    // per-wave staging area (STAGE): rows_cap query positions, then rows_cap + 2W reference cells
+    // per-wave staging area (STAGE): rows_cap query positions (u16), then rows_cap + 2W reference
+    // cells as nibbles, two per byte
     extern __shared__ unsigned char baq_stage[];
-    uint16_t *rowd = (uint16_t *)(baq_stage + (threadIdx.x >> 6) * (a.rows_cap * 128 + (a.rows_cap + 2 * W) * 64));
+    const uint32_t ref_bytes = (a.rows_cap + 2 * W + 1) / 2;
+    uint16_t *rowd = (uint16_t *)(baq_stage + (threadIdx.x >> 6) * (a.rows_cap * 128 + ref_bytes * 64));
     uint8_t *refd = (uint8_t *)(rowd + (uint64_t)a.rows_cap * 64);
+    auto refn = [&](int i) -> uint32_t { return (refd[(i >> 1) * 64 + (threadIdx.x & 63)] >> ((i & 1) * 4)) & 15; };
 
     auto qprob = [&](uint8_t q) -> double { return s_q2p[q < 4 ? 4 : q]; };
     // Compiler fence: makes a value materialise here (emits no instruction).  The row loops decode
@@ -527,7 +530,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
                     uint64_t v = 0;
 #pragma unroll
                     for (int j = 0; j < 16; j++)
-                        if (w * 16 + j < W) v |= (uint64_t)refd[(idx0 + w * 16 + j + W) * 64 + lane] << (4 * j);
+                        if (w * 16 + j < W) v |= (uint64_t)refn(idx0 + w * 16 + j + W) << (4 * j);
                     win[w] = v;
                 }
                 return;
@@ -590,7 +593,12 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
                 rowd[q * 64 + lane] = (uint16_t)(b.quals[qqual + q] | (base << 8));
             }
 #pragma unroll 4
-            for (int k = -bw; k <= qLen + W - 1 - bw; k++) refd[(k + W) * 64 + lane] = nibble(ref_word(hoff + k), k);
+            // cells k = -bw .. qLen + W - 1 - bw at index k + W, two per byte (an odd first index also
+            // fills the cell below the window, never read)
+            for (int bi = (W - bw) >> 1; bi <= (qLen + 2 * W - 1 - bw) >> 1; bi++) {
+                const int k0 = 2 * bi - W, k1 = k0 + 1;
+                refd[bi * 64 + lane] = (uint8_t)(nibble(ref_word(hoff + k0), k0) | (nibble(ref_word(hoff + k1), k1) << 4));
+            }
         }
 
         double M[W], I[W], D[W];
@@ -643,7 +651,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             const uint32_t v = rowd[q1 * 64 + lane];
             cqb = v >> 8;
             cp = qprob((uint8_t)(v & 255));
-            crb = refd[(2 - bw - 1 + (W - 1) + W) * 64 + lane];
+            crb = refn(2 - bw - 1 + (W - 1) + W);
         } else {
             cqb = (read_word(b.reads, qbase + q1) >> (((qbase + q1) & 7) * 4)) & 15;
             cp = qprob(b.quals[qqual + q1]);
@@ -656,7 +664,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             uint8_t nqq = 0;
             if constexpr (STAGE) {
                 nv = rowd[in_ * 64 + lane];
-                nrw = refd[(nidx + W) * 64 + lane];
+                nrw = refn(nidx + W);
             } else {
                 nqw = read_word(b.reads, qbase + in_);
                 nqq = b.quals[qqual + in_];
@@ -775,7 +783,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         // inputs of backward row qLen - 1, then loaded a row ahead and decoded as in the forward pass
         if constexpr (STAGE) {
             cqb = rowd[qlast * 64 + lane] >> 8;
-            crb = refd[(qLen - 1 - bw + W) * 64 + lane];
+            crb = refn(qLen - 1 - bw + W);
         } else {
             cqb = (read_word(b.reads, qbase + qlast) >> (((qbase + qlast) & 7) * 4)) & 15;
             crb = nibble(ref_word(hoff + (qLen - 1 - bw)), qLen - 1 - bw);
@@ -789,7 +797,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             uint8_t nqq = 0;
             if constexpr (STAGE) {
                 nv = rowd[in_ * 64 + lane];
-                nrw = refd[(nidx + W) * 64 + lane];
+                nrw = refn(nidx + W);
                 nqq = (uint8_t)(nv & 255);
             } else {
                 nqw = read_word(b.reads, qbase + in_);
@@ -1043,10 +1051,10 @@ extern "C" hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t
 static int g_baq_lds_limit = 64 * 1024;
 
 // LDS bytes of a STAGE band workgroup (4 waves): per wave rows_cap u16 query cells and
-// rows_cap + 2W reference cells
+// rows_cap + 2W reference cells as nibbles
 static size_t baq_stage_bytes(int W, uint32_t rows_cap)
 {
-    return 4 * ((size_t)rows_cap * 128 + ((size_t)rows_cap + 2 * W) * 64);
+    return 4 * ((size_t)rows_cap * 128 + (((size_t)rows_cap + 2 * W + 1) / 2) * 64);
 }
 
 extern "C" hipError_t fpd_baq_prepare()

@@ -14,7 +14,7 @@
  * launch order or the order atomics land in, and no update waits for a
  * returned value.
  *
- * Work layout: reads are grouped by read group; a workgroup of 16 waves takes a
+ * Work layout: reads are grouped by read group; a workgroup of 8 waves takes a
  * tile of up to 1024 reads of one group and keeps that group's table entries in
  * LDS: every count, and the exact sums of the quality, context and cycle I/D
  * entries.  Only the sums of the cycle table's mismatch entries (few nonzero,
@@ -197,7 +197,7 @@ __device__ inline void global_add(const TablesDev &t, int tab, uint32_t key, uin
 //   [1218, 1218+2*ncodes)  cycle I/D: 2 x cycle codes
 static constexpr uint32_t E_CM = 66, E_CID = 66 + 1024, E_YID = 66 + 1024 + 128;
 static constexpr uint32_t LDS_ENTRY_BYTES = 3 * 8 + 4;
-static constexpr uint32_t COV_WAVES = 16;
+static constexpr uint32_t COV_WAVES = 8;
 static constexpr uint32_t TILE = 1024;                     // reads per workgroup tile
 static constexpr uint32_t STAGE_BYTES = COV_WAVES * 256;   // per-wave copies of the window's read bases
 
