@@ -44,7 +44,8 @@ struct CovArgs {
     uint32_t mcodes;             // cycle codes whose M counts are kept in LDS
     const uint32_t *rg_list;     // active reads grouped by read group
     const uint32_t *rg_offset;   // num_rg + 1
-    const uint32_t *tile_offset; // num_rg + 1 (tiles of TILE reads)
+    const uint32_t *tile_offset; // num_rg + 1 (tiles of `tile` reads)
+    uint32_t tile;               // reads per workgroup tile
     // pair mode (sort aggregation): one (key, value) pair per observation and table
     uint32_t *pkey[3];
     double *pval[3];
@@ -198,7 +199,8 @@ __device__ inline void global_add(const TablesDev &t, int tab, uint32_t key, uin
 static constexpr uint32_t E_CM = 66, E_CID = 66 + 1024, E_YID = 66 + 1024 + 128;
 static constexpr uint32_t LDS_ENTRY_BYTES = 3 * 8 + 4;
 static constexpr uint32_t COV_WAVES = 8;
-static constexpr uint32_t TILE = 1024;                     // reads per workgroup tile
+static constexpr uint32_t TILE = 1024;                     // reads per workgroup tile (large batches)
+static constexpr uint32_t TILE_MIN = 128;                  // small batches: more, smaller tiles
 static constexpr uint32_t STAGE_BYTES = COV_WAVES * 256;   // per-wave copies of the window's read bases
 
 // the covariate keys of one base and their accumulation
@@ -603,7 +605,7 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
     __builtin_amdgcn_wave_barrier();
 }
 
-// one workgroup per (read group, tile of TILE reads of that group)
+// one workgroup per (read group, tile of a.tile reads of that group)
 template <bool PAIRS>
 __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
 {
@@ -635,8 +637,8 @@ __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
     }
     const uint32_t rg = lo_rg;
     const uint32_t tile = blk - a.tile_offset[rg];
-    const uint32_t first = a.rg_offset[rg] + tile * TILE;
-    const uint32_t last = min(a.rg_offset[rg + 1], first + TILE);
+    const uint32_t first = a.rg_offset[rg] + tile * a.tile;
+    const uint32_t last = min(a.rg_offset[rg + 1], first + a.tile);
 
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if constexpr (PAIRS) {
@@ -772,7 +774,7 @@ __global__ void __launch_bounds__(256) k_rg_count(uint32_t n, const uint32_t *li
 }
 
 __global__ void __launch_bounds__(256) k_rg_offsets(uint32_t num_rg, const uint32_t *rg_count, uint32_t *rg_offset,
-                                                    uint32_t *tile_offset, uint32_t *rg_cursor)
+                                                    uint32_t *tile_offset, uint32_t *rg_cursor, uint32_t tile)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     uint32_t o = 0, t = 0;
@@ -781,7 +783,7 @@ __global__ void __launch_bounds__(256) k_rg_offsets(uint32_t num_rg, const uint3
         tile_offset[g] = t;
         rg_cursor[g] = 0;
         o += rg_count[g];
-        t += (rg_count[g] + TILE - 1) / TILE;
+        t += (rg_count[g] + tile - 1) / tile;
     }
     rg_offset[num_rg] = o;
     tile_offset[num_rg] = t;
@@ -870,13 +872,16 @@ extern "C" hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStat
     const uint32_t rg_blocks = std::min<uint32_t>((n + 255) / 256, fpd::RG_BLOCKS);
     hipLaunchKernelGGL(fpd::k_rg_count, dim3(rg_blocks), dim3(256), 0, st, n, b->live, s->status, b->read_group,
                        nrg, cnt, counters + 4);
-    hipLaunchKernelGGL(fpd::k_rg_offsets, dim3(1), dim3(64), 0, st, nrg, cnt, off, toff, cur);
+    // tiles of 1024 reads for large batches; smaller ones (down to 128) keep a small batch spread
+    // over the whole device
+    const uint32_t tile = std::max<uint32_t>(fpd::TILE_MIN, std::min<uint32_t>(fpd::TILE, (n / 1024 + 63) & ~63u));
+    hipLaunchKernelGGL(fpd::k_rg_offsets, dim3(1), dim3(64), 0, st, nrg, cnt, off, toff, cur, tile);
     hipLaunchKernelGGL(fpd::k_rg_scatter, dim3(rg_blocks), dim3(256), 0, st, n, b->live, s->status,
                        b->read_group, nrg, off, cur, list);
     fpd::CovArgs a{};
     a.b = *b; a.s = *s; a.t = *t; a.frac_f64 = frac_f64; a.keep_frac = keep_frac;
-    a.rg_list = list; a.rg_offset = off; a.tile_offset = toff;
-    const uint32_t blocks = (n + fpd::TILE - 1) / fpd::TILE + nrg;
+    a.rg_list = list; a.rg_offset = off; a.tile_offset = toff; a.tile = tile;
+    const uint32_t blocks = (n + tile - 1) / tile + nrg;
     if (pkey) {
         // pair mode (sort aggregation): pairs out instead of table updates; LDS holds the base stages
         for (int i = 0; i < 3; i++) { a.pkey[i] = pkey[i]; a.pval[i] = pval[i]; }
