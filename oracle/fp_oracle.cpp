@@ -1842,6 +1842,27 @@ int fpo_process_batch(fpo_ctx *c, const fp_batch_view *b)
     return FP_OK;
 }
 
+// gather_intermediates (pipeline_interface.cu:52-58,137-160) followed by the covariate re-pack of
+// postprocess_covariates (covariates.cu:265-282): with exact sums the merge is a per-key add
+int fpo_gather(fpo_ctx *dst, fpo_ctx *src)
+{
+    if (!dst || !src) return FP_ERR_INVALID_ARG;
+    if (dst->accum_mode != FPO_ACCUM_EXACT || src->accum_mode != FPO_ACCUM_EXACT) return FP_ERR_UNSUPPORTED;
+    for (int t = 0; t < 3; t++) {
+        for (const auto &kv : src->exact[t]) {
+            ExactAcc &a = dst->exact[t][kv.first];
+            a.obs += kv.second.obs;
+            a.mism += kv.second.mism;
+        }
+    }
+    dst->stats.total_reads += src->stats.total_reads;
+    dst->stats.filtered_reads += src->stats.filtered_reads;
+    dst->stats.baq_reads += src->stats.baq_reads;
+    dst->stats.num_batches += src->stats.num_batches;
+    dst->postprocessed = false;
+    return FP_OK;
+}
+
 int fpo_postprocess(fpo_ctx *c)
 {
     if (!c) return FP_ERR_INVALID_ARG;

@@ -36,6 +36,9 @@ int fpo_postprocess(fpo_ctx *c);
 int fpo_get_table(fpo_ctx *c, int table_id, fp_table_out *out);
 int fpo_write_report(fpo_ctx *c, const char *path);
 int fpo_stats(fpo_ctx *c, fp_stats *out);
+/* adds src's observation tables and read counters into dst (exact accumulation only): the
+ * multi-device concat + reduce of gather_intermediates (pipeline_interface.cu:52-58,137-160) */
+int fpo_gather(fpo_ctx *dst, fpo_ctx *src);
 int fpo_debug_size(fpo_ctx *c, int item, uint64_t *bytes);
 int fpo_debug_fetch(fpo_ctx *c, int item, void *dst, uint64_t bytes);
 /* BAQ MAP quality of the last batch as the real value before truncation,

@@ -30,6 +30,7 @@ def lib():
                                 C.POINTER(abi.FpReferenceView), C.POINTER(abi.FpVariantsView)]
         L.fpo_process_batch.argtypes = [C.c_void_p, C.POINTER(abi.FpBatchView)]
         L.fpo_postprocess.argtypes = [C.c_void_p]
+        L.fpo_gather.argtypes = [C.c_void_p, C.c_void_p]
         L.fpo_get_table.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.FpTableOut)]
         L.fpo_write_report.argtypes = [C.c_void_p, C.c_char_p]
         L.fpo_stats.argtypes = [C.c_void_p, C.POINTER(abi.FpStats)]
@@ -96,6 +97,12 @@ class Oracle:
 
     def postprocess(self):
         lib().fpo_postprocess(self._h)
+
+    def gather(self, other):
+        """Add other's observation tables and read counters into this oracle (exact mode)."""
+        rc = lib().fpo_gather(self._h, other._h)
+        if rc:
+            raise RuntimeError(f"fpo_gather: {rc}")
 
     def table(self, table_id):
         return read_table(lib().fpo_get_table, self._h, table_id)
