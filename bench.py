@@ -320,6 +320,9 @@ def main():
     ap.add_argument("--aggregation", choices=("direct", "sort"), default="direct",
                     help="covariate aggregation of the profile-only pass")
     ap.add_argument("--no-sort", action="store_true", help="skip the sort-aggregation pass")
+    ap.add_argument("--workload", choices=("config1", "mixed"), default="config1",
+                    help="workload of the profile-only pass (mixed: the first --mixed-reads reads of the "
+                         "50-250 bp set)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-reads", type=int, default=2000000)
     ap.add_argument("--no-served", action="store_true")
@@ -346,8 +349,11 @@ def main():
         sys.exit(2)
 
     import torch
-    gs, rs, total = synth.named_spec(1)
+    mixed_profile = args.profile_only and args.workload == "mixed"
+    gs, rs, total = synth.named_spec(2 if mixed_profile else 1)
     genome = synth.Genome(gs, rs.num_read_groups)
+    if mixed_profile:
+        total = min(total, args.mixed_reads)
     B = args.batch_reads
     if args.steps:
         total = min(total, args.steps * B)
@@ -361,6 +367,8 @@ def main():
     batched_opts = dict(engine_kind=abi.FP_ENGINE_BATCHED, max_read_length=101)
     if args.profile_only:
         popts = dict(batched_opts)
+        if mixed_profile:
+            popts.update(max_read_length=250, frac_error_f64=1)
         if args.aggregation == "sort":
             popts.update(aggregation=abi.FP_AGG_SORT, sort_digit_bits=8)
         pdt, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=args.wpe or 1, profile_stages=1, **popts)

@@ -435,13 +435,19 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
     uint64_t *Ost = (uint64_t *)(Sst + (uint64_t)(a.rows_cap + 2) * 64);  // [row][lane] MAP state | quality << 32
     const BatchDev &b = a.b;
     uint64_t ncell = 0;
-    // per-wave staging area (STAGE): rows_cap query positions (u16), then rows_cap + 2W reference
-    // cells as nibbles, two per byte
+    // per-wave staging area (STAGE), [index][lane]: rows_cap query qualities (bytes), the query
+    // bases and the rows_cap + 2W reference cells as nibbles, two per byte
     extern __shared__ unsigned char baq_stage[];
-    const uint32_t ref_bytes = (a.rows_cap + 2 * W + 1) / 2;
-    uint16_t *rowd = (uint16_t *)(baq_stage + (threadIdx.x >> 6) * (a.rows_cap * 128 + ref_bytes * 64));
-    uint8_t *refd = (uint8_t *)(rowd + (uint64_t)a.rows_cap * 64);
+    const uint32_t base_bytes = (a.rows_cap + 1) / 2, ref_bytes = (a.rows_cap + 2 * W + 1) / 2;
+    uint8_t *rowq = baq_stage + (threadIdx.x >> 6) * ((a.rows_cap + base_bytes + ref_bytes) * 64);
+    uint8_t *rowb = rowq + (uint64_t)a.rows_cap * 64;
+    uint8_t *refd = rowb + (uint64_t)base_bytes * 64;
     auto refn = [&](int i) -> uint32_t { return (refd[(i >> 1) * 64 + (threadIdx.x & 63)] >> ((i & 1) * 4)) & 15; };
+    // query position q as quality | base << 8
+    auto rowv = [&](int q) -> uint32_t {
+        const uint32_t l = threadIdx.x & 63;
+        return rowq[q * 64 + l] | (((rowb[(q >> 1) * 64 + l] >> ((q & 1) * 4)) & 15) << 8);
+    };
 
     auto qprob = [&](uint8_t q) -> double { return s_q2p[q < 4 ? 4 : q]; };
     // Compiler fence: makes a value materialise here (emits no instruction).  The row loops decode
@@ -587,10 +593,14 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         if constexpr (STAGE) {
             // this read's inputs into the wave's LDS area (each lane writes and later reads only its own column)
 #pragma unroll 4
-            for (int q = 0; q < qLen; q++) {
+            for (int q = 0; q < qLen; q++) rowq[q * 64 + lane] = b.quals[qqual + q];
+#pragma unroll 4
+            for (int q = 0; q < qLen; q += 2) {
                 const uint32_t w = b.reads[(qbase + q) >> 3];
-                const uint32_t base = (w >> (((qbase + q) & 7) * 4)) & 15;
-                rowd[q * 64 + lane] = (uint16_t)(b.quals[qqual + q] | (base << 8));
+                const uint32_t b0 = (w >> (((qbase + q) & 7) * 4)) & 15;
+                const uint64_t i1 = qbase + (q + 1 < qLen ? q + 1 : q);   // never past the window
+                const uint32_t b1 = q + 1 < qLen ? (b.reads[i1 >> 3] >> ((i1 & 7) * 4)) & 15 : 15u;
+                rowb[(q >> 1) * 64 + lane] = (uint8_t)(b0 | (b1 << 4));
             }
 #pragma unroll 4
             // cells k = -bw .. qLen + W - 1 - bw at index k + W, two per byte (an odd first index also
@@ -608,7 +618,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             uint8_t qb;
             double p;
             if constexpr (STAGE) {
-                const uint32_t v = rowd[lane];
+                const uint32_t v = rowv(0);
                 qb = (uint8_t)(v >> 8);
                 p = qprob((uint8_t)(v & 255));
             } else {
@@ -648,7 +658,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         uint32_t cqb, crb;
         double cp;
         if constexpr (STAGE) {
-            const uint32_t v = rowd[q1 * 64 + lane];
+            const uint32_t v = rowv(q1);
             cqb = v >> 8;
             cp = qprob((uint8_t)(v & 255));
             crb = refn(2 - bw - 1 + (W - 1) + W);
@@ -663,7 +673,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             uint32_t nqw = 0, nrw = 0, nv = 0;
             uint8_t nqq = 0;
             if constexpr (STAGE) {
-                nv = rowd[in_ * 64 + lane];
+                nv = rowv(in_);
                 nrw = refn(nidx + W);
             } else {
                 nqw = read_word(b.reads, qbase + in_);
@@ -764,7 +774,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         };
         load_f(qLen);
         // backward row i emits query index i (the row below it); map_row(i) needs quality index i - 1
-        const uint8_t lastq = STAGE ? (uint8_t)(rowd[qlast * 64 + lane] & 255) : b.quals[qqual + qlast];
+        const uint8_t lastq = STAGE ? (uint8_t)(rowv(qlast) & 255) : b.quals[qqual + qlast];
         {
             const double sq = Sst[(uint64_t)qLen * 64 + lane];
             const int klo = 1 > qLen - bw ? 1 : qLen - bw;
@@ -782,7 +792,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         win_fill(qLen - bw);                     // backward cell r of row i: window index i - bw + r
         // inputs of backward row qLen - 1, then loaded a row ahead and decoded as in the forward pass
         if constexpr (STAGE) {
-            cqb = rowd[qlast * 64 + lane] >> 8;
+            cqb = rowv(qlast) >> 8;
             crb = refn(qLen - 1 - bw + W);
         } else {
             cqb = (read_word(b.reads, qbase + qlast) >> (((qbase + qlast) & 7) * 4)) & 15;
@@ -796,7 +806,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             uint32_t nqw = 0, nrw = 0, nv = 0;
             uint8_t nqq = 0;
             if constexpr (STAGE) {
-                nv = rowd[in_ * 64 + lane];
+                nv = rowv(in_);
                 nrw = refn(nidx + W);
                 nqq = (uint8_t)(nv & 255);
             } else {
@@ -1050,11 +1060,11 @@ extern "C" hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t
 
 static int g_baq_lds_limit = 64 * 1024;
 
-// LDS bytes of a STAGE band workgroup (4 waves): per wave rows_cap u16 query cells and
-// rows_cap + 2W reference cells as nibbles
+// LDS bytes of a STAGE band workgroup (4 waves): per wave rows_cap quality bytes, rows_cap query
+// base nibbles and rows_cap + 2W reference nibbles
 static size_t baq_stage_bytes(int W, uint32_t rows_cap)
 {
-    return 4 * ((size_t)rows_cap * 128 + (((size_t)rows_cap + 2 * W + 1) / 2) * 64);
+    return 4 * ((size_t)rows_cap + ((size_t)rows_cap + 1) / 2 + ((size_t)rows_cap + 2 * W + 1) / 2) * 64;
 }
 
 extern "C" hipError_t fpd_baq_prepare()
