@@ -8,7 +8,7 @@
  *   a segmented reduce_by_key of the sorted runs into the resident direct-indexed tables.
  *
  * Each workgroup of a fixed grid owns one contiguous range of the elements (a whole number of
- * 2048-element tiles).  One radix pass is k_sort_hist (the range's digit counts, digit-major, one
+ * 4096-element tiles).  One radix pass is k_sort_hist (the range's digit counts, digit-major, one
  * row of the grid per digit) -> k_sort_totals + k_sort_offsets (exclusive scan = each range's
  * global output offset per digit) -> k_sort_scatter.  The scatter walks its range tile by tile and
  * ranks a tile's elements stably in LDS: equal digits inside a wavefront are matched with D ballots
@@ -31,9 +31,10 @@
 
 namespace fpd {
 
-static constexpr uint32_t SORT_THREADS = 256;
+static constexpr uint32_t SORT_THREADS = 512;
+static constexpr uint32_t SORT_WAVES = SORT_THREADS / 64;
 static constexpr uint32_t SORT_ITEMS = 8;
-static constexpr uint32_t SORT_TILE = SORT_THREADS * SORT_ITEMS;   // 2048 elements
+static constexpr uint32_t SORT_TILE = SORT_THREADS * SORT_ITEMS;   // 4096 elements
 static constexpr uint32_t SORT_SENTINEL = 0xffffffffu;
 static constexpr uint32_t SORT_GRID = 2048;                          // workgroups of hist / scatter
 static constexpr uint32_t RED_LANE = 32;                             // consecutive pairs per reduce lane
@@ -90,7 +91,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_hist(SortPass p)
 {
     constexpr uint32_t R = 1u << BITS, mask = R - 1;
     constexpr uint32_t G = (1024 / R) < 4 ? 4 : ((1024 / R) > 16 ? 16 : (1024 / R));   // groups per wave
-    constexpr uint32_t C = 4 * G;                                                      // sub-histograms
+    constexpr uint32_t C = SORT_WAVES * G;                                             // sub-histograms
     __shared__ uint32_t h[C][R + 1];   // +1: the copies of one digit fall in different banks
     const uint32_t n = pass_n(p);
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -154,7 +155,7 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *wtot, uint32_t 
 // totals[d] = sum over the ranges of hist[d][range]; one workgroup per digit
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_totals(SortPass p, uint32_t grid)
 {
-    __shared__ uint32_t wtot[4];
+    __shared__ uint32_t wtot[SORT_WAVES];
     const uint32_t d = blockIdx.x;
     const uint32_t *h = p.hist + (uint64_t)d * grid;
     uint32_t s = 0;
@@ -168,7 +169,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_totals(SortPass p, uint32
 // consecutive ranges per thread and step
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_offsets(SortPass p, uint32_t grid)
 {
-    __shared__ uint32_t wtot[4];
+    __shared__ uint32_t wtot[SORT_WAVES];
     const uint32_t d = blockIdx.x;
     const uint32_t R = gridDim.x;       // one workgroup per digit
     uint32_t below = 0, all = 0;
@@ -202,10 +203,10 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_offsets(SortPass p, uint3
 template <uint32_t BITS>
 __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(SortPass p)
 {
-    __shared__ uint32_t cnt[4][256];    // per-wave running digit counters, then per-wave digit bases
+    __shared__ uint32_t cnt[SORT_WAVES][256];   // per-wave running digit counters, then per-wave bases
     __shared__ uint32_t gofs[256];      // the range's running global offset per digit
     __shared__ uint32_t lofs[256];      // start of each digit in the tile's digit order
-    __shared__ uint32_t wtot[4];
+    __shared__ uint32_t wtot[SORT_WAVES];
     __shared__ uint32_t kept_s;
     __shared__ uint32_t lk[SORT_TILE];
     __shared__ double lv[SORT_TILE];
@@ -218,7 +219,7 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(SortPass p)
     if (lo >= hi) return;
     for (uint32_t d = threadIdx.x; d < R; d += SORT_THREADS) gofs[d] = p.hist[(uint64_t)d * gridDim.x + blockIdx.x];
     for (uint32_t base = lo; base < hi; base += SORT_TILE) {
-        for (uint32_t i = threadIdx.x; i < 4 * 256; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
+        for (uint32_t i = threadIdx.x; i < SORT_WAVES * 256; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
         __syncthreads();
         // stable ranks: element order inside the tile is (wave, item, lane)
         const uint32_t t0 = base + wave * (SORT_ITEMS * 64);
@@ -254,14 +255,14 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(SortPass p)
         uint32_t tot = 0;
         {
             const uint32_t d = threadIdx.x;
-            uint32_t t[4] = {0, 0, 0, 0};
+            uint32_t t[SORT_WAVES] = {};
             if (d < R) {
-                for (int w = 0; w < 4; w++) { t[w] = cnt[w][d]; tot += t[w]; }
+                for (uint32_t w = 0; w < SORT_WAVES; w++) { t[w] = cnt[w][d]; tot += t[w]; }
             }
             __syncthreads();
             if (d < R) {
                 uint32_t run = 0;
-                for (int w = 0; w < 4; w++) { cnt[w][d] = run; run += t[w]; }
+                for (uint32_t w = 0; w < SORT_WAVES; w++) { cnt[w][d] = run; run += t[w]; }
             }
             uint32_t kept;
             const uint32_t start = block_excl_scan(d < R ? tot : 0u, wtot, &kept);
