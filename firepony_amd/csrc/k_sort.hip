@@ -356,8 +356,11 @@ __global__ void __launch_bounds__(256) k_seg_reduce(ReduceArgs a)
             for (int j = 0; j < 8; j++) {
                 const uint32_t k = kb[j];
                 if (k == SORT_SENTINEL) continue;     // past the end
-                uint64_t l0, l1, l2;
-                fixed_split(fixed_from_double(vb[j]), l0, l1, l2);
+                // most per-base errors are exactly 0 or 1 (1.0 = 2^80: limbs 0, 0, 2^26)
+                uint64_t l0 = 0, l1 = 0, l2 = 0;
+                const double v = vb[j];
+                if (v == 1.0) l2 = 1ull << 26;
+                else if (v != 0.0) fixed_split(fixed_from_double(v), l0, l1, l2);
                 if (k != cur) {
                     if (cur != SORT_SENTINEL) {
                         if (single) { hs = cs; single = false; }            // the slice's first run ended
