@@ -16,6 +16,7 @@ FP_ERR_BUDGET = 5
 FP_ERR_UNSUPPORTED = 6
 FP_ERR_IO = 7
 FP_ERR_INTERNAL = 8
+FP_ERR_FORMAT = 9
 
 FP_ENGINE_PLAIN = 0
 FP_ENGINE_BATCHED = 1
@@ -195,3 +196,35 @@ class FpsynReadSpec(C.Structure):
         ("insert_sd", C.c_double),
         ("edge_case_frac", C.c_double),
     ]
+
+
+def view_arrays(v):
+    """numpy views of every array of an fp_batch_view over host memory (no copies)."""
+    import numpy as np
+    n = v.num_reads
+
+    def arr(ptr, count, dtype):
+        if count == 0 or not ptr:
+            return np.zeros(0, dtype=dtype)
+        return np.ctypeslib.as_array(ptr, shape=(count,)).view(dtype)
+
+    return {
+        "chromosome": arr(v.chromosome, n, np.uint16),
+        "alignment_start": arr(v.alignment_start, n, np.uint32),
+        "alignment_stop": arr(v.alignment_stop, n, np.uint32),
+        "mate_chromosome": arr(v.mate_chromosome, n, np.uint32),
+        "mate_alignment_start": arr(v.mate_alignment_start, n, np.uint32),
+        "inferred_insert_size": arr(v.inferred_insert_size, n, np.int32),
+        "cigars": arr(v.cigars, v.num_cigar_ops, np.uint32),
+        "cigar_start": arr(v.cigar_start, n, np.uint32),
+        "cigar_len": arr(v.cigar_len, n, np.uint32),
+        "reads": arr(v.reads, v.num_read_bases // 8, np.uint32),
+        "read_start": arr(v.read_start, n, np.uint32),
+        "read_len": arr(v.read_len, n, np.uint32),
+        "qualities": arr(v.qualities, v.num_qualities, np.uint8),
+        "qual_start": arr(v.qual_start, n, np.uint32),
+        "qual_len": arr(v.qual_len, n, np.uint32),
+        "flags": arr(v.flags, n, np.uint16),
+        "mapq": arr(v.mapq, n, np.uint8),
+        "read_group": arr(v.read_group, n, np.uint32),
+    }

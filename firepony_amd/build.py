@@ -6,6 +6,7 @@
 Outputs:
     firepony_amd/_lib/libfirepony_amd.so   HIP engine + C ABI (hipcc, gfx950)
     firepony_amd/_lib/libfpsynth.so        synthetic workload generator (g++)
+    firepony_amd/_lib/libfirepony_amd_io.so  FASTA / VCF / SAM / BAM loaders (g++, zlib)
     oracle/_build/libfp_oracle.so          CPU oracle (g++), built by oracle/build.sh
 """
 import os
@@ -57,6 +58,15 @@ def build_synth():
     return out
 
 
+def build_io():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libfirepony_amd_io.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"),
+          os.path.join(CSRC, "loaders.cpp"), "-lz", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_engine():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libfirepony_amd.so")
@@ -79,6 +89,7 @@ def build_oracle():
 
 def main(argv):
     build_synth()
+    build_io()
     if "--oracle" in argv or "--all" in argv:
         build_oracle()
     if "--no-engine" not in argv:

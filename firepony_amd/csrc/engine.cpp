@@ -1180,8 +1180,19 @@ const char *fp_status_string(fp_status s)
     case FP_ERR_BUDGET: return "refused by the HBM budget";
     case FP_ERR_UNSUPPORTED: return "unsupported";
     case FP_ERR_IO: return "I/O error";
+    case FP_ERR_FORMAT: return "malformed input file";
     default: return "internal error";
     }
+}
+
+fp_status fp_engine_set_read_group_names(fp_engine *e, uint32_t n, const char *const *names)
+{
+    fp_status rc = check_engine(e, true);
+    if (rc != FP_OK) return rc;
+    if (e->opts.serve_rg_slots) return fail(FP_ERR_STATE, "serving engines take read-group names per job");
+    if (n > e->num_rg || (n && !names)) return fail(FP_ERR_INVALID_ARG, "more names than read groups");
+    for (uint32_t i = 0; i < n; i++) e->rg_names[i] = names[i] ? names[i] : "";
+    return FP_OK;
 }
 
 fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **out)

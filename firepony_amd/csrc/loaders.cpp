@@ -1,0 +1,967 @@
+// Host-side input loaders (include/firepony_amd_io.h): FASTA reference, VCF
+// known sites, SAM/BAM alignments -> the engine's boundary views.
+//
+// Files are read through zlib's gz* stream, which passes plain files through
+// and inflates gzip and BGZF (concatenated gzip members) alike, so SAM, SAM.gz,
+// BAM, VCF and VCF.gz share one reader.  The SAM and BAM record grammar is the
+// SAM/BAM specification's; the reference parses it with htslib
+// (nsubtil/htslib@fix-signed-one-bit-bitfield, an htslib 1.2 fork, not vendored):
+// the htslib behaviours the reference depends on are restated where they are
+// used, each with the reference call site.
+#include "firepony_amd_io.h"
+
+#include <zlib.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+fp_status fail(fp_status s, const std::string &msg)
+{
+    g_err = msg;
+    return s;
+}
+
+// string_database::hash (string_database.cu:83-94): h = h*101 + c over the
+// bytes up to the first NUL, each char sign-extended as the reference's
+// (uint32)(char) conversion does on x86.
+uint32_t name_hash(const char *s)
+{
+    uint32_t h = 0;
+    for (; *s; s++) h = h * 101u + (uint32_t)(int32_t)(signed char)*s;
+    return h;
+}
+
+// string_database (string_database.cu:33-76): ids in insertion order, looked up
+// by the hash of the name (two names with one hash share an id, as there).
+struct NameDb {
+    std::unordered_map<uint32_t, uint32_t> by_hash;
+    std::vector<std::string> names;
+
+    uint32_t lookup(const char *s) const
+    {
+        auto it = by_hash.find(name_hash(s));
+        return it == by_hash.end() ? UINT32_MAX : it->second;
+    }
+    uint32_t insert(const std::string &s)
+    {
+        const uint32_t h = name_hash(s.c_str());
+        auto it = by_hash.find(h);
+        if (it != by_hash.end()) return it->second;
+        const uint32_t id = (uint32_t)names.size();
+        by_hash.emplace(h, id);
+        names.push_back(s);
+        return id;
+    }
+};
+
+// Buffered byte/line reader over a gz stream.
+class Reader {
+public:
+    ~Reader() { if (f_) gzclose(f_); }
+
+    bool open(const char *path)
+    {
+        path_ = path;
+        f_ = gzopen(path, "rb");
+        if (!f_) return false;
+        gzbuffer(f_, 1u << 20);
+        buf_.resize(4u << 20);
+        return true;
+    }
+    const std::string &path() const { return path_; }
+    bool error() const { return err_; }
+
+    // makes at least n bytes available if the stream holds them; returns the count available
+    size_t want(size_t n)
+    {
+        if (e_ - b_ >= n) return e_ - b_;
+        if (b_) {
+            std::memmove(buf_.data(), buf_.data() + b_, e_ - b_);
+            e_ -= b_;
+            b_ = 0;
+        }
+        if (buf_.size() < n) buf_.resize(std::max(n, 2 * buf_.size()));
+        while (e_ < n && !eof_) {
+            const size_t room = std::min<size_t>(buf_.size() - e_, 1u << 30);
+            const int r = gzread(f_, buf_.data() + e_, (unsigned)room);
+            if (r < 0) { err_ = true; eof_ = true; break; }
+            if (r == 0) { eof_ = true; break; }
+            e_ += (size_t)r;
+        }
+        return e_ - b_;
+    }
+    const char *ptr() const { return buf_.data() + b_; }
+    void skip(size_t n) { b_ += n; }
+
+    // next line without its '\n' and without one trailing '\r'; false at end of stream
+    bool line(std::string &out)
+    {
+        size_t scan = 0;
+        for (;;) {
+            const char *p = (const char *)std::memchr(buf_.data() + b_ + scan, '\n', e_ - b_ - scan);
+            if (p) {
+                const size_t n = (size_t)(p - (buf_.data() + b_));
+                take(out, n);
+                b_ += 1;
+                return true;
+            }
+            scan = e_ - b_;
+            if (want(scan + 1) == scan) {
+                if (scan == 0) return false;
+                take(out, scan);
+                return true;
+            }
+        }
+    }
+    // peeks the next byte without consuming it; -1 at end of stream
+    int peek()
+    {
+        if (want(1) < 1) return -1;
+        return (unsigned char)buf_[b_];
+    }
+
+private:
+    void take(std::string &out, size_t n)
+    {
+        out.assign(buf_.data() + b_, n);
+        b_ += n;
+        if (!out.empty() && out.back() == '\r') out.pop_back();
+    }
+
+    gzFile f_ = nullptr;
+    std::string path_;
+    std::vector<char> buf_;
+    size_t b_ = 0, e_ = 0;
+    bool eof_ = false, err_ = false;
+};
+
+std::vector<std::string> split(const std::string &s, char sep)
+{
+    std::vector<std::string> out;
+    size_t a = 0;
+    for (;;) {
+        const size_t p = s.find(sep, a);
+        out.push_back(s.substr(a, p == std::string::npos ? std::string::npos : p - a));
+        if (p == std::string::npos) break;
+        a = p + 1;
+    }
+    return out;
+}
+
+// IUPAC16 code of an upper-cased reference base: A C G T, else N (loader/reference.cu:288-325)
+inline uint32_t ref_code(char c)
+{
+    switch (c) {
+    case 'A': case 'a': return 1;
+    case 'C': case 'c': return 2;
+    case 'G': case 'g': return 4;
+    case 'T': case 't': return 8;
+    default: return 15;
+    }
+}
+
+// seq_nt16_table of the SAM specification: "=ACMGRSVTWYHKDBN", either case, every other
+// character N.
+struct Nt16 {
+    uint8_t t[256];
+    Nt16()
+    {
+        std::memset(t, 15, sizeof t);
+        const char *codes = "=ACMGRSVTWYHKDBN";
+        for (int i = 0; i < 16; i++) {
+            t[(unsigned char)codes[i]] = (uint8_t)i;
+            t[(unsigned char)std::tolower(codes[i])] = (uint8_t)i;
+        }
+    }
+};
+const Nt16 NT16;
+
+// CIGAR operation letters in BAM code order; B (9) is turned into X by the reference
+// (loader/alignments.cu:200-212).
+int cigar_code(char c)
+{
+    static const char *ops = "MIDNSHP=XB";
+    const char *p = std::strchr(ops, c);
+    return (c && p) ? (int)(p - ops) : -1;
+}
+
+// bam_cigar_type bits of the SAM specification: bit 0 consumes query, bit 1 consumes reference.
+inline uint32_t cigar_type(uint32_t op)
+{
+    static const uint32_t t[16] = {3, 1, 2, 2, 1, 0, 0, 3, 3, 0, 0, 0, 0, 0, 0, 0};
+    return t[op & 15];
+}
+
+} // namespace
+
+// ===========================================================================
+// reference
+
+struct fp_sequence_db {
+    NameDb names;
+    std::vector<uint64_t> word_offset;
+    std::vector<uint32_t> length;
+    std::vector<uint32_t> words;
+};
+
+namespace {
+
+// faidx index names (loader/reference.cu:74-108): the reference keys them by the hash of the first
+// whitespace-separated field.
+bool load_fai(const std::string &path, std::unordered_set<uint32_t> &names)
+{
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[65536];
+    while (std::fgets(line, sizeof line, f)) {
+        char *p = line;
+        while (*p == ' ' || *p == '\t') p++;
+        char *q = p;
+        while (*q && *q != ' ' && *q != '\t' && *q != '\n' && *q != '\r') q++;
+        *q = 0;
+        names.insert(name_hash(p));
+    }
+    std::fclose(f);
+    return true;
+}
+
+} // namespace
+
+extern "C" fp_status fp_reference_load(const char *fasta_path, fp_sequence_db **out)
+{
+    if (!fasta_path || !out) return fail(FP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    Reader rd;
+    if (!rd.open(fasta_path)) return fail(FP_ERR_IO, std::string("cannot open ") + fasta_path);
+    std::unordered_set<uint32_t> indexed;
+    const bool have_index = load_fai(std::string(fasta_path) + ".fai", indexed);
+
+    fp_sequence_db *db = new fp_sequence_db();
+    std::string line;
+    bool have_header = rd.line(line);
+    while (have_header) {
+        if (line.empty() || line[0] != '>') {
+            // load_next_sequence accepts only a '>' header here (loader/reference.cu:240-244)
+            if (db->length.empty()) {
+                delete db;
+                return fail(FP_ERR_FORMAT, std::string(fasta_path) + ": not a FASTA file");
+            }
+            break;
+        }
+        // name: after '>', up to the first space (loader/reference.cu:246-252)
+        std::string name = line.substr(1);
+        const size_t sp = name.find(' ');
+        if (sp != std::string::npos) name.resize(sp);
+        const bool take = !have_index || indexed.count(name_hash(name.c_str()));
+        const uint64_t woff = db->words.size();
+        uint64_t n = 0;
+        have_header = false;
+        while (rd.line(line)) {
+            if (!line.empty() && line[0] == '>') { have_header = true; break; }
+            if (!take) continue;
+            for (char c : line) {
+                if ((n & 7) == 0) db->words.push_back(0);
+                db->words.back() |= ref_code(c) << ((n & 7) * 4);
+                n++;
+            }
+        }
+        if (rd.error()) {
+            delete db;
+            return fail(FP_ERR_IO, std::string(fasta_path) + ": read error");
+        }
+        if (take) {
+            if (n > UINT32_MAX) {
+                delete db;
+                return fail(FP_ERR_UNSUPPORTED, "contig " + name + " is longer than 2^32-1 bases");
+            }
+            const uint32_t before = (uint32_t)db->names.names.size();
+            const uint32_t id = db->names.insert(name);
+            if (id == before) {
+                db->word_offset.push_back(woff);
+                db->length.push_back((uint32_t)n);
+            } else {
+                // a second sequence of the same name hash: the first stays (string_database::insert)
+                db->words.resize(woff);
+            }
+            // whole-file load stops at the first empty sequence (loader/reference.cu:123-127,324-326)
+            if (!have_index && n == 0) break;
+        }
+    }
+    *out = db;
+    return FP_OK;
+}
+
+extern "C" fp_status fp_reference_get_view(const fp_sequence_db *db, fp_reference_view *out)
+{
+    if (!db || !out) return fail(FP_ERR_INVALID_ARG, "null argument");
+    out->num_sequences = (uint32_t)db->length.size();
+    out->word_offset = db->word_offset.data();
+    out->length = db->length.data();
+    out->words = db->words.data();
+    out->total_words = db->words.size();
+    return FP_OK;
+}
+
+extern "C" uint32_t fp_reference_lookup(const fp_sequence_db *db, const char *name)
+{
+    if (!db || !name) return UINT32_MAX;
+    return db->names.lookup(name);
+}
+
+extern "C" fp_status fp_reference_name(const fp_sequence_db *db, uint32_t id, char *buf, size_t buf_len)
+{
+    if (!db || !buf || !buf_len) return fail(FP_ERR_INVALID_ARG, "null argument");
+    if (id >= db->names.names.size()) return fail(FP_ERR_INVALID_ARG, "no such contig id");
+    const std::string &s = db->names.names[id];
+    const size_t n = std::min(s.size(), buf_len - 1);
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+    return FP_OK;
+}
+
+extern "C" void fp_reference_free(fp_sequence_db *db) { delete db; }
+
+// ===========================================================================
+// known sites
+
+struct fp_variant_db {
+    std::vector<uint64_t> offset;
+    std::vector<uint32_t> start, stop, max_end;
+    uint64_t records = 0, skipped = 0;
+};
+
+extern "C" fp_status fp_variants_load(const char *vcf_path, const fp_sequence_db *db, fp_variant_db **out)
+{
+    if (!vcf_path || !db || !out) return fail(FP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    Reader rd;
+    if (!rd.open(vcf_path)) return fail(FP_ERR_IO, std::string("cannot open ") + vcf_path);
+    const uint32_t nseq = (uint32_t)db->length.size();
+    std::vector<std::vector<uint32_t>> fs(nseq), fe(nseq);
+    fp_variant_db *v = new fp_variant_db();
+    std::string line;
+    uint64_t lineno = 0;
+    auto bad = [&](const char *why) {
+        delete v;
+        return fail(FP_ERR_FORMAT, std::string(vcf_path) + ":" + std::to_string(lineno) + ": " + why);
+    };
+    while (rd.line(line)) {
+        lineno++;
+        if (line.empty() || line[0] == '#') continue;
+        const std::vector<std::string> f = split(line, '\t');
+        if (f.size() < 8) return bad("fewer than 8 columns");
+        char *end = nullptr;
+        errno = 0;
+        const long long pos1 = std::strtoll(f[1].c_str(), &end, 10);
+        if (errno || end == f[1].c_str() || *end || pos1 < 1 || pos1 > (long long)UINT32_MAX)
+            return bad("bad POS");
+        if (f[3].empty()) return bad("empty REF");
+        const uint32_t pos = (uint32_t)(pos1 - 1);
+        // bcf1_t.rlen: the REF length, or END - POS + 1 when INFO holds END
+        int64_t rlen = (int64_t)f[3].size();
+        for (const std::string &kv : split(f[7], ';')) {
+            if (kv.compare(0, 4, "END=") == 0) {
+                const long long endv = std::strtoll(kv.c_str() + 4, &end, 10);
+                if (end != kv.c_str() + 4 && !*end) rlen = endv - (long long)pos;
+                break;
+            }
+        }
+        v->records++;
+        // records on contigs the reference does not hold are skipped (loader/variants.cu:104-110)
+        const uint32_t id = db->names.lookup(f[0].c_str());
+        if (id == UINT32_MAX || id >= nseq) {
+            v->skipped++;
+            continue;
+        }
+        fs[id].push_back(pos);
+        fe[id].push_back((uint32_t)((int64_t)pos + rlen - 1));
+    }
+    if (rd.error()) return bad("read error");
+    v->offset.assign(nseq + 1, 0);
+    for (uint32_t s = 0; s < nseq; s++) {
+        v->offset[s + 1] = v->offset[s] + fs[s].size();
+        v->start.insert(v->start.end(), fs[s].begin(), fs[s].end());
+        v->stop.insert(v->stop.end(), fe[s].begin(), fe[s].end());
+        // inclusive max-scan of the stops (loader/variants.cu:125-134)
+        uint32_t m = 0;
+        for (size_t i = 0; i < fe[s].size(); i++) {
+            m = i ? std::max(m, fe[s][i]) : fe[s][i];
+            v->max_end.push_back(m);
+        }
+    }
+    *out = v;
+    return FP_OK;
+}
+
+extern "C" fp_status fp_variants_get_view(const fp_variant_db *v, fp_variants_view *out)
+{
+    if (!v || !out) return fail(FP_ERR_INVALID_ARG, "null argument");
+    out->num_sequences = (uint32_t)(v->offset.size() - 1);
+    out->feature_offset = v->offset.data();
+    out->feature_start = v->start.data();
+    out->feature_stop = v->stop.data();
+    out->max_end_point_left = v->max_end.data();
+    out->num_features = v->start.size();
+    return FP_OK;
+}
+
+extern "C" fp_status fp_variants_counts(const fp_variant_db *v, uint64_t *records, uint64_t *skipped)
+{
+    if (!v) return fail(FP_ERR_INVALID_ARG, "null argument");
+    if (records) *records = v->records;
+    if (skipped) *skipped = v->skipped;
+    return FP_OK;
+}
+
+extern "C" void fp_variants_free(fp_variant_db *v) { delete v; }
+
+// ===========================================================================
+// alignments
+
+namespace {
+
+struct BatchStore {
+    std::vector<uint16_t> chromosome;
+    std::vector<uint32_t> alignment_start, alignment_stop, mate_chromosome, mate_alignment_start;
+    std::vector<int32_t> inferred_insert_size;
+    std::vector<uint32_t> cigars, cigar_start, cigar_len;
+    std::vector<uint32_t> reads, read_start, read_len;
+    uint64_t read_bases = 0;
+    std::vector<uint8_t> qualities;
+    std::vector<uint32_t> qual_start, qual_len;
+    std::vector<uint16_t> flags;
+    std::vector<uint8_t> mapq;
+    std::vector<uint32_t> read_group;
+
+    void clear()
+    {
+        chromosome.clear(); alignment_start.clear(); alignment_stop.clear(); mate_chromosome.clear();
+        mate_alignment_start.clear(); inferred_insert_size.clear(); cigars.clear(); cigar_start.clear();
+        cigar_len.clear(); reads.clear(); read_start.clear(); read_len.clear(); read_bases = 0;
+        qualities.clear(); qual_start.clear(); qual_len.clear(); flags.clear(); mapq.clear();
+        read_group.clear();
+    }
+    void view(fp_batch_view *o) const
+    {
+        std::memset(o, 0, sizeof *o);
+        o->num_reads = (uint32_t)chromosome.size();
+        o->chromosome = chromosome.data();
+        o->alignment_start = alignment_start.data();
+        o->alignment_stop = alignment_stop.data();
+        o->mate_chromosome = mate_chromosome.data();
+        o->mate_alignment_start = mate_alignment_start.data();
+        o->inferred_insert_size = inferred_insert_size.data();
+        o->cigars = cigars.data();
+        o->num_cigar_ops = cigars.size();
+        o->cigar_start = cigar_start.data();
+        o->cigar_len = cigar_len.data();
+        o->reads = reads.data();
+        o->num_read_bases = read_bases;
+        o->read_start = read_start.data();
+        o->read_len = read_len.data();
+        o->qualities = qualities.data();
+        o->num_qualities = qualities.size();
+        o->qual_start = qual_start.data();
+        o->qual_len = qual_len.data();
+        o->flags = flags.data();
+        o->mapq = mapq.data();
+        o->read_group = read_group.data();
+    }
+};
+
+// One alignment record in BAM's in-memory terms (bam1_core_t), whichever file form it came from.
+struct Record {
+    int32_t tid = -1, pos = -1, mtid = -1, mpos = -1, isize = 0;
+    uint32_t flag = 0, mapq = 0;
+    std::vector<uint32_t> cigar;   // BAM words: len << 4 | op
+    std::vector<uint8_t> seq;      // one nt16 code per base
+    std::vector<uint8_t> qual;
+    bool has_rg = false;
+    std::string rg;
+};
+
+// read_group(header_line) (loader/alignments.cu:62-118): the record is split at tabs and every
+// field that is FOLLOWED by a tab is read as TG:value; the last field of the line is not read.
+void parse_rg_line(const std::string &line, std::string &id, std::string &pu)
+{
+    id.clear();
+    pu.clear();
+    size_t a = 0;
+    for (;;) {
+        const size_t p = line.find('\t', a);
+        if (p == std::string::npos) break;
+        const std::string tok = line.substr(a, p - a);
+        a = p + 1;
+        if (tok.size() < 3) continue;
+        if (tok[0] == 'I' && tok[1] == 'D') id = tok.substr(3);
+        if (tok[0] == 'P' && tok[1] == 'U') pu = tok.substr(3);
+    }
+}
+
+} // namespace
+
+struct fp_alignment_file {
+    Reader rd;
+    const fp_sequence_db *db = nullptr;
+    bool bam = false;
+    uint64_t records = 0, unknown_rg = 0, lineno = 0;
+    std::string header_text;
+    std::vector<std::string> target_name;
+    std::vector<uint32_t> target_len;
+    std::unordered_map<std::string, uint32_t> sam_target;   // SAM: RNAME -> tid
+    std::vector<uint32_t> target_contig;                    // tid -> contig id of db, or UINT32_MAX
+    std::unordered_map<std::string, std::string> rg_id_to_name;
+    uint32_t max_read_groups = 0;
+    NameDb read_groups;                                     // alignment_header_host::read_groups_db
+    std::vector<const char *> rg_name_ptrs;
+    BatchStore batch[2];   // alternate, so a batch stays valid while the next one loads
+    unsigned cur = 0;
+    Record rec;
+    std::string line, pending;
+    bool have_pending = false;
+};
+
+namespace {
+
+fp_status fmt_error(fp_alignment_file *f, const std::string &why)
+{
+    const std::string where = f->bam ? " record " + std::to_string(f->records + 1)
+                                     : ":" + std::to_string(f->lineno);
+    return fail(FP_ERR_FORMAT, f->rd.path() + where + ": " + why);
+}
+
+// alignment_file::init (loader/alignments.cu:151-198): read-group map from every "@RG" of the
+// header text (found anywhere, up to the next newline).
+void build_rg_map(fp_alignment_file *f)
+{
+    const std::string &h = f->header_text;
+    std::unordered_set<uint32_t> distinct;
+    size_t at = 0;
+    for (;;) {
+        at = h.find("@RG", at);
+        if (at == std::string::npos) break;
+        size_t end = h.find('\n', at + 1);
+        const std::string rec = h.substr(at, end == std::string::npos ? std::string::npos : end - at);
+        std::string id, pu;
+        parse_rg_line(rec, id, pu);
+        const std::string name = pu.empty() ? id : pu;
+        f->rg_id_to_name[id] = name;
+        if (end == std::string::npos) break;
+        at = end + 1;
+    }
+    for (auto &kv : f->rg_id_to_name) distinct.insert(name_hash(kv.second.c_str()));
+    f->max_read_groups = (uint32_t)distinct.size();
+}
+
+void map_targets(fp_alignment_file *f)
+{
+    f->target_contig.resize(f->target_name.size());
+    for (size_t i = 0; i < f->target_name.size(); i++)
+        f->target_contig[i] = f->db->names.lookup(f->target_name[i].c_str());
+}
+
+fp_status open_sam_header(fp_alignment_file *f)
+{
+    while (f->rd.peek() == '@') {
+        f->rd.line(f->line);
+        f->lineno++;
+        f->header_text += f->line;
+        f->header_text += '\n';
+        if (f->line.compare(0, 4, "@SQ\t") == 0) {
+            std::string sn;
+            long long ln = -1;
+            for (const std::string &t : split(f->line, '\t')) {
+                if (t.compare(0, 3, "SN:") == 0) sn = t.substr(3);
+                if (t.compare(0, 3, "LN:") == 0) ln = std::strtoll(t.c_str() + 3, nullptr, 10);
+            }
+            if (sn.empty() || ln < 0) return fmt_error(f, "@SQ line without SN or LN");
+            if (!f->sam_target.count(sn)) {
+                f->sam_target[sn] = (uint32_t)f->target_name.size();
+                f->target_name.push_back(sn);
+                f->target_len.push_back((uint32_t)ln);
+            }
+        }
+    }
+    return FP_OK;
+}
+
+template <class T> T rd_le(const char *p)
+{
+    T v;
+    std::memcpy(&v, p, sizeof v);
+    return v;
+}
+
+fp_status open_bam_header(fp_alignment_file *f)
+{
+    Reader &r = f->rd;
+    if (r.want(12) < 12) return fmt_error(f, "truncated BAM header");
+    r.skip(4);
+    const int32_t l_text = rd_le<int32_t>(r.ptr());
+    r.skip(4);
+    if (l_text < 0 || r.want((size_t)l_text + 4) < (size_t)l_text + 4) return fmt_error(f, "truncated BAM header");
+    f->header_text.assign(r.ptr(), (size_t)l_text);
+    r.skip((size_t)l_text);
+    const int32_t n_ref = rd_le<int32_t>(r.ptr());
+    r.skip(4);
+    if (n_ref < 0) return fmt_error(f, "negative reference count");
+    for (int32_t i = 0; i < n_ref; i++) {
+        if (r.want(4) < 4) return fmt_error(f, "truncated BAM header");
+        const int32_t l_name = rd_le<int32_t>(r.ptr());
+        r.skip(4);
+        if (l_name < 1 || r.want((size_t)l_name + 4) < (size_t)l_name + 4) return fmt_error(f, "truncated BAM header");
+        f->target_name.emplace_back(r.ptr(), strnlen(r.ptr(), (size_t)l_name));
+        r.skip((size_t)l_name);
+        f->target_len.push_back(rd_le<uint32_t>(r.ptr()));
+        r.skip(4);
+    }
+    return FP_OK;
+}
+
+inline int32_t parse_i32(const char *s, const char *e, bool &ok, int base = 10)
+{
+    char *end = nullptr;
+    const long long v = std::strtoll(s, &end, base);
+    ok = end == e && s != e;
+    return (int32_t)v;
+}
+
+// One SAM text line into rec (the SAM specification's mandatory fields and the RG tag).
+// Where htslib's sam_parse1 (1.2) gives a field a defined meaning, the same meaning is used:
+// RNAME "*" or not among the @SQ names -> tid -1; CIGAR "*" -> no ops and the read flagged
+// unmapped; RNEXT "=" -> tid; SEQ "*" -> no bases; QUAL "*" -> 0xFF per base; FLAG in any
+// strtol base.  A record htslib rejects is an error here (the reference ends the batch there).
+fp_status parse_sam(fp_alignment_file *f, const std::string &l, Record &rec)
+{
+    const char *p = l.c_str();
+    const char *fld[11], *fend[11];
+    const char *q = p;
+    for (int i = 0; i < 11; i++) {
+        fld[i] = q;
+        const char *t = std::strchr(q, '\t');
+        if (!t) {
+            if (i < 10) return fmt_error(f, "fewer than 11 fields");
+            t = q + std::strlen(q);
+        }
+        fend[i] = t;
+        q = *t ? t + 1 : t;
+    }
+    const char *tags = (*fend[10] == '\t') ? fend[10] + 1 : nullptr;
+    auto field = [&](int i) { return std::string(fld[i], fend[i]); };
+    bool ok;
+    rec.flag = (uint32_t)parse_i32(fld[1], fend[1], ok, 0);
+    if (!ok) return fmt_error(f, "bad FLAG");
+    const std::string rname = field(2);
+    if (rname == "*") {
+        rec.tid = -1;
+    } else {
+        if (f->target_name.empty()) return fmt_error(f, "missing SAM header");
+        auto it = f->sam_target.find(rname);
+        rec.tid = it == f->sam_target.end() ? -1 : (int32_t)it->second;
+    }
+    rec.pos = parse_i32(fld[3], fend[3], ok) - 1;
+    if (!ok) return fmt_error(f, "bad POS");
+    rec.mapq = (uint32_t)parse_i32(fld[4], fend[4], ok) & 0xff;
+    if (!ok) return fmt_error(f, "bad MAPQ");
+    rec.cigar.clear();
+    if (fend[5] - fld[5] == 1 && *fld[5] == '*') {
+        rec.flag |= 4;
+    } else {
+        const char *c = fld[5];
+        while (c < fend[5]) {
+            char *e = nullptr;
+            const unsigned long long len = std::strtoull(c, &e, 10);
+            if (e == c || e >= fend[5]) return fmt_error(f, "bad CIGAR");
+            const int op = cigar_code(*e);
+            if (op < 0) return fmt_error(f, "unrecognized CIGAR operator");
+            rec.cigar.push_back((uint32_t)(len << 4) | (uint32_t)op);
+            c = e + 1;
+        }
+    }
+    const std::string rnext = field(6);
+    if (rnext == "=") {
+        rec.mtid = rec.tid;
+    } else if (rnext == "*") {
+        rec.mtid = -1;
+    } else {
+        auto it = f->sam_target.find(rnext);
+        rec.mtid = it == f->sam_target.end() ? -1 : (int32_t)it->second;
+    }
+    rec.mpos = parse_i32(fld[7], fend[7], ok) - 1;
+    if (!ok) return fmt_error(f, "bad PNEXT");
+    rec.isize = parse_i32(fld[8], fend[8], ok);
+    if (!ok) return fmt_error(f, "bad TLEN");
+    rec.seq.clear();
+    const size_t lseq = (fend[9] - fld[9] == 1 && *fld[9] == '*') ? 0 : (size_t)(fend[9] - fld[9]);
+    if (lseq) {
+        uint64_t qlen = 0;
+        for (uint32_t w : rec.cigar)
+            if (cigar_type(w & 15) & 1) qlen += w >> 4;
+        if (!rec.cigar.empty() && qlen != lseq) return fmt_error(f, "SEQ and CIGAR are inconsistent");
+        rec.seq.resize(lseq);
+        for (size_t i = 0; i < lseq; i++) rec.seq[i] = NT16.t[(unsigned char)fld[9][i]];
+    }
+    rec.qual.assign(lseq, 0xff);
+    if (!(fend[10] - fld[10] == 1 && *fld[10] == '*')) {
+        if ((size_t)(fend[10] - fld[10]) != lseq) return fmt_error(f, "SEQ and QUAL are of different length");
+        for (size_t i = 0; i < lseq; i++) rec.qual[i] = (uint8_t)(fld[10][i] - 33);
+    }
+    // first RG tag (bam_aux_get); a non-Z RG is treated as absent
+    rec.has_rg = false;
+    for (const char *t = tags; t && *t;) {
+        const char *e = std::strchr(t, '\t');
+        const size_t n = e ? (size_t)(e - t) : std::strlen(t);
+        if (n >= 5 && t[0] == 'R' && t[1] == 'G' && t[2] == ':') {
+            if (t[3] == 'Z' && t[4] == ':') {
+                rec.has_rg = true;
+                rec.rg.assign(t + 5, n - 5);
+            }
+            break;
+        }
+        t = e ? e + 1 : nullptr;
+    }
+    return FP_OK;
+}
+
+// One BAM record (SAM specification 4.2) into rec.
+fp_status parse_bam(fp_alignment_file *f, const char *p, size_t n, Record &rec)
+{
+    if (n < 32) return fmt_error(f, "record shorter than its fixed fields");
+    rec.tid = rd_le<int32_t>(p);
+    rec.pos = rd_le<int32_t>(p + 4);
+    const uint32_t l_read_name = (uint8_t)p[8];
+    rec.mapq = (uint8_t)p[9];
+    const uint32_t n_cigar = rd_le<uint16_t>(p + 12);
+    rec.flag = rd_le<uint16_t>(p + 14);
+    const int32_t l_seq = rd_le<int32_t>(p + 16);
+    rec.mtid = rd_le<int32_t>(p + 20);
+    rec.mpos = rd_le<int32_t>(p + 24);
+    rec.isize = rd_le<int32_t>(p + 28);
+    if (l_seq < 0) return fmt_error(f, "negative sequence length");
+    const uint64_t need = 32ull + l_read_name + 4ull * n_cigar + (uint64_t)(l_seq + 1) / 2 + (uint64_t)l_seq;
+    if (need > n) return fmt_error(f, "record shorter than its fields");
+    const char *c = p + 32 + l_read_name;
+    rec.cigar.resize(n_cigar);
+    for (uint32_t i = 0; i < n_cigar; i++) rec.cigar[i] = rd_le<uint32_t>(c + 4 * i);
+    const uint8_t *s = (const uint8_t *)(c + 4 * n_cigar);
+    rec.seq.resize((size_t)l_seq);
+    for (int32_t i = 0; i < l_seq; i++) rec.seq[i] = (s[i >> 1] >> ((~i & 1) << 2)) & 15;   // bam_seqi
+    const uint8_t *qs = s + (l_seq + 1) / 2;
+    rec.qual.assign(qs, qs + l_seq);
+    // aux fields: first RG (bam_aux_get), its string if of type Z (bam_aux2Z)
+    rec.has_rg = false;
+    const char *a = (const char *)(qs + l_seq), *ae = p + n;
+    while (a + 3 <= ae) {
+        const char t0 = a[0], t1 = a[1], ty = a[2];
+        const char *v = a + 3;
+        size_t sz;
+        switch (ty) {
+        case 'A': case 'c': case 'C': sz = 1; break;
+        case 's': case 'S': sz = 2; break;
+        case 'i': case 'I': case 'f': sz = 4; break;
+        case 'd': sz = 8; break;
+        case 'Z': case 'H': {
+            const char *z = (const char *)std::memchr(v, 0, (size_t)(ae - v));
+            if (!z) return fmt_error(f, "unterminated string tag");
+            sz = (size_t)(z - v) + 1;
+            break;
+        }
+        case 'B': {
+            if (v + 5 > ae) return fmt_error(f, "truncated array tag");
+            const char sub = v[0];
+            const uint32_t cnt = rd_le<uint32_t>(v + 1);
+            const size_t es = (sub == 'c' || sub == 'C') ? 1 : (sub == 's' || sub == 'S') ? 2 : 4;
+            sz = 5 + (size_t)cnt * es;
+            break;
+        }
+        default: return fmt_error(f, "unknown tag type");
+        }
+        if (v + sz > ae) return fmt_error(f, "truncated tag");
+        if (t0 == 'R' && t1 == 'G') {
+            if (ty == 'Z') {
+                rec.has_rg = true;
+                rec.rg.assign(v, sz - 1);
+            }
+            break;
+        }
+        a = v + sz;
+    }
+    return FP_OK;
+}
+
+// bam_endpos of htslib 1.2: pos + reference length of the CIGAR for a mapped read with CIGAR
+// ops, else pos + 1.  The reference stores bam_endpos - 1 (loader/alignments.cu:302-309).
+int32_t bam_endpos(const Record &r)
+{
+    if (!(r.flag & 4) && !r.cigar.empty()) {
+        int64_t l = 0;
+        for (uint32_t w : r.cigar)
+            if (cigar_type(w & 15) & 2) l += w >> 4;
+        return (int32_t)(r.pos + l);
+    }
+    return r.pos + 1;
+}
+
+// alignment_file::next_batch body for one record (loader/alignments.cu:250-404)
+fp_status append(fp_alignment_file *f, const Record &r)
+{
+    BatchStore &b = f->batch[f->cur];
+    const uint32_t ntarget = (uint32_t)f->target_name.size();
+    // chromosome: contig id of the record's reference name; a named contig missing from the
+    // reference is an error, no contig at all loads as 0xFFFF for the filters to drop
+    uint16_t chrom = 0xffff;
+    if (r.tid >= 0 && (uint32_t)r.tid < ntarget) {
+        const uint32_t id = f->target_contig[(uint32_t)r.tid];
+        if (id == UINT32_MAX)
+            return fail(FP_ERR_FORMAT, "sequence " + f->target_name[(uint32_t)r.tid] + " not found in reference file");
+        if (id >= 0xffff) return fail(FP_ERR_UNSUPPORTED, "more than 65535 reference contigs");
+        chrom = (uint16_t)id;
+    }
+    b.chromosome.push_back(chrom);
+    b.alignment_start.push_back((uint32_t)r.pos);
+    b.alignment_stop.push_back((uint32_t)(bam_endpos(r) - 1));
+    uint32_t mchrom = UINT32_MAX;
+    if (r.mtid >= 0 && (uint32_t)r.mtid < ntarget) mchrom = f->target_contig[(uint32_t)r.mtid];
+    b.mate_chromosome.push_back(mchrom);
+    b.mate_alignment_start.push_back((uint32_t)r.mpos);
+    b.inferred_insert_size.push_back(r.isize);
+    // cigar_op{len:24, op:4}: B becomes X (loader/alignments.cu:200-212, 318-334)
+    b.cigar_start.push_back((uint32_t)b.cigars.size());
+    b.cigar_len.push_back((uint32_t)r.cigar.size());
+    for (uint32_t w : r.cigar) {
+        uint32_t op = w & 15;
+        if (op == 9) op = 8;
+        b.cigars.push_back(((w >> 4) & 0xffffffu) | (op << 24));
+    }
+    // bases: 4-bit codes, each read starting on a word, padded to 8 (loader/alignments.cu:337-367)
+    const uint32_t n = (uint32_t)r.seq.size();
+    b.read_start.push_back((uint32_t)b.read_bases);
+    b.read_len.push_back(n);
+    const uint64_t padded = ((uint64_t)n + 7) / 8 * 8;
+    const size_t w0 = b.reads.size();
+    b.reads.resize(w0 + padded / 8, 0);
+    for (uint32_t i = 0; i < n; i++) b.reads[w0 + i / 8] |= (uint32_t)r.seq[i] << ((i & 7) * 4);
+    b.read_bases += padded;
+    b.qual_start.push_back((uint32_t)b.qualities.size());
+    b.qual_len.push_back(n);
+    b.qualities.insert(b.qualities.end(), r.qual.begin(), r.qual.end());
+    // the twelve SAM flag bits map one to one onto AlignmentFlags (loader/alignments.cu:214-255)
+    b.flags.push_back((uint16_t)(r.flag & 0xfff));
+    b.mapq.push_back((uint8_t)r.mapq);
+    // read group: RG tag -> @RG name -> id by first appearance (loader/alignments.cu:383-403)
+    uint32_t rg = UINT32_MAX;
+    if (r.has_rg) {
+        auto it = f->rg_id_to_name.find(r.rg);
+        if (it == f->rg_id_to_name.end()) {
+            f->unknown_rg++;
+        } else {
+            rg = f->read_groups.insert(it->second);
+        }
+    }
+    b.read_group.push_back(rg);
+    return FP_OK;
+}
+
+} // namespace
+
+extern "C" fp_status fp_alignments_open(const char *path, const fp_sequence_db *db, fp_alignment_file **out)
+{
+    if (!path || !db || !out) return fail(FP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    fp_alignment_file *f = new fp_alignment_file();
+    f->db = db;
+    if (!f->rd.open(path)) {
+        delete f;
+        return fail(FP_ERR_IO, std::string("cannot open ") + path);
+    }
+    f->bam = f->rd.want(4) >= 4 && std::memcmp(f->rd.ptr(), "BAM\1", 4) == 0;
+    fp_status rc = f->bam ? open_bam_header(f) : open_sam_header(f);
+    if (rc != FP_OK) {
+        delete f;
+        return rc;
+    }
+    build_rg_map(f);
+    map_targets(f);
+    *out = f;
+    return FP_OK;
+}
+
+extern "C" fp_status fp_alignments_next_batch(fp_alignment_file *f, uint32_t batch_size, fp_batch_view *out)
+{
+    if (!f || !out) return fail(FP_ERR_INVALID_ARG, "null argument");
+    f->cur ^= 1;
+    BatchStore &b = f->batch[f->cur];
+    b.clear();
+    for (uint32_t i = 0; i < batch_size; i++) {
+        fp_status rc;
+        if (f->bam) {
+            Reader &r = f->rd;
+            if (r.want(4) < 4) break;
+            const int32_t bs = rd_le<int32_t>(r.ptr());
+            if (bs < 0 || r.want(4 + (size_t)bs) < 4 + (size_t)bs) {
+                b.clear();
+                b.view(out);
+                return fmt_error(f, "truncated record");
+            }
+            rc = parse_bam(f, r.ptr() + 4, (size_t)bs, f->rec);
+            r.skip(4 + (size_t)bs);
+        } else {
+            if (!f->rd.line(f->line)) break;
+            f->lineno++;
+            if (f->line.empty()) { i--; continue; }
+            rc = parse_sam(f, f->line, f->rec);
+        }
+        if (rc == FP_OK) rc = append(f, f->rec);
+        if (rc != FP_OK) {
+            b.clear();
+            b.view(out);
+            return rc;
+        }
+        f->records++;
+    }
+    if (f->rd.error()) {
+        b.clear();
+        b.view(out);
+        return fail(FP_ERR_IO, f->rd.path() + ": read error");
+    }
+    b.view(out);
+    return FP_OK;
+}
+
+extern "C" fp_status fp_alignments_header(const fp_alignment_file *cf, fp_header_view *out, uint32_t *max_read_groups)
+{
+    if (!cf || !out) return fail(FP_ERR_INVALID_ARG, "null argument");
+    fp_alignment_file *f = const_cast<fp_alignment_file *>(cf);
+    f->rg_name_ptrs.clear();
+    for (const std::string &s : f->read_groups.names) f->rg_name_ptrs.push_back(s.c_str());
+    out->num_chromosomes = (uint32_t)f->target_len.size();
+    out->chromosome_lengths = f->target_len.data();
+    out->num_read_groups = (uint32_t)f->rg_name_ptrs.size();
+    out->read_group_names = f->rg_name_ptrs.data();
+    if (max_read_groups) *max_read_groups = f->max_read_groups;
+    return FP_OK;
+}
+
+extern "C" fp_status fp_alignments_counts(const fp_alignment_file *f, uint64_t *records, uint64_t *unknown_read_group)
+{
+    if (!f) return fail(FP_ERR_INVALID_ARG, "null argument");
+    if (records) *records = f->records;
+    if (unknown_read_group) *unknown_read_group = f->unknown_rg;
+    return FP_OK;
+}
+
+extern "C" void fp_alignments_close(fp_alignment_file *f) { delete f; }
+
+extern "C" const char *fp_io_last_error(void) { return g_err.c_str(); }

@@ -24,6 +24,7 @@ EXPORTED = [
     "fp_engine_debug_fetch", "fp_engine_reset", "fp_engine_destroy", "fp_last_error",
     "fp_status_string", "fp_job_open", "fp_job_submit", "fp_job_submit_device", "fp_job_close",
     "fp_job_write_report", "fp_job_get_table", "fp_job_stats", "fp_job_release",
+    "fp_engine_set_read_group_names",
 ]
 
 
@@ -72,6 +73,7 @@ def lib():
         L.fp_job_stats.argtypes = [vp, C.POINTER(abi.FpStats)]
         L.fp_job_release.argtypes = [vp]
         L.fp_job_release.restype = None
+        L.fp_engine_set_read_group_names.argtypes = [vp, C.c_uint32, C.POINTER(C.c_char_p)]
         for name in EXPORTED:
             f = getattr(L, name)
             if f.restype is C.c_int and name not in ("fp_options_init",):
@@ -124,6 +126,11 @@ class Engine:
 
     def reset(self):
         _check(lib().fp_engine_reset(self._h))
+
+    def set_read_group_names(self, names):
+        """Names by read-group id, as assigned while the reads were loaded."""
+        arr = (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+        _check(lib().fp_engine_set_read_group_names(self._h, len(names), arr))
 
     def table(self, table_id):
         out = abi.FpTableOut()

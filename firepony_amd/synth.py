@@ -75,34 +75,7 @@ class Batch:
 
     def arrays(self):
         """numpy views of every array of the batch (no copies)."""
-        v = self.view
-        n = v.num_reads
-
-        def arr(ptr, count, dtype):
-            if count == 0:
-                return np.zeros(0, dtype=dtype)
-            return np.ctypeslib.as_array(ptr, shape=(count,)).view(dtype)
-
-        return {
-            "chromosome": arr(v.chromosome, n, np.uint16),
-            "alignment_start": arr(v.alignment_start, n, np.uint32),
-            "alignment_stop": arr(v.alignment_stop, n, np.uint32),
-            "mate_chromosome": arr(v.mate_chromosome, n, np.uint32),
-            "mate_alignment_start": arr(v.mate_alignment_start, n, np.uint32),
-            "inferred_insert_size": arr(v.inferred_insert_size, n, np.int32),
-            "cigars": arr(v.cigars, v.num_cigar_ops, np.uint32),
-            "cigar_start": arr(v.cigar_start, n, np.uint32),
-            "cigar_len": arr(v.cigar_len, n, np.uint32),
-            "reads": arr(v.reads, v.num_read_bases // 8, np.uint32),
-            "read_start": arr(v.read_start, n, np.uint32),
-            "read_len": arr(v.read_len, n, np.uint32),
-            "qualities": arr(v.qualities, v.num_qualities, np.uint8),
-            "qual_start": arr(v.qual_start, n, np.uint32),
-            "qual_len": arr(v.qual_len, n, np.uint32),
-            "flags": arr(v.flags, n, np.uint16),
-            "mapq": arr(v.mapq, n, np.uint8),
-            "read_group": arr(v.read_group, n, np.uint32),
-        }
+        return abi.view_arrays(self.view)
 
     def __del__(self):
         if getattr(self, "_h", None):

@@ -42,7 +42,8 @@ typedef enum fp_status {
     FP_ERR_BUDGET = 5,        /* serving: job refused by the HBM admission budget   */
     FP_ERR_UNSUPPORTED = 6,   /* option or input outside what the engine supports  */
     FP_ERR_IO = 7,            /* report file could not be written                  */
-    FP_ERR_INTERNAL = 8
+    FP_ERR_INTERNAL = 8,
+    FP_ERR_FORMAT = 9         /* input file malformed (loaders, firepony_amd_io.h)  */
 } fp_status;
 
 /* engine flavours (both produce bit-identical tables) */
@@ -270,6 +271,11 @@ fp_status fp_engine_postprocess(fp_engine *e);
  * (device/pipeline.cu:150-204, read_group_table.cu:122-143, covariates.cu:285-311) */
 fp_status fp_engine_write_report(fp_engine *e, const char *path);
 fp_status fp_engine_get_table(fp_engine *e, int table_id, fp_table_out *out);
+/* alignment_header_host::read_groups_db (alignment_data.h:51-54): the
+ * reference assigns read-group ids while it loads reads and prints their names
+ * at the end; this replaces the names given to fp_engine_setup (n <= the count
+ * given there; ids >= n keep theirs).  Not for serving engines. */
+fp_status fp_engine_set_read_group_names(fp_engine *e, uint32_t n, const char *const *names);
 /* firepony_pipeline::statistics */
 fp_status fp_engine_stats(fp_engine *e, fp_stats *out);
 /* firepony_pipeline::get_name / get_total_memory */

@@ -1760,6 +1760,13 @@ fpo_ctx *fpo_create(int frac_error_f64, int accumulate_mode, int disable_output_
     return c;
 }
 
+int fpo_set_read_group_names(fpo_ctx *c, uint32_t n, const char *const *names)
+{
+    if (!c || n > c->rg_names.size() || (n && !names)) return FP_ERR_INVALID_ARG;
+    for (uint32 i = 0; i < n; i++) c->rg_names[i] = names[i] ? names[i] : "";
+    return FP_OK;
+}
+
 int fpo_setup(fpo_ctx *c, const fp_header_view *h, const fp_reference_view *r, const fp_variants_view *v)
 {
     if (!c || !h || !r || !v) return FP_ERR_INVALID_ARG;

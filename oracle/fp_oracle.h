@@ -39,6 +39,9 @@ int fpo_stats(fpo_ctx *c, fp_stats *out);
 /* adds src's observation tables and read counters into dst (exact accumulation only): the
  * multi-device concat + reduce of gather_intermediates (pipeline_interface.cu:52-58,137-160) */
 int fpo_gather(fpo_ctx *dst, fpo_ctx *src);
+/* read-group names by id as the reference's read_groups_db holds them when the report is
+ * written (alignment_data.h:51-54): replaces the first n names given to fpo_setup */
+int fpo_set_read_group_names(fpo_ctx *c, uint32_t n, const char *const *names);
 int fpo_debug_size(fpo_ctx *c, int item, uint64_t *bytes);
 int fpo_debug_fetch(fpo_ctx *c, int item, void *dst, uint64_t bytes);
 /* BAQ MAP quality of the last batch as the real value before truncation,

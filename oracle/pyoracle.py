@@ -31,6 +31,7 @@ def lib():
         L.fpo_process_batch.argtypes = [C.c_void_p, C.POINTER(abi.FpBatchView)]
         L.fpo_postprocess.argtypes = [C.c_void_p]
         L.fpo_gather.argtypes = [C.c_void_p, C.c_void_p]
+        L.fpo_set_read_group_names.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_char_p)]
         L.fpo_get_table.argtypes = [C.c_void_p, C.c_int, C.POINTER(abi.FpTableOut)]
         L.fpo_write_report.argtypes = [C.c_void_p, C.c_char_p]
         L.fpo_stats.argtypes = [C.c_void_p, C.POINTER(abi.FpStats)]
@@ -103,6 +104,12 @@ class Oracle:
         rc = lib().fpo_gather(self._h, other._h)
         if rc:
             raise RuntimeError(f"fpo_gather: {rc}")
+
+    def set_read_group_names(self, names):
+        arr = (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+        rc = lib().fpo_set_read_group_names(self._h, len(names), arr)
+        if rc:
+            raise RuntimeError(f"fpo_set_read_group_names: {rc}")
 
     def table(self, table_id):
         return read_table(lib().fpo_get_table, self._h, table_id)
