@@ -156,7 +156,21 @@ def cpu_baseline(genome, rs, sample_reads, threads):
     dt = time.perf_counter() - t0
     return {"value": sample_reads / dt, "unit": "reads/s", "cores": threads, "kind": "port",
             "sample": f"first {sample_reads} reads of the workload, batches of 1000, {threads} threads, "
-                      f"{dt:.2f} s"}
+                      f"{dt:.2f} s", "host": host_cpu()}
+
+
+def host_cpu():
+    """CPU model and logical CPU count of this host (lscpu's 'Model name' and 'CPU(s)')."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs"
 
 
 def upload(genome, rs, total, batch, torch):
@@ -325,6 +339,8 @@ def main():
                          "50-250 bp set)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-reads", type=int, default=2000000)
+    ap.add_argument("--no-genome3g", action="store_true")
+    ap.add_argument("--genome3g-reads", type=int, default=4000000)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-jobs", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=400000, help="reads of the CPU baseline sample")
@@ -438,6 +454,24 @@ def main():
                                "baq_waves_per_simd": int(bst2.baq_waves_per_simd)}
         log(f"mixed lengths: plain {n2 / pdt2:.4g} reads/s, batched {n2 / bdt2:.4g} reads/s")
         del dev2
+        torch.cuda.empty_cache()
+
+    if not args.no_genome3g:
+        gs3, rs3, total3 = synth.named_spec(4)
+        t0 = time.perf_counter()
+        genome3 = synth.Genome(gs3, rs3.num_read_groups)
+        n3 = min(total3, args.genome3g_reads)
+        log(f"3 Gbp genome in {time.perf_counter() - t0:.1f} s; generating {n3} paired 150 bp reads")
+        dev3 = upload(genome3, rs3, n3, B, torch)
+        bdt3, bst3 = timed_pass(genome3, dev3, args.warmup, torch, engine_kind=abi.FP_ENGINE_BATCHED,
+                                max_read_length=150)
+        out["genome_3gbp"] = {"reads": n3, "value": n3 / bdt3, "unit": "reads/s",
+                              "workload": "3 Gbp / 24 contigs resident in HBM (1.5 GB packed) with 1M known "
+                                          "sites, paired 150 bp reads, 16 read groups, 4 % indels, clips 0-40; "
+                                          "first reads of the 600M-read set",
+                              "baq_waves_per_simd": int(bst3.baq_waves_per_simd)}
+        log(f"3 Gbp genome: batched {n3 / bdt3:.4g} reads/s")
+        del dev3, genome3
         torch.cuda.empty_cache()
 
     if not args.no_served:

@@ -148,6 +148,14 @@ def test_long_reads(tmp_path):
     run_parity(tmp_path, g, rs, n, 750)
 
 
+def test_genome_3gbp(tmp_path):
+    say("parity: 3 Gbp reference (24 contigs) and 1M known sites resident, paired 150 bp reads, 16 read groups")
+    gs, rs, n = synth.named_spec(4)
+    g = synth.Genome(gs, rs.num_read_groups)
+    assert g.reference.total_words == 375_000_000
+    run_parity(tmp_path, g, rs, 20000, 10000)
+
+
 def test_batch_size_invariance(tmp_path):
     say("invariance: one batch vs many batches give identical tables")
     g, rs, n = synth.small_workload(seed=5, num_reads=3000, edge=0.1)
