@@ -284,14 +284,15 @@ def served(args, wpe, torch):
     return res
 
 
-def sort_pass(args, genome, dev, wpe, batched_opts, reads, torch):
+def sort_pass(args, genome, dev, wpe, ckpt, batched_opts, reads, torch):
     """The batched engine with the sort aggregation (covariate pairs -> LDS-tiled radix sort ->
     segmented reduce_by_key): reads/s with the digit width autotuned during the warm-up pass, then a
     per-stage pass at that width for the sort/reduce time and its HBM rate."""
     sopts = dict(batched_opts, aggregation=abi.FP_AGG_SORT)
-    dt, st = timed_pass(genome, dev, max(args.warmup, 1), torch, baq_waves_per_simd=wpe, **sopts)
+    dt, st = timed_pass(genome, dev, max(args.warmup, 1), torch, baq_waves_per_simd=wpe,
+                        baq_checkpoint_rows=ckpt, **sopts)
     bits = int(st.sort_digit_bits)
-    _, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=wpe, profile_stages=1,
+    _, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=wpe, baq_checkpoint_rows=ckpt, profile_stages=1,
                         **dict(sopts, sort_digit_bits=bits))
     batches = max(int(pst.cov_launches), 1)
     sort_s = pst.covariates_sort_ms / 1e3
@@ -334,6 +335,8 @@ def main():
     ap.add_argument("--aggregation", choices=("direct", "sort"), default="direct",
                     help="covariate aggregation of the profile-only pass")
     ap.add_argument("--no-sort", action="store_true", help="skip the sort-aggregation pass")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched engine's timed pass (graph replays), for kernel-overlap traces")
     ap.add_argument("--workload", choices=("config1", "mixed"), default="config1",
                     help="workload of the profile-only pass (mixed: the first --mixed-reads reads of the "
                          "50-250 bp set)")
@@ -346,6 +349,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400000, help="reads of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--wpe", type=int, default=0, help="BAQ waves per SIMD (0 = autotuned)")
+    ap.add_argument("--ckpt", type=int, default=0,
+                    help="rows per BAQ forward checkpoint, 1 or 2 (0 = autotuned; profile-only pass: 1)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -387,7 +392,8 @@ def main():
             popts.update(max_read_length=250, frac_error_f64=1)
         if args.aggregation == "sort":
             popts.update(aggregation=abi.FP_AGG_SORT, sort_digit_bits=8)
-        pdt, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=args.wpe or 1, profile_stages=1, **popts)
+        pdt, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=args.wpe or 1,
+                              baq_checkpoint_rows=args.ckpt or 1, profile_stages=1, **popts)
         log(f"profile: {reads} reads in {pdt:.3f} s; prep {pst.prep_kernel_ms:.1f} ms, baq_hmm "
             f"{pst.baq_hmm_kernel_ms:.1f} ms ({pst.baq_hmm_launches} launches), cov {pst.cov_kernel_ms:.1f} ms")
         print(json.dumps({"profile_only": True, "reads": reads, "seconds": pdt, "kernel_ms": kernel_ms(pst),
@@ -400,13 +406,20 @@ def main():
         out_plain = {"value": reads / dt, "seconds": dt, "kernel_ms": kernel_ms(st)}
         log(f"plain: {reads} reads in {dt:.3f} s -> {reads / dt:.4g} reads/s")
 
-    dt, st = timed_pass(genome, dev, args.warmup, torch, baq_waves_per_simd=args.wpe, **batched_opts)
+    dt, st = timed_pass(genome, dev, args.warmup, torch, baq_waves_per_simd=args.wpe,
+                        baq_checkpoint_rows=args.ckpt, **batched_opts)
     value = reads / dt
     wpe = int(st.baq_waves_per_simd)
-    log(f"batched: {reads} reads in {dt:.3f} s -> {value:.4g} reads/s (BAQ waves/SIMD {wpe}); "
+    ckpt = int(st.baq_checkpoint_rows)
+    if args.batched_only:
+        log(f"batched: {reads} reads in {dt:.3f} s -> {value:.4g} reads/s")
+        print(json.dumps({"batched_only": True, "reads": reads, "seconds": dt, "value": value}), flush=True)
+        return
+    log(f"batched: {reads} reads in {dt:.3f} s -> {value:.4g} reads/s (BAQ waves/SIMD {wpe}, checkpoint rows {ckpt}); "
         f"stage 1 {st.prep_kernel_ms:.1f} ms, stage 2 {st.baq_ms:.1f} ms, postprocess {st.postprocessing_ms:.1f} ms")
 
-    pdt, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=wpe, profile_stages=1, **batched_opts)
+    pdt, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=wpe, baq_checkpoint_rows=ckpt,
+                          profile_stages=1, **batched_opts)
     kname, roof = roofline(pst, reads, bases)
     log(f"profile: prep {pst.prep_kernel_ms:.1f} ms, baq_hmm {pst.baq_hmm_kernel_ms:.1f} ms "
         f"({pst.baq_hmm_launches} launches), cov {pst.cov_kernel_ms:.1f} ms; baq reads {pst.baq_reads}, "
@@ -426,7 +439,8 @@ def main():
         "data": "synthetic (seeded generator, 30 Mbp genome, 10M x 101 bp reads)",
         "config": {"workload": "30 Mbp / 3 contigs, 10M SE 101 bp reads, 4 read groups, 30k known sites, "
                                "BAQ band 7, input batches of 20,000 reads", "reads": reads, "batch_reads": B,
-                   "engine": "batched (1M-read staging buffers, hipGraph replays)", "baq_waves_per_simd": wpe},
+                   "engine": "batched (1M-read staging buffers, hipGraph replays)", "baq_waves_per_simd": wpe,
+                   "baq_checkpoint_rows": ckpt},
         "kernel_ms": kernel_ms(pst),
         "roofline": dict(roof, kernel=kname),
     }
@@ -434,7 +448,7 @@ def main():
         out["plain"] = out_plain
         out["batched_over_plain"] = value / out_plain["value"]
     if not args.no_sort:
-        out["sort_aggregation"] = sort_pass(args, genome, dev, wpe, batched_opts, reads, torch)
+        out["sort_aggregation"] = sort_pass(args, genome, dev, wpe, ckpt, batched_opts, reads, torch)
     del dev
     torch.cuda.empty_cache()
 
@@ -451,7 +465,8 @@ def main():
                                                         "1-10 bp, clips of 0-30 bp on every read, fp64 "
                                                         "fractional errors; first reads of the 20M-read set",
                                "plain": n2 / pdt2, "batched": n2 / bdt2, "batched_over_plain": pdt2 / bdt2,
-                               "baq_waves_per_simd": int(bst2.baq_waves_per_simd)}
+                               "baq_waves_per_simd": int(bst2.baq_waves_per_simd),
+                               "baq_checkpoint_rows": int(bst2.baq_checkpoint_rows)}
         log(f"mixed lengths: plain {n2 / pdt2:.4g} reads/s, batched {n2 / bdt2:.4g} reads/s")
         del dev2
         torch.cuda.empty_cache()
@@ -469,7 +484,8 @@ def main():
                               "workload": "3 Gbp / 24 contigs resident in HBM (1.5 GB packed) with 1M known "
                                           "sites, paired 150 bp reads, 16 read groups, 4 % indels, clips 0-40; "
                                           "first reads of the 600M-read set",
-                              "baq_waves_per_simd": int(bst3.baq_waves_per_simd)}
+                              "baq_waves_per_simd": int(bst3.baq_waves_per_simd),
+                              "baq_checkpoint_rows": int(bst3.baq_checkpoint_rows)}
         log(f"3 Gbp genome: batched {n3 / bdt3:.4g} reads/s")
         del dev3, genome3
         torch.cuda.empty_cache()

@@ -57,7 +57,8 @@ class FpOptions(C.Structure):
         ("profile_stages", C.c_uint32),
         ("serve_rg_slots", C.c_uint32),
         ("disable_baq_staging", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("baq_checkpoint_rows", C.c_uint32),
+        ("reserved", C.c_uint32 * 1),
     ]
 
 
@@ -161,6 +162,7 @@ class FpStats(C.Structure):
         ("sort_slots", C.c_uint64),
         ("sort_pairs", C.c_uint64),
         ("sort_passes", C.c_uint64),
+        ("baq_checkpoint_rows", C.c_uint64),
     ]
 
 

@@ -33,7 +33,7 @@ hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t *live, con
 hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const fpd::GenomeDev *g, const fpd::ReadStateDev *s,
                               const uint32_t *list, const uint32_t *count, double *scratch, uint64_t slot_doubles,
                               uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves, unsigned long long *cells,
-                              int wpe, int allow_stage, hipStream_t st);
+                              int wpe, int allow_stage, int recompute, hipStream_t st);
 hipError_t fpd_baq_prepare();
 hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *s, uint32_t band_done,
                                hipStream_t st);
@@ -42,6 +42,8 @@ hipError_t fpd_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
 hipError_t fpd_rebase(uint32_t n, uint32_t *cigar_start, uint32_t *read_start, uint32_t *qual_start,
                       uint32_t *read_group, uint32_t cig_off, uint32_t base_off, uint32_t qual_off, uint32_t rg_off,
                       uint32_t rg_limit, hipStream_t st);
+hipError_t fpd_append(const void *const *src, void *const *dst, const uint64_t *bytes, uint32_t cig_off,
+                      uint32_t base_off, uint32_t qual_off, uint32_t rg_off, uint32_t rg_limit, hipStream_t st);
 hipError_t fpd_cov_prepare();
 hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
                           uint32_t frac_f64, uint32_t keep_frac, uint32_t *counters, uint32_t *rg_buf,
@@ -198,6 +200,7 @@ struct fp_engine {
     std::string name;
     HostConstants consts;
     int wpe = 1;   // waves per SIMD of the 15-cell BAQ band kernel (1-4)
+    int ckpt = 1;  // rows per forward checkpoint of the 15-cell band kernel (1: every row spilled; 2: recompute)
 
     // setup state
     bool is_setup = false;
@@ -247,7 +250,7 @@ struct fp_engine {
         uint64_t ncig = 0, nbases = 0, nquals = 0;
         hipEvent_t filled = nullptr, consumed = nullptr;
         hipGraphExec_t gfront = nullptr, gbaq = nullptr, gcov = nullptr;
-        int g_wpe = 0, g_bits = 0;
+        int g_wpe = 0, g_ckpt = 0, g_bits = 0;
         uint64_t g_epoch = 0;
         BatchState bs;                  // this buffer's per-batch state
         hipStream_t stream = nullptr;   // this buffer's compute stream
@@ -269,7 +272,7 @@ struct fp_engine {
     int tune_pending = -1;              // candidate whose stage-2 timing is in flight
     int tune_kind = 0;                  // 0: BAQ waves per SIMD, 1: sort digit width
     uint32_t tune_pending_reads = 0;
-    double tune_ms_per_read[4] = {0, 0, 0, 0};
+    double tune_ms_per_read[4] = {0, 0, 0, 0};   // one per BAQ candidate
     hipEvent_t bev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool back_pending = false;          // stage-2 time of the last staged batch not yet collected
     uint64_t eager_filtered = 0, eager_baq = 0, eager_cells = 0;   // oversized batches run eagerly
@@ -496,7 +499,7 @@ fp_status collect_sort_counts(fp_engine *e, int sort_bits)
 }
 
 // stage 2a (graph-safe): the BAQ band classes and cap/recode
-fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, int wpe,
+fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, int wpe, int ckpt,
                            uint32_t *counters, hipStream_t st, hipEvent_t mark, const uint32_t *known_counts)
 {
     const uint32_t cap = std::max<uint32_t>(p.n, 1);
@@ -506,7 +509,7 @@ fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, 
                                    S.baq_counts.as<uint32_t>() + c, e->scratch.as<double>(),
                                    band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c],
                                    band_waves(wpe, c, p.n), (unsigned long long *)(counters + 8), wpe,
-                                   e->opts.disable_baq_staging ? 0 : 1, st));
+                                   e->opts.disable_baq_staging ? 0 : 1, ckpt > 1 ? 1 : 0, st));
     }
     if (mark) HIP_TRY(hipEventRecord(mark, st));
     HIP_TRY(fpd_launch_baq_post(&b, &S.rs, 1u, st));
@@ -514,10 +517,11 @@ fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, 
 }
 
 // stage 2 = 2a + 2b (covariates); marks: after the BAQ kernels, after cap/recode, after pair emission
-fp_status launch_back(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, int wpe, int sort_bits,
-                      uint32_t *counters, hipStream_t st, hipEvent_t *marks, const uint32_t *known_counts)
+fp_status launch_back(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, int wpe, int ckpt,
+                      int sort_bits, uint32_t *counters, hipStream_t st, hipEvent_t *marks,
+                      const uint32_t *known_counts)
 {
-    fp_status rc = launch_baq_stage(e, S, b, p, wpe, counters, st, marks ? marks[0] : nullptr, known_counts);
+    fp_status rc = launch_baq_stage(e, S, b, p, wpe, ckpt, counters, st, marks ? marks[0] : nullptr, known_counts);
     if (rc != FP_OK) return rc;
     if (marks) HIP_TRY(hipEventRecord(marks[1], st));
     return launch_cov_stage(e, S, b, p, sort_bits, counters, st, marks ? marks[2] : nullptr);
@@ -534,7 +538,7 @@ fp_status launch_generic(fp_engine *e, BatchState &S, const fpd::BatchDev &b, co
     HIP_TRY(fpd_launch_baq_hmm(3, &b, &e->genome, &S.rs,
                                S.baq_lists.as<uint32_t>() + 3ull * std::max<uint32_t>(p.n, 1),
                                S.baq_counts.as<uint32_t>() + 3, e->scratch_gen.as<double>(), slot, p.rows_cap, wgen,
-                               waves, (unsigned long long *)(counters + 8), e->wpe, 0, st));
+                               waves, (unsigned long long *)(counters + 8), e->wpe, 0, 0, st));
     return FP_OK;
 }
 
@@ -576,7 +580,7 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     rc = launch_generic(e, S, b, p, hc[3], hc[4], counters, st);
     if (rc != FP_OK) return rc;
     hipEvent_t marks[3] = {e->ev[9], e->ev[2], e->ev[10]};
-    rc = launch_back(e, S, b, p, e->wpe, e->sort_bits, counters, st, marks, hc);
+    rc = launch_back(e, S, b, p, e->wpe, e->ckpt, e->sort_bits, counters, st, marks, hc);
     if (rc != FP_OK) return rc;
     HIP_TRY(hipEventRecord(e->ev[3], st));
     uint32_t *post = e->host_counts + 16;
@@ -616,7 +620,12 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
 // ---------------------------------------------------------------------------
 // batched engine
 
-static const int TUNE_CANDIDATES[3] = {1, 2, 3};
+// BAQ candidates of the autotuner: (waves per SIMD, rows per forward checkpoint).  The checkpointed
+// kernel is not among them: measured slower on every workload here (it holds two rows in registers,
+// which saves a quarter of the forward-row traffic for a second forward pass), so it runs only when
+// asked for.
+static const int TUNE_CANDIDATES[3][2] = {{1, 1}, {2, 1}, {3, 1}};
+static const int NUM_TUNE = 3;
 static const int SORT_CANDIDATES[5] = {4, 5, 6, 7, 8};
 
 fpd::BatchDev stage_dev(fp_engine *e, fp_engine::Stage &s)
@@ -722,10 +731,13 @@ void tune_record(fp_engine *e, int kind, int cand, double per_read)
 {
     if (kind == 0) {
         e->tune_ms_per_read[cand] = per_read;
-        if (e->tune_ms_per_read[0] > 0 && e->tune_ms_per_read[1] > 0 && e->tune_ms_per_read[2] > 0) {
+        bool all = true;
+        for (int i = 0; i < NUM_TUNE; i++) all = all && e->tune_ms_per_read[i] > 0;
+        if (all) {
             int best = 0;
-            for (int i = 1; i < 3; i++) if (e->tune_ms_per_read[i] < e->tune_ms_per_read[best]) best = i;
-            e->wpe = TUNE_CANDIDATES[best];
+            for (int i = 1; i < NUM_TUNE; i++) if (e->tune_ms_per_read[i] < e->tune_ms_per_read[best]) best = i;
+            e->wpe = TUNE_CANDIDATES[best][0];
+            e->ckpt = TUNE_CANDIDATES[best][1];
             e->tuned = true;
         }
     } else {
@@ -813,11 +825,11 @@ fp_status batched_flush(fp_engine *e)
 
     // autotuner: BAQ waves per SIMD first, then the radix digit width of the sort aggregation; each
     // candidate runs one staged batch eagerly, alone on the device, and is timed per read
-    int wpe = e->wpe, bits = e->sort_bits;
+    int wpe = e->wpe, ckpt = e->ckpt, bits = e->sort_bits;
     int cand = -1, kind = 0;
     if (!e->tuned) {
-        for (int i = 0; i < 3; i++) if (e->tune_ms_per_read[i] == 0) { cand = i; break; }
-        if (cand >= 0) wpe = TUNE_CANDIDATES[cand];
+        for (int i = 0; i < NUM_TUNE; i++) if (e->tune_ms_per_read[i] == 0) { cand = i; break; }
+        if (cand >= 0) { wpe = TUNE_CANDIDATES[cand][0]; ckpt = TUNE_CANDIDATES[cand][1]; }
     } else if (!e->sort_tuned) {
         for (int i = 0; i < 5; i++) if (e->sort_tune_ms[i] == 0) { cand = i; break; }
         if (cand >= 0) { bits = SORT_CANDIDATES[cand]; kind = 1; }
@@ -829,7 +841,7 @@ fp_status batched_flush(fp_engine *e)
         rc = launch_generic(e, S, b, p, hc[3], hc[4], counters, st);
         if (rc != FP_OK) return rc;
         hipEvent_t marks[3] = {e->ev[9], e->ev[2], e->ev[10]};
-        rc = launch_back(e, S, b, p, wpe, bits, counters, st, marks, hc);
+        rc = launch_back(e, S, b, p, wpe, ckpt, bits, counters, st, marks, hc);
         if (rc != FP_OK) return rc;
         HIP_TRY(hipEventRecord(e->ev[3], st));
         HIP_TRY(hipStreamSynchronize(st));
@@ -858,14 +870,15 @@ fp_status batched_flush(fp_engine *e)
     rc = launch_generic(e, S, b, p, hc[3], hc[4], counters, st);
     if (rc != FP_OK) return rc;
     if (cand < 0) {
-        if (!s.gbaq || s.g_wpe != wpe || !s.gcov || s.g_bits != bits) {
+        if (!s.gbaq || s.g_wpe != wpe || s.g_ckpt != ckpt || !s.gcov || s.g_bits != bits) {
             rc = capture(st, &s.gbaq,
-                         [&] { return launch_baq_stage(e, S, b, p, wpe, counters, st, nullptr, nullptr); });
+                         [&] { return launch_baq_stage(e, S, b, p, wpe, ckpt, counters, st, nullptr, nullptr); });
             if (rc != FP_OK) return rc;
             rc = capture(st, &s.gcov,
                          [&] { return launch_cov_stage(e, S, b, p, bits, counters, st, nullptr); });
             if (rc != FP_OK) return rc;
             s.g_wpe = wpe;
+            s.g_ckpt = ckpt;
             s.g_bits = bits;
         }
         HIP_TRY(hipGraphLaunch(s.gbaq, st));
@@ -876,7 +889,7 @@ fp_status batched_flush(fp_engine *e)
         HIP_TRY(hipEventRecord(s.t1, st));
         s.back_pending = true;
     } else {
-        rc = launch_baq_stage(e, S, b, p, wpe, counters, st, nullptr, nullptr);   // timed candidate, eager
+        rc = launch_baq_stage(e, S, b, p, wpe, ckpt, counters, st, nullptr, nullptr);   // timed candidate, eager
         if (rc != FP_OK) return rc;
         HIP_TRY(hipEventRecord(s.baq_done, st));
         rc = launch_cov_stage(e, S, b, p, bits, counters, st, nullptr);
@@ -959,17 +972,27 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
                            v->cigar_len, v->reads, v->read_start, v->read_len, v->qualities, v->qual_start,
                            v->qual_len, v->flags, v->mapq, v->read_group};
     const uint64_t elem[18] = {2, 4, 4, 4, 4, 4, 0, 4, 4, 0, 4, 4, 0, 4, 4, 2, 1, 4};
+    void *dst[18];
+    uint64_t len[18];
     for (int i = 0; i < 18; i++) {
-        uint64_t off, bytes;
-        if (i == 6) { off = 4 * s->ncig; bytes = 4 * v->num_cigar_ops; }
-        else if (i == 9) { off = s->nbases / 2; bytes = v->num_read_bases / 2; }
-        else if (i == 12) { off = s->nquals; bytes = v->num_qualities; }
-        else { off = elem[i] * s->n; bytes = elem[i] * k; }
-        if (bytes) HIP_TRY(hipMemcpyAsync((char *)s->arr[i].p + off, src[i], bytes, kind, cs));
+        uint64_t off;
+        if (i == 6) { off = 4 * s->ncig; len[i] = 4 * v->num_cigar_ops; }
+        else if (i == 9) { off = s->nbases / 2; len[i] = v->num_read_bases / 2; }
+        else if (i == 12) { off = s->nquals; len[i] = v->num_qualities; }
+        else { off = elem[i] * s->n; len[i] = elem[i] * k; }
+        dst[i] = (char *)s->arr[i].p + off;
     }
-    HIP_TRY(fpd_rebase(k, s->arr[7].as<uint32_t>() + s->n, s->arr[10].as<uint32_t>() + s->n,
-                       s->arr[13].as<uint32_t>() + s->n, s->arr[17].as<uint32_t>() + s->n, (uint32_t)s->ncig,
-                       (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off, rg_limit, cs));
+    if (on_device) {
+        // one launch copies and rebases the whole batch (a copy per array costs a launch each)
+        HIP_TRY(fpd_append(src, dst, len, (uint32_t)s->ncig, (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off,
+                           rg_limit, cs));
+    } else {
+        for (int i = 0; i < 18; i++)
+            if (len[i]) HIP_TRY(hipMemcpyAsync(dst[i], src[i], len[i], kind, cs));
+        HIP_TRY(fpd_rebase(k, s->arr[7].as<uint32_t>() + s->n, s->arr[10].as<uint32_t>() + s->n,
+                           s->arr[13].as<uint32_t>() + s->n, s->arr[17].as<uint32_t>() + s->n, (uint32_t)s->ncig,
+                           (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off, rg_limit, cs));
+    }
     s->n += k;
     s->ncig += v->num_cigar_ops;
     s->nbases += v->num_read_bases;
@@ -1207,6 +1230,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     }
     if (o.aggregation != FP_AGG_DIRECT && o.aggregation != FP_AGG_SORT)
         return fail(FP_ERR_INVALID_ARG, "aggregation must be FP_AGG_DIRECT or FP_AGG_SORT");
+    if (o.baq_checkpoint_rows > 2) return fail(FP_ERR_INVALID_ARG, "baq_checkpoint_rows must be 0, 1 or 2");
     if (o.sort_digit_bits != 0 && (o.sort_digit_bits < 4 || o.sort_digit_bits > 8))
         return fail(FP_ERR_INVALID_ARG, "sort_digit_bits must be 0 (autotune) or 4..8");
     int ndev = 0;
@@ -1217,6 +1241,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     e->device = hip_device;
     e->opts = o;
     e->wpe = o.baq_waves_per_simd >= 1 && o.baq_waves_per_simd <= 4 ? (int)o.baq_waves_per_simd : 1;
+    e->ckpt = o.baq_checkpoint_rows == 2 ? 2 : 1;
     e->sort_bits = o.sort_digit_bits ? (int)o.sort_digit_bits : 8;
     // the digit width is tuned on the batched engine's first staged batches unless it is given
     e->sort_tuned = o.sort_digit_bits != 0 || o.aggregation != FP_AGG_SORT || o.engine_kind != FP_ENGINE_BATCHED;
@@ -1237,7 +1262,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
         e->cap_bases = (uint64_t)e->cap_reads * ((e->rows_cap + 7) & ~7u);
         e->cap_quals = (uint64_t)e->cap_reads * e->rows_cap;
         e->cap_cig = (uint64_t)e->cap_reads * 8;
-        e->tuned = o.baq_waves_per_simd != 0;
+        e->tuned = o.baq_waves_per_simd != 0 || o.baq_checkpoint_rows != 0;
         he = hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking);
         if (he != hipSuccess) { fp_engine_destroy(e); return fail(FP_ERR_HIP, hipGetErrorString(he)); }
     }
@@ -1513,6 +1538,7 @@ fp_status fp_engine_stats(fp_engine *e, fp_stats *out)
     }
     *out = e->stats;
     out->baq_waves_per_simd = (uint64_t)e->wpe;
+    out->baq_checkpoint_rows = (uint64_t)e->ckpt;
     out->sort_digit_bits = e->opts.aggregation == FP_AGG_SORT ? (uint64_t)e->sort_bits : 0;
     return FP_OK;
 }

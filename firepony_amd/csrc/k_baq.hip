@@ -474,15 +474,23 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
 
     for (uint32_t chunk = gwave; (uint64_t)chunk * 64 < total; chunk += nwaves) {
         const uint32_t idx = chunk * 64 + lane;
-        if (idx >= total) continue;
-        const uint32_t rd = a.list[idx];
+        const bool act = idx < total;
+        const uint32_t rd = act ? a.list[idx] : 0u;
+        const uint32_t rwv = act ? a.s.rwc[rd] : 0u;
+        const int qStart = lo16(rwv);
+        const int qLen = act ? (int)hi16(rwv) - qStart + 1 : 0;
+        // the backward loop runs on the wave's longest window, so all lanes touch the same row at once
+        int qmax = qLen;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const int o = __shfl_xor(qmax, off);
+            qmax = o > qmax ? o : qmax;
+        }
+        if (!act) continue;
 
         // ---- setup (baq.cu:260-354)
         const uint32_t hw = a.s.hmmw[rd];
         const int hx = (int16_t)lo16(hw), hy = (int16_t)hi16(hw);
-        const uint32_t rwv = a.s.rwc[rd];
-        const int qStart = lo16(rwv);
-        const int qLen = (int)hi16(rwv) - qStart + 1;
         const int bw = a.s.bandwidth[rd];
         const uint32_t astart = b.alignment_start[rd];
         const uint32_t chrom = b.chromosome[rd];
@@ -800,7 +808,8 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         }
         cp = qprob(lastq);
         double cyy = 1.0 / Sst[(uint64_t)(qLen >= 2 ? qLen - 1 : qLen) * 64 + lane];   // 1 / s[i]
-        for (int i = qLen - 1; i >= 1; i--) {
+        for (int i = qmax - 1; i >= 1; i--) {
+            if (i >= qLen) continue;            // rows above this lane's window
             const int in_ = i - 1;
             const int nidx = i - 1 - bw;
             uint32_t nqw = 0, nrw = 0, nv = 0;
@@ -868,6 +877,405 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         for (; j < qLen; j++) out[j] = good ? (uint8_t)orow[(uint64_t)j * 64] : (uint8_t)64;
     }
     // one counter update per wave
+    for (int off = 32; off >= 1; off >>= 1) ncell += __shfl_xor(ncell, off);
+    if (lane == 0 && ncell) atomicAdd(a.cells, (unsigned long long)ncell);
+}
+
+// ---------------------------------------------------------------------------
+// Checkpointed band kernel (inputs staged in LDS, one wave per SIMD).
+//
+// The forward pass spills every row's scale factor but the full state (M, I, D) of only every K-th
+// row.  The backward pass walks K-row segments from the top: it recomputes a segment's forward
+// rows from the segment's checkpoint into registers, with the forward pass's own row code (so the
+// values are the same bits), and consumes them in reverse for the MAP step.  Forward-row traffic
+// drops from 32 B per cell (M and I of every row written and read back) to 48/K B; the recompute
+// repeats the forward arithmetic once, on FP64 units the spill-bound form leaves idle.  The
+// segment loop runs on a wave-uniform counter, so all lanes of a wave load the same checkpoint row
+// at once (coalesced) even when their reads' lengths differ.
+template <int W, int K, bool PREFETCH>
+__global__ void __launch_bounds__(256, 1) k_baq_seg(BaqArgs a)
+{
+    constexpr int NWIN = (W + 15) / 16;
+    constexpr int TOP = (W - 1) - 16 * (NWIN - 1);
+    constexpr uint64_t TOPMASK = TOP == 15 ? ~0ull : ((1ull << (4 * (TOP + 1))) - 1);
+
+    __shared__ double s_q2p[256];
+    __shared__ double s_thr[128];
+    for (uint32_t t = threadIdx.x; t < 256; t += blockDim.x) s_q2p[t] = c_qual2prob[t];
+    for (uint32_t t = threadIdx.x; t < 128; t += blockDim.x) s_thr[t] = c_map_thr[t];
+    __syncthreads();
+
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t gwave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t total = *a.count;
+    double *slot = a.scratch + (uint64_t)gwave * a.slot_doubles;
+    // checkpoints of rows 1, 1 + K, 1 + 2K, ...: (M, I) [j][r][lane], then D [j][r][lane]; they fit in
+    // the forward-row area of the every-row kernel, and the scale factors and cap codes keep its offsets
+    const uint32_t nck = (a.rows_cap + K - 1) / K;
+    double2 *Cmi = (double2 *)slot;
+    double *Cd = slot + (uint64_t)nck * W * 2 * 64;
+    double *Sst = slot + (uint64_t)(a.rows_cap + 1) * W * 2 * 64;         // [row][lane]
+    uint64_t *Ost = (uint64_t *)(Sst + (uint64_t)(a.rows_cap + 2) * 64);  // [row][lane] MAP state | quality << 32
+    const BatchDev &b = a.b;
+    uint64_t ncell = 0;
+    extern __shared__ unsigned char baq_stage[];
+    const uint32_t base_bytes = (a.rows_cap + 1) / 2, ref_bytes = (a.rows_cap + 2 * W + 1) / 2;
+    uint8_t *rowq = baq_stage + (threadIdx.x >> 6) * ((a.rows_cap + base_bytes + ref_bytes) * 64);
+    uint8_t *rowb = rowq + (uint64_t)a.rows_cap * 64;
+    uint8_t *refd = rowb + (uint64_t)base_bytes * 64;
+    auto refn = [&](int i) -> uint32_t { return (refd[(i >> 1) * 64 + lane] >> ((i & 1) * 4)) & 15; };
+    auto rowv = [&](int q) -> uint32_t {
+        return rowq[q * 64 + lane] | (((rowb[(q >> 1) * 64 + lane] >> ((q & 1) * 4)) & 15) << 8);
+    };
+    auto qprob = [&](uint8_t q) -> double { return s_q2p[q < 4 ? 4 : q]; };
+    auto pin = [](auto &v) { asm volatile("" : "+v"(v)); };
+    auto mapq = [&](double y) -> uint8_t {
+        if (y == 0.0) return 4;
+        int k = (int)(-4.343 * log(y) + 0.499);
+        k = k < 4 ? 4 : (k > 100 ? 100 : k);
+        const double t0 = s_thr[k], t1 = s_thr[k + 1];
+        if (k >= 5 && !(y <= t0)) {
+            k--;
+            while (k >= 5 && !(y <= s_thr[k])) k--;
+        } else if (y <= t1) {
+            k++;
+            while (k + 1 <= 101 && y <= s_thr[k + 1]) k++;
+        }
+        if (k <= 4) return 4;
+        if (k >= 101) return 99;
+        return (uint8_t)k;
+    };
+    // band reference windows: nibble r = HMM window index idx0 + r
+    auto win_fill = [&](uint64_t (&win)[NWIN], int idx0) {
+#pragma unroll
+        for (int w = 0; w < NWIN; w++) {
+            uint64_t v = 0;
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                if (w * 16 + j < W) v |= (uint64_t)refn(idx0 + w * 16 + j + W) << (4 * j);
+            win[w] = v;
+        }
+    };
+    auto win_shr = [&](uint64_t (&win)[NWIN], uint32_t nb) {     // band moves right: append cell W-1
+#pragma unroll
+        for (int w = 0; w < NWIN; w++) {
+            const uint64_t in = (w + 1 < NWIN) ? (win[w + 1 < NWIN ? w + 1 : w] & 15) : (uint64_t)nb;
+            const int top = (w + 1 < NWIN) ? 15 : TOP;
+            win[w] = (win[w] >> 4) | (in << (4 * top));
+        }
+    };
+    auto win_shl = [&](uint64_t (&win)[NWIN], uint32_t nb) {     // band moves left: insert cell 0
+#pragma unroll
+        for (int w = NWIN - 1; w >= 0; w--) {
+            const uint64_t in = w > 0 ? (win[w > 0 ? w - 1 : 0] >> 60) & 15 : (uint64_t)nb;
+            const uint64_t keep = (w + 1 < NWIN) ? ~0ull : TOPMASK;
+            win[w] = ((win[w] << 4) | in) & keep;
+        }
+    };
+
+    for (uint32_t chunk = gwave; (uint64_t)chunk * 64 < total; chunk += nwaves) {
+        const uint32_t idx = chunk * 64 + lane;
+        const bool act = idx < total;
+        const uint32_t rd = act ? a.list[idx] : 0u;
+        const uint32_t rwv = act ? a.s.rwc[rd] : 0u;
+        const int qStart = lo16(rwv);
+        const int qLen = act ? (int)hi16(rwv) - qStart + 1 : 0;
+        int qmax = qLen;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const int o = __shfl_xor(qmax, off);
+            qmax = o > qmax ? o : qmax;
+        }
+        if (!act) continue;
+
+        // ---- setup (baq.cu:260-354)
+        const uint32_t hw = a.s.hmmw[rd];
+        const int hx = (int16_t)lo16(hw), hy = (int16_t)hi16(hw);
+        const int bw = a.s.bandwidth[rd];
+        const uint32_t astart = b.alignment_start[rd];
+        const uint32_t chrom = b.chromosome[rd];
+        const int refLenW = hy - hx + 1;
+        const uint32_t ahx = hx < 0 ? (uint32_t)(-hx) : (uint32_t)hx;
+        const bool clamp = hx < 0 && astart < ahx;
+        const int refLen = clamp ? refLenW - 1 : refLenW;
+        const int64_t hoff = clamp ? 0 : (int64_t)astart + hx;
+
+        const double GOP = c_gap_open;
+        const double sM = 1.0 / (2 * qLen + 2);
+        const double sI = sM;
+        const double bM = (1 - GOP) / refLenW;
+        const double bI = GOP / refLenW;
+        const double m0 = (1 - GOP - GOP) * (1 - sM);
+        const double m1 = GOP * (1 - sM);
+        const double m2 = m1;
+        const double m3 = (1 - GEP) * (1 - sI);
+        const double m4 = GEP * (1 - sI);
+        const double m6 = 1 - GEP;
+        const double m8 = GEP;
+        const double EIm1 = EI * m1;
+        const double EIm4 = EI * m4;
+
+        const uint64_t qbase = (uint64_t)b.read_start[rd] + qStart;
+        const uint32_t qqual = b.qual_start[rd] + qStart;
+        const uint32_t *cw = a.g.words + a.g.word_offset[chrom];
+        const int64_t clen = a.g.length[chrom];
+        const int64_t wlast = clen > 0 ? ((clen + 7) >> 3) - 1 : 0;
+        auto ref_word = [&](int64_t p) -> uint32_t {
+            int64_t wi = p >> 3;
+            wi = wi < 0 ? 0 : (wi > wlast ? wlast : wi);
+            return cw[wi];
+        };
+        auto nibble = [&](uint32_t w, int idx0) -> uint8_t {
+            const uint32_t ok = (uint32_t)(idx0 >= 0) & (uint32_t)(idx0 < refLen) & (uint32_t)(hoff + idx0 < clen);
+            const uint32_t nb = (w >> (((uint32_t)(hoff + idx0) & 7) * 4)) & 15;
+            return (uint8_t)((nb & (0u - ok)) | (15u & (ok - 1u)));
+        };
+
+        const bool good = cap_walk(b, a.s, rd, [&](uint16_t p, uint32_t kind, uint32_t ep) {
+            Ost[(uint64_t)((int)p - qStart + 1) * 64 + lane] = (uint64_t)ep | ((uint64_t)kind << 32);
+        });
+
+        // this read's inputs into the wave's LDS area (each lane writes and later reads only its own column)
+#pragma unroll 4
+        for (int q = 0; q < qLen; q++) rowq[q * 64 + lane] = b.quals[qqual + q];
+#pragma unroll 4
+        for (int q = 0; q < qLen; q += 2) {
+            const uint32_t w = b.reads[(qbase + q) >> 3];
+            const uint32_t b0 = (w >> (((qbase + q) & 7) * 4)) & 15;
+            const uint64_t i1 = qbase + (q + 1 < qLen ? q + 1 : q);
+            const uint32_t b1 = q + 1 < qLen ? (b.reads[i1 >> 3] >> ((i1 & 7) * 4)) & 15 : 15u;
+            rowb[(q >> 1) * 64 + lane] = (uint8_t)(b0 | (b1 << 4));
+        }
+#pragma unroll 4
+        for (int bi = (W - bw) >> 1; bi <= (qLen + 2 * W - 1 - bw) >> 1; bi++) {
+            const int k0 = 2 * bi - W, k1 = k0 + 1;
+            refd[bi * 64 + lane] = (uint8_t)(nibble(ref_word(hoff + k0), k0) | (nibble(ref_word(hoff + k1), k1) << 4));
+        }
+
+        // forward row i >= 2 from row i - 1 (baq.cu:444-512): unscaled values and their sum.  The
+        // forward pass and the segment recompute both run this code, so they produce the same bits.
+        auto fwd_row = [&](int i, double (&M)[W], double (&I)[W], double (&D)[W], const uint64_t (&win)[NWIN],
+                           uint32_t qb, double p) -> double {
+            const double em_eq = 1 - p, em_ne = p * EM;
+            const int klo = 1 > i - bw ? 1 : i - bw;
+            const int khi = refLen < i + bw ? refLen : i + bw;
+            double sum = 0.0, cM = 0.0, cD = 0.0;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = i - bw + r;
+                const bool in = k >= klo && k <= khi;
+                const double pM = M[r], pI = I[r], pD = D[r];
+                const double nM = (r + 1 < W) ? M[r + 1 < W ? r + 1 : r] : 0.0;
+                const double nI = (r + 1 < W) ? I[r + 1 < W ? r + 1 : r] : 0.0;
+                const uint32_t rb = (uint32_t)((win[r >> 4] >> (4 * (r & 15))) & 15);
+                const double e = (rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne);
+                const double mv = in ? e * (m0 * pM + m3 * pI + m6 * pD) : 0.0;
+                const double iv = in ? EI * (m1 * nM + m4 * nI) : 0.0;
+                const double dv = in ? m2 * cM + m8 * cD : 0.0;
+                sum += in ? mv + iv + dv : 0.0;
+                M[r] = mv; I[r] = iv; D[r] = dv;
+                cM = mv; cD = dv;
+            }
+            return sum;
+        };
+        auto fwd_scale = [&](double (&M)[W], double (&I)[W], double (&D)[W], double sum) {
+            const double inv = 1. / sum;
+#pragma unroll
+            for (int r = 0; r < W; r++) { M[r] = M[r] * inv; I[r] = I[r] * inv; D[r] = D[r] * inv; }
+        };
+        auto store_ckpt = [&](int j, const double (&M)[W], const double (&I)[W], const double (&D)[W]) {
+            double2 *cmi = Cmi + (uint64_t)j * W * 64 + lane;
+            double *cd = Cd + (uint64_t)j * W * 64 + lane;
+#pragma unroll
+            for (int r = 0; r < W; r++) { cmi[r * 64] = make_double2(M[r], I[r]); cd[r * 64] = D[r]; }
+        };
+
+        double M[W], I[W], D[W];
+        uint64_t win[NWIN];
+        // ---- forward row 1 (baq.cu:396-442)
+        {
+            const uint32_t v = rowv(0);
+            const uint8_t qb = (uint8_t)(v >> 8);
+            const double p = qprob((uint8_t)(v & 255));
+            const double em_eq = 1 - p, em_ne = p * EM;
+            const int khi = refLen < bw + 1 ? refLen : bw + 1;
+            ncell += khi >= 1 ? (uint64_t)khi : 0;
+            win_fill(win, -bw);
+            double sum = 0.0;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = 1 - bw + r;
+                const bool in = k >= 1 && k <= khi;
+                const uint8_t rb = (uint8_t)((win[r >> 4] >> (4 * (r & 15))) & 15);
+                const double e = (rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne);
+                const double mv = in ? e * bM : 0.0;
+                const double iv = in ? EI * bI : 0.0;
+                sum += in ? mv + iv : 0.0;
+                M[r] = mv; I[r] = iv; D[r] = 0.0;
+            }
+            Sst[64 + lane] = sum;
+#pragma unroll
+            for (int r = 0; r < W; r++) { M[r] = M[r] / sum; I[r] = I[r] / sum; D[r] = D[r] / sum; }
+            store_ckpt(0, M, I, D);
+        }
+        // ---- forward rows 2..qLen: next row's LDS inputs read a row ahead
+        const int qlast = qLen - 1;
+        const int q1 = 1 < qlast ? 1 : qlast;
+        uint32_t cv = rowv(q1);
+        uint32_t crb = refn(2 - bw - 1 + (W - 1) + W);
+        for (int i = 2; i <= qLen; i++) {
+            const int in_ = i < qlast ? i : qlast;
+            const uint32_t nv = rowv(in_);
+            const uint32_t nrw = refn(i + 1 - bw - 1 + (W - 1) + W);
+            win_shr(win, crb);
+            const int klo = 1 > i - bw ? 1 : i - bw;
+            const int khi = refLen < i + bw ? refLen : i + bw;
+            if (khi >= klo) ncell += (uint64_t)(khi - klo + 1);
+            const double sum = fwd_row(i, M, I, D, win, cv >> 8, qprob((uint8_t)(cv & 255)));
+            cv = nv;
+            crb = nrw;
+            pin(cv); pin(crb);
+            Sst[(uint64_t)i * 64 + lane] = sum;
+            fwd_scale(M, I, D, sum);
+            if ((i - 1) % K == 0) store_ckpt((i - 1) / K, M, I, D);
+        }
+        // ---- s[qLen+1] (baq.cu:514-525)
+        double sq1 = 0.0;
+        {
+            const int klo = 1 > qLen - bw ? 1 : qLen - bw;
+            const int khi = refLen < qLen + bw ? refLen : qLen + bw;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = qLen - bw + r;
+                const double t = M[r] * sM + I[r] * sI;
+                sq1 += (k >= klo && k <= khi) ? t : 0.0;
+            }
+        }
+
+        // ---- backward + MAP (baq.cu:528-681), K rows at a time from the top
+        auto load_ckpt = [&](int j, double (&M)[W], double (&I)[W], double (&D)[W]) {
+            const double2 *cmi = Cmi + (uint64_t)j * W * 64 + lane;
+            const double *cd = Cd + (uint64_t)j * W * 64 + lane;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const double2 f = cmi[r * 64];
+                M[r] = f.x; I[r] = f.y; D[r] = cd[r * 64];
+            }
+        };
+        const int nseg = (qLen - 1) / K + 1;
+        const int nseg_w = (qmax - 1) / K + 1;
+        // backward row qLen (baq.cu:528-545), before the segment walk reaches it
+        double bMv[W], bIv[W];
+        uint64_t bwin[NWIN];
+        {
+            const double sq = Sst[(uint64_t)qLen * 64 + lane];
+            const int klo = 1 > qLen - bw ? 1 : qLen - bw;
+            const int khi = refLen < qLen + bw ? refLen : qLen + bw;
+            const double vM = sM / sq / sq1, vI = sI / sq / sq1;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = qLen - bw + r;
+                const bool in = k >= klo && k <= khi;
+                bMv[r] = in ? vM : 0.0; bIv[r] = in ? vI : 0.0;
+            }
+            win_fill(bwin, qLen - bw);
+        }
+        load_ckpt(nseg - 1, M, I, D);          // the forward state is dead: its registers hold the recompute
+        for (int j = nseg_w - 1; j >= 0; j--) {
+            const int s0 = 1 + j * K;
+            if (s0 > qLen) continue;
+            double2 F[K][W];
+#pragma unroll
+            for (int r = 0; r < W; r++) F[0][r] = make_double2(M[r], I[r]);
+            uint64_t fwin[NWIN];
+            win_fill(fwin, s0 - bw);           // forward row s0 + 1: cell r at window index s0 - bw + r
+#pragma unroll
+            for (int t = 1; t < K; t++) {
+                const int i = s0 + t;
+                if (i <= qLen) {
+                    if (t > 1) win_shr(fwin, refn(i - bw + W - 2 + W));
+                    const uint32_t v = rowv(i - 1);
+                    const double sum = fwd_row(i, M, I, D, fwin, v >> 8, qprob((uint8_t)(v & 255)));
+                    fwd_scale(M, I, D, sum);
+#pragma unroll
+                    for (int r = 0; r < W; r++) F[t][r] = make_double2(M[r], I[r]);
+                }
+            }
+            double Sv[K];
+            uint64_t Cv[K];
+#pragma unroll
+            for (int t = 0; t < K; t++) {
+                const int i = s0 + t <= qLen ? s0 + t : qLen;
+                Sv[t] = Sst[(uint64_t)i * 64 + lane];
+                Cv[t] = Ost[(uint64_t)i * 64 + lane];
+            }
+            if (PREFETCH && j > 0) load_ckpt(j - 1, M, I, D);   // arrives while this segment's backward rows run
+#pragma unroll
+            for (int t = K - 1; t >= 0; t--) {
+                const int i = s0 + t;
+                if (i > qLen) continue;
+                if (i < qLen) {
+                    win_shl(bwin, refn(i - bw + W));
+                    const uint32_t v = rowv(i);
+                    const uint8_t qb = (uint8_t)(v >> 8);
+                    const double p = qprob((uint8_t)(v & 255));
+                    const double y = (i > 1) ? 1. : 0.;
+                    const double em_eq = 1 - p, em_ne = p * EM;
+                    const int klo = 1 > i - bw ? 1 : i - bw;
+                    const int khi = refLen < i + bw ? refLen : i + bw;
+                    double cD = 0.0;
+#pragma unroll
+                    for (int r = W - 1; r >= 0; r--) {
+                        const int k = i - bw + r;
+                        const bool in = k >= klo && k <= khi;
+                        const double pM = bMv[r];
+                        const double pIl = r >= 1 ? bIv[r >= 1 ? r - 1 : 0] : 0.0;
+                        const uint8_t rb = (uint8_t)((bwin[r >> 4] >> (4 * (r & 15))) & 15);
+                        const double e0 = ((rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne)) * pM;
+                        const double e = k >= refLen ? 0.0 : e0;
+                        const double mv = in ? e * m0 + EIm1 * pIl + m2 * cD : 0.0;
+                        const double iv = in ? e * m3 + EIm4 * pIl : 0.0;
+                        const double dv = in ? (e * m6 + m8 * cD) * y : 0.0;
+                        bMv[r] = mv; bIv[r] = iv;
+                        cD = dv;
+                    }
+                    const double yy = 1.0 / Sv[t];
+#pragma unroll
+                    for (int r = 0; r < W; r++) { bMv[r] = bMv[r] * yy; bIv[r] = bIv[r] * yy; }
+                }
+                // MAP step of row i (quality of query position i - 1)
+                double sum = 0.0, mx = 0.0;
+                int32_t stt = -1;
+#pragma unroll
+                for (int r = 0; r < W; r++) {
+                    const int k = i - bw + r;
+                    const double2 f = F[t][r];
+                    double z = f.x * bMv[r];
+                    sum += z;
+                    if (z > mx) { mx = z; stt = (k - 1) << 2 | 0; }
+                    z = f.y * bIv[r];
+                    sum += z;
+                    if (z > mx) { mx = z; stt = (k - 1) << 2 | 1; }
+                }
+                mx /= sum;
+                const uint64_t code = Cv[t];
+                const uint8_t oq = (uint8_t)(rowv(i - 1) & 255);
+                Ost[(uint64_t)i * 64 + lane] = cap_tag((uint32_t)(code >> 32), (uint32_t)code, oq, mapq(1.0 - mx), (uint32_t)stt);
+            }
+        }
+        uint8_t *out = a.s.baq + b.qual_start[rd] + qStart;
+        const uint64_t *orow = Ost + 64 + lane;
+        int j = 0;
+        for (; j + 8 <= qLen; j += 8) {
+            uint64_t t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) t[u] = orow[(uint64_t)(j + u) * 64];
+#pragma unroll
+            for (int u = 0; u < 8; u++) out[j + u] = good ? (uint8_t)t[u] : (uint8_t)64;
+        }
+        for (; j < qLen; j++) out[j] = good ? (uint8_t)orow[(uint64_t)j * 64] : (uint8_t)64;
+    }
     for (int off = 32; off >= 1; off >>= 1) ncell += __shfl_xor(ncell, off);
     if (lane == 0 && ncell) atomicAdd(a.cells, (unsigned long long)ncell);
 }
@@ -1080,6 +1488,7 @@ extern "C" hipError_t fpd_baq_prepare()
     (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<15, 1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<23, 1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<33, 1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_seg<15, 2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     return hipSuccess;
 }
 
@@ -1096,7 +1505,8 @@ extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const 
                                          const fpd::ReadStateDev *s, const uint32_t *list,
                                          const uint32_t *count, double *scratch, uint64_t slot_doubles,
                                          uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves,
-                                         unsigned long long *cells, int wpe, int allow_stage, hipStream_t st)
+                                         unsigned long long *cells, int wpe, int allow_stage, int recompute,
+                                         hipStream_t st)
 {
     fpd::BaqArgs a{};
     a.b = *b; a.g = *g; a.s = *s; a.list = list; a.count = count; a.scratch = scratch;
@@ -1108,6 +1518,7 @@ extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const 
     const size_t lds = stage ? baq_stage_bytes(cls == 0 ? 15 : (cls == 1 ? 23 : 33), rows_cap) : 0;
     switch (cls) {
     case 0:
+        if (stage && recompute) { hipLaunchKernelGGL((fpd::k_baq_seg<15, 2, true>), g4, b4, lds, st, a); break; }
         if (stage) { hipLaunchKernelGGL((fpd::k_baq_band<15, 1, true>), g4, b4, lds, st, a); break; }
         switch (wpe) {
         case 1: hipLaunchKernelGGL((fpd::k_baq_band<15, 1, false>), g4, b4, 0, st, a); break;

@@ -779,6 +779,75 @@ __global__ void __launch_bounds__(256) k_rebase(uint32_t n, uint32_t *cigar_star
 }
 } // namespace fpd
 
+namespace fpd {
+// append of a device-resident batch in one launch: blockIdx.y picks one of the 18 batch arrays, whose
+// bytes the blocks along x copy; the three start-offset arrays and the read groups are rebased on the
+// way (as k_rebase does), everything else is copied in 16-, 4- or 1-byte units as alignment allows
+struct AppendArgs {
+    const uint8_t *src[18];
+    uint8_t *dst[18];
+    uint64_t bytes[18];
+    uint32_t cig_off, base_off, qual_off, rg_off, rg_limit;
+};
+
+__global__ void __launch_bounds__(256) k_append(AppendArgs a)
+{
+    const int i = blockIdx.y;
+    const uint64_t nb = a.bytes[i];
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint8_t *s = a.src[i];
+    uint8_t *d = a.dst[i];
+    if (i == 7 || i == 10 || i == 13 || i == 17) {
+        const uint32_t *s4 = (const uint32_t *)s;
+        uint32_t *d4 = (uint32_t *)d;
+        const bool remap = a.rg_off || a.rg_limit != 0xffffffffu;
+        for (uint64_t r = tid; r < nb / 4; r += stride) {
+            uint32_t v = s4[r];
+            if (i == 7) v += a.cig_off;
+            else if (i == 10) v += a.base_off;
+            else if (i == 13) v += a.qual_off;
+            else if (remap) v = v < a.rg_limit ? v + a.rg_off : 0xffffffffu;
+            d4[r] = v;
+        }
+        return;
+    }
+    const uintptr_t al = (uintptr_t)s | (uintptr_t)d;
+    uint64_t done = 0;
+    if ((al & 15) == 0) {
+        const uint4 *s16 = (const uint4 *)s;
+        uint4 *d16 = (uint4 *)d;
+        for (uint64_t r = tid; r < nb / 16; r += stride) d16[r] = s16[r];
+        done = nb / 16 * 16;
+    } else if ((al & 3) == 0) {
+        const uint32_t *s4 = (const uint32_t *)s;
+        uint32_t *d4 = (uint32_t *)d;
+        for (uint64_t r = tid; r < nb / 4; r += stride) d4[r] = s4[r];
+        done = nb / 4 * 4;
+    }
+    for (uint64_t r = done + tid; r < nb; r += stride) d[r] = s[r];
+}
+} // namespace fpd
+
+extern "C" hipError_t fpd_append(const void *const *src, void *const *dst, const uint64_t *bytes, uint32_t cig_off,
+                                 uint32_t base_off, uint32_t qual_off, uint32_t rg_off, uint32_t rg_limit,
+                                 hipStream_t st)
+{
+    fpd::AppendArgs a{};
+    uint64_t most = 0;
+    for (int i = 0; i < 18; i++) {
+        a.src[i] = (const uint8_t *)src[i];
+        a.dst[i] = (uint8_t *)dst[i];
+        a.bytes[i] = src[i] && dst[i] ? bytes[i] : 0;
+        most = std::max(most, a.bytes[i]);
+    }
+    if (most == 0) return hipSuccess;
+    a.cig_off = cig_off; a.base_off = base_off; a.qual_off = qual_off; a.rg_off = rg_off; a.rg_limit = rg_limit;
+    const uint32_t gx = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, most / 16384));
+    hipLaunchKernelGGL(fpd::k_append, dim3(gx, 18), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t fpd_rebase(uint32_t n, uint32_t *cigar_start, uint32_t *read_start, uint32_t *qual_start,
                                  uint32_t *read_group, uint32_t cig_off, uint32_t base_off, uint32_t qual_off,
                                  uint32_t rg_off, uint32_t rg_limit, hipStream_t st)

@@ -88,7 +88,13 @@ typedef struct fp_options {
                                         from HBM on every row instead of staging them
                                         in LDS once per read (0 = stage when the batch's
                                         longest read fits the LDS budget)              */
-    uint32_t reserved[2];
+    uint32_t baq_checkpoint_rows;    /* 15-cell BAQ band kernel: 1 = every forward row's
+                                        state spilled to HBM for the backward pass; 2 =
+                                        every second row's, the rows between recomputed
+                                        (same results, a quarter less forward-row
+                                        traffic, a second forward pass); 0 = 1.  Used
+                                        with LDS staging at one wave per SIMD.          */
+    uint32_t reserved[1];
 } fp_options;
 
 /* how covariate observations reach the resident tables (identical results) */
@@ -223,6 +229,7 @@ typedef struct fp_stats {
     uint64_t sort_slots;         /* FP_AGG_SORT, profile_stages or plain engine: pair    */
     uint64_t sort_pairs;         /*   slots emitted, real pairs sorted, and radix       */
     uint64_t sort_passes;        /*   passes run (summed over tables and batches)       */
+    uint64_t baq_checkpoint_rows; /* rows per forward checkpoint of the 15-cell BAQ kernel */
 } fp_stats;
 
 /* Per-batch intermediates of the last processed batch, for stage-by-stage

@@ -36,13 +36,14 @@ def mixed_workload(n=20000, seed=21):
                                 contigs=(400000, 300000), sites=800, clip=1.0, indel=0.05)
 
 
-@pytest.mark.parametrize("wpe", [1, 0])
-def test_batched_equals_plain_with_replays(tmp_path, wpe):
-    say(f"batched vs plain: mixed lengths, staging buffers of 4096 reads, BAQ waves/SIMD {wpe or 'autotuned'}")
+@pytest.mark.parametrize("wpe,ckpt", [(1, 1), (0, 0), (1, 2)])
+def test_batched_equals_plain_with_replays(tmp_path, wpe, ckpt):
+    say(f"batched vs plain: mixed lengths, staging buffers of 4096 reads, BAQ waves/SIMD {wpe or 'autotuned'}, "
+        f"forward checkpoint rows {ckpt or 'autotuned'}")
     g, rs, n = mixed_workload()
     plain = Engine(g, frac_error_f64=1)
     bat = Engine(g, frac_error_f64=1, engine_kind=abi.FP_ENGINE_BATCHED, max_batch_reads=4096,
-                 max_read_length=250, baq_waves_per_simd=wpe)
+                 max_read_length=250, baq_waves_per_simd=wpe, baq_checkpoint_rows=ckpt)
     step = 1500
     for first in range(0, n, step):
         b = synth.Batch(g, rs, first, min(step, n - first))

@@ -117,28 +117,32 @@ def run_parity(tmp_path, genome, rs, n, batch_size, per_batch=True, **engine_opt
     e.close()
 
 
-@pytest.mark.parametrize("wpe,nostage", [(1, 0), (1, 1), (2, 0), (3, 0), (4, 0)])
-def test_config0_slice(tmp_path, wpe, nostage):
-    say(f"parity: config 0 first 4000 reads, BAQ register budget for {wpe} waves/SIMD, LDS staging {1 - nostage}")
+@pytest.mark.parametrize("wpe,nostage,ckpt", [(1, 0, 1), (1, 1, 1), (2, 0, 1), (3, 0, 1), (4, 0, 1), (1, 0, 2)])
+def test_config0_slice(tmp_path, wpe, nostage, ckpt):
+    say(f"parity: config 0 first 4000 reads, BAQ register budget for {wpe} waves/SIMD, LDS staging {1 - nostage}, "
+        f"forward checkpoint every {ckpt} rows")
     gs, rs, n = synth.named_spec(0)
     g = synth.Genome(gs, rs.num_read_groups)
-    run_parity(tmp_path, g, rs, 4000, 1000, baq_waves_per_simd=wpe, disable_baq_staging=nostage)
+    run_parity(tmp_path, g, rs, 4000, 1000, baq_waves_per_simd=wpe, disable_baq_staging=nostage,
+               baq_checkpoint_rows=ckpt)
 
 
-def test_edge_cases(tmp_path):
-    say("parity: edge-case menu, paired, N runs, multi-base sites")
+@pytest.mark.parametrize("ckpt", [1, 2])
+def test_edge_cases(tmp_path, ckpt):
+    say(f"parity: edge-case menu, paired, N runs, multi-base sites, forward checkpoint every {ckpt} rows")
     g, rs, n = synth.small_workload(seed=7, num_reads=3000, edge=0.35, lengths=(36, 100, 151), read_groups=3,
                                     contigs=(60000, 40000), sites=300, clip=0.3, indel=0.2, paired=1,
                                     n_runs=0.01, multi_sites=0.3)
-    run_parity(tmp_path, g, rs, n, 1000)
+    run_parity(tmp_path, g, rs, n, 1000, baq_checkpoint_rows=ckpt)
 
 
-@pytest.mark.parametrize("nostage", [0, 1])
-def test_mixed_lengths_f64(tmp_path, nostage):
-    say(f"parity: mixed lengths, large indels, fp64 fractional errors, LDS staging {1 - nostage}")
+@pytest.mark.parametrize("nostage,ckpt", [(0, 1), (1, 1), (0, 2)])
+def test_mixed_lengths_f64(tmp_path, nostage, ckpt):
+    say(f"parity: mixed lengths, large indels, fp64 fractional errors, LDS staging {1 - nostage}, "
+        f"forward checkpoint every {ckpt} rows")
     g, rs, n = synth.small_workload(seed=11, num_reads=3000, lengths=(50, 76, 101, 151, 250), read_groups=8,
                                     contigs=(300000,), sites=600, clip=1.0, indel=0.3)
-    run_parity(tmp_path, g, rs, n, 1500, frac_error_f64=1, disable_baq_staging=nostage)
+    run_parity(tmp_path, g, rs, n, 1500, frac_error_f64=1, disable_baq_staging=nostage, baq_checkpoint_rows=ckpt)
 
 
 def test_long_reads(tmp_path):
