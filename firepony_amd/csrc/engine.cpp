@@ -248,7 +248,6 @@ struct fp_engine {
         uint32_t *live_host = nullptr;  // pinned
         uint32_t n = 0;
         uint64_t ncig = 0, nbases = 0, nquals = 0;
-        hipEvent_t filled = nullptr, consumed = nullptr;
         hipGraphExec_t gfront = nullptr, gbaq = nullptr, gcov = nullptr;
         int g_wpe = 0, g_ckpt = 0, g_bits = 0;
         uint64_t g_epoch = 0;
@@ -258,10 +257,14 @@ struct fp_engine {
         hipEvent_t cov_done = nullptr;  // after its covariate stage (sort buffers are shared)
         hipEvent_t t0 = nullptr, t1 = nullptr;   // around its stage 2 (timing)
         bool back_pending = false;      // stage-2 time not yet collected
-    } stg[2];
+    };
+    // staging buffers: one fills and runs its prep and covariates while the other's BAQ runs (a third
+    // buffer lets more kernels overlap but measured no faster: the BAQ kernels hold the HBM
+    // bandwidth, so kernels running beside them slow them down)
+    static constexpr int NSTAGE = 2;
+    Stage stg[NSTAGE];
     bool batched = false;
     int cur = 0;
-    hipStream_t copy_stream = nullptr;
     uint32_t cap_reads = 0, rows_cap = 0;
     uint64_t cap_cig = 0, cap_bases = 0, cap_quals = 0;
     DevBuf bcounters;                   // prep / BAQ counters accumulated over staged batches
@@ -666,16 +669,14 @@ fp_status batched_alloc(fp_engine *e)
         for (int i = 0; i < 18; i++) HIP_TRY(s.arr[i].ensure(sz[i] + 64));
         HIP_TRY(s.live.ensure(16));
         if (!s.live_host) HIP_TRY(hipHostMalloc((void **)&s.live_host, 16, hipHostMallocDefault));
-        if (!s.filled) HIP_TRY(hipEventCreateWithFlags(&s.filled, hipEventDisableTiming));
-        if (!s.consumed) HIP_TRY(hipEventCreateWithFlags(&s.consumed, hipEventDisableTiming));
         if (!s.baq_done) HIP_TRY(hipEventCreateWithFlags(&s.baq_done, hipEventDisableTiming));
         if (!s.cov_done) HIP_TRY(hipEventCreateWithFlags(&s.cov_done, hipEventDisableTiming));
         if (!s.t0) HIP_TRY(hipEventCreate(&s.t0));
         if (!s.t1) HIP_TRY(hipEventCreate(&s.t1));
-        // The first buffer runs on the engine stream and the second on a stream of its own: a
-        // process has few hardware queues (streams beyond them share one and run in turn), so
-        // the engine adds one compute stream, not two.  profile_stages keeps every kernel on the
-        // engine stream (clean per-kernel times).
+        // The first buffer runs on the engine stream and the others on streams of their own, which
+        // also carry their appends (so a buffer refills only once its last batch is consumed).  A
+        // process has four hardware queues; streams beyond them share one and run in turn.
+        // profile_stages keeps every kernel on the engine stream (clean per-kernel times).
         if (!s.stream) {
             if (&s == &e->stg[0] || e->opts.profile_stages) s.stream = e->stream;
             else HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
@@ -753,7 +754,7 @@ void tune_record(fp_engine *e, int kind, int cand, double per_read)
     }
 }
 
-// finish a staging buffer's turn: its batch is consumed and the other buffer fills next
+// finish a staging buffer's turn: the next buffer fills
 static void stage_done(fp_engine *e, fp_engine::Stage &s)
 {
     e->last_num_reads = s.n;
@@ -762,19 +763,20 @@ static void stage_done(fp_engine *e, fp_engine::Stage &s)
     e->last = &s.bs;
     s.n = 0;
     s.ncig = s.nbases = s.nquals = 0;
-    e->cur ^= 1;
+    e->cur = (e->cur + 1) % fp_engine::NSTAGE;
     e->postprocessed = false;
 }
 
-// Run the current staging buffer (if it holds reads) and switch to the other one.  Each buffer has
-// its own compute stream and per-batch state, so one buffer's covariate stage and the other's prep
-// and BAQ run at the same time.  The BAQ kernels of the two buffers share the band scratch and take
-// turns (event baq_done); so do the covariate stages of the sort aggregation (its pair buffers are
-// shared, event cov_done).  The direct tables take both buffers' atomic adds in any order.
+// Run the current staging buffer (if it holds reads) and switch to the next one.  Each buffer has
+// its own compute stream and per-batch state, so one buffer's covariate stage and prep run at the
+// same time as the other's BAQ.  The BAQ kernels of all buffers share the band
+// scratch and take turns (each waits for the previous buffer's event baq_done); so do the covariate
+// stages of the sort aggregation (its pair buffers are shared, event cov_done).  The direct tables
+// take every buffer's atomic adds in any order.
 fp_status batched_flush(fp_engine *e)
 {
     fp_engine::Stage &s = e->stg[e->cur];
-    fp_engine::Stage &o = e->stg[e->cur ^ 1];
+    fp_engine::Stage &o = e->stg[(e->cur + fp_engine::NSTAGE - 1) % fp_engine::NSTAGE];
     if (s.n == 0) return FP_OK;
     fp_status rc = batched_alloc(e);
     if (rc != FP_OK) return rc;
@@ -786,9 +788,7 @@ fp_status batched_flush(fp_engine *e)
     s.live_host[1] = (uint32_t)s.nquals;
     s.live_host[2] = (uint32_t)(s.nquals >> 32);
     s.live_host[3] = 0;
-    HIP_TRY(hipMemcpyAsync(s.live.p, s.live_host, 16, hipMemcpyHostToDevice, e->copy_stream));
-    HIP_TRY(hipEventRecord(s.filled, e->copy_stream));
-    HIP_TRY(hipStreamWaitEvent(st, s.filled, 0));
+    HIP_TRY(hipMemcpyAsync(s.live.p, s.live_host, 16, hipMemcpyHostToDevice, st));
 
     const fpd::BatchDev b = stage_dev(e, s);
     const Plan p{e->cap_reads, e->cap_quals, s.live.as<uint32_t>(), e->rows_cap};
@@ -859,11 +859,12 @@ fp_status batched_flush(fp_engine *e)
         e->stats.cov_launches += 1;
         e->stats.baq_ms += elapsed(e->ev[8], e->ev[2]);
         e->stats.covariates_ms += elapsed(e->ev[2], e->ev[3]);
-        HIP_TRY(hipEventRecord(s.consumed, st));
         stage_done(e, s);
         return FP_OK;
     }
-    if (cand >= 0) HIP_TRY(hipStreamSynchronize(o.stream));   // the candidate runs alone
+    if (cand >= 0)   // the candidate runs alone
+        for (auto &x : e->stg)
+            if (&x != &s && x.stream) HIP_TRY(hipStreamSynchronize(x.stream));
     // the band scratch is free once the other buffer's BAQ kernels are done
     HIP_TRY(hipStreamWaitEvent(st, o.baq_done, 0));
     HIP_TRY(hipEventRecord(s.t0, st));
@@ -901,7 +902,6 @@ fp_status batched_flush(fp_engine *e)
         e->stats.baq_ms += ms;
         tune_record(e, kind, cand, ms / std::max<uint32_t>(s.n, 1));
     }
-    HIP_TRY(hipEventRecord(s.consumed, st));
     stage_done(e, s);
     return FP_OK;
 }
@@ -911,7 +911,6 @@ fp_status batched_sync(fp_engine *e)
 {
     fp_status rc = batched_flush(e);
     if (rc != FP_OK) return rc;
-    HIP_TRY(hipStreamSynchronize(e->copy_stream));
     for (auto &s : e->stg)
         if (s.stream) HIP_TRY(hipStreamSynchronize(s.stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -964,8 +963,8 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
     }
     fp_status rc = batched_alloc(e);
     if (rc != FP_OK) return rc;
-    hipStream_t cs = e->copy_stream;
-    if (s->n == 0) HIP_TRY(hipStreamWaitEvent(cs, s->consumed, 0));   // the buffer's last batch is done
+    // the buffer's own stream: its appends follow its last batch's kernels
+    hipStream_t cs = e->opts.profile_stages ? e->stream : s->stream;
     const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const void *src[18] = {v->chromosome, v->alignment_start, v->alignment_stop, v->mate_chromosome,
                            v->mate_alignment_start, v->inferred_insert_size, v->cigars, v->cigar_start,
@@ -1263,8 +1262,6 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
         e->cap_quals = (uint64_t)e->cap_reads * e->rows_cap;
         e->cap_cig = (uint64_t)e->cap_reads * 8;
         e->tuned = o.baq_waves_per_simd != 0 || o.baq_checkpoint_rows != 0;
-        he = hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking);
-        if (he != hipSuccess) { fp_engine_destroy(e); return fail(FP_ERR_HIP, hipGetErrorString(he)); }
     }
     compute_constants(e->consts);
     he = fpd_baq_set_constants(e->consts.qual2prob, e->consts.map_thr, e->consts.gap_open);
@@ -1664,6 +1661,8 @@ void fp_engine_destroy(fp_engine *e)
     for (auto &s : e->stg)
         if (s.stream) (void)hipStreamSynchronize(s.stream);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto &s : e->stg)
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
     DevBuf *bufs[] = {&e->g_word_offset, &e->g_length, &e->g_words, &e->g_foff, &e->g_fstart, &e->g_fstop,
                       &e->g_mel, &e->counters, &e->scratch, &e->tk, &e->tv, &e->shist, &e->stot, &e->sctr};
     for (DevBuf *b : bufs) b->release();
@@ -1675,8 +1674,6 @@ void fp_engine_destroy(fp_engine *e)
         for (auto &b : s.arr) b.release();
         s.live.release();
         if (s.live_host) (void)hipHostFree(s.live_host);
-        if (s.filled) (void)hipEventDestroy(s.filled);
-        if (s.consumed) (void)hipEventDestroy(s.consumed);
         if (s.gfront) (void)hipGraphExecDestroy(s.gfront);
         if (s.gbaq) (void)hipGraphExecDestroy(s.gbaq);
         if (s.gcov) (void)hipGraphExecDestroy(s.gcov);
@@ -1686,7 +1683,6 @@ void fp_engine_destroy(fp_engine *e)
         s.bs.release();
     }
     for (auto &ev : e->bev) if (ev) (void)hipEventDestroy(ev);
-    if (e->copy_stream) { (void)hipStreamSynchronize(e->copy_stream); (void)hipStreamDestroy(e->copy_stream); }
     if (e->host_counts) (void)hipHostFree(e->host_counts);
     for (auto &b : e->in) b.release();
     for (auto &t : e->tab) for (auto &b : t) b.release();

@@ -813,19 +813,33 @@ __global__ void __launch_bounds__(256) k_append(AppendArgs a)
         return;
     }
     const uintptr_t al = (uintptr_t)s | (uintptr_t)d;
-    uint64_t done = 0;
     if ((al & 15) == 0) {
         const uint4 *s16 = (const uint4 *)s;
         uint4 *d16 = (uint4 *)d;
         for (uint64_t r = tid; r < nb / 16; r += stride) d16[r] = s16[r];
-        done = nb / 16 * 16;
-    } else if ((al & 3) == 0) {
+        for (uint64_t r = nb / 16 * 16 + tid; r < nb; r += stride) d[r] = s[r];
+        return;
+    }
+    if ((al & 3) == 0) {
         const uint32_t *s4 = (const uint32_t *)s;
         uint32_t *d4 = (uint32_t *)d;
         for (uint64_t r = tid; r < nb / 4; r += stride) d4[r] = s4[r];
-        done = nb / 4 * 4;
+        for (uint64_t r = nb / 4 * 4 + tid; r < nb; r += stride) d[r] = s[r];
+        return;
     }
-    for (uint64_t r = done + tid; r < nb; r += stride) d[r] = s[r];
+    // source and destination aligned differently (qualities appended at any byte offset): whole
+    // destination words built from four source bytes, the unaligned head and tail byte by byte
+    const uint64_t lead = (4 - ((uintptr_t)d & 3)) & 3;
+    const uint64_t head = lead < nb ? lead : nb;
+    for (uint64_t r = tid; r < head; r += stride) d[r] = s[r];
+    const uint8_t *sw = s + head;
+    uint32_t *dw = (uint32_t *)(d + head);
+    const uint64_t words = (nb - head) / 4;
+    for (uint64_t r = tid; r < words; r += stride) {
+        const uint8_t *p = sw + 4 * r;
+        dw[r] = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+    }
+    for (uint64_t r = head + 4 * words + tid; r < nb; r += stride) d[r] = s[r];
 }
 } // namespace fpd
 
