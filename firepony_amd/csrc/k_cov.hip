@@ -180,6 +180,17 @@ __device__ inline void lds_add_n(const LdsAcc &L, uint32_t e, uint32_t n, const 
 
 __device__ inline void lds_add(const LdsAcc &L, uint32_t e, const Limbs &v) { lds_add_n(L, e, 1, v); }
 
+// the sums of an entry whose count is kept by its partner (an I/D key pair counts every base in both;
+// the D entry's count is read from the I entry's when the tile is flushed)
+__device__ inline void lds_add_sums(const LdsAcc &L, uint32_t e, const Limbs &v)
+{
+    if (nonzero(v)) {
+        atomicAdd(&L.p0[e], (unsigned long long)v.p0);
+        atomicAdd(&L.p1[e], (unsigned long long)v.p1);
+        atomicAdd(&L.p2[e], (unsigned long long)v.p2);
+    }
+}
+
 __device__ inline void global_add(const TablesDev &t, int tab, uint32_t key, uint64_t count, const Limbs &v)
 {
     atomicAdd((unsigned long long *)&t.count[tab][key], (unsigned long long)count);
@@ -215,7 +226,7 @@ __device__ inline void emit_base(const CovArgs &a, const LdsAcc &L, uint32_t rg,
     else global_add(a.t, 0, 0u | (q << 2) | (rg << 10), 1, f0);
     if (id_in_lds) {   // the wave path reduces these across lanes itself
         lds_add(L, 64, f1);
-        lds_add(L, 65, f2);
+        lds_add_sums(L, 65, f2);
     }
     // cycle: EventTracker(2) | Cycle(10) | QualityScore(8) | ReadGroup(8)
     uint32_t cm, cid;
@@ -235,7 +246,7 @@ __device__ inline void emit_base(const CovArgs &a, const LdsAcc &L, uint32_t rg,
     if (cid != 1023u) {
         if (cid < ncodes) {
             lds_add(L, E_YID + cid, f1);
-            lds_add(L, E_YID + ncodes + cid, f2);
+            lds_add_sums(L, E_YID + ncodes + cid, f2);
         } else {
             global_add(a.t, 1, 1u | (cid << 2) | (45u << 12) | (rg << 20), 1, f1);
             global_add(a.t, 1, 2u | (cid << 2) | (45u << 12) | (rg << 20), 1, f2);
@@ -250,7 +261,7 @@ __device__ inline void emit_base(const CovArgs &a, const LdsAcc &L, uint32_t rg,
     }
     if (xid != 2047u) {
         lds_add(L, E_CID + (xid >> 4), f1);
-        lds_add(L, E_CID + 64 + (xid >> 4), f2);
+        lds_add_sums(L, E_CID + 64 + (xid >> 4), f2);
     }
 }
 
@@ -449,8 +460,8 @@ __device__ inline uint64_t wave_sum(uint64_t v)
     return v;
 }
 
-// one read on one wavefront (clipped window of at most 256 bases); r is wave-uniform
-template <bool PAIRS>
+// one read on one wavefront (clipped window of at most 64 * NC bases); r is wave-uniform
+template <bool PAIRS, int NC>
 __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegion &pr, uint8_t *stage, uint32_t r,
                               uint32_t rg, uint32_t ncodes)
 {
@@ -468,10 +479,11 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
         for (uint32_t k = lane; k < L_read; k += 64)
             if (k < rwx || k > rwy) { double *z = a.s.frac + 3ull * (qs + k); z[0] = 0.0; z[1] = 0.0; z[2] = 0.0; }
 
-    uint8_t tag[4], fl[4], qv[4];
-    uint64_t IB[4], ES[4], EI_[4], ED[4], LQ[4];
+    uint8_t tag[NC], fl[NC], qv[NC];
+    uint64_t IB[4] = {0, 0, 0, 0}, ES[4] = {0, 0, 0, 0}, EI_[4] = {0, 0, 0, 0}, ED[4] = {0, 0, 0, 0},
+             LQ[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < NC; c++) {
         const int j = lane + 64 * c;
         const bool v = j < len;
         const uint32_t p = rwx + j;
@@ -562,7 +574,7 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
     if constexpr (PAIRS) {
         // every lane stays in the loop: the appends are wavefront collectives
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < NC; c++) {
             if (64 * c >= len) break;
             const int j = lane + 64 * c;
             double vs = 0.0, vi = 0.0, vd = 0.0;
@@ -579,7 +591,7 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
     uint32_t nid = 0;
     uint64_t i0 = 0, i1 = 0, i2 = 0, d0 = 0, d1 = 0, d2 = 0;
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < NC; c++) {
         const int j = lane + 64 * c;
         if (j >= len) continue;
         double vs, vi, vd;
@@ -599,7 +611,7 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
     const Limbs td = {wave_sum(d0), wave_sum(d1), wave_sum(d2)};
     if (lane == 0 && nid) {
         lds_add_n(L, 64, nid, ti);
-        lds_add_n(L, 65, nid, td);
+        lds_add_sums(L, 65, td);
     }
     // the stage is reused by the wave's next read
     __builtin_amdgcn_wave_barrier();
@@ -676,7 +688,8 @@ __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
         const uint32_t r = __builtin_amdgcn_readfirstlane(a.rg_list[li]);
         const uint32_t rw = a.s.rwc[r];
         const int len = (int)hi16(rw) - (int)lo16(rw) + 1;
-        if (len <= 256) cov_read_wave<PAIRS>(a, L, pr, stage, r, rg, ncodes);
+        if (len <= 128) cov_read_wave<PAIRS, 2>(a, L, pr, stage, r, rg, ncodes);
+        else if (len <= 256) cov_read_wave<PAIRS, 4>(a, L, pr, stage, r, rg, ncodes);
         else if (lane == 0) cov_read_serial<PAIRS>(a, L, pr, r, rg, ncodes);
     }
     __syncthreads();
@@ -688,9 +701,12 @@ __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
         return;
     }
 
-    // flush the tile's entries into the resident tables
+    // flush the tile's entries into the resident tables (a D entry's count is its I entry's)
     for (uint32_t e = threadIdx.x; e < E; e += blockDim.x) {
-        const uint32_t c = L.cnt[e];
+        const uint32_t ce = e == 65 ? 64u
+                          : (e >= E_CID + 64 && e < E_YID) ? e - 64
+                          : (e >= E_YID + ncodes) ? e - ncodes : e;
+        const uint32_t c = L.cnt[ce];
         if (!c) continue;
         int tab;
         uint32_t key;
