@@ -61,7 +61,7 @@ def build_synth():
 def build_io():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libfirepony_amd_io.so")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"),
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-Wall", "-I", os.path.join(ROOT, "include"),
           os.path.join(CSRC, "loaders.cpp"), "-lz", "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out

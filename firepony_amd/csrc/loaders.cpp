@@ -13,12 +13,14 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -66,7 +68,96 @@ struct NameDb {
     }
 };
 
-// Buffered byte/line reader over a gz stream.
+// BGZF (the blocked gzip of BAM and bgzip files): independent deflate blocks of at most 64 KiB,
+// each a gzip member whose extra field "BC" gives its size (SAM/BAM specification, section 4.1).
+// Blocks are inflated in parallel, a chunk of them at a time, and appended in order.
+class Bgzf {
+public:
+    ~Bgzf() { if (f_) std::fclose(f_); }
+
+    // true if path is BGZF (the file is then kept open)
+    bool open(const char *path)
+    {
+        f_ = std::fopen(path, "rb");
+        if (!f_) return false;
+        uint8_t h[18];
+        const size_t got = std::fread(h, 1, 18, f_);
+        const bool bgzf = got == 18 && h[0] == 31 && h[1] == 139 && h[2] == 8 && (h[3] & 4) &&
+                          h[10] == 6 && h[11] == 0 && h[12] == 'B' && h[13] == 'C' && h[14] == 2 && h[15] == 0;
+        if (!bgzf) { std::fclose(f_); f_ = nullptr; return false; }
+        std::fseek(f_, 0, SEEK_SET);
+        unsigned t = std::thread::hardware_concurrency();
+        threads_ = t == 0 ? 4 : (t > 16 ? 16 : t);
+        return true;
+    }
+
+    // inflates up to one chunk of blocks and appends their bytes to out; false at the end of the file
+    // (err set on a malformed or truncated block)
+    bool next(std::vector<char> &out, size_t &end, bool &err)
+    {
+        if (done_) return false;
+        // read whole blocks, up to CHUNK of them
+        struct Block { size_t coff, clen; uint32_t isize; size_t ooff; };
+        std::vector<Block> blocks;
+        comp_.clear();
+        size_t total_out = 0;
+        while (blocks.size() < CHUNK) {
+            uint8_t h[18];
+            const size_t got = std::fread(h, 1, 18, f_);
+            if (got == 0) { done_ = true; break; }
+            if (got < 18 || h[0] != 31 || h[1] != 139 || h[2] != 8 || !(h[3] & 4)) { err = true; done_ = true; return false; }
+            const uint32_t xlen = h[10] | (h[11] << 8);
+            // the BC subfield is the first of the extra field in files written to the specification
+            if (xlen < 6 || h[12] != 'B' || h[13] != 'C') { err = true; done_ = true; return false; }
+            const uint32_t bsize = (uint32_t)(h[16] | (h[17] << 8)) + 1;
+            if (bsize < 18 + xlen - 6 + 8 + 2) { err = true; done_ = true; return false; }
+            const size_t rest = bsize - 18;
+            const size_t at = comp_.size();
+            comp_.resize(at + rest);
+            if (std::fread(comp_.data() + at, 1, rest, f_) != rest) { err = true; done_ = true; return false; }
+            const uint8_t *tail = comp_.data() + at + rest - 4;
+            const uint32_t isize = tail[0] | (tail[1] << 8) | (tail[2] << 16) | ((uint32_t)tail[3] << 24);
+            const size_t skip = xlen - 6;   // any further extra subfields
+            blocks.push_back({at + skip, rest - skip - 8, isize, total_out});
+            total_out += isize;
+        }
+        if (blocks.empty()) return false;
+        if (out.size() < end + total_out) out.resize(std::max(end + total_out, 2 * out.size()));
+        std::atomic<bool> bad{false};
+        auto work = [&](size_t t) {
+            z_stream z;
+            for (size_t i = t; i < blocks.size(); i += threads_) {
+                const Block &bk = blocks[i];
+                std::memset(&z, 0, sizeof z);
+                if (inflateInit2(&z, -15) != Z_OK) { bad = true; return; }
+                z.next_in = comp_.data() + bk.coff;
+                z.avail_in = (uInt)bk.clen;
+                z.next_out = (Bytef *)out.data() + end + bk.ooff;
+                z.avail_out = bk.isize;
+                const int rc = inflate(&z, Z_FINISH);
+                if (rc != Z_STREAM_END || z.total_out != bk.isize) bad = true;
+                inflateEnd(&z);
+            }
+        };
+        const size_t nt = std::min<size_t>(threads_, blocks.size());
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; t++) pool.emplace_back(work, t);
+        work(0);
+        for (auto &th : pool) th.join();
+        if (bad) { err = true; done_ = true; return false; }
+        end += total_out;
+        return true;
+    }
+
+private:
+    static constexpr size_t CHUNK = 256;   // blocks per parallel step (up to 16 MiB inflated)
+    FILE *f_ = nullptr;
+    unsigned threads_ = 1;
+    bool done_ = false;
+    std::vector<uint8_t> comp_;
+};
+
+// Buffered byte/line reader: BGZF through Bgzf, anything else (plain, gzip) through zlib's gz stream.
 class Reader {
 public:
     ~Reader() { if (f_) gzclose(f_); }
@@ -74,10 +165,11 @@ public:
     bool open(const char *path)
     {
         path_ = path;
+        buf_.resize(4u << 20);
+        if (bgzf_.open(path)) { use_bgzf_ = true; return true; }
         f_ = gzopen(path, "rb");
         if (!f_) return false;
         gzbuffer(f_, 1u << 20);
-        buf_.resize(4u << 20);
         return true;
     }
     const std::string &path() const { return path_; }
@@ -94,6 +186,10 @@ public:
         }
         if (buf_.size() < n) buf_.resize(std::max(n, 2 * buf_.size()));
         while (e_ < n && !eof_) {
+            if (use_bgzf_) {
+                if (!bgzf_.next(buf_, e_, err_)) eof_ = true;
+                continue;
+            }
             const size_t room = std::min<size_t>(buf_.size() - e_, 1u << 30);
             const int r = gzread(f_, buf_.data() + e_, (unsigned)room);
             if (r < 0) { err_ = true; eof_ = true; break; }
@@ -141,6 +237,8 @@ private:
     }
 
     gzFile f_ = nullptr;
+    Bgzf bgzf_;
+    bool use_bgzf_ = false;
     std::string path_;
     std::vector<char> buf_;
     size_t b_ = 0, e_ = 0;

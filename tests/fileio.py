@@ -151,7 +151,7 @@ def bam_record(arrays, i, nc, extra_tags=b""):
     return struct.pack("<i", len(rec)) + rec
 
 
-def write_bam(path, arrays, names, lengths, rg_lines, extra_tags=b""):
+def write_bam(path, arrays, names, lengths, rg_lines, extra_tags=b"", block=65280):
     text = header_text(names, lengths, rg_lines).encode()
     data = bytearray(b"BAM\1" + struct.pack("<i", len(text)) + text + struct.pack("<i", len(names)))
     for n, l in zip(names, lengths):
@@ -160,4 +160,4 @@ def write_bam(path, arrays, names, lengths, rg_lines, extra_tags=b""):
     for i in range(len(arrays["chromosome"])):
         data += bam_record(arrays, i, len(names), extra_tags)
     with open(path, "wb") as f:
-        f.write(bgzf_blocks(bytes(data)))
+        f.write(bgzf_blocks(bytes(data), block))

@@ -102,8 +102,9 @@ def test_round_trip_crlf_and_extra_tags(tmp_path):
     a = b.arrays()
     names = [ref.name(i) for i in range(len(ref))]
     tags = b"XAi" + (7).to_bytes(4, "little") + b"XBB" + b"s" + (3).to_bytes(4, "little") + bytes(6) + b"XZZabc\0"
+    # 300-byte BGZF blocks: more blocks than one parallel step inflates, records split across blocks
     fileio.write_bam(bam, a, names, [int(g.reference.length[i]) for i in range(len(names))],
-                     fileio.default_rg_lines(3), extra_tags=tags)
+                     fileio.default_rg_lines(3), extra_tags=tags, block=300)
     aln2 = Alignments(bam, ref)
     check_round_trip(load_all(aln2, 128), a, aln2.header()[1], len(ref))
 
