@@ -584,8 +584,12 @@ struct Record {
     int32_t tid = -1, pos = -1, mtid = -1, mpos = -1, isize = 0;
     uint32_t flag = 0, mapq = 0;
     std::vector<uint32_t> cigar;   // BAM words: len << 4 | op
-    std::vector<uint8_t> seq;      // one nt16 code per base
-    std::vector<uint8_t> qual;
+    std::vector<uint8_t> seq;      // SAM: one nt16 code per base
+    std::vector<uint8_t> qual;     // SAM
+    // BAM: the record's own packed bases (two per byte, first base in the high nibble) and
+    // qualities, in the reader's buffer until the record is appended
+    const uint8_t *bseq = nullptr, *bqual = nullptr;
+    uint32_t lseq = 0;
     bool has_rg = false;
     std::string rg;
 };
@@ -801,6 +805,7 @@ fp_status parse_sam(fp_alignment_file *f, const std::string &l, Record &rec)
     if (!ok) return fmt_error(f, "bad PNEXT");
     rec.isize = parse_i32(fld[8], fend[8], ok);
     if (!ok) return fmt_error(f, "bad TLEN");
+    rec.bseq = rec.bqual = nullptr;
     rec.seq.clear();
     const size_t lseq = (fend[9] - fld[9] == 1 && *fld[9] == '*') ? 0 : (size_t)(fend[9] - fld[9]);
     if (lseq) {
@@ -854,10 +859,10 @@ fp_status parse_bam(fp_alignment_file *f, const char *p, size_t n, Record &rec)
     rec.cigar.resize(n_cigar);
     for (uint32_t i = 0; i < n_cigar; i++) rec.cigar[i] = rd_le<uint32_t>(c + 4 * i);
     const uint8_t *s = (const uint8_t *)(c + 4 * n_cigar);
-    rec.seq.resize((size_t)l_seq);
-    for (int32_t i = 0; i < l_seq; i++) rec.seq[i] = (s[i >> 1] >> ((~i & 1) << 2)) & 15;   // bam_seqi
     const uint8_t *qs = s + (l_seq + 1) / 2;
-    rec.qual.assign(qs, qs + l_seq);
+    rec.bseq = s;
+    rec.bqual = qs;
+    rec.lseq = (uint32_t)l_seq;
     // aux fields: first RG (bam_aux_get), its string if of type Z (bam_aux2Z)
     rec.has_rg = false;
     const char *a = (const char *)(qs + l_seq), *ae = p + n;
@@ -944,17 +949,27 @@ fp_status append(fp_alignment_file *f, const Record &r)
         b.cigars.push_back(((w >> 4) & 0xffffffu) | (op << 24));
     }
     // bases: 4-bit codes, each read starting on a word, padded to 8 (loader/alignments.cu:337-367)
-    const uint32_t n = (uint32_t)r.seq.size();
+    const uint32_t n = r.bseq ? r.lseq : (uint32_t)r.seq.size();
     b.read_start.push_back((uint32_t)b.read_bases);
     b.read_len.push_back(n);
     const uint64_t padded = ((uint64_t)n + 7) / 8 * 8;
     const size_t w0 = b.reads.size();
     b.reads.resize(w0 + padded / 8, 0);
-    for (uint32_t i = 0; i < n; i++) b.reads[w0 + i / 8] |= (uint32_t)r.seq[i] << ((i & 7) * 4);
+    if (r.bseq) {
+        // BAM byte k holds bases 2k (high nibble) and 2k+1 (low); the batch's byte k (little-endian
+        // words) holds them the other way round, so each byte has its nibbles swapped
+        uint8_t *dst = (uint8_t *)(b.reads.data() + w0);
+        const uint32_t nb = (n + 1) / 2;
+        for (uint32_t k = 0; k < nb; k++) dst[k] = (uint8_t)((r.bseq[k] >> 4) | (r.bseq[k] << 4));
+        if (n & 1) dst[nb - 1] &= 15;   // the unused half of BAM's last byte stays padding
+    } else {
+        for (uint32_t i = 0; i < n; i++) b.reads[w0 + i / 8] |= (uint32_t)r.seq[i] << ((i & 7) * 4);
+    }
     b.read_bases += padded;
     b.qual_start.push_back((uint32_t)b.qualities.size());
     b.qual_len.push_back(n);
-    b.qualities.insert(b.qualities.end(), r.qual.begin(), r.qual.end());
+    if (r.bqual) b.qualities.insert(b.qualities.end(), r.bqual, r.bqual + n);
+    else b.qualities.insert(b.qualities.end(), r.qual.begin(), r.qual.end());
     // the twelve SAM flag bits map one to one onto AlignmentFlags (loader/alignments.cu:214-255)
     b.flags.push_back((uint16_t)(r.flag & 0xfff));
     b.mapq.push_back((uint8_t)r.mapq);
@@ -1014,7 +1029,12 @@ extern "C" fp_status fp_alignments_next_batch(fp_alignment_file *f, uint32_t bat
                 return fmt_error(f, "truncated record");
             }
             rc = parse_bam(f, r.ptr() + 4, (size_t)bs, f->rec);
+            if (rc == FP_OK) rc = append(f, f->rec);   // while the record is still in the buffer
             r.skip(4 + (size_t)bs);
+            if (rc == FP_OK) { f->records++; continue; }
+            b.clear();
+            b.view(out);
+            return rc;
         } else {
             if (!f->rd.line(f->line)) break;
             f->lineno++;
