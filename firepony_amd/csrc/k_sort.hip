@@ -316,64 +316,73 @@ struct RunSum {
     uint64_t c, v0, v1, v2;
 };
 
-__global__ void __launch_bounds__(256) k_seg_reduce(ReduceArgs a)
+// One wave per workgroup and per chunk of 64 x RED_LANE sorted pairs.  The chunk is loaded into LDS
+// with coalesced 16-byte loads (a lane's own RED_LANE consecutive pairs would be 64 scattered
+// partial lines per load instruction), padded by one entry per RED_LANE so that the lanes' reads of
+// their consecutive slices fall in different banks; then each lane sums its slice run by run.
+__global__ void __launch_bounds__(64) k_seg_reduce(ReduceArgs a)
 {
+    constexpr uint32_t CE = 64 * RED_LANE;
+    constexpr uint32_t PADDED = CE + CE / RED_LANE;
+    __shared__ uint32_t sk[PADDED];
+    __shared__ double sv[PADDED];
     const uint64_t nn = *a.n;
     const uint32_t n = (uint32_t)(nn < a.n_cap ? nn : a.n_cap);
-    const uint32_t chunk_elems = 64 * RED_LANE;
-    const uint32_t chunks = (n + chunk_elems - 1) / chunk_elems;
+    const uint32_t chunks = (n + CE - 1) / CE;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t chunk = gw; chunk < chunks; chunk += nw) {
+    auto slot = [](uint32_t e) { return e + e / RED_LANE; };
+    for (uint32_t chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
+        const uint32_t base = chunk * CE;
+        if (base + CE <= n) {
+            const uint4 *k4 = (const uint4 *)(a.keys + base);
+            const double2 *v2 = (const double2 *)(a.vals + base);
+#pragma unroll
+            for (uint32_t i = 0; i < CE / 256; i++) {
+                const uint32_t e = 4 * (lane + 64 * i);
+                const uint4 k = k4[lane + 64 * i];
+                sk[slot(e)] = k.x; sk[slot(e + 1)] = k.y; sk[slot(e + 2)] = k.z; sk[slot(e + 3)] = k.w;
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < CE / 128; i++) {
+                const uint32_t e = 2 * (lane + 64 * i);
+                const double2 v = v2[lane + 64 * i];
+                sv[slot(e)] = v.x; sv[slot(e + 1)] = v.y;
+            }
+        } else {
+            for (uint32_t e = lane; e < CE; e += 64) {
+                const bool in = base + e < n;
+                sk[slot(e)] = in ? a.keys[base + e] : SORT_SENTINEL;
+                sv[slot(e)] = in ? a.vals[base + e] : 0.0;
+            }
+        }
+        __syncthreads();
         // this lane's slice, summed run by run
-        const uint32_t e0 = chunk * chunk_elems + lane * RED_LANE;
         uint32_t hk = SORT_SENTINEL, cur = SORT_SENTINEL;
         RunSum hs = {0, 0, 0, 0}, cs = {0, 0, 0, 0};
         bool single = true;     // the slice holds one run so far
-        for (uint32_t q = 0; q < RED_LANE; q += 8) {
-            uint32_t kb[8];
-            double vb[8];
-            if (e0 + q + 8 <= n) {
-                const uint4 k0 = *(const uint4 *)(a.keys + e0 + q), k1 = *(const uint4 *)(a.keys + e0 + q + 4);
-                kb[0] = k0.x; kb[1] = k0.y; kb[2] = k0.z; kb[3] = k0.w;
-                kb[4] = k1.x; kb[5] = k1.y; kb[6] = k1.z; kb[7] = k1.w;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const double2 v = *(const double2 *)(a.vals + e0 + q + 2 * j);
-                    vb[2 * j] = v.x;
-                    vb[2 * j + 1] = v.y;
+        const uint32_t s0 = lane * (RED_LANE + 1);
+#pragma unroll 8
+        for (uint32_t q = 0; q < RED_LANE; q++) {
+            const uint32_t k = sk[s0 + q];
+            if (k == SORT_SENTINEL) continue;     // past the end
+            // most per-base errors are exactly 0 or 1 (1.0 = 2^80: limbs 0, 0, 2^26)
+            uint64_t l0 = 0, l1 = 0, l2 = 0;
+            const double v = sv[s0 + q];
+            if (v == 1.0) l2 = 1ull << 26;
+            else if (v != 0.0) fixed_split(fixed_from_double(v), l0, l1, l2);
+            if (k != cur) {
+                if (cur != SORT_SENTINEL) {
+                    if (single) { hs = cs; single = false; }            // the slice's first run ended
+                    else table_add(a, cur, cs.c, cs.v0, cs.v1, cs.v2);  // a run inside the slice
+                } else {
+                    hk = k;
                 }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    const bool in = e0 + q + j < n;
-                    kb[j] = in ? a.keys[e0 + q + j] : SORT_SENTINEL;
-                    vb[j] = in ? a.vals[e0 + q + j] : 0.0;
-                }
+                cur = k;
+                cs = {0, 0, 0, 0};
             }
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const uint32_t k = kb[j];
-                if (k == SORT_SENTINEL) continue;     // past the end
-                // most per-base errors are exactly 0 or 1 (1.0 = 2^80: limbs 0, 0, 2^26)
-                uint64_t l0 = 0, l1 = 0, l2 = 0;
-                const double v = vb[j];
-                if (v == 1.0) l2 = 1ull << 26;
-                else if (v != 0.0) fixed_split(fixed_from_double(v), l0, l1, l2);
-                if (k != cur) {
-                    if (cur != SORT_SENTINEL) {
-                        if (single) { hs = cs; single = false; }            // the slice's first run ended
-                        else table_add(a, cur, cs.c, cs.v0, cs.v1, cs.v2);  // a run inside the slice
-                    } else {
-                        hk = k;
-                    }
-                    cur = k;
-                    cs = {0, 0, 0, 0};
-                }
-                cs.c += 1; cs.v0 += l0; cs.v1 += l1; cs.v2 += l2;
-            }
+            cs.c += 1; cs.v0 += l0; cs.v1 += l1; cs.v2 += l2;
         }
+        __syncthreads();   // the next chunk overwrites the LDS copy
         // lane summary: head run (hk, hs) unless single; tail run (cur, cs) in every lane
         if (single) hk = cur;
         const uint32_t tk = cur;
@@ -455,7 +464,7 @@ extern "C" hipError_t fpd_sort_reduce(uint32_t *keys, double *vals, uint32_t *tk
     r.keys = ka; r.vals = va; r.n = n_kept; r.n_cap = cap; r.key_limit = table_entries;
     r.count = count; r.p0 = parts[0]; r.p1 = parts[1]; r.p2 = parts[2];
     const uint32_t chunks = (uint32_t)((cap + 64 * fpd::RED_LANE - 1) / (64 * fpd::RED_LANE));
-    const uint32_t rgrid = std::max<uint32_t>(1, std::min<uint32_t>((chunks + 3) / 4, 2048));
-    hipLaunchKernelGGL(fpd::k_seg_reduce, dim3(rgrid), dim3(256), 0, st, r);
+    const uint32_t rgrid = std::max<uint32_t>(1, std::min<uint32_t>(chunks, 8192));
+    hipLaunchKernelGGL(fpd::k_seg_reduce, dim3(rgrid), dim3(64), 0, st, r);
     return hipGetLastError();
 }
