@@ -122,8 +122,15 @@ def roofline(stats, total_reads, bases):
         # backward 16 + rescale 3, MAP 4; baq.cu:374-681)
         flops = BAQ_FLOPS_PER_CELL * stats.baq_cells / max(launches, 1)
         ach = flops / avg_s / 1e12
-        return name, {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                      "frac": ach / FP64_PEAK_TFLOPS, "traffic": traffic, "avg_launch_ms": avg_s * 1e3}
+        out = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+               "frac": ach / FP64_PEAK_TFLOPS, "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
+               "note": "vector FP64 (no MFMA form exists for this recurrence); the kernel is held by its "
+                       "HBM traffic, see traffic_rate"}
+        if traffic:
+            # measured HBM bytes per launch over the launch time, against the peak and a plain copy
+            rate = traffic / avg_s / 1e9
+            out.update(traffic_rate_gbs=rate, traffic_frac_of_peak=rate / HBM_PEAK_GBS)
+        return name, out
     # scan stages: compulsory bytes per read (inputs read + per-read outputs written)
     per_read = {"prep": 49 + 13 * 4 + 101 + 4 * 1.2 + 54, "cov": 101 + 13 * 4 + 101 + 101 + 8}[name]
     byts = per_read * total_reads / max(launches, 1)
