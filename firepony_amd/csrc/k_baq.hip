@@ -385,6 +385,16 @@ __device__ inline bool cap_walk(const BatchDev &b, const ReadStateDev &s, uint32
     return readI == (uint32_t)(rwy - rwx + 1);
 }
 
+// a window position's cap-walk code in 32 bits: kind in the top two bits, the expected MAP state
+// below.  An expected state that does not fit is stored as CAP_EP_MASK, which no MAP state with
+// state & 3 == 0 reaches (band positions are far below 2^30), so the comparison in cap_tag keeps
+// its outcome.
+static constexpr uint32_t CAP_EP_MASK = 0x3fffffffu;
+__device__ inline uint32_t cap_code(uint32_t kind, uint32_t ep)
+{
+    return (kind << 30) | (ep < CAP_EP_MASK ? ep : CAP_EP_MASK);
+}
+
 // the capped quality of one window position (cap_baq_qualities) recoded into its BQ tag (recode, baq.cu:1058-1081)
 __device__ inline uint8_t cap_tag(uint32_t kind, uint32_t expectedPos, uint8_t oq, uint8_t mapq, uint32_t state)
 {
@@ -411,7 +421,7 @@ __device__ inline uint8_t cap_tag(uint32_t kind, uint32_t expectedPos, uint8_t o
 // read nothing from global memory but the spilled forward rows.  Per-lane global reads of a few
 // bytes per row are otherwise fetched again from HBM on every row, because the forward-row stream
 // evicts them from the caches in between.
-template <int W, int WPE, bool STAGE>
+template <int W, int WPE, bool STAGE, bool TAGL>
 __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
 {
     constexpr int NWIN = (W + 15) / 16;       // u64 words of band reference nibbles
@@ -432,16 +442,20 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
     double *slot = a.scratch + (uint64_t)gwave * a.slot_doubles;
     double2 *Fst = (double2 *)slot;                                  // [row][r][lane] (M, I)
     double *Sst = slot + (uint64_t)(a.rows_cap + 1) * W * 2 * 64;    // [row][lane]
-    uint64_t *Ost = (uint64_t *)(Sst + (uint64_t)(a.rows_cap + 2) * 64);  // [row][lane] MAP state | quality << 32
+    // [row][lane] the cap walk's code (kind << 30 | expected state, capped) and then, unless TAGL, the tag
+    uint32_t *Ost = (uint32_t *)(Sst + (uint64_t)(a.rows_cap + 2) * 64);
     const BatchDev &b = a.b;
     uint64_t ncell = 0;
     // per-wave staging area (STAGE), [index][lane]: rows_cap query qualities (bytes), the query
-    // bases and the rows_cap + 2W reference cells as nibbles, two per byte
+    // bases and the rows_cap + 2W reference cells as nibbles, two per byte, and (TAGL) the read's
+    // BQ tags as the MAP steps produce them
     extern __shared__ unsigned char baq_stage[];
     const uint32_t base_bytes = (a.rows_cap + 1) / 2, ref_bytes = (a.rows_cap + 2 * W + 1) / 2;
-    uint8_t *rowq = baq_stage + (threadIdx.x >> 6) * ((a.rows_cap + base_bytes + ref_bytes) * 64);
+    const uint32_t wave_bytes = (a.rows_cap + base_bytes + ref_bytes + (TAGL ? a.rows_cap : 0u)) * 64;
+    uint8_t *rowq = baq_stage + (threadIdx.x >> 6) * wave_bytes;
     uint8_t *rowb = rowq + (uint64_t)a.rows_cap * 64;
     uint8_t *refd = rowb + (uint64_t)base_bytes * 64;
+    uint8_t *tagl = refd + (uint64_t)ref_bytes * 64;
     auto refn = [&](int i) -> uint32_t { return (refd[(i >> 1) * 64 + (threadIdx.x & 63)] >> ((i & 1) * 4)) & 15; };
     // query position q as quality | base << 8
     auto rowv = [&](int q) -> uint32_t {
@@ -595,7 +609,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         // expected MAP state of every window position (the cap_baq_qualities walk), staged per row;
         // the MAP step turns it into the position's final tag
         const bool good = cap_walk(b, a.s, rd, [&](uint16_t p, uint32_t kind, uint32_t ep) {
-            Ost[(uint64_t)((int)p - qStart + 1) * 64 + lane] = (uint64_t)ep | ((uint64_t)kind << 32);
+            Ost[(uint64_t)((int)p - qStart + 1) * 64 + lane] = cap_code(kind, ep);
         });
 
         if constexpr (STAGE) {
@@ -748,7 +762,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         // consumes is loaded right after the current one is used, a full backward row ahead
         constexpr bool PREF = WPE == 1;
         double2 Fp[PREF ? W : 1];
-        uint64_t Cp = 0;
+        uint32_t Cp = 0;
         auto load_f = [&](int i) {
             if constexpr (PREF) {
                 const double2 *fr = Fst + (uint64_t)i * W * 64 + lane;
@@ -775,10 +789,11 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
                 if (z > mx) { mx = z; stt = (k - 1) << 2 | 1; }
             }
             mx /= sum;
-            const uint64_t code = PREF ? Cp : Ost[(uint64_t)i * 64 + lane];
-            // the final tag replaces the code in the slot (one coalesced row store)
-            Ost[(uint64_t)i * 64 + lane] =
-                cap_tag((uint32_t)(code >> 32), (uint32_t)code, oq, mapq(1.0 - mx), (uint32_t)stt);
+            const uint32_t code = PREF ? Cp : Ost[(uint64_t)i * 64 + lane];
+            const uint8_t tag = cap_tag(code >> 30, code & CAP_EP_MASK, oq, mapq(1.0 - mx), (uint32_t)stt);
+            // the final tag goes to LDS (TAGL) or replaces the code in the slot (one coalesced row store)
+            if constexpr (TAGL) tagl[(i - 1) * 64 + lane] = tag;
+            else Ost[(uint64_t)i * 64 + lane] = tag;
         };
         load_f(qLen);
         // backward row i emits query index i (the row below it); map_row(i) needs quality index i - 1
@@ -865,16 +880,21 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         // the read's tags, row by row in lockstep across the wave: coalesced slot loads, and each lane
         // writes its read's window as one burst of consecutive bytes
         uint8_t *out = a.s.baq + b.qual_start[rd] + qStart;
-        const uint64_t *orow = Ost + 64 + lane;                 // row j + 1 holds position qStart + j
-        int j = 0;
-        for (; j + 8 <= qLen; j += 8) {
-            uint64_t t[8];
+        if constexpr (TAGL) {
+            const uint8_t *trow = tagl + lane;                  // index j holds position qStart + j
+            for (int j = 0; j < qLen; j++) out[j] = good ? trow[j * 64] : (uint8_t)64;
+        } else {
+            const uint32_t *orow = Ost + 64 + lane;             // row j + 1 holds position qStart + j
+            int j = 0;
+            for (; j + 8 <= qLen; j += 8) {
+                uint32_t t[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) t[u] = orow[(uint64_t)(j + u) * 64];
+                for (int u = 0; u < 8; u++) t[u] = orow[(uint64_t)(j + u) * 64];
 #pragma unroll
-            for (int u = 0; u < 8; u++) out[j + u] = good ? (uint8_t)t[u] : (uint8_t)64;
+                for (int u = 0; u < 8; u++) out[j + u] = good ? (uint8_t)t[u] : (uint8_t)64;
+            }
+            for (; j < qLen; j++) out[j] = good ? (uint8_t)orow[(uint64_t)j * 64] : (uint8_t)64;
         }
-        for (; j < qLen; j++) out[j] = good ? (uint8_t)orow[(uint64_t)j * 64] : (uint8_t)64;
     }
     // one counter update per wave
     for (int off = 32; off >= 1; off >>= 1) ncell += __shfl_xor(ncell, off);
@@ -1469,10 +1489,11 @@ extern "C" hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t
 static int g_baq_lds_limit = 64 * 1024;
 
 // LDS bytes of a STAGE band workgroup (4 waves): per wave rows_cap quality bytes, rows_cap query
-// base nibbles and rows_cap + 2W reference nibbles
-static size_t baq_stage_bytes(int W, uint32_t rows_cap)
+// base nibbles and rows_cap + 2W reference nibbles, and with tags rows_cap tag bytes
+static size_t baq_stage_bytes(int W, uint32_t rows_cap, bool tags = false)
 {
-    return 4 * ((size_t)rows_cap + ((size_t)rows_cap + 1) / 2 + ((size_t)rows_cap + 2 * W + 1) / 2) * 64;
+    return 4 * ((size_t)rows_cap + ((size_t)rows_cap + 1) / 2 + ((size_t)rows_cap + 2 * W + 1) / 2 +
+                (tags ? (size_t)rows_cap : 0)) * 64;
 }
 
 extern "C" hipError_t fpd_baq_prepare()
@@ -1485,9 +1506,12 @@ extern "C" hipError_t fpd_baq_prepare()
     // the static tables (qualities and thresholds, 3 KiB) share the limit
     g_baq_lds_limit = (v > 0 ? v : 64 * 1024) - 4096;
     const int lim = g_baq_lds_limit;
-    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<15, 1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<23, 1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<33, 1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<15, 1, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<23, 1, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<33, 1, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<15, 1, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<23, 1, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<33, 1, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void *)fpd::k_baq_seg<15, 2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     return hipSuccess;
 }
@@ -1515,25 +1539,34 @@ extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const 
     // band kernels: 4 waves per workgroup (the caller sizes waves as a multiple of 4)
     const dim3 g4((waves + 3) / 4), b4(256);
     const bool stage = fpd_baq_staged(cls, rows_cap, wpe, allow_stage) != 0;
-    const size_t lds = stage ? baq_stage_bytes(cls == 0 ? 15 : (cls == 1 ? 23 : 33), rows_cap) : 0;
+    const int W = cls == 0 ? 15 : (cls == 1 ? 23 : 33);
+    // the tags stay in LDS too when the budget holds them (short reads)
+    const bool tags = stage && cls <= 2 && baq_stage_bytes(W, rows_cap, true) <= (size_t)g_baq_lds_limit;
+    const size_t lds = stage ? baq_stage_bytes(W, rows_cap, tags) : 0;
     switch (cls) {
     case 0:
-        if (stage && recompute) { hipLaunchKernelGGL((fpd::k_baq_seg<15, 2, true>), g4, b4, lds, st, a); break; }
-        if (stage) { hipLaunchKernelGGL((fpd::k_baq_band<15, 1, true>), g4, b4, lds, st, a); break; }
+        if (stage && recompute) {
+            hipLaunchKernelGGL((fpd::k_baq_seg<15, 2, true>), g4, b4, baq_stage_bytes(15, rows_cap), st, a);
+            break;
+        }
+        if (tags) { hipLaunchKernelGGL((fpd::k_baq_band<15, 1, true, true>), g4, b4, lds, st, a); break; }
+        if (stage) { hipLaunchKernelGGL((fpd::k_baq_band<15, 1, true, false>), g4, b4, lds, st, a); break; }
         switch (wpe) {
-        case 1: hipLaunchKernelGGL((fpd::k_baq_band<15, 1, false>), g4, b4, 0, st, a); break;
-        case 3: hipLaunchKernelGGL((fpd::k_baq_band<15, 3, false>), g4, b4, 0, st, a); break;
-        case 4: hipLaunchKernelGGL((fpd::k_baq_band<15, 4, false>), g4, b4, 0, st, a); break;
-        default: hipLaunchKernelGGL((fpd::k_baq_band<15, 2, false>), g4, b4, 0, st, a); break;
+        case 1: hipLaunchKernelGGL((fpd::k_baq_band<15, 1, false, false>), g4, b4, 0, st, a); break;
+        case 3: hipLaunchKernelGGL((fpd::k_baq_band<15, 3, false, false>), g4, b4, 0, st, a); break;
+        case 4: hipLaunchKernelGGL((fpd::k_baq_band<15, 4, false, false>), g4, b4, 0, st, a); break;
+        default: hipLaunchKernelGGL((fpd::k_baq_band<15, 2, false, false>), g4, b4, 0, st, a); break;
         }
         break;
     case 1:
-        if (stage) hipLaunchKernelGGL((fpd::k_baq_band<23, 1, true>), g4, b4, lds, st, a);
-        else hipLaunchKernelGGL((fpd::k_baq_band<23, 1, false>), g4, b4, 0, st, a);
+        if (tags) hipLaunchKernelGGL((fpd::k_baq_band<23, 1, true, true>), g4, b4, lds, st, a);
+        else if (stage) hipLaunchKernelGGL((fpd::k_baq_band<23, 1, true, false>), g4, b4, lds, st, a);
+        else hipLaunchKernelGGL((fpd::k_baq_band<23, 1, false, false>), g4, b4, 0, st, a);
         break;
     case 2:
-        if (stage) hipLaunchKernelGGL((fpd::k_baq_band<33, 1, true>), g4, b4, lds, st, a);
-        else hipLaunchKernelGGL((fpd::k_baq_band<33, 1, false>), g4, b4, 0, st, a);
+        if (tags) hipLaunchKernelGGL((fpd::k_baq_band<33, 1, true, true>), g4, b4, lds, st, a);
+        else if (stage) hipLaunchKernelGGL((fpd::k_baq_band<33, 1, true, false>), g4, b4, lds, st, a);
+        else hipLaunchKernelGGL((fpd::k_baq_band<33, 1, false, false>), g4, b4, 0, st, a);
         break;
     default: hipLaunchKernelGGL((fpd::k_baq_hmm<0, 2>), dim3(waves), dim3(64), 0, st, a); break;
     }
