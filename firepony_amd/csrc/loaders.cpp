@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -91,9 +92,12 @@ public:
         return true;
     }
 
-    // inflates up to one chunk of blocks and appends their bytes to out; false at the end of the file
-    // (err set on a malformed or truncated block)
-    bool next(std::vector<char> &out, size_t &end, bool &err)
+    static constexpr size_t CHUNK = 256;                    // blocks per parallel step
+    static constexpr size_t MAX_CHUNK_BYTES = CHUNK * 65536; // inflated bytes of one step at most
+
+    // inflates up to one chunk of blocks to out + end (room for MAX_CHUNK_BYTES); false at the end
+    // of the file (err set on a malformed or truncated block)
+    bool next(char *out, size_t &end, bool &err)
     {
         if (done_) return false;
         // read whole blocks, up to CHUNK of them
@@ -117,12 +121,12 @@ public:
             if (std::fread(comp_.data() + at, 1, rest, f_) != rest) { err = true; done_ = true; return false; }
             const uint8_t *tail = comp_.data() + at + rest - 4;
             const uint32_t isize = tail[0] | (tail[1] << 8) | (tail[2] << 16) | ((uint32_t)tail[3] << 24);
+            if (isize > 65536) { err = true; done_ = true; return false; }
             const size_t skip = xlen - 6;   // any further extra subfields
             blocks.push_back({at + skip, rest - skip - 8, isize, total_out});
             total_out += isize;
         }
         if (blocks.empty()) return false;
-        if (out.size() < end + total_out) out.resize(std::max(end + total_out, 2 * out.size()));
         std::atomic<bool> bad{false};
         auto work = [&](size_t t) {
             z_stream z;
@@ -132,7 +136,7 @@ public:
                 if (inflateInit2(&z, -15) != Z_OK) { bad = true; return; }
                 z.next_in = comp_.data() + bk.coff;
                 z.avail_in = (uInt)bk.clen;
-                z.next_out = (Bytef *)out.data() + end + bk.ooff;
+                z.next_out = (Bytef *)out + end + bk.ooff;
                 z.avail_out = bk.isize;
                 const int rc = inflate(&z, Z_FINISH);
                 if (rc != Z_STREAM_END || z.total_out != bk.isize) bad = true;
@@ -150,7 +154,6 @@ public:
     }
 
 private:
-    static constexpr size_t CHUNK = 256;   // blocks per parallel step (up to 16 MiB inflated)
     FILE *f_ = nullptr;
     unsigned threads_ = 1;
     bool done_ = false;
@@ -165,7 +168,7 @@ public:
     bool open(const char *path)
     {
         path_ = path;
-        buf_.resize(4u << 20);
+        reserve(4u << 20);
         if (bgzf_.open(path)) { use_bgzf_ = true; return true; }
         f_ = gzopen(path, "rb");
         if (!f_) return false;
@@ -180,25 +183,26 @@ public:
     {
         if (e_ - b_ >= n) return e_ - b_;
         if (b_) {
-            std::memmove(buf_.data(), buf_.data() + b_, e_ - b_);
+            std::memmove(buf_.get(), buf_.get() + b_, e_ - b_);
             e_ -= b_;
             b_ = 0;
         }
-        if (buf_.size() < n) buf_.resize(std::max(n, 2 * buf_.size()));
+        reserve(n);
         while (e_ < n && !eof_) {
             if (use_bgzf_) {
-                if (!bgzf_.next(buf_, e_, err_)) eof_ = true;
+                reserve(e_ + Bgzf::MAX_CHUNK_BYTES);
+                if (!bgzf_.next(buf_.get(), e_, err_)) eof_ = true;
                 continue;
             }
-            const size_t room = std::min<size_t>(buf_.size() - e_, 1u << 30);
-            const int r = gzread(f_, buf_.data() + e_, (unsigned)room);
+            const size_t room = std::min<size_t>(cap_ - e_, 1u << 30);
+            const int r = gzread(f_, buf_.get() + e_, (unsigned)room);
             if (r < 0) { err_ = true; eof_ = true; break; }
             if (r == 0) { eof_ = true; break; }
             e_ += (size_t)r;
         }
         return e_ - b_;
     }
-    const char *ptr() const { return buf_.data() + b_; }
+    const char *ptr() const { return buf_.get() + b_; }
     void skip(size_t n) { b_ += n; }
 
     // next line without its '\n' and without one trailing '\r'; false at end of stream
@@ -206,9 +210,9 @@ public:
     {
         size_t scan = 0;
         for (;;) {
-            const char *p = (const char *)std::memchr(buf_.data() + b_ + scan, '\n', e_ - b_ - scan);
+            const char *p = (const char *)std::memchr(buf_.get() + b_ + scan, '\n', e_ - b_ - scan);
             if (p) {
-                const size_t n = (size_t)(p - (buf_.data() + b_));
+                const size_t n = (size_t)(p - (buf_.get() + b_));
                 take(out, n);
                 b_ += 1;
                 return true;
@@ -217,6 +221,33 @@ public:
             if (want(scan + 1) == scan) {
                 if (scan == 0) return false;
                 take(out, scan);
+                return true;
+            }
+        }
+    }
+    // the line starting c bytes into the unread data, without consuming it: its length (without
+    // '\n' and one trailing '\r') and the offset after it; false at the end of the stream
+    bool line_at(size_t c, size_t &len, size_t &next)
+    {
+        size_t scan = c;
+        for (;;) {
+            const size_t have = e_ - b_;
+            if (scan < have) {
+                const char *p = (const char *)std::memchr(buf_.get() + b_ + scan, '\n', have - scan);
+                if (p) {
+                    const size_t end = (size_t)(p - (buf_.get() + b_));
+                    len = end - c;
+                    next = end + 1;
+                    if (len && buf_[b_ + c + len - 1] == '\r') len--;
+                    return true;
+                }
+                scan = have;
+            }
+            if (want(have + 1) == have) {
+                if (have <= c) return false;
+                len = have - c;
+                next = have;
+                if (len && buf_[b_ + c + len - 1] == '\r') len--;
                 return true;
             }
         }
@@ -231,16 +262,31 @@ public:
 private:
     void take(std::string &out, size_t n)
     {
-        out.assign(buf_.data() + b_, n);
+        out.assign(buf_.get() + b_, n);
         b_ += n;
         if (!out.empty() && out.back() == '\r') out.pop_back();
+    }
+
+    // capacity for n unread bytes, keeping the unread ones (allocated uninitialised, and grown
+    // rarely: fresh pages cost more to fault in than the bytes cost to inflate)
+    void reserve(size_t n)
+    {
+        if (n <= cap_) return;
+        const size_t nc = std::max(n, 2 * cap_);
+        std::unique_ptr<char[]> nb(new char[nc]);
+        if (e_ > b_) std::memcpy(nb.get(), buf_.get() + b_, e_ - b_);
+        e_ -= b_;
+        b_ = 0;
+        buf_ = std::move(nb);
+        cap_ = nc;
     }
 
     gzFile f_ = nullptr;
     Bgzf bgzf_;
     bool use_bgzf_ = false;
     std::string path_;
-    std::vector<char> buf_;
+    std::unique_ptr<char[]> buf_;
+    size_t cap_ = 0;
     size_t b_ = 0, e_ = 0;
     bool eof_ = false, err_ = false;
 };
@@ -624,23 +670,28 @@ struct fp_alignment_file {
     std::vector<uint32_t> target_len;
     std::unordered_map<std::string, uint32_t> sam_target;   // SAM: RNAME -> tid
     std::vector<uint32_t> target_contig;                    // tid -> contig id of db, or UINT32_MAX
-    std::unordered_map<std::string, std::string> rg_id_to_name;
+    std::unordered_map<std::string, uint32_t> rg_id_to_name;   // @RG ID -> index into rg_names
+    std::vector<std::string> rg_names;                         // name = PU if read, else ID
     uint32_t max_read_groups = 0;
     NameDb read_groups;                                     // alignment_header_host::read_groups_db
     std::vector<const char *> rg_name_ptrs;
     BatchStore batch[2];   // alternate, so a batch stays valid while the next one loads
     unsigned cur = 0;
-    Record rec;
-    std::string line, pending;
-    bool have_pending = false;
+    unsigned threads = 1;
+    // per parsing thread: its records' arrays and read-group names
+    std::vector<BatchStore> part;
+    std::vector<std::vector<uint32_t>> part_rg;
+    std::vector<size_t> span_off, span_len;      // records of the batch being read: byte spans
+    std::vector<uint64_t> span_line;             // SAM: line numbers
+    std::string line;                            // header lines
 };
 
 namespace {
 
+// a malformed header: the file and (SAM) the line
 fp_status fmt_error(fp_alignment_file *f, const std::string &why)
 {
-    const std::string where = f->bam ? " record " + std::to_string(f->records + 1)
-                                     : ":" + std::to_string(f->lineno);
+    const std::string where = f->bam ? " header" : ":" + std::to_string(f->lineno);
     return fail(FP_ERR_FORMAT, f->rd.path() + where + ": " + why);
 }
 
@@ -658,12 +709,12 @@ void build_rg_map(fp_alignment_file *f)
         const std::string rec = h.substr(at, end == std::string::npos ? std::string::npos : end - at);
         std::string id, pu;
         parse_rg_line(rec, id, pu);
-        const std::string name = pu.empty() ? id : pu;
-        f->rg_id_to_name[id] = name;
+        f->rg_names.push_back(pu.empty() ? id : pu);
+        f->rg_id_to_name[id] = (uint32_t)f->rg_names.size() - 1;
         if (end == std::string::npos) break;
         at = end + 1;
     }
-    for (auto &kv : f->rg_id_to_name) distinct.insert(name_hash(kv.second.c_str()));
+    for (auto &kv : f->rg_id_to_name) distinct.insert(name_hash(f->rg_names[kv.second].c_str()));
     f->max_read_groups = (uint32_t)distinct.size();
 }
 
@@ -740,12 +791,19 @@ inline int32_t parse_i32(const char *s, const char *e, bool &ok, int base = 10)
     return (int32_t)v;
 }
 
+// a record the grammar rejects: the reason, for the caller to place (file and line or record)
+inline fp_status bad(std::string &why, const char *what)
+{
+    why = what;
+    return FP_ERR_FORMAT;
+}
+
 // One SAM text line into rec (the SAM specification's mandatory fields and the RG tag).
 // Where htslib's sam_parse1 (1.2) gives a field a defined meaning, the same meaning is used:
 // RNAME "*" or not among the @SQ names -> tid -1; CIGAR "*" -> no ops and the read flagged
 // unmapped; RNEXT "=" -> tid; SEQ "*" -> no bases; QUAL "*" -> 0xFF per base; FLAG in any
 // strtol base.  A record htslib rejects is an error here (the reference ends the batch there).
-fp_status parse_sam(fp_alignment_file *f, const std::string &l, Record &rec)
+fp_status parse_sam(const fp_alignment_file *f, const std::string &l, Record &rec, std::string &why)
 {
     const char *p = l.c_str();
     const char *fld[11], *fend[11];
@@ -754,7 +812,7 @@ fp_status parse_sam(fp_alignment_file *f, const std::string &l, Record &rec)
         fld[i] = q;
         const char *t = std::strchr(q, '\t');
         if (!t) {
-            if (i < 10) return fmt_error(f, "fewer than 11 fields");
+            if (i < 10) return bad(why, "fewer than 11 fields");
             t = q + std::strlen(q);
         }
         fend[i] = t;
@@ -764,19 +822,19 @@ fp_status parse_sam(fp_alignment_file *f, const std::string &l, Record &rec)
     auto field = [&](int i) { return std::string(fld[i], fend[i]); };
     bool ok;
     rec.flag = (uint32_t)parse_i32(fld[1], fend[1], ok, 0);
-    if (!ok) return fmt_error(f, "bad FLAG");
+    if (!ok) return bad(why, "bad FLAG");
     const std::string rname = field(2);
     if (rname == "*") {
         rec.tid = -1;
     } else {
-        if (f->target_name.empty()) return fmt_error(f, "missing SAM header");
+        if (f->target_name.empty()) return bad(why, "missing SAM header");
         auto it = f->sam_target.find(rname);
         rec.tid = it == f->sam_target.end() ? -1 : (int32_t)it->second;
     }
     rec.pos = parse_i32(fld[3], fend[3], ok) - 1;
-    if (!ok) return fmt_error(f, "bad POS");
+    if (!ok) return bad(why, "bad POS");
     rec.mapq = (uint32_t)parse_i32(fld[4], fend[4], ok) & 0xff;
-    if (!ok) return fmt_error(f, "bad MAPQ");
+    if (!ok) return bad(why, "bad MAPQ");
     rec.cigar.clear();
     if (fend[5] - fld[5] == 1 && *fld[5] == '*') {
         rec.flag |= 4;
@@ -785,9 +843,9 @@ fp_status parse_sam(fp_alignment_file *f, const std::string &l, Record &rec)
         while (c < fend[5]) {
             char *e = nullptr;
             const unsigned long long len = std::strtoull(c, &e, 10);
-            if (e == c || e >= fend[5]) return fmt_error(f, "bad CIGAR");
+            if (e == c || e >= fend[5]) return bad(why, "bad CIGAR");
             const int op = cigar_code(*e);
-            if (op < 0) return fmt_error(f, "unrecognized CIGAR operator");
+            if (op < 0) return bad(why, "unrecognized CIGAR operator");
             rec.cigar.push_back((uint32_t)(len << 4) | (uint32_t)op);
             c = e + 1;
         }
@@ -802,9 +860,9 @@ fp_status parse_sam(fp_alignment_file *f, const std::string &l, Record &rec)
         rec.mtid = it == f->sam_target.end() ? -1 : (int32_t)it->second;
     }
     rec.mpos = parse_i32(fld[7], fend[7], ok) - 1;
-    if (!ok) return fmt_error(f, "bad PNEXT");
+    if (!ok) return bad(why, "bad PNEXT");
     rec.isize = parse_i32(fld[8], fend[8], ok);
-    if (!ok) return fmt_error(f, "bad TLEN");
+    if (!ok) return bad(why, "bad TLEN");
     rec.bseq = rec.bqual = nullptr;
     rec.seq.clear();
     const size_t lseq = (fend[9] - fld[9] == 1 && *fld[9] == '*') ? 0 : (size_t)(fend[9] - fld[9]);
@@ -812,13 +870,13 @@ fp_status parse_sam(fp_alignment_file *f, const std::string &l, Record &rec)
         uint64_t qlen = 0;
         for (uint32_t w : rec.cigar)
             if (cigar_type(w & 15) & 1) qlen += w >> 4;
-        if (!rec.cigar.empty() && qlen != lseq) return fmt_error(f, "SEQ and CIGAR are inconsistent");
+        if (!rec.cigar.empty() && qlen != lseq) return bad(why, "SEQ and CIGAR are inconsistent");
         rec.seq.resize(lseq);
         for (size_t i = 0; i < lseq; i++) rec.seq[i] = NT16.t[(unsigned char)fld[9][i]];
     }
     rec.qual.assign(lseq, 0xff);
     if (!(fend[10] - fld[10] == 1 && *fld[10] == '*')) {
-        if ((size_t)(fend[10] - fld[10]) != lseq) return fmt_error(f, "SEQ and QUAL are of different length");
+        if ((size_t)(fend[10] - fld[10]) != lseq) return bad(why, "SEQ and QUAL are of different length");
         for (size_t i = 0; i < lseq; i++) rec.qual[i] = (uint8_t)(fld[10][i] - 33);
     }
     // first RG tag (bam_aux_get); a non-Z RG is treated as absent
@@ -839,9 +897,9 @@ fp_status parse_sam(fp_alignment_file *f, const std::string &l, Record &rec)
 }
 
 // One BAM record (SAM specification 4.2) into rec.
-fp_status parse_bam(fp_alignment_file *f, const char *p, size_t n, Record &rec)
+fp_status parse_bam(const char *p, size_t n, Record &rec, std::string &why)
 {
-    if (n < 32) return fmt_error(f, "record shorter than its fixed fields");
+    if (n < 32) return bad(why, "record shorter than its fixed fields");
     rec.tid = rd_le<int32_t>(p);
     rec.pos = rd_le<int32_t>(p + 4);
     const uint32_t l_read_name = (uint8_t)p[8];
@@ -852,9 +910,9 @@ fp_status parse_bam(fp_alignment_file *f, const char *p, size_t n, Record &rec)
     rec.mtid = rd_le<int32_t>(p + 20);
     rec.mpos = rd_le<int32_t>(p + 24);
     rec.isize = rd_le<int32_t>(p + 28);
-    if (l_seq < 0) return fmt_error(f, "negative sequence length");
+    if (l_seq < 0) return bad(why, "negative sequence length");
     const uint64_t need = 32ull + l_read_name + 4ull * n_cigar + (uint64_t)(l_seq + 1) / 2 + (uint64_t)l_seq;
-    if (need > n) return fmt_error(f, "record shorter than its fields");
+    if (need > n) return bad(why, "record shorter than its fields");
     const char *c = p + 32 + l_read_name;
     rec.cigar.resize(n_cigar);
     for (uint32_t i = 0; i < n_cigar; i++) rec.cigar[i] = rd_le<uint32_t>(c + 4 * i);
@@ -877,21 +935,21 @@ fp_status parse_bam(fp_alignment_file *f, const char *p, size_t n, Record &rec)
         case 'd': sz = 8; break;
         case 'Z': case 'H': {
             const char *z = (const char *)std::memchr(v, 0, (size_t)(ae - v));
-            if (!z) return fmt_error(f, "unterminated string tag");
+            if (!z) return bad(why, "unterminated string tag");
             sz = (size_t)(z - v) + 1;
             break;
         }
         case 'B': {
-            if (v + 5 > ae) return fmt_error(f, "truncated array tag");
+            if (v + 5 > ae) return bad(why, "truncated array tag");
             const char sub = v[0];
             const uint32_t cnt = rd_le<uint32_t>(v + 1);
             const size_t es = (sub == 'c' || sub == 'C') ? 1 : (sub == 's' || sub == 'S') ? 2 : 4;
             sz = 5 + (size_t)cnt * es;
             break;
         }
-        default: return fmt_error(f, "unknown tag type");
+        default: return bad(why, "unknown tag type");
         }
-        if (v + sz > ae) return fmt_error(f, "truncated tag");
+        if (v + sz > ae) return bad(why, "truncated tag");
         if (t0 == 'R' && t1 == 'G') {
             if (ty == 'Z') {
                 rec.has_rg = true;
@@ -917,19 +975,26 @@ int32_t bam_endpos(const Record &r)
     return r.pos + 1;
 }
 
-// alignment_file::next_batch body for one record (loader/alignments.cu:250-404)
-fp_status append(fp_alignment_file *f, const Record &r)
+// rg_name of a record: an index into fp_alignment_file::rg_names, or one of these
+constexpr uint32_t RGN_ABSENT = 0xffffffffu;    // no RG tag (or one not of type Z)
+constexpr uint32_t RGN_UNKNOWN = 0xfffffffeu;   // an RG tag naming no @RG record
+
+// alignment_file::next_batch body for one record (loader/alignments.cu:250-404), into b; the read
+// group is left as rg_name, and becomes an id by first appearance when the batch is assembled
+fp_status append_local(const fp_alignment_file *f, const Record &r, BatchStore &b, uint32_t &rg_name,
+                       std::string &why)
 {
-    BatchStore &b = f->batch[f->cur];
     const uint32_t ntarget = (uint32_t)f->target_name.size();
     // chromosome: contig id of the record's reference name; a named contig missing from the
     // reference is an error, no contig at all loads as 0xFFFF for the filters to drop
     uint16_t chrom = 0xffff;
     if (r.tid >= 0 && (uint32_t)r.tid < ntarget) {
         const uint32_t id = f->target_contig[(uint32_t)r.tid];
-        if (id == UINT32_MAX)
-            return fail(FP_ERR_FORMAT, "sequence " + f->target_name[(uint32_t)r.tid] + " not found in reference file");
-        if (id >= 0xffff) return fail(FP_ERR_UNSUPPORTED, "more than 65535 reference contigs");
+        if (id == UINT32_MAX) {
+            why = "sequence " + f->target_name[(uint32_t)r.tid] + " not found in reference file";
+            return FP_ERR_FORMAT;
+        }
+        if (id >= 0xffff) { why = "more than 65535 reference contigs"; return FP_ERR_UNSUPPORTED; }
         chrom = (uint16_t)id;
     }
     b.chromosome.push_back(chrom);
@@ -973,17 +1038,13 @@ fp_status append(fp_alignment_file *f, const Record &r)
     // the twelve SAM flag bits map one to one onto AlignmentFlags (loader/alignments.cu:214-255)
     b.flags.push_back((uint16_t)(r.flag & 0xfff));
     b.mapq.push_back((uint8_t)r.mapq);
-    // read group: RG tag -> @RG name -> id by first appearance (loader/alignments.cu:383-403)
-    uint32_t rg = UINT32_MAX;
+    // read group: RG tag -> @RG name (loader/alignments.cu:383-403)
+    rg_name = RGN_ABSENT;
     if (r.has_rg) {
         auto it = f->rg_id_to_name.find(r.rg);
-        if (it == f->rg_id_to_name.end()) {
-            f->unknown_rg++;
-        } else {
-            rg = f->read_groups.insert(it->second);
-        }
+        rg_name = it == f->rg_id_to_name.end() ? RGN_UNKNOWN : it->second;
     }
-    b.read_group.push_back(rg);
+    b.read_group.push_back(UINT32_MAX);
     return FP_OK;
 }
 
@@ -1007,53 +1068,137 @@ extern "C" fp_status fp_alignments_open(const char *path, const fp_sequence_db *
     }
     build_rg_map(f);
     map_targets(f);
+    // parsing threads: FP_IO_THREADS if set, else the host's CPUs (at most 16)
+    const unsigned hw = std::thread::hardware_concurrency();
+    f->threads = hw == 0 ? 4 : (hw > 16 ? 16 : hw);
+    if (const char *env = std::getenv("FP_IO_THREADS")) {
+        const long v = std::strtol(env, nullptr, 10);
+        if (v >= 1 && v <= 64) f->threads = (unsigned)v;
+    }
     *out = f;
     return FP_OK;
 }
 
+// alignment_file::next_batch (loader/alignments.cu:238-407).  The records' byte spans are found in
+// order; their fields are then parsed on several threads, each into its own arrays, which are joined
+// in record order, where the read groups get their ids by first appearance.
 extern "C" fp_status fp_alignments_next_batch(fp_alignment_file *f, uint32_t batch_size, fp_batch_view *out)
 {
     if (!f || !out) return fail(FP_ERR_INVALID_ARG, "null argument");
     f->cur ^= 1;
     BatchStore &b = f->batch[f->cur];
     b.clear();
-    for (uint32_t i = 0; i < batch_size; i++) {
-        fp_status rc;
+    Reader &rd = f->rd;
+    f->span_off.clear();
+    f->span_len.clear();
+    f->span_line.clear();
+    size_t c = 0;
+    while (f->span_off.size() < batch_size) {
         if (f->bam) {
-            Reader &r = f->rd;
-            if (r.want(4) < 4) break;
-            const int32_t bs = rd_le<int32_t>(r.ptr());
-            if (bs < 0 || r.want(4 + (size_t)bs) < 4 + (size_t)bs) {
-                b.clear();
+            if (rd.want(c + 4) < c + 4) break;
+            const int32_t bs = rd_le<int32_t>(rd.ptr() + c);
+            if (bs < 0 || rd.want(c + 4 + (size_t)bs) < c + 4 + (size_t)bs) {
                 b.view(out);
-                return fmt_error(f, "truncated record");
+                return fail(FP_ERR_FORMAT, rd.path() + " record " + std::to_string(f->records + f->span_off.size() + 1) +
+                                               ": truncated record");
             }
-            rc = parse_bam(f, r.ptr() + 4, (size_t)bs, f->rec);
-            if (rc == FP_OK) rc = append(f, f->rec);   // while the record is still in the buffer
-            r.skip(4 + (size_t)bs);
-            if (rc == FP_OK) { f->records++; continue; }
-            b.clear();
-            b.view(out);
-            return rc;
+            f->span_off.push_back(c + 4);
+            f->span_len.push_back((size_t)bs);
+            c += 4 + (size_t)bs;
         } else {
-            if (!f->rd.line(f->line)) break;
+            size_t len, next;
+            if (!rd.line_at(c, len, next)) break;
             f->lineno++;
-            if (f->line.empty()) { i--; continue; }
-            rc = parse_sam(f, f->line, f->rec);
+            if (len) {
+                f->span_off.push_back(c);
+                f->span_len.push_back(len);
+                f->span_line.push_back(f->lineno);
+            }
+            c = next;
         }
-        if (rc == FP_OK) rc = append(f, f->rec);
-        if (rc != FP_OK) {
-            b.clear();
-            b.view(out);
-            return rc;
-        }
-        f->records++;
     }
-    if (f->rd.error()) {
+    if (rd.error()) {
+        b.view(out);
+        return fail(FP_ERR_IO, rd.path() + ": read error");
+    }
+    const size_t nrec = f->span_off.size();
+    const unsigned T = nrec >= 4096 ? f->threads : 1u;
+    if (f->part.size() < T) { f->part.resize(T); f->part_rg.resize(T); }
+    std::vector<size_t> fail_at(T, SIZE_MAX);
+    std::vector<fp_status> fail_rc(T, FP_OK);
+    std::vector<std::string> fail_why(T);
+    const char *base = rd.ptr();
+    auto work = [&](unsigned t) {
+        const size_t k0 = nrec * t / T, k1 = nrec * (t + 1) / T;
+        BatchStore &lb = f->part[t];
+        std::vector<uint32_t> &lrg = f->part_rg[t];
+        lb.clear();
+        lrg.clear();
+        Record rec;
+        std::string line, why;
+        for (size_t k = k0; k < k1; k++) {
+            fp_status rc;
+            if (f->bam) {
+                rc = parse_bam(base + f->span_off[k], f->span_len[k], rec, why);
+            } else {
+                line.assign(base + f->span_off[k], f->span_len[k]);
+                rc = parse_sam(f, line, rec, why);
+            }
+            uint32_t rgn = RGN_ABSENT;
+            if (rc == FP_OK) rc = append_local(f, rec, lb, rgn, why);
+            if (rc != FP_OK) { fail_at[t] = k; fail_rc[t] = rc; fail_why[t] = why; return; }
+            lrg.push_back(rgn);
+        }
+    };
+    {
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < T; t++) pool.emplace_back(work, t);
+        work(0);
+        for (auto &th : pool) th.join();
+    }
+    rd.skip(c);
+    for (unsigned t = 0; t < T; t++) {
+        if (fail_at[t] == SIZE_MAX) continue;
+        const size_t k = fail_at[t];
+        const std::string where = f->bam ? " record " + std::to_string(f->records + k + 1)
+                                         : ":" + std::to_string(f->span_line[k]);
         b.clear();
         b.view(out);
-        return fail(FP_ERR_IO, f->rd.path() + ": read error");
+        return fail(fail_rc[t], rd.path() + where + ": " + fail_why[t]);
     }
+    // join the threads' arrays in record order
+    for (unsigned t = 0; t < T; t++) {
+        const BatchStore &L = f->part[t];
+        const uint32_t co = (uint32_t)b.cigars.size(), qo = (uint32_t)b.qualities.size();
+        const uint32_t bo = (uint32_t)b.read_bases;
+        auto cat = [](auto &dst, const auto &src) { dst.insert(dst.end(), src.begin(), src.end()); };
+        cat(b.chromosome, L.chromosome);
+        cat(b.alignment_start, L.alignment_start);
+        cat(b.alignment_stop, L.alignment_stop);
+        cat(b.mate_chromosome, L.mate_chromosome);
+        cat(b.mate_alignment_start, L.mate_alignment_start);
+        cat(b.inferred_insert_size, L.inferred_insert_size);
+        cat(b.cigar_len, L.cigar_len);
+        cat(b.read_len, L.read_len);
+        cat(b.qual_len, L.qual_len);
+        cat(b.flags, L.flags);
+        cat(b.mapq, L.mapq);
+        for (uint32_t x : L.cigar_start) b.cigar_start.push_back(x + co);
+        for (uint32_t x : L.read_start) b.read_start.push_back(x + bo);
+        for (uint32_t x : L.qual_start) b.qual_start.push_back(x + qo);
+        cat(b.cigars, L.cigars);
+        cat(b.reads, L.reads);
+        cat(b.qualities, L.qualities);
+        b.read_bases += L.read_bases;
+        // read groups: ids by first appearance (loader/alignments.cu:396-401)
+        for (uint32_t rgn : f->part_rg[t]) {
+            uint32_t id = UINT32_MAX;
+            if (rgn == RGN_UNKNOWN) f->unknown_rg++;
+            else if (rgn != RGN_ABSENT) id = f->read_groups.insert(f->rg_names[rgn]);
+            b.read_group.push_back(id);
+        }
+    }
+    f->records += nrec;
     b.view(out);
     return FP_OK;
 }
