@@ -460,19 +460,61 @@ __device__ inline uint64_t wave_sum(uint64_t v)
     return v;
 }
 
-// one read on one wavefront (clipped window of at most 64 * NC bases); r is wave-uniform
+// the per-read fields a wave needs before it touches a read's bases.  A wave loads them for up
+// to 64 reads at once, one read per lane, and takes read k's with readlane: one round trip per
+// 64 reads instead of a chain of dependent loads in front of every read.
+struct ReadMeta {
+    uint32_t r, rw, rs, qs, len, cs, cl, flags;
+};
+
+__device__ inline ReadMeta meta_lane(const CovArgs &a, uint32_t li, bool v)
+{
+    ReadMeta m{};
+    if (v) {
+        m.r = a.rg_list[li];
+        m.rw = a.s.rwc[m.r];
+        m.rs = a.b.read_start[m.r];
+        m.qs = a.b.qual_start[m.r];
+        m.len = a.b.read_len[m.r];
+        m.cs = a.b.cigar_start[m.r];
+        m.cl = a.b.cigar_len[m.r];
+        m.flags = a.b.flags[m.r];
+    }
+    return m;
+}
+
+__device__ inline ReadMeta meta_of(const ReadMeta &m, uint32_t k)
+{
+    ReadMeta u;
+    u.r = __builtin_amdgcn_readlane(m.r, k);
+    u.rw = __builtin_amdgcn_readlane(m.rw, k);
+    u.rs = __builtin_amdgcn_readlane(m.rs, k);
+    u.qs = __builtin_amdgcn_readlane(m.qs, k);
+    u.len = __builtin_amdgcn_readlane(m.len, k);
+    u.cs = __builtin_amdgcn_readlane(m.cs, k);
+    u.cl = __builtin_amdgcn_readlane(m.cl, k);
+    u.flags = __builtin_amdgcn_readlane(m.flags, k);
+    return u;
+}
+
+// one read on one wavefront (clipped window of at most 64 * NC bases); m is wave-uniform
 template <bool PAIRS, int NC>
-__device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegion &pr, uint8_t *stage, uint32_t r,
-                              uint32_t rg, uint32_t ncodes)
+__device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegion &pr, uint8_t *stage,
+                              const ReadMeta &m, uint32_t rg, uint32_t ncodes)
 {
     const BatchDev &b = a.b;
     const int lane = threadIdx.x & 63;
-    const uint32_t rw = a.s.rwc[r];
+    const uint32_t r = m.r;
+    const uint32_t rw = m.rw;
     const uint16_t rwx = lo16(rw), rwy = hi16(rw);
-    const uint32_t rs = b.read_start[r];
-    const uint32_t qs = b.qual_start[r];
-    const uint32_t L_read = b.read_len[r];
+    const uint32_t rs = m.rs;
+    const uint32_t qs = m.qs;
+    const uint32_t L_read = m.len;
     const int len = (int)rwy - (int)rwx + 1;
+    // the read's CIGAR ops, one per lane, loaded beside its bases (longer CIGARs are walked from memory)
+    const uint32_t *ops = b.cigars + m.cs;
+    const uint32_t nops = m.cl;
+    const uint32_t op_l = (uint32_t)lane < nops ? ops[lane] : 0u;
 
     // zero the fractional errors outside the window (defined output for every quality of the read)
     if (a.keep_frac)
@@ -491,6 +533,10 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
         fl[c] = v ? a.s.base_flags[rs + p] : 0;
         qv[c] = v ? b.quals[qs + p] : 255;
         if (v) stage[j] = read_base(b.reads, (uint64_t)rs + p);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const bool v = lane + 64 * c < len;
         IB[c] = __ballot(v && tag[c] != 64);
         ES[c] = __ballot(v && (fl[c] & BF_SNP));
         EI_[c] = __ballot(v && (fl[c] & BF_INS));
@@ -505,16 +551,14 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
     const uint16_t hx = (uint16_t)(rwx + hxr), hy = (uint16_t)(rwx + hyr);
     if (lane == 0) a.s.hqw[r] = pack2(hx, hy);
 
-    const uint16_t flags = b.flags[r];
+    const uint16_t flags = (uint16_t)m.flags;
     const bool neg = (flags & F_REVERSE) != 0;
     // soft-clipped window positions (S ops emit nothing), from a wave-uniform walk of the CIGAR
     uint64_t SM[4] = {0, 0, 0, 0};
     {
-        const uint32_t *ops = b.cigars + b.cigar_start[r];
-        const uint32_t nops = b.cigar_len[r];
         int ro = 0;
         for (uint32_t oi = 0; oi < nops; oi++) {
-            const uint32_t w = ops[oi];
+            const uint32_t w = oi < 64 ? __builtin_amdgcn_readlane(op_l, oi) : ops[oi];
             const uint32_t op = op_code(w), l = op_len(w);
             if (!op_reads(op)) continue;
             if (op_event(op) == EV_S) mset_range(SM, ro - (int)rwx, ro + (int)l - (int)rwx);
@@ -593,22 +637,29 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int j = lane + 64 * c;
-        if (j >= len) continue;
-        double vs, vi, vd;
-        frac_at(j, c, vs, vi, vd);
-        const uint16_t bp = (uint16_t)(rwx + j);
-        if (fl[c] & BF_MASKED) continue;
-        if (bit_at(SM, c)) continue;
-        const Limbs f0 = to_limbs(vs, a.frac_f64), f1 = to_limbs(vi, a.frac_f64), f2 = to_limbs(vd, a.frac_f64);
-        nid++;
-        i0 += f1.p0; i1 += f1.p1; i2 += f1.p2;
-        d0 += f2.p0; d1 += f2.p1; d2 += f2.p2;
-        emit_base(a, L, rg, ncodes, flags, neg, base, rwx, rwy, hx, hy, bp, qv[c], f0, f1, f2, false);
+        bool em = false;
+        if (j < len) {
+            double vs, vi, vd;
+            frac_at(j, c, vs, vi, vd);
+            const uint16_t bp = (uint16_t)(rwx + j);
+            if (!(fl[c] & BF_MASKED) && !bit_at(SM, c)) {
+                const Limbs f0 = to_limbs(vs, a.frac_f64), f1 = to_limbs(vi, a.frac_f64),
+                            f2 = to_limbs(vd, a.frac_f64);
+                em = true;
+                i0 += f1.p0; i1 += f1.p1; i2 += f1.p2;
+                d0 += f2.p0; d1 += f2.p1; d2 += f2.p2;
+                emit_base(a, L, rg, ncodes, flags, neg, base, rwx, rwy, hx, hy, bp, qv[c], f0, f1, f2, false);
+            }
+        }
+        nid += (uint32_t)__popcll(__ballot(em));
     }
-    // the quality-table I and D keys are the same for every base of the read: one LDS update per read
-    nid = (uint32_t)wave_sum(nid);
-    const Limbs ti = {wave_sum(i0), wave_sum(i1), wave_sum(i2)};
-    const Limbs td = {wave_sum(d0), wave_sum(d1), wave_sum(d2)};
+    // the quality-table I and D keys are the same for every base of the read: one LDS update per read;
+    // most reads have no insertion or deletion error anywhere and skip the cross-lane sums
+    Limbs ti = {0, 0, 0}, td = {0, 0, 0};
+    if (__ballot((i0 | i1 | i2 | d0 | d1 | d2) != 0)) {
+        ti = {wave_sum(i0), wave_sum(i1), wave_sum(i2)};
+        td = {wave_sum(d0), wave_sum(d1), wave_sum(d2)};
+    }
     if (lane == 0 && nid) {
         lds_add_n(L, 64, nid, ti);
         lds_add_sums(L, 65, td);
@@ -652,7 +703,7 @@ __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
     const uint32_t first = a.rg_offset[rg] + tile * a.tile;
     const uint32_t last = min(a.rg_offset[rg + 1], first + a.tile);
 
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if constexpr (PAIRS) {
         // this tile's slice of each table's pair array: 3 slots per window base bound its pairs
         if (threadIdx.x == 0) { pr_bound = 0; pr_over = 0; }
@@ -684,13 +735,19 @@ __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
         __syncthreads();
     }
 
-    for (uint32_t li = first + wave; li < last; li += COV_WAVES) {
-        const uint32_t r = __builtin_amdgcn_readfirstlane(a.rg_list[li]);
-        const uint32_t rw = a.s.rwc[r];
-        const int len = (int)hi16(rw) - (int)lo16(rw) + 1;
-        if (len <= 128) cov_read_wave<PAIRS, 2>(a, L, pr, stage, r, rg, ncodes);
-        else if (len <= 256) cov_read_wave<PAIRS, 4>(a, L, pr, stage, r, rg, ncodes);
-        else if (lane == 0) cov_read_serial<PAIRS>(a, L, pr, r, rg, ncodes);
+    // each wave takes a contiguous run of the tile's reads, 64 reads' fields at a time
+    const uint32_t per = (last - first + COV_WAVES - 1) / COV_WAVES;
+    const uint32_t w0 = first + wave * per, w1 = min(last, w0 + per);
+    for (uint32_t g = w0; g < w1; g += 64) {
+        const uint32_t cnt = min(64u, w1 - g);
+        const ReadMeta ml = meta_lane(a, g + lane, lane < cnt);
+        for (uint32_t k = 0; k < cnt; k++) {
+            const ReadMeta m = meta_of(ml, k);
+            const int len = (int)hi16(m.rw) - (int)lo16(m.rw) + 1;
+            if (len <= 128) cov_read_wave<PAIRS, 2>(a, L, pr, stage, m, rg, ncodes);
+            else if (len <= 256) cov_read_wave<PAIRS, 4>(a, L, pr, stage, m, rg, ncodes);
+            else if (lane == 0) cov_read_serial<PAIRS>(a, L, pr, m.r, rg, ncodes);
+        }
     }
     __syncthreads();
     if constexpr (PAIRS) {
