@@ -583,6 +583,116 @@ __device__ uint8_t prep_read_lane(const PrepArgs &a, uint32_t r, uint16_t &nerr)
 
 __device__ inline uint32_t rl(uint32_t v, int j) { return (uint32_t)__builtin_amdgcn_readlane((int)v, j); }
 
+// one read of the per-base pass, its fields wave-uniform
+struct PrepRead {
+    uint32_t rs, qs, ast, clen, cwo, nops, rwx, rwy, fwx, fwy, L, pj;
+    bool neg;
+    const uint32_t *ops;
+};
+
+// Flags of the read's bases (positions lane + 64c, c < NC) into its LDS row; returns the read's error
+// count (SNPs inside the read window, plus the insertions and deletions counted by the walk).
+// The bases, qualities and CIGAR ops (one per lane) are loaded together; the walk itself only
+// reads registers and records each base's reference offset, and the reference words are then
+// loaded together: two round trips per read, whatever the number of ops.
+template <int NC>
+__device__ uint32_t prep_bases(const PrepArgs &a, const PrepRead &p, uint8_t *row)
+{
+    const BatchDev &b = a.b;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t op_l = lane < p.nops ? p.ops[lane] : 0u;
+    uint32_t rb[NC];
+    uint8_t f[NC];
+    int32_t rf[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const uint32_t i = lane + 64 * c;
+        rb[c] = 0;
+        f[c] = 0;
+        rf[c] = -1;
+        if (i < p.L) {
+            rb[c] = (b.reads[(p.rs + i) >> 3] >> (((p.rs + i) & 7) * 4)) & 15;
+            const bool regular = rb[c] == B_A || rb[c] == B_C || rb[c] == B_G || rb[c] == B_T;
+            if (!regular || b.quals[p.qs + i] < 6) f[c] |= BF_MASKED;
+        }
+    }
+    // the error walk with the lanes' bases side by side: current_bp_idx (bp) and the read /
+    // reference offsets are the same in every lane
+    uint32_t cnt = 0;
+    uint16_t ro = 0, fo = 0, bp = 0;
+    for (uint32_t oi = 0; oi < p.nops; oi++) {
+        const uint32_t wop = oi < 64 ? rl(op_l, (int)oi) : p.ops[oi];
+        const uint32_t op = op_code(wop), l = op_len(wop);
+        if (op == OP_M) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const uint32_t i = lane + 64 * c;
+                if (i >= ro && i < (uint32_t)ro + l) rf[c] = (uint16_t)(fo + (i - ro));
+            }
+            if (l) bp = (uint16_t)(ro + l - 1);
+        } else if (op == OP_I) {
+            if (l) {
+                bp = ro;
+                if (bp >= p.rwx && bp <= p.rwy) {
+                    const int off = p.neg ? (int)bp + (int)l : (int)bp - 1;
+                    if (off >= 0 && off <= (int)p.rwy) {
+#pragma unroll
+                        for (int c = 0; c < NC; c++)
+                            if ((int)(lane + 64 * c) == off) f[c] |= BF_INS;
+                        cnt++;
+                    }
+                }
+            }
+        } else if (op == OP_D) {
+            if (l) {
+                const uint16_t cri = fo;
+                if (cri >= p.fwx && cri <= p.fwy) {
+                    if (!p.neg) {
+#pragma unroll
+                        for (int c = 0; c < NC; c++)
+                            if (lane + 64 * c == bp) f[c] |= BF_DEL;
+                        cnt++;
+                    } else {
+                        const uint16_t off = (uint16_t)(bp + 1);
+                        if (off < p.L) {
+#pragma unroll
+                            for (int c = 0; c < NC; c++)
+                                if (lane + 64 * c == off) f[c] |= BF_DEL;
+                            cnt++;
+                        }
+                    }
+                }
+            }
+        }
+        if (op_reads(op)) ro = (uint16_t)(ro + l);
+        if (op_refs(op)) fo = (uint16_t)(fo + l);
+    }
+    // SNPs: the aligned bases against the reference (past the contig: N)
+    uint32_t fw[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        fw[c] = 0;
+        const uint32_t pos = p.ast + (uint32_t)rf[c];
+        if (rf[c] >= 0 && pos < p.clen) fw[c] = a.g.words[p.cwo + (pos >> 3)];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const uint32_t i = lane + 64 * c;
+        bool snp_in = false;
+        if (rf[c] >= 0) {
+            const uint32_t pos = p.ast + (uint32_t)rf[c];
+            const uint32_t fb = pos < p.clen ? (fw[c] >> ((pos & 7) * 4)) & 15 : (uint32_t)B_N;
+            if (fb != rb[c]) {
+                f[c] |= BF_SNP;
+                snp_in = i >= p.rwx && i <= p.rwy;
+            }
+        }
+        cnt += (uint32_t)__popcll(__ballot(snp_in));
+        if (i < p.pj) row[i] = f[c];
+    }
+    return cnt;
+}
+
 // One wave per 64 reads.  The CIGAR walks (filters, windows, known sites, HMM window) run one lane
 // per read; the per-base work (base masks, SNP / insertion / deletion flags) runs one read at a time
 // across the wave, lanes over bases, so its loads are coalesced.  Flags are built in LDS rows of
@@ -618,6 +728,11 @@ __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
         const uint32_t neg = valid ? ((b.flags[r] & F_REVERSE) != 0) : 0u;
         const uint64_t vmask = __ballot(valid), okmask = __ballot(ok);
 
+        // the contig of each lane's read: its length and first reference word, taken by readlane below
+        const bool has_contig = valid && chrom < a.g.num_contigs;
+        const uint32_t clen = has_contig ? a.g.length[chrom] : 0u;
+        const uint32_t cwo = has_contig ? a.g.word_offset[chrom] : 0u;
+
         // per-base pass, one read at a time (read_filters.cu:376-438, cigar.cu:612-733)
         for (uint64_t m = vmask; m; m &= m - 1) {
             const int j = __builtin_ctzll(m);
@@ -627,71 +742,14 @@ __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
                 for (uint32_t i = lane; i < pj; i += 64) row[i] = 0;
                 continue;
             }
-            const uint32_t rsj = rl(rs, j), qsj = rl(qs, j), astj = rl(astart, j), chj = rl(chrom, j);
-            const uint32_t nopsj = rl(nops, j);
-            const bool negj = rl(neg, j) != 0;
-            const uint32_t *opsj = b.cigars + rl(cs, j);
-            const uint32_t rwxj = rl(w.rwx, j), rwyj = rl(w.rwy, j), fwxj = rl(w.fwx, j), fwyj = rl(w.fwy, j);
-            uint32_t cnt = 0;
-            for (uint32_t i0 = 0; i0 < pj; i0 += 64) {
-                const uint32_t i = i0 + lane;
-                uint8_t f = 0;
-                uint32_t rb = 0;
-                if (i < Lj) {
-                    rb = (b.reads[(rsj + i) >> 3] >> (((rsj + i) & 7) * 4)) & 15;
-                    const bool regular = rb == B_A || rb == B_C || rb == B_G || rb == B_T;
-                    if (!regular || b.quals[qsj + i] < 6) f |= BF_MASKED;
-                }
-                // the error walk with the lanes' bases side by side: current_bp_idx (bp) and the
-                // read / reference offsets are the same in every lane
-                uint16_t ro = 0, fo = 0, bp = 0;
-                bool snp_in = false;
-                for (uint32_t oi = 0; oi < nopsj; oi++) {
-                    const uint32_t wop = opsj[oi];
-                    const uint32_t op = op_code(wop), l = op_len(wop);
-                    if (op == OP_M) {
-                        if (i >= ro && i < (uint32_t)ro + l) {
-                            const uint8_t fb = contig_base(a.g, chj, astj + (uint16_t)(fo + (i - ro)));
-                            if (fb != rb) {
-                                f |= BF_SNP;
-                                snp_in = i >= rwxj && i <= rwyj;
-                            }
-                        }
-                        if (l) bp = (uint16_t)(ro + l - 1);
-                    } else if (op == OP_I) {
-                        if (l) {
-                            bp = ro;
-                            if (bp >= rwxj && bp <= rwyj) {
-                                const int off = negj ? (int)bp + (int)l : (int)bp - 1;
-                                if (off >= 0 && off <= (int)rwyj) {
-                                    if ((int)i == off) f |= BF_INS;
-                                    cnt += i0 == 0;
-                                }
-                            }
-                        }
-                    } else if (op == OP_D) {
-                        if (l) {
-                            const uint16_t cri = fo;
-                            if (cri >= fwxj && cri <= fwyj) {
-                                if (!negj) {
-                                    if (i == bp) f |= BF_DEL;
-                                    cnt += i0 == 0;
-                                } else {
-                                    const uint16_t off = (uint16_t)(bp + 1);
-                                    if (off < Lj) {
-                                        if (i == off) f |= BF_DEL;
-                                        cnt += i0 == 0;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    if (op_reads(op)) ro = (uint16_t)(ro + l);
-                    if (op_refs(op)) fo = (uint16_t)(fo + l);
-                }
-                cnt += (uint32_t)__popcll(__ballot(snp_in));
-                if (i < pj) row[i] = f;
-            }
+            PrepRead p;
+            p.rs = rl(rs, j); p.qs = rl(qs, j); p.ast = rl(astart, j);
+            p.clen = rl(clen, j); p.cwo = rl(cwo, j);
+            p.nops = rl(nops, j); p.neg = rl(neg, j) != 0;
+            p.ops = b.cigars + rl(cs, j);
+            p.rwx = rl(w.rwx, j); p.rwy = rl(w.rwy, j); p.fwx = rl(w.fwx, j); p.fwy = rl(w.fwy, j);
+            p.L = Lj; p.pj = pj;
+            const uint32_t cnt = pj <= 128 ? prep_bases<2>(a, p, row) : prep_bases<4>(a, p, row);
             if ((int)lane == j) nerr = (uint16_t)cnt;
         }
         __syncthreads();
