@@ -590,38 +590,54 @@ struct PrepRead {
     const uint32_t *ops;
 };
 
-// Flags of the read's bases (positions lane + 64c, c < NC) into its LDS row; returns the read's error
-// count (SNPs inside the read window, plus the insertions and deletions counted by the walk).
-// The bases, qualities and CIGAR ops (one per lane) are loaded together; the walk itself only
-// reads registers and records each base's reference offset, and the reference words are then
-// loaded together: two round trips per read, whatever the number of ops.
+// the per-base loads of one read: bases, qualities and the CIGAR ops, one op per lane
 template <int NC>
-__device__ uint32_t prep_bases(const PrepArgs &a, const PrepRead &p, uint8_t *row)
+struct BaseLoads {
+    uint32_t rb[NC], q[NC], op;
+};
+
+template <int NC>
+__device__ inline BaseLoads<NC> load_bases(const BatchDev &b, const PrepRead &p)
 {
-    const BatchDev &b = a.b;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t op_l = lane < p.nops ? p.ops[lane] : 0u;
-    uint32_t rb[NC];
-    uint8_t f[NC];
-    int32_t rf[NC];
+    BaseLoads<NC> x;
+    x.op = lane < p.nops ? p.ops[lane] : 0u;
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const uint32_t i = lane + 64 * c;
-        rb[c] = 0;
+        x.rb[c] = 0;
+        x.q[c] = 255;
+        if (i < p.L) {
+            x.rb[c] = (b.reads[(p.rs + i) >> 3] >> (((p.rs + i) & 7) * 4)) & 15;
+            x.q[c] = b.quals[p.qs + i];
+        }
+    }
+    return x;
+}
+
+// The error walk of one read with the lanes' bases side by side (positions lane + 64c): the
+// current_bp_idx (bp) and the read / reference offsets are the same in every lane, and the walk
+// only reads registers.  Sets the mask, insertion and deletion flags and each aligned base's
+// reference offset (rf, -1 when not aligned); returns the insertions and deletions counted.
+template <int NC>
+__device__ uint32_t error_walk(const PrepRead &p, const BaseLoads<NC> &x, uint8_t *f, int32_t *rf)
+{
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const uint32_t i = lane + 64 * c;
         f[c] = 0;
         rf[c] = -1;
         if (i < p.L) {
-            rb[c] = (b.reads[(p.rs + i) >> 3] >> (((p.rs + i) & 7) * 4)) & 15;
-            const bool regular = rb[c] == B_A || rb[c] == B_C || rb[c] == B_G || rb[c] == B_T;
-            if (!regular || b.quals[p.qs + i] < 6) f[c] |= BF_MASKED;
+            const uint32_t rb = x.rb[c];
+            const bool regular = rb == B_A || rb == B_C || rb == B_G || rb == B_T;
+            if (!regular || x.q[c] < 6) f[c] |= BF_MASKED;
         }
     }
-    // the error walk with the lanes' bases side by side: current_bp_idx (bp) and the read /
-    // reference offsets are the same in every lane
     uint32_t cnt = 0;
     uint16_t ro = 0, fo = 0, bp = 0;
     for (uint32_t oi = 0; oi < p.nops; oi++) {
-        const uint32_t wop = oi < 64 ? rl(op_l, (int)oi) : p.ops[oi];
+        const uint32_t wop = oi < 64 ? rl(x.op, (int)oi) : p.ops[oi];
         const uint32_t op = op_code(wop), l = op_len(wop);
         if (op == OP_M) {
 #pragma unroll
@@ -667,30 +683,64 @@ __device__ uint32_t prep_bases(const PrepArgs &a, const PrepRead &p, uint8_t *ro
         if (op_reads(op)) ro = (uint16_t)(ro + l);
         if (op_refs(op)) fo = (uint16_t)(fo + l);
     }
-    // SNPs: the aligned bases against the reference (past the contig: N)
-    uint32_t fw[NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        fw[c] = 0;
-        const uint32_t pos = p.ast + (uint32_t)rf[c];
-        if (rf[c] >= 0 && pos < p.clen) fw[c] = a.g.words[p.cwo + (pos >> 3)];
-    }
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-        const uint32_t i = lane + 64 * c;
-        bool snp_in = false;
-        if (rf[c] >= 0) {
-            const uint32_t pos = p.ast + (uint32_t)rf[c];
-            const uint32_t fb = pos < p.clen ? (fw[c] >> ((pos & 7) * 4)) & 15 : (uint32_t)B_N;
-            if (fb != rb[c]) {
-                f[c] |= BF_SNP;
-                snp_in = i >= p.rwx && i <= p.rwy;
-            }
-        }
-        cnt += (uint32_t)__popcll(__ballot(snp_in));
-        if (i < p.pj) row[i] = f[c];
-    }
     return cnt;
+}
+
+// The per-base pass over the wave's reads that passed the filters (read_filters.cu:376-438,
+// cigar.cu:612-733), one read at a time across the wave: flags into the read's LDS row and its
+// error count (SNPs inside the read window plus the walk's insertions and deletions) into nerr of
+// the read's lane.  Software-pipelined: the next read's bases, qualities and ops are in flight
+// while the current read's reference words return and are compared, so a read costs about one
+// round trip to memory.
+template <int NC, class ReadOf>
+__device__ void prep_pass(const PrepArgs &a, uint64_t okmask, ReadOf read_of, uint8_t *rows, uint32_t RL,
+                          uint16_t &nerr)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t m = okmask;
+    if (!m) return;
+    int j = __builtin_ctzll(m);
+    PrepRead p = read_of(j);
+    BaseLoads<NC> cur = load_bases<NC>(a.b, p);
+    for (;;) {
+        uint8_t f[NC];
+        int32_t rf[NC];
+        uint32_t cnt = error_walk<NC>(p, cur, f, rf);
+        // the reference words of the aligned bases (past the contig: N)
+        uint32_t fw[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            fw[c] = 0;
+            const uint32_t pos = p.ast + (uint32_t)rf[c];
+            if (rf[c] >= 0 && pos < p.clen) fw[c] = a.g.words[p.cwo + (pos >> 3)];
+        }
+        m &= m - 1;
+        const bool more = m != 0;
+        const int jn = more ? __builtin_ctzll(m) : j;
+        const PrepRead pn = read_of(jn);
+        const BaseLoads<NC> nxt = load_bases<NC>(a.b, pn);   // in flight during the comparisons
+        uint8_t *row = rows + (uint32_t)j * RL;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const uint32_t i = lane + 64 * c;
+            bool snp_in = false;
+            if (rf[c] >= 0) {
+                const uint32_t pos = p.ast + (uint32_t)rf[c];
+                const uint32_t fb = pos < p.clen ? (fw[c] >> ((pos & 7) * 4)) & 15 : (uint32_t)B_N;
+                if (fb != cur.rb[c]) {
+                    f[c] |= BF_SNP;
+                    snp_in = i >= p.rwx && i <= p.rwy;
+                }
+            }
+            cnt += (uint32_t)__popcll(__ballot(snp_in));
+            if (i < p.pj) row[i] = f[c];
+        }
+        if ((int)lane == j) nerr = (uint16_t)cnt;
+        if (!more) break;
+        p = pn;
+        cur = nxt;
+        j = jn;
+    }
 }
 
 // One wave per 64 reads.  The CIGAR walks (filters, windows, known sites, HMM window) run one lane
@@ -733,25 +783,25 @@ __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
         const uint32_t clen = has_contig ? a.g.length[chrom] : 0u;
         const uint32_t cwo = has_contig ? a.g.word_offset[chrom] : 0u;
 
-        // per-base pass, one read at a time (read_filters.cu:376-438, cigar.cu:612-733)
-        for (uint64_t m = vmask; m; m &= m - 1) {
+        // rows of valid reads that the filters dropped are all zero
+        for (uint64_t m = vmask & ~okmask; m; m &= m - 1) {
             const int j = __builtin_ctzll(m);
-            const uint32_t Lj = rl(L, j), pj = (Lj + 7) & ~7u;
+            const uint32_t pj = (rl(L, j) + 7) & ~7u;
             uint8_t *row = prep_rows + (uint32_t)j * RL;
-            if (!((okmask >> j) & 1)) {
-                for (uint32_t i = lane; i < pj; i += 64) row[i] = 0;
-                continue;
-            }
+            for (uint32_t i = lane; i < pj; i += 64) row[i] = 0;
+        }
+        auto read_of = [&](int j) {
             PrepRead p;
             p.rs = rl(rs, j); p.qs = rl(qs, j); p.ast = rl(astart, j);
             p.clen = rl(clen, j); p.cwo = rl(cwo, j);
             p.nops = rl(nops, j); p.neg = rl(neg, j) != 0;
             p.ops = b.cigars + rl(cs, j);
             p.rwx = rl(w.rwx, j); p.rwy = rl(w.rwy, j); p.fwx = rl(w.fwx, j); p.fwy = rl(w.fwy, j);
-            p.L = Lj; p.pj = pj;
-            const uint32_t cnt = pj <= 128 ? prep_bases<2>(a, p, row) : prep_bases<4>(a, p, row);
-            if ((int)lane == j) nerr = (uint16_t)cnt;
-        }
+            p.L = rl(L, j); p.pj = (p.L + 7) & ~7u;
+            return p;
+        };
+        if (!__any(ok && L > 128)) prep_pass<2>(a, okmask, read_of, prep_rows, RL, nerr);
+        else prep_pass<4>(a, okmask, read_of, prep_rows, RL, nerr);
         __syncthreads();
 
         if (ok) {
