@@ -218,24 +218,32 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(SortPass p)
     block_range(n, blockIdx.x, gridDim.x, lo, hi);
     if (lo >= hi) return;
     for (uint32_t d = threadIdx.x; d < R; d += SORT_THREADS) gofs[d] = p.hist[(uint64_t)d * gridDim.x + blockIdx.x];
+    // element order inside a tile is (wave, item, lane); the next tile's elements are loaded while
+    // this one is ranked, reordered and written
+    uint32_t key[SORT_ITEMS], nkey[SORT_ITEMS];
+    double val[SORT_ITEMS], nval[SORT_ITEMS];
+    auto load_tile = [&](uint32_t base, uint32_t (&k_)[SORT_ITEMS], double (&v_)[SORT_ITEMS]) {
+        const uint32_t t0 = base + wave * (SORT_ITEMS * 64);
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
+            const uint32_t i = t0 + k * 64 + lane;
+            k_[k] = i < hi ? p.kin[i] : SORT_SENTINEL;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
+            const uint32_t i = t0 + k * 64 + lane;
+            v_[k] = i < hi ? p.vin[i] : 0.0;
+        }
+    };
+    load_tile(lo, key, val);
     for (uint32_t base = lo; base < hi; base += SORT_TILE) {
         for (uint32_t i = threadIdx.x; i < SORT_WAVES * 256; i += SORT_THREADS) (&cnt[0][0])[i] = 0;
+        if (base + SORT_TILE < hi) load_tile(base + SORT_TILE, nkey, nval);
         __syncthreads();
-        // stable ranks: element order inside the tile is (wave, item, lane)
+        // stable ranks
         const uint32_t t0 = base + wave * (SORT_ITEMS * 64);
-        uint32_t key[SORT_ITEMS], rank[SORT_ITEMS];
-        double val[SORT_ITEMS];
+        uint32_t rank[SORT_ITEMS];
         uint32_t vmask = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
-            const uint32_t i = t0 + k * 64 + lane;
-            key[k] = i < hi ? p.kin[i] : SORT_SENTINEL;
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < SORT_ITEMS; k++) {
-            const uint32_t i = t0 + k * 64 + lane;
-            val[k] = i < hi ? p.vin[i] : 0.0;
-        }
 #pragma unroll
         for (uint32_t k = 0; k < SORT_ITEMS; k++) {
             const uint32_t i = t0 + k * 64 + lane;
@@ -289,6 +297,8 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(SortPass p)
         }
         __syncthreads();
         if (threadIdx.x < R) gofs[threadIdx.x] += tot;
+#pragma unroll
+        for (uint32_t k = 0; k < SORT_ITEMS; k++) { key[k] = nkey[k]; val[k] = nval[k]; }
     }
 }
 
@@ -331,21 +341,31 @@ __global__ void __launch_bounds__(64) k_seg_reduce(ReduceArgs a)
     const uint32_t chunks = (n + CE - 1) / CE;
     const uint32_t lane = threadIdx.x & 63;
     auto slot = [](uint32_t e) { return e + e / RED_LANE; };
+    // a full chunk's pairs in registers: the next chunk is loaded while this one is summed
+    uint4 rk[CE / 256];
+    double2 rv[CE / 128];
+    auto load_chunk = [&](uint32_t base) {
+        const uint4 *k4 = (const uint4 *)(a.keys + base);
+        const double2 *v2 = (const double2 *)(a.vals + base);
+#pragma unroll
+        for (uint32_t i = 0; i < CE / 256; i++) rk[i] = k4[lane + 64 * i];
+#pragma unroll
+        for (uint32_t i = 0; i < CE / 128; i++) rv[i] = v2[lane + 64 * i];
+    };
+    if (blockIdx.x < chunks && (uint64_t)(blockIdx.x + 1) * CE <= n) load_chunk(blockIdx.x * CE);
     for (uint32_t chunk = blockIdx.x; chunk < chunks; chunk += gridDim.x) {
         const uint32_t base = chunk * CE;
         if (base + CE <= n) {
-            const uint4 *k4 = (const uint4 *)(a.keys + base);
-            const double2 *v2 = (const double2 *)(a.vals + base);
 #pragma unroll
             for (uint32_t i = 0; i < CE / 256; i++) {
                 const uint32_t e = 4 * (lane + 64 * i);
-                const uint4 k = k4[lane + 64 * i];
+                const uint4 k = rk[i];
                 sk[slot(e)] = k.x; sk[slot(e + 1)] = k.y; sk[slot(e + 2)] = k.z; sk[slot(e + 3)] = k.w;
             }
 #pragma unroll
             for (uint32_t i = 0; i < CE / 128; i++) {
                 const uint32_t e = 2 * (lane + 64 * i);
-                const double2 v = v2[lane + 64 * i];
+                const double2 v = rv[i];
                 sv[slot(e)] = v.x; sv[slot(e + 1)] = v.y;
             }
         } else {
@@ -356,6 +376,10 @@ __global__ void __launch_bounds__(64) k_seg_reduce(ReduceArgs a)
             }
         }
         __syncthreads();
+        {
+            const uint32_t next = chunk + gridDim.x;
+            if (next < chunks && (uint64_t)(next + 1) * CE <= n) load_chunk(next * CE);
+        }
         // this lane's slice, summed run by run
         uint32_t hk = SORT_SENTINEL, cur = SORT_SENTINEL;
         RunSum hs = {0, 0, 0, 0}, cs = {0, 0, 0, 0};
