@@ -338,7 +338,7 @@ __device__ inline void emit_pairs(const CovArgs &a, const PairRegion &pr, uint32
 
 // one read on one lane (windows longer than 256 bases)
 template <bool PAIRS>
-__device__ void cov_read_serial(const CovArgs &a, const LdsAcc &L, const PairRegion &pr, uint32_t r, uint32_t rg,
+__device__ __forceinline__ void cov_read_serial(const CovArgs &a, const LdsAcc &L, const PairRegion &pr, uint32_t r, uint32_t rg,
                                 uint32_t ncodes)
 {
     const BatchDev &b = a.b;
@@ -497,10 +497,40 @@ __device__ inline ReadMeta meta_of(const ReadMeta &m, uint32_t k)
     return u;
 }
 
+// the per-base inputs of one read's window positions lane + 64c (c < NC) and its CIGAR ops, one
+// per lane: issued for the next read while the current one is processed
+template <int NC>
+struct CovLoads {
+    uint32_t tag[NC], fl[NC], qv[NC], rb[NC];
+    uint32_t op;
+};
+
+template <int NC>
+__device__ inline void cov_load(const CovArgs &a, const ReadMeta &m, CovLoads<NC> &x)
+{
+    const BatchDev &b = a.b;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t rwx = lo16(m.rw);
+    const int len = (int)hi16(m.rw) - (int)rwx + 1;
+    x.op = lane < m.cl ? b.cigars[m.cs + lane] : 0u;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const int j = (int)lane + 64 * c;
+        const bool v = j < len;
+        const uint32_t p = rwx + j;
+        x.tag[c] = v ? a.s.baq[m.qs + p] : 64u;
+        x.fl[c] = v ? a.s.base_flags[m.rs + p] : 0u;
+        x.qv[c] = v ? b.quals[m.qs + p] : 255u;
+        x.rb[c] = v ? read_base(b.reads, (uint64_t)m.rs + p) : 0u;
+    }
+}
+
 // one read on one wavefront (clipped window of at most 64 * NC bases); m is wave-uniform
-template <bool PAIRS, int NC>
-__device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegion &pr, uint8_t *stage,
-                              const ReadMeta &m, uint32_t rg, uint32_t ncodes)
+// (x: its loads; with PF, once its masks are built, the loads of read mn start into nx)
+template <bool PAIRS, int NC, bool PF>
+__device__ __forceinline__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegion &pr, uint8_t *stage,
+                                              const ReadMeta &m, const CovLoads<NC> &x, uint32_t rg, uint32_t ncodes,
+                                              const ReadMeta &mn, CovLoads<NC> &nx)
 {
     const BatchDev &b = a.b;
     const int lane = threadIdx.x & 63;
@@ -514,7 +544,7 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
     // the read's CIGAR ops, one per lane, loaded beside its bases (longer CIGARs are walked from memory)
     const uint32_t *ops = b.cigars + m.cs;
     const uint32_t nops = m.cl;
-    const uint32_t op_l = (uint32_t)lane < nops ? ops[lane] : 0u;
+    const uint32_t op_l = x.op;
 
     // zero the fractional errors outside the window (defined output for every quality of the read)
     if (a.keep_frac)
@@ -527,12 +557,10 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int j = lane + 64 * c;
-        const bool v = j < len;
-        const uint32_t p = rwx + j;
-        tag[c] = v ? a.s.baq[qs + p] : 64;
-        fl[c] = v ? a.s.base_flags[rs + p] : 0;
-        qv[c] = v ? b.quals[qs + p] : 255;
-        if (v) stage[j] = read_base(b.reads, (uint64_t)rs + p);
+        tag[c] = (uint8_t)x.tag[c];
+        fl[c] = (uint8_t)x.fl[c];
+        qv[c] = (uint8_t)x.qv[c];
+        if (j < len) stage[j] = (uint8_t)x.rb[c];
     }
 #pragma unroll
     for (int c = 0; c < NC; c++) {
@@ -550,6 +578,7 @@ __device__ void cov_read_wave(const CovArgs &a, const LdsAcc &L, const PairRegio
     if (hyr < hxr) hyr = hxr;
     const uint16_t hx = (uint16_t)(rwx + hxr), hy = (uint16_t)(rwx + hyr);
     if (lane == 0) a.s.hqw[r] = pack2(hx, hy);
+    if constexpr (PF) cov_load<NC>(a, mn, nx);
 
     const uint16_t flags = (uint16_t)m.flags;
     const bool neg = (flags & F_REVERSE) != 0;
@@ -741,12 +770,35 @@ __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
     for (uint32_t g = w0; g < w1; g += 64) {
         const uint32_t cnt = min(64u, w1 - g);
         const ReadMeta ml = meta_lane(a, g + lane, lane < cnt);
+        const int len_l = (int)hi16(ml.rw) - (int)lo16(ml.rw) + 1;
+        if (!__any(lane < cnt && len_l > 128)) {
+            // reads of up to 128 window bases: software-pipelined, the next read's inputs are
+            // in flight while this one is processed
+            ReadMeta m = meta_of(ml, 0);
+            CovLoads<2> cur, nxt;
+            cov_load<2>(a, m, cur);
+            for (uint32_t k = 0; k < cnt; k++) {
+                const ReadMeta mn = meta_of(ml, k + 1 < cnt ? k + 1 : k);
+                cov_read_wave<PAIRS, 2, true>(a, L, pr, stage, m, cur, rg, ncodes, mn, nxt);
+                m = mn;
+                cur = nxt;
+            }
+            continue;
+        }
         for (uint32_t k = 0; k < cnt; k++) {
             const ReadMeta m = meta_of(ml, k);
             const int len = (int)hi16(m.rw) - (int)lo16(m.rw) + 1;
-            if (len <= 128) cov_read_wave<PAIRS, 2>(a, L, pr, stage, m, rg, ncodes);
-            else if (len <= 256) cov_read_wave<PAIRS, 4>(a, L, pr, stage, m, rg, ncodes);
-            else if (lane == 0) cov_read_serial<PAIRS>(a, L, pr, m.r, rg, ncodes);
+            if (len <= 128) {
+                CovLoads<2> x;
+                cov_load<2>(a, m, x);
+                cov_read_wave<PAIRS, 2, false>(a, L, pr, stage, m, x, rg, ncodes, m, x);
+            } else if (len <= 256) {
+                CovLoads<4> x;
+                cov_load<4>(a, m, x);
+                cov_read_wave<PAIRS, 4, false>(a, L, pr, stage, m, x, rg, ncodes, m, x);
+            } else if (lane == 0) {
+                cov_read_serial<PAIRS>(a, L, pr, m.r, rg, ncodes);
+            }
         }
     }
     __syncthreads();
