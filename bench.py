@@ -355,6 +355,8 @@ def main():
     ap.add_argument("--served-jobs", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=400000, help="reads of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stage-reads", type=int, default=0,
+                    help="reads per staging buffer of the batched engine (0 = the engine's default, 2^20)")
     ap.add_argument("--wpe", type=int, default=0, help="BAQ waves per SIMD (0 = autotuned)")
     ap.add_argument("--ckpt", type=int, default=0,
                     help="rows per BAQ forward checkpoint, 1 or 2 (0 = autotuned; profile-only pass: 1)")
@@ -393,6 +395,8 @@ def main():
     reads = sum(d.num_reads for d in dev)
     bases = sum(d.num_bases for d in dev)
     batched_opts = dict(engine_kind=abi.FP_ENGINE_BATCHED, max_read_length=101)
+    if args.stage_reads:
+        batched_opts.update(max_batch_reads=args.stage_reads)
     if args.profile_only:
         popts = dict(batched_opts)
         if mixed_profile:

@@ -33,7 +33,7 @@ hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t *live, con
 hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const fpd::GenomeDev *g, const fpd::ReadStateDev *s,
                               const uint32_t *list, const uint32_t *count, double *scratch, uint64_t slot_doubles,
                               uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves, unsigned long long *cells,
-                              int wpe, int allow_stage, int recompute, hipStream_t st);
+                              int wpe, int allow_stage, int recompute, uint32_t *next_chunk, hipStream_t st);
 hipError_t fpd_baq_prepare();
 hipError_t fpd_launch_baq_post(const fpd::BatchDev *b, const fpd::ReadStateDev *s, uint32_t band_done,
                                hipStream_t st);
@@ -185,11 +185,18 @@ struct BatchState {
     DevBuf st_status, st_rwc, st_noins, st_refw, st_nerr, st_hmmw, st_bw, st_hqw, st_bflags, st_baq, st_state, st_frac;
     DevBuf baq_lists, baq_counts, rg_buf;
     fpd::ReadStateDev rs{};
+    // the BAQ stage's second stream: the 23- and 33-cell band classes run on it beside the 15-cell
+    // class (fork and join events around them)
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     void release()
     {
         DevBuf *all[] = {&st_status, &st_rwc, &st_noins, &st_refw, &st_nerr, &st_hmmw, &st_bw, &st_hqw,
                          &st_bflags, &st_baq, &st_state, &st_frac, &baq_lists, &baq_counts, &rg_buf};
         for (DevBuf *b : all) b->release();
+        if (side) { (void)hipStreamSynchronize(side); (void)hipStreamDestroy(side); side = nullptr; }
+        if (fork) { (void)hipEventDestroy(fork); fork = nullptr; }
+        if (join) { (void)hipEventDestroy(join); join = nullptr; }
     }
 };
 
@@ -322,6 +329,9 @@ fp_status alloc_tables(fp_engine *e)
 fp_status ensure_state(fp_engine *e, BatchState &S, uint32_t n, uint64_t nbases, uint64_t nquals,
                        bool need_frac = true)
 {
+    if (!S.side) HIP_TRY(hipStreamCreateWithFlags(&S.side, hipStreamNonBlocking));
+    if (!S.fork) HIP_TRY(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
+    if (!S.join) HIP_TRY(hipEventCreateWithFlags(&S.join, hipEventDisableTiming));
     HIP_TRY(S.st_status.ensure(n));
     HIP_TRY(S.st_rwc.ensure(4ull * n));
     HIP_TRY(S.st_noins.ensure(4ull * n));
@@ -388,6 +398,9 @@ float elapsed(hipEvent_t a, hipEvent_t b)
 }
 
 static const uint32_t BAQ_CLASS_W[4] = {15, 23, 33, 0};
+// words 8..10 of a batch's BAQ class counts (zeroed with them by the front stage): the chunk counters of
+// the three band kernels
+static constexpr uint32_t BAQ_CHUNK_COUNTERS = 8;
 
 // what the launch sequence needs to know about a batch
 struct Plan {
@@ -410,11 +423,16 @@ static uint64_t band_slot(int c, uint32_t rows_cap, uint32_t wgen)
     // forward rows (M, I) | scale factors | staged cap codes and tags (band kernels) or band state (generic)
     return ((uint64_t)(rows_cap + 1) * W * 2 + (rows_cap + 2) + (c == 3 ? 3ull * W : rows_cap + 1ull)) * 64;
 }
+// the band classes run at the same time (two streams), so each has its own region of the scratch
+static uint64_t band_offset(int wpe, int c, uint32_t n, uint32_t rows_cap)
+{
+    uint64_t off = 0;
+    for (int k = 0; k < c; k++) off += band_slot(k, rows_cap, 0) * band_waves(wpe, k, n);
+    return off;
+}
 static uint64_t band_scratch_doubles(int wpe, uint32_t n, uint32_t rows_cap)
 {
-    uint64_t need = 0;
-    for (int c = 0; c < 3; c++) need = std::max(need, band_slot(c, rows_cap, 0) * band_waves(wpe, c, n));
-    return need;
+    return band_offset(wpe, 3, n, rows_cap);
 }
 
 // stage 1 (graph-safe): per-read prep, BAQ tag reset, band classification
@@ -506,13 +524,31 @@ fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, 
                            uint32_t *counters, hipStream_t st, hipEvent_t mark, const uint32_t *known_counts)
 {
     const uint32_t cap = std::max<uint32_t>(p.n, 1);
-    for (int c = 0; c < 3; c++) {
-        if (known_counts && known_counts[c] == 0) continue;   // eager launches skip empty classes
-        HIP_TRY(fpd_launch_baq_hmm(c, &b, &e->genome, &S.rs, S.baq_lists.as<uint32_t>() + (uint64_t)c * cap,
-                                   S.baq_counts.as<uint32_t>() + c, e->scratch.as<double>(),
-                                   band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c],
-                                   band_waves(wpe, c, p.n), (unsigned long long *)(counters + 8), wpe,
-                                   e->opts.disable_baq_staging ? 0 : 1, ckpt > 1 ? 1 : 0, st));
+    auto launch_class = [&](int c, hipStream_t s) -> hipError_t {
+        // the class's chunk counter: a word of the class counts that the front stage zeroes
+        return fpd_launch_baq_hmm(c, &b, &e->genome, &S.rs, S.baq_lists.as<uint32_t>() + (uint64_t)c * cap,
+                                  S.baq_counts.as<uint32_t>() + c, e->scratch.as<double>() + band_offset(wpe, c, p.n, p.rows_cap),
+                                  band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c], band_waves(wpe, c, p.n),
+                                  (unsigned long long *)(counters + 8), wpe, e->opts.disable_baq_staging ? 0 : 1,
+                                  ckpt > 1 ? 1 : 0, S.baq_counts.as<uint32_t>() + BAQ_CHUNK_COUNTERS + c, s);
+    };
+    // The 15-cell class keeps every SIMD busy until its last round of chunks; the wider classes
+    // run on the second stream beside it and fill that round (all three take their chunks from
+    // counters, so waves that start late take fewer).  Eager launches skip empty classes.
+    bool forked = false;
+    for (int c = 1; c < 3; c++) {
+        if (known_counts && known_counts[c] == 0) continue;
+        if (!forked) {
+            HIP_TRY(hipEventRecord(S.fork, st));
+            HIP_TRY(hipStreamWaitEvent(S.side, S.fork, 0));
+            forked = true;
+        }
+        HIP_TRY(launch_class(c, S.side));
+    }
+    if (!(known_counts && known_counts[0] == 0)) HIP_TRY(launch_class(0, st));
+    if (forked) {
+        HIP_TRY(hipEventRecord(S.join, S.side));
+        HIP_TRY(hipStreamWaitEvent(st, S.join, 0));
     }
     if (mark) HIP_TRY(hipEventRecord(mark, st));
     HIP_TRY(fpd_launch_baq_post(&b, &S.rs, 1u, st));
@@ -541,7 +577,7 @@ fp_status launch_generic(fp_engine *e, BatchState &S, const fpd::BatchDev &b, co
     HIP_TRY(fpd_launch_baq_hmm(3, &b, &e->genome, &S.rs,
                                S.baq_lists.as<uint32_t>() + 3ull * std::max<uint32_t>(p.n, 1),
                                S.baq_counts.as<uint32_t>() + 3, e->scratch_gen.as<double>(), slot, p.rows_cap, wgen,
-                               waves, (unsigned long long *)(counters + 8), e->wpe, 0, 0, st));
+                               waves, (unsigned long long *)(counters + 8), e->wpe, 0, 0, nullptr, st));
     return FP_OK;
 }
 

@@ -58,6 +58,7 @@ struct BaqArgs {
     uint32_t rows_cap;        // >= max query length
     uint32_t wmax_runtime;    // band cells for the generic kernel
     unsigned long long *cells;  // total band cells evaluated (algorithmic work)
+    uint32_t *next;           // band kernels: chunk counter for dynamic chunk order (zero at launch), or null
 };
 
 // WPE: waves per SIMD the register budget is sized for (1: band fully in
@@ -486,7 +487,9 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
         return (uint8_t)k;
     };
 
-    for (uint32_t chunk = gwave; (uint64_t)chunk * 64 < total; chunk += nwaves) {
+    // one chunk of 64 list entries, one per lane (a lambda: lanes past the list leave it early and
+    // rejoin the wave when it returns)
+    auto run_chunk = [&](uint32_t chunk) {
         const uint32_t idx = chunk * 64 + lane;
         const bool act = idx < total;
         const uint32_t rd = act ? a.list[idx] : 0u;
@@ -500,7 +503,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             const int o = __shfl_xor(qmax, off);
             qmax = o > qmax ? o : qmax;
         }
-        if (!act) continue;
+        if (!act) return;
 
         // ---- setup (baq.cu:260-354)
         const uint32_t hw = a.s.hmmw[rd];
@@ -895,7 +898,18 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
             }
             for (; j < qLen; j++) out[j] = good ? (uint8_t)orow[(uint64_t)j * 64] : (uint8_t)64;
         }
-    }
+    };
+    // Chunks in a fixed round-robin over the resident waves, or, with a.next, taken one at a time
+    // from a counter (zero at launch) in descending list order, longest windows first: waves that
+    // start late, beside another kernel, then take fewer chunks and all finish together.
+    const uint32_t nchunks = (uint32_t)((total + 63) / 64);
+    auto take = [&]() -> uint32_t {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(a.next, 1u);
+        return __builtin_amdgcn_readfirstlane(c);
+    };
+    for (uint32_t k = a.next ? take() : gwave; k < nchunks; k = a.next ? take() : k + nwaves)
+        run_chunk(a.next ? nchunks - 1 - k : k);
     // one counter update per wave
     for (int off = 32; off >= 1; off >>= 1) ncell += __shfl_xor(ncell, off);
     if (lane == 0 && ncell) atomicAdd(a.cells, (unsigned long long)ncell);
@@ -1530,11 +1544,12 @@ extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const 
                                          const uint32_t *count, double *scratch, uint64_t slot_doubles,
                                          uint32_t rows_cap, uint32_t wmax_runtime, uint32_t waves,
                                          unsigned long long *cells, int wpe, int allow_stage, int recompute,
-                                         hipStream_t st)
+                                         uint32_t *next_chunk, hipStream_t st)
 {
     fpd::BaqArgs a{};
     a.b = *b; a.g = *g; a.s = *s; a.list = list; a.count = count; a.scratch = scratch;
     a.slot_doubles = slot_doubles; a.rows_cap = rows_cap; a.wmax_runtime = wmax_runtime; a.cells = cells;
+    a.next = next_chunk;
     if (waves == 0) return hipSuccess;
     // band kernels: 4 waves per workgroup (the caller sizes waves as a multiple of 4)
     const dim3 g4((waves + 3) / 4), b4(256);
