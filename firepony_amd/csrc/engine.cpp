@@ -185,16 +185,14 @@ struct BatchState {
     DevBuf st_status, st_rwc, st_noins, st_refw, st_nerr, st_hmmw, st_bw, st_hqw, st_bflags, st_baq, st_state, st_frac;
     DevBuf baq_lists, baq_counts, rg_buf;
     fpd::ReadStateDev rs{};
-    // the BAQ stage's second stream: the 23- and 33-cell band classes run on it beside the 15-cell
-    // class (fork and join events around them)
-    hipStream_t side = nullptr;
+    // fork and join events around the 23- and 33-cell band classes, which run on the engine's second
+    // BAQ stream beside the 15-cell class
     hipEvent_t fork = nullptr, join = nullptr;
     void release()
     {
         DevBuf *all[] = {&st_status, &st_rwc, &st_noins, &st_refw, &st_nerr, &st_hmmw, &st_bw, &st_hqw,
                          &st_bflags, &st_baq, &st_state, &st_frac, &baq_lists, &baq_counts, &rg_buf};
         for (DevBuf *b : all) b->release();
-        if (side) { (void)hipStreamSynchronize(side); (void)hipStreamDestroy(side); side = nullptr; }
         if (fork) { (void)hipEventDestroy(fork); fork = nullptr; }
         if (join) { (void)hipEventDestroy(join); join = nullptr; }
     }
@@ -204,6 +202,9 @@ struct fp_engine {
     int device = 0;
     fp_options opts;
     hipStream_t stream = nullptr;
+    // the BAQ stage's second stream, shared by every batch state (the BAQ stages of the staging buffers
+    // take turns): a process has few hardware queues, and streams beyond them share one
+    hipStream_t side = nullptr;
     std::string name;
     HostConstants consts;
     int wpe = 1;   // waves per SIMD of the 15-cell BAQ band kernel (1-4)
@@ -329,7 +330,7 @@ fp_status alloc_tables(fp_engine *e)
 fp_status ensure_state(fp_engine *e, BatchState &S, uint32_t n, uint64_t nbases, uint64_t nquals,
                        bool need_frac = true)
 {
-    if (!S.side) HIP_TRY(hipStreamCreateWithFlags(&S.side, hipStreamNonBlocking));
+    if (!e->side) HIP_TRY(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
     if (!S.fork) HIP_TRY(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
     if (!S.join) HIP_TRY(hipEventCreateWithFlags(&S.join, hipEventDisableTiming));
     HIP_TRY(S.st_status.ensure(n));
@@ -540,14 +541,14 @@ fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, 
         if (known_counts && known_counts[c] == 0) continue;
         if (!forked) {
             HIP_TRY(hipEventRecord(S.fork, st));
-            HIP_TRY(hipStreamWaitEvent(S.side, S.fork, 0));
+            HIP_TRY(hipStreamWaitEvent(e->side, S.fork, 0));
             forked = true;
         }
-        HIP_TRY(launch_class(c, S.side));
+        HIP_TRY(launch_class(c, e->side));
     }
     if (!(known_counts && known_counts[0] == 0)) HIP_TRY(launch_class(0, st));
     if (forked) {
-        HIP_TRY(hipEventRecord(S.join, S.side));
+        HIP_TRY(hipEventRecord(S.join, e->side));
         HIP_TRY(hipStreamWaitEvent(st, S.join, 0));
     }
     if (mark) HIP_TRY(hipEventRecord(mark, st));
@@ -1699,6 +1700,7 @@ void fp_engine_destroy(fp_engine *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (auto &s : e->stg)
         if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (e->side) (void)hipStreamSynchronize(e->side);
     DevBuf *bufs[] = {&e->g_word_offset, &e->g_length, &e->g_words, &e->g_foff, &e->g_fstart, &e->g_fstop,
                       &e->g_mel, &e->counters, &e->scratch, &e->tk, &e->tv, &e->shist, &e->stot, &e->sctr};
     for (DevBuf *b : bufs) b->release();
@@ -1723,6 +1725,7 @@ void fp_engine_destroy(fp_engine *e)
     for (auto &b : e->in) b.release();
     for (auto &t : e->tab) for (auto &b : t) b.release();
     for (int i = 0; i < 16; i++) if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+    if (e->side) (void)hipStreamDestroy(e->side);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
