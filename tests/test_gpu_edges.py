@@ -5,7 +5,9 @@
 - at the full size of the 10M-read workload the batched engine (staging buffers, two streams,
   graph replays) and the plain engine give identical tables: the sums are exact, so any lost,
   doubled or raced update would show;
-- 2M reads of the same workload against the oracle fed the same batches (16 host threads).
+- 2M reads of the same workload against the oracle fed the same batches (16 host threads);
+- batches whose BAQ reads all have bands wider than 15 cells (the 15-cell kernel has nothing to do
+  while the wider classes run on their own stream), and batches with only wider-than-33 bands.
 """
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -158,3 +160,55 @@ def test_two_million_reads_against_the_oracle(tmp_path):
     bat.postprocess()
     assert oracles[0].report(str(tmp_path / "o.txt")) == bat.report(str(tmp_path / "e.txt"))
     bat.close()
+
+
+def long_indel_reads(contig, rng, n):
+    """101 bp reads with one 9-15 bp deletion or insertion and a few mismatches: their BAQ bands
+    are wider than the 15-cell class."""
+    other = {"A": "C", "C": "G", "G": "T", "T": "A"}
+    reads = []
+    for i in range(n):
+        L, dl, a = 101, int(rng.integers(9, 16)), int(rng.integers(30, 60))
+        start = int(rng.integers(10, len(contig) - L - dl - 10))
+        if i % 2 == 0:
+            ref = contig[start:start + L + dl]
+            seq, cig = ref[:a] + ref[a + dl:a + dl + (L - a)], f"{a}M{dl}D{L - a}M"
+        else:
+            ref = contig[start:start + L - dl]
+            seq = ref[:a] + "".join(rng.choice(list("ACGT"), size=dl)) + ref[a:a + (L - a - dl)]
+            cig = f"{a}M{dl}I{L - a - dl}M"
+        seq = "".join(other[c] if rng.random() < 0.02 else c for c in seq)
+        reads.append({"seq": seq, "qual": [int(q) for q in rng.integers(10, 41, size=L)], "cigar": cig,
+                      "start": start, "flags": int(rng.choice([0, 16]))})
+    return reads
+
+
+@pytest.mark.parametrize("kind", [abi.FP_ENGINE_PLAIN, abi.FP_ENGINE_BATCHED])
+def test_batches_without_the_15_cell_class(tmp_path, kind):
+    say(f"BAQ bands of 17-33 cells only, then wider-than-33 only (no 15-cell reads), engine kind {kind}")
+    rng = np.random.default_rng(9)
+    contig = random_genome_string(rng, 20000)
+    g = HandGenome([contig])
+    cand = long_indel_reads(contig, rng, 400)
+    probe = Oracle(g)
+    probe.process(HandBatch(cand))
+    w = 2 * probe.debug(abi.FP_DBG_BANDWIDTH, np.uint16).astype(np.int64) + 1
+    nerr = probe.debug(abi.FP_DBG_NUM_ERRORS, np.uint16)
+    mid = [cand[i] for i in range(len(cand)) if nerr[i] and 17 <= w[i] <= 33]
+    wide = [cand[i] for i in range(len(cand)) if nerr[i] and w[i] > 33]
+    assert len(mid) > 20 and len(wide) > 20
+    batches = [HandBatch(mid), HandBatch(wide), HandBatch(mid[:7])]
+    o = Oracle(g)
+    opts = dict(engine_kind=kind)
+    if kind == abi.FP_ENGINE_BATCHED:
+        opts.update(max_batch_reads=128, max_read_length=101)
+    e = Engine(g, **opts)
+    for b in batches:
+        o.process(b)
+        e.process(b)
+    e.synchronize()
+    same_tables(o, e, "wide bands only")
+    o.postprocess()
+    e.postprocess()
+    assert e.report(str(tmp_path / "e.txt")) == o.report(str(tmp_path / "o.txt"))
+    e.close()
