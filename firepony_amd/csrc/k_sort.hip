@@ -220,8 +220,8 @@ __global__ void __launch_bounds__(SORT_THREADS) k_sort_scatter(SortPass p)
     for (uint32_t d = threadIdx.x; d < R; d += SORT_THREADS) gofs[d] = p.hist[(uint64_t)d * gridDim.x + blockIdx.x];
     // element order inside a tile is (wave, item, lane); the next tile's elements are loaded while
     // this one is ranked, reordered and written
-    uint32_t key[SORT_ITEMS], nkey[SORT_ITEMS];
-    double val[SORT_ITEMS], nval[SORT_ITEMS];
+    uint32_t key[SORT_ITEMS], nkey[SORT_ITEMS] = {};
+    double val[SORT_ITEMS], nval[SORT_ITEMS] = {};
     auto load_tile = [&](uint32_t base, uint32_t (&k_)[SORT_ITEMS], double (&v_)[SORT_ITEMS]) {
         const uint32_t t0 = base + wave * (SORT_ITEMS * 64);
 #pragma unroll
