@@ -29,6 +29,12 @@
  * BAQ blocks and error bits; the context keys read neighbouring bases from a
  * per-wave LDS copy of the window.  Windows longer than 256 bases take a
  * serial path on one lane.
+ *
+ * A wave loads the per-read fields (list entry, windows, starts, lengths, flags,
+ * CIGAR position) for 64 reads at once, one read per lane, and takes read k's
+ * with readlane; a read's CIGAR ops arrive one per lane beside its bases.  In
+ * groups of reads of at most 128 window bases the next read's inputs are loaded
+ * while the current read is processed.
  */
 #include "fp_device.h"
 
