@@ -11,7 +11,7 @@ postprocess, which is how the reference defines reads/s
 
 Passes over the same input batches:
   plain     each batch runs as it arrives, one launch per stage;
-  batched   batches are staged into 1M-read device buffers and each buffer
+  batched   batches are staged into 2M-read device buffers and each buffer
             runs as captured hipGraph replays; the BAQ variant is autotuned
             during one untimed warm-up pass over the input.  This is `value`.
   profile   the batched engine with an event between kernels (no graphs):
@@ -291,7 +291,7 @@ def served(args, wpe, torch):
     return res
 
 
-def sort_pass(args, genome, dev, wpe, ckpt, batched_opts, reads, torch):
+def sort_pass(args, genome, dev, wpe, ckpt, batched_opts, profile_opts, reads, torch):
     """The batched engine with the sort aggregation (covariate pairs -> LDS-tiled radix sort ->
     segmented reduce_by_key): reads/s with the digit width autotuned during the warm-up pass, then a
     per-stage pass at that width for the sort/reduce time and its HBM rate."""
@@ -299,8 +299,9 @@ def sort_pass(args, genome, dev, wpe, ckpt, batched_opts, reads, torch):
     dt, st = timed_pass(genome, dev, max(args.warmup, 1), torch, baq_waves_per_simd=wpe,
                         baq_checkpoint_rows=ckpt, **sopts)
     bits = int(st.sort_digit_bits)
+    # the per-stage pass at 1M-read staged batches, the batch size of the committed traffic profile
     _, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=wpe, baq_checkpoint_rows=ckpt, profile_stages=1,
-                        **dict(sopts, sort_digit_bits=bits))
+                        **dict(profile_opts, aggregation=abi.FP_AGG_SORT, sort_digit_bits=bits))
     batches = max(int(pst.cov_launches), 1)
     sort_s = pst.covariates_sort_ms / 1e3
     # algorithmic bytes: first radix pass reads every emitted slot's key twice and value once and
@@ -356,8 +357,9 @@ def main():
     ap.add_argument("--served-jobs", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=400000, help="reads of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--stage-reads", type=int, default=0,
-                    help="reads per staging buffer of the batched engine (0 = the engine's default, 2^20)")
+    ap.add_argument("--stage-reads", type=int, default=1 << 21,
+                    help="reads per staging buffer of the batched passes (0 = the engine's default, 2^20); "
+                         "the per-kernel profile and roofline passes keep the default")
     ap.add_argument("--wpe", type=int, default=0, help="BAQ waves per SIMD (0 = autotuned)")
     ap.add_argument("--ckpt", type=int, default=0,
                     help="rows per BAQ forward checkpoint, 1 or 2 (0 = autotuned; profile-only pass: 1)")
@@ -395,11 +397,13 @@ def main():
     K = len(dev)
     reads = sum(d.num_reads for d in dev)
     bases = sum(d.num_bases for d in dev)
-    batched_opts = dict(engine_kind=abi.FP_ENGINE_BATCHED, max_read_length=101)
+    # per-kernel passes (profile, roofline) stage 2^20 reads per launch, as the committed profiles do
+    profile_opts = dict(engine_kind=abi.FP_ENGINE_BATCHED, max_read_length=101)
+    batched_opts = dict(profile_opts)
     if args.stage_reads:
         batched_opts.update(max_batch_reads=args.stage_reads)
     if args.profile_only:
-        popts = dict(batched_opts)
+        popts = dict(profile_opts)
         if mixed_profile:
             popts.update(max_read_length=250, frac_error_f64=1)
         if args.aggregation == "sort":
@@ -432,7 +436,7 @@ def main():
         f"stage 1 {st.prep_kernel_ms:.1f} ms, stage 2 {st.baq_ms:.1f} ms, postprocess {st.postprocessing_ms:.1f} ms")
 
     pdt, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=wpe, baq_checkpoint_rows=ckpt,
-                          profile_stages=1, **batched_opts)
+                          profile_stages=1, **profile_opts)
     kname, roof = roofline(pst, reads, bases)
     log(f"profile: prep {pst.prep_kernel_ms:.1f} ms, baq_hmm {pst.baq_hmm_kernel_ms:.1f} ms "
         f"({pst.baq_hmm_launches} launches), cov {pst.cov_kernel_ms:.1f} ms; baq reads {pst.baq_reads}, "
@@ -452,7 +456,8 @@ def main():
         "data": "synthetic (seeded generator, 30 Mbp genome, 10M x 101 bp reads)",
         "config": {"workload": "30 Mbp / 3 contigs, 10M SE 101 bp reads, 4 read groups, 30k known sites, "
                                "BAQ band 7, input batches of 20,000 reads", "reads": reads, "batch_reads": B,
-                   "engine": "batched (1M-read staging buffers, hipGraph replays)", "baq_waves_per_simd": wpe,
+                   "engine": f"batched ({(args.stage_reads or (1 << 20)) / 2**20:g}M-read staging buffers, hipGraph replays; "
+                             "per-kernel profile and roofline at 1M-read launches)", "baq_waves_per_simd": wpe,
                    "baq_checkpoint_rows": ckpt},
         "kernel_ms": kernel_ms(pst),
         "roofline": dict(roof, kernel=kname),
@@ -461,7 +466,7 @@ def main():
         out["plain"] = out_plain
         out["batched_over_plain"] = value / out_plain["value"]
     if not args.no_sort:
-        out["sort_aggregation"] = sort_pass(args, genome, dev, wpe, ckpt, batched_opts, reads, torch)
+        out["sort_aggregation"] = sort_pass(args, genome, dev, wpe, ckpt, batched_opts, profile_opts, reads, torch)
     del dev
     torch.cuda.empty_cache()
 
