@@ -477,6 +477,8 @@ def main():
         log(f"mixed lengths: generating {n2} reads in batches of {B}")
         dev2 = upload(genome2, rs2, n2, B, torch)
         mopts = dict(engine_kind=abi.FP_ENGINE_BATCHED, max_read_length=250, frac_error_f64=1)
+        if args.stage_reads:
+            mopts.update(max_batch_reads=args.stage_reads)
         pdt2, _ = timed_pass(genome2, dev2, args.warmup, torch, baq_waves_per_simd=1, frac_error_f64=1)
         bdt2, bst2 = timed_pass(genome2, dev2, args.warmup, torch, **mopts)
         out["mixed_length"] = {"reads": n2, "workload": "50-250 bp reads, 8 read groups, 5 % indel reads of "
