@@ -58,7 +58,7 @@ class FpOptions(C.Structure):
         ("serve_rg_slots", C.c_uint32),
         ("disable_baq_staging", C.c_uint32),
         ("baq_checkpoint_rows", C.c_uint32),
-        ("reserved", C.c_uint32 * 1),
+        ("serve_hbm_budget_mib", C.c_uint32),
     ]
 
 
@@ -163,6 +163,9 @@ class FpStats(C.Structure):
         ("sort_pairs", C.c_uint64),
         ("sort_passes", C.c_uint64),
         ("baq_checkpoint_rows", C.c_uint64),
+        ("baq15_kernel_ms", C.c_double),
+        ("baq15_launches", C.c_uint64),
+        ("baq15_cells", C.c_uint64),
     ]
 
 

@@ -26,7 +26,7 @@
 // kernels (k_*.hip)
 extern "C" {
 hipError_t fpd_launch_prep(const fpd::BatchDev *b, const fpd::GenomeDev *g, const fpd::ReadStateDev *s,
-                           uint32_t *counters, uint32_t max_read_len, hipStream_t st);
+                           uint32_t *batch, uint32_t num_rg, uint32_t max_read_len, hipStream_t st);
 hipError_t fpd_baq_set_constants(const double *qual2prob, const double *thr, double gap_open);
 hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t *live, const fpd::ReadStateDev *s,
                                    uint32_t *lists, uint32_t *counts, uint32_t capacity, hipStream_t st);
@@ -46,7 +46,7 @@ hipError_t fpd_append(const void *const *src, void *const *dst, const uint64_t *
                       uint32_t base_off, uint32_t qual_off, uint32_t rg_off, uint32_t rg_limit, hipStream_t st);
 hipError_t fpd_cov_prepare();
 hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
-                          uint32_t frac_f64, uint32_t keep_frac, uint32_t *counters, uint32_t *rg_buf,
+                          uint32_t frac_f64, uint32_t keep_frac, uint32_t *batch, uint32_t *rg_buf,
                           uint32_t max_read_len, uint32_t *const *pkey, double *const *pval, uint32_t *pcursor,
                           uint64_t pcap, hipStream_t st);
 hipError_t fpd_sort_reduce(uint32_t *keys, double *vals, uint32_t *tkeys, double *tvals, uint64_t cap,
@@ -60,12 +60,13 @@ hipError_t fpd_compact_scatter(const uint64_t *count, const uint64_t *const *par
 hipError_t fpd_table_add(uint64_t *dc, uint64_t *const *dparts, const uint64_t *sc, const uint64_t *const *sparts,
                          uint64_t n, hipStream_t st);
 hipError_t fpd_post_set_constants(const double *qual_err, const double *prior, const double *log1mp, double log10e);
-hipError_t fpd_launch_empirical(uint64_t n, const uint32_t *key, const uint64_t *count, const uint64_t *lo,
-                                const uint64_t *hi, int qual_shift, double *mism, double *expected, double *estimated,
-                                double *empirical, hipStream_t st);
+hipError_t fpd_launch_obs_values(uint64_t n, const uint32_t *key, const uint64_t *count, const uint64_t *lo,
+                                 const uint64_t *hi, int qual_shift, double *mism, double *expected, hipStream_t st);
+hipError_t fpd_launch_empirical(uint64_t n, const uint64_t *count, const double *mism, const double *estimated,
+                                const double *coeff, uint32_t need_rounding, double *empirical, hipStream_t st);
 hipError_t fpd_launch_rg_table(uint32_t num_rg, const uint64_t *count, const uint64_t *const *parts,
-                               uint32_t *valid, uint64_t *obs, double *mism, double *expected, double *estimated,
-                               double *empirical, hipStream_t st);
+                               uint32_t *valid, uint64_t *obs, double *mism, double *expected, hipStream_t st);
+hipError_t fpd_check_batch(const fpd::BatchDev *b, uint32_t rg_limit, uint32_t *out, hipStream_t st);
 }
 
 namespace {
@@ -265,6 +266,8 @@ struct fp_engine {
         hipEvent_t cov_done = nullptr;  // after its covariate stage (sort buffers are shared)
         hipEvent_t t0 = nullptr, t1 = nullptr;   // around its stage 2 (timing)
         bool back_pending = false;      // stage-2 time not yet collected
+        uint64_t fill = 1;              // which fill of this buffer is staging now (a job records the
+                                        // fills that hold its reads)
     };
     // staging buffers: one fills and runs its prep and covariates while the other's BAQ runs (a third
     // buffer lets more kernels overlap but measured no faster: the BAQ kernels hold the HBM
@@ -286,11 +289,19 @@ struct fp_engine {
     double tune_ms_per_read[4] = {0, 0, 0, 0};   // one per BAQ candidate
     hipEvent_t bev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool back_pending = false;          // stage-2 time of the last staged batch not yet collected
-    uint64_t eager_filtered = 0, eager_baq = 0, eager_cells = 0;   // oversized batches run eagerly
+    uint64_t eager_cells = 0, eager_cells15 = 0;   // BAQ cells of oversized batches run eagerly
+    bool band15_timed = false;          // profile_stages: events 11/12 bracket a 15-cell band launch
 
-    // serving: owner of each read-group table slot (-1 = free)
+    // serving: owner of each read-group table slot (-1 = free), the HBM byte budget of the open jobs
     std::vector<int64_t> slot_owner;
     int64_t next_job_id = 1;
+    uint64_t budget_bytes = 0, bytes_in_use = 0;
+    // job postprocess, release and device-batch checks run on a stream of their own, ordered after
+    // the staging buffers that hold the job's reads (events), so they overlap other jobs' batches
+    hipStream_t post = nullptr;
+    // postprocess working buffers, grown and kept: freeing device memory waits for the whole device
+    DevBuf pw[12];
+    DevBuf check;
 };
 
 // a serving job: its read groups occupy table slots [base, base + nrg)
@@ -303,6 +314,8 @@ struct fp_job {
     uint64_t reads = 0, batches = 0;
     double post_ms = 0;
     TableRows rows[4];
+    uint64_t fill[2] = {0, 0};   // per staging buffer: the last fill that holds the job's reads (0: none)
+    uint64_t bytes = 0;          // HBM footprint counted against the engine's budget
 };
 
 namespace {
@@ -398,6 +411,15 @@ float elapsed(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+// the 15-cell band kernel's time in the last profiled BAQ stage (its stream work has completed)
+void collect_band15_time(fp_engine *e)
+{
+    if (!e->band15_timed) return;
+    e->stats.baq15_kernel_ms += elapsed(e->ev[11], e->ev[12]);
+    e->stats.baq15_launches += 1;
+    e->band15_timed = false;
+}
+
 static const uint32_t BAQ_CLASS_W[4] = {15, 23, 33, 0};
 // words 8..10 of a batch's BAQ class counts (zeroed with them by the front stage): the chunk counters of
 // the three band kernels
@@ -439,8 +461,9 @@ static uint64_t band_scratch_doubles(int wpe, uint32_t n, uint32_t rows_cap)
 // stage 1 (graph-safe): per-read prep, BAQ tag reset, band classification
 fp_status launch_front(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, uint32_t *counters, hipStream_t st)
 {
-    HIP_TRY(fpd_fill_u32(S.baq_counts.as<uint32_t>(), 144, 0u, st));
-    HIP_TRY(fpd_launch_prep(&b, &e->genome, &S.rs, counters, p.rows_cap, st));
+    (void)counters;
+    HIP_TRY(fpd_fill_u32(S.baq_counts.as<uint32_t>(), 144, 0u, st));   // class sizes and the batch's counter words
+    HIP_TRY(fpd_launch_prep(&b, &e->genome, &S.rs, S.baq_counts.as<uint32_t>(), e->num_rg, p.rows_cap, st));
     HIP_TRY(fpd_fill_u8(S.rs.baq, p.nquals, p.live, 0xff, st));
     HIP_TRY(fpd_launch_baq_classify(p.n, p.live, &S.rs, S.baq_lists.as<uint32_t>(), S.baq_counts.as<uint32_t>(),
                                     std::max<uint32_t>(p.n, 1), st));
@@ -484,7 +507,7 @@ fp_status launch_cov_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, 
     const uint32_t keep = e->opts.debug_arrays ? 1u : 0u;
     const uint32_t maxlen = std::min<uint32_t>(p.rows_cap, 499);
     if (e->opts.aggregation != FP_AGG_SORT) {
-        HIP_TRY(fpd_launch_cov(&b, &S.rs, &e->tables, e->opts.frac_error_f64, keep, counters,
+        HIP_TRY(fpd_launch_cov(&b, &S.rs, &e->tables, e->opts.frac_error_f64, keep, S.baq_counts.as<uint32_t>(),
                                S.rg_buf.as<uint32_t>(), maxlen, nullptr, nullptr, nullptr, 0, st));
         return FP_OK;
     }
@@ -493,8 +516,8 @@ fp_status launch_cov_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, 
     uint32_t *pkeys[3];
     double *pvals[3];
     for (int t = 0; t < 3; t++) { pkeys[t] = e->pk[t].as<uint32_t>(); pvals[t] = e->pv[t].as<double>(); }
-    HIP_TRY(fpd_launch_cov(&b, &S.rs, &e->tables, e->opts.frac_error_f64, keep, counters, S.rg_buf.as<uint32_t>(),
-                           maxlen, pkeys, pvals, sc, e->pair_cap, st));
+    HIP_TRY(fpd_launch_cov(&b, &S.rs, &e->tables, e->opts.frac_error_f64, keep, S.baq_counts.as<uint32_t>(),
+                           S.rg_buf.as<uint32_t>(), maxlen, pkeys, pvals, sc, e->pair_cap, st));
     if (sort_mark) HIP_TRY(hipEventRecord(sort_mark, st));
     for (int t = 0; t < 3; t++) {
         unsigned long long *parts[3];
@@ -520,7 +543,8 @@ fp_status collect_sort_counts(fp_engine *e, int sort_bits)
     return FP_OK;
 }
 
-// stage 2a (graph-safe): the BAQ band classes and cap/recode
+// stage 2a (graph-safe): the BAQ band classes and cap/recode.  The 15-cell class counts its band
+// cells in counter words 10..11, the other classes in 8..9.
 fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, int wpe, int ckpt,
                            uint32_t *counters, hipStream_t st, hipEvent_t mark, const uint32_t *known_counts)
 {
@@ -530,9 +554,24 @@ fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, 
         return fpd_launch_baq_hmm(c, &b, &e->genome, &S.rs, S.baq_lists.as<uint32_t>() + (uint64_t)c * cap,
                                   S.baq_counts.as<uint32_t>() + c, e->scratch.as<double>() + band_offset(wpe, c, p.n, p.rows_cap),
                                   band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c], band_waves(wpe, c, p.n),
-                                  (unsigned long long *)(counters + 8), wpe, e->opts.disable_baq_staging ? 0 : 1,
-                                  ckpt > 1 ? 1 : 0, S.baq_counts.as<uint32_t>() + BAQ_CHUNK_COUNTERS + c, s);
+                                  (unsigned long long *)(counters + (c == 0 ? 10 : 8)), wpe,
+                                  e->opts.disable_baq_staging ? 0 : 1, ckpt > 1 ? 1 : 0,
+                                  S.baq_counts.as<uint32_t>() + BAQ_CHUNK_COUNTERS + c, s);
     };
+    if (e->opts.profile_stages) {
+        // per-kernel timing: the classes in turn on one stream, events around the 15-cell class
+        for (int c = 1; c < 3; c++)
+            if (!(known_counts && known_counts[c] == 0)) HIP_TRY(launch_class(c, st));
+        if (!(known_counts && known_counts[0] == 0)) {
+            HIP_TRY(hipEventRecord(e->ev[11], st));
+            HIP_TRY(launch_class(0, st));
+            HIP_TRY(hipEventRecord(e->ev[12], st));
+            e->band15_timed = true;
+        }
+        if (mark) HIP_TRY(hipEventRecord(mark, st));
+        HIP_TRY(fpd_launch_baq_post(&b, &S.rs, 1u, st));
+        return FP_OK;
+    }
     // The 15-cell class keeps every SIMD busy until its last round of chunks; the wider classes
     // run on the second stream beside it and fill that round (all three take their chunks from
     // counters, so waves that start late take fewer).  Eager launches skip empty classes.
@@ -604,11 +643,11 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     rc = launch_front(e, S, b, p, counters, st);
     if (rc != FP_OK) return rc;
     HIP_TRY(hipEventRecord(e->ev[7], st));
-    uint32_t *hc = e->host_counts;
-    HIP_TRY(hipMemcpyAsync(hc, S.baq_counts.p, 32, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(hc + 8, counters, 32, hipMemcpyDeviceToHost, st));
+    uint32_t *hc = e->host_counts;   // the batch's class sizes and counter words
+    HIP_TRY(hipMemcpyAsync(hc, S.baq_counts.p, 4 * fpd::BW_WORDS, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    p.rows_cap = std::max<uint32_t>(std::max(max_read_len_hint, hc[8 + 3]), 1);
+    if (hc[fpd::BW_BAD_RG]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
+    p.rows_cap = std::max<uint32_t>(std::max(max_read_len_hint, hc[fpd::BW_LONGEST]), 1);
     HIP_TRY(e->scratch.ensure(band_scratch_doubles(e->wpe, n, p.rows_cap) * 8 + 8));
     const bool sorting = e->opts.aggregation == FP_AGG_SORT;
     if (sorting) {
@@ -625,17 +664,20 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     HIP_TRY(hipEventRecord(e->ev[3], st));
     uint32_t *post = e->host_counts + 16;
     HIP_TRY(hipMemcpyAsync(post, counters, 64, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(post + 16, S.baq_counts.as<uint32_t>() + fpd::BW_OVERFLOW, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if (post[4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
-    if (post[6]) return fail(FP_ERR_INTERNAL, "sort aggregation: observation pairs overflowed their buffer");
+    if (post[16]) return fail(FP_ERR_INTERNAL, "sort aggregation: observation pairs overflowed their buffer");
 
     e->stats.total_reads += n;
     e->stats.num_batches += 1;
-    e->stats.filtered_reads += post[1];
-    e->stats.baq_reads += post[2];
-    uint64_t cells;
+    e->stats.filtered_reads += hc[fpd::BW_FILTERED];
+    e->stats.baq_reads += hc[fpd::BW_BAQ];
+    uint64_t cells, cells15;
     memcpy(&cells, post + 8, 8);
-    e->stats.baq_cells += cells;
+    memcpy(&cells15, post + 10, 8);
+    e->stats.baq_cells += cells + cells15;
+    e->stats.baq15_cells += cells15;
+    collect_band15_time(e);
     const uint32_t hmm_launches = (hc[0] ? 1 : 0) + (hc[1] ? 1 : 0) + (hc[2] ? 1 : 0) + (hc[3] ? 1 : 0);
     e->stats.prep_kernel_ms += elapsed(e->ev[6], e->ev[7]);
     e->stats.prep_launches += 1;
@@ -721,6 +763,7 @@ fp_status batched_alloc(fp_engine *e)
         fp_status rc = ensure_state(e, s.bs, e->cap_reads, e->cap_bases, e->cap_quals,
                                     e->opts.debug_arrays != 0 || e->rows_cap > 256);
         if (rc != FP_OK) return rc;
+        HIP_TRY(fpd_fill_u32(s.bs.baq_counts.as<uint32_t>(), 144, 0u, e->stream));   // read before the first batch
     }
     fp_status rc = FP_OK;
     HIP_TRY(e->scratch.ensure(band_scratch_doubles(std::max(3, e->wpe), e->cap_reads, e->rows_cap) * 8 + 8));
@@ -791,9 +834,19 @@ void tune_record(fp_engine *e, int kind, int cand, double per_read)
     }
 }
 
+// empty a staging buffer without running its reads (its batch failed a check); the buffer refills
+static void stage_drop(fp_engine *e, fp_engine::Stage &s)
+{
+    e->stats.total_reads -= s.n;
+    s.n = 0;
+    s.ncig = s.nbases = s.nquals = 0;
+    s.fill += 1;
+}
+
 // finish a staging buffer's turn: the next buffer fills
 static void stage_done(fp_engine *e, fp_engine::Stage &s)
 {
+    s.fill += 1;
     e->last_num_reads = s.n;
     e->last_num_bases = s.nbases;
     e->last_num_quals = s.nquals;
@@ -826,6 +879,9 @@ fp_status batched_flush(fp_engine *e)
     s.live_host[2] = (uint32_t)(s.nquals >> 32);
     s.live_host[3] = 0;
     HIP_TRY(hipMemcpyAsync(s.live.p, s.live_host, 16, hipMemcpyHostToDevice, st));
+    // the pair-overflow word of this buffer's previous batch, read before the front stage zeroes it
+    uint32_t *prev = e->host_counts + 64;
+    HIP_TRY(hipMemcpyAsync(prev, S.baq_counts.as<uint32_t>() + fpd::BW_OVERFLOW, 4, hipMemcpyDeviceToHost, st));
 
     const fpd::BatchDev b = stage_dev(e, s);
     const Plan p{e->cap_reads, e->cap_quals, s.live.as<uint32_t>(), e->rows_cap};
@@ -845,20 +901,30 @@ fp_status batched_flush(fp_engine *e)
         HIP_TRY(hipGraphLaunch(s.gfront, st));
     }
     HIP_TRY(hipEventRecord(e->bev[3], st));
-    // the band classes and the longest read: one small readback per staged batch
+    // the band classes, the batch's read counts, longest read and error words: one small readback
     uint32_t *hc = e->host_counts;
-    HIP_TRY(hipMemcpyAsync(hc, S.baq_counts.p, 32, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(hc + 8, counters, 64, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hc, S.baq_counts.p, 4 * fpd::BW_WORDS, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     e->stats.read_filter_ms += elapsed(e->bev[2], e->bev[3]);
     e->stats.prep_kernel_ms += elapsed(e->bev[2], e->bev[3]);
     e->stats.prep_launches += 1;
-    if (hc[8 + 4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
-    if (hc[8 + 6]) return fail(FP_ERR_INTERNAL, "sort aggregation: observation pairs overflowed their buffer");
-    if (hc[8 + 3] > e->rows_cap)
-        return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(e->rows_cap) + ")");
-    e->stats.filtered_reads = e->eager_filtered + hc[8 + 1];
-    e->stats.baq_reads = e->eager_baq + hc[8 + 2];
+    // A failed check drops this buffer's reads before they touch the tables; the error is reported
+    // once and the engine goes on with the next batch.
+    if (prev[0]) {
+        stage_drop(e, s);
+        return fail(FP_ERR_INTERNAL, "sort aggregation: observation pairs overflowed their buffer");
+    }
+    if (hc[fpd::BW_BAD_RG]) {
+        stage_drop(e, s);
+        return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups (staged batch dropped)");
+    }
+    if (hc[fpd::BW_LONGEST] > e->rows_cap) {
+        stage_drop(e, s);
+        return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(e->rows_cap) +
+                                            "; staged batch dropped)");
+    }
+    e->stats.filtered_reads += hc[fpd::BW_FILTERED];
+    e->stats.baq_reads += hc[fpd::BW_BAQ];
 
     // autotuner: BAQ waves per SIMD first, then the radix digit width of the sort aggregation; each
     // candidate runs one staged batch eagerly, alone on the device, and is timed per read
@@ -881,7 +947,10 @@ fp_status batched_flush(fp_engine *e)
         rc = launch_back(e, S, b, p, wpe, ckpt, bits, counters, st, marks, hc);
         if (rc != FP_OK) return rc;
         HIP_TRY(hipEventRecord(e->ev[3], st));
+        HIP_TRY(hipEventRecord(s.baq_done, st));
+        HIP_TRY(hipEventRecord(s.cov_done, st));
         HIP_TRY(hipStreamSynchronize(st));
+        collect_band15_time(e);
         const uint32_t launches = (hc[0] ? 1 : 0) + (hc[1] ? 1 : 0) + (hc[2] ? 1 : 0) + (hc[3] ? 1 : 0);
         e->stats.baq_hmm_kernel_ms += elapsed(e->ev[8], e->ev[9]);
         e->stats.baq_hmm_launches += launches;
@@ -955,13 +1024,24 @@ fp_status batched_sync(fp_engine *e)
     for (auto &s : e->stg) collect_back_time(e, s);
     uint32_t *hc = e->host_counts + 32;
     HIP_TRY(hipMemcpy(hc, e->bcounters.p, 64, hipMemcpyDeviceToHost));
-    if (hc[4]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
-    if (hc[6]) return fail(FP_ERR_INTERNAL, "sort aggregation: observation pairs overflowed their buffer");
-    e->stats.filtered_reads = e->eager_filtered + hc[1];
-    e->stats.baq_reads = e->eager_baq + hc[2];
-    uint64_t cells;
+    uint64_t cells, cells15;
     memcpy(&cells, hc + 8, 8);
-    e->stats.baq_cells = e->eager_cells + cells;
+    memcpy(&cells15, hc + 10, 8);
+    e->stats.baq_cells = e->eager_cells + cells + cells15;
+    e->stats.baq15_cells = e->eager_cells15 + cells15;
+    if (e->opts.aggregation == FP_AGG_SORT) {
+        // a staged batch's pair overflow: reported once, then cleared
+        for (auto &s : e->stg) {
+            uint32_t *w = s.bs.baq_counts.as<uint32_t>() + fpd::BW_OVERFLOW;
+            uint32_t ov = 0;
+            HIP_TRY(hipMemcpy(&ov, w, 4, hipMemcpyDeviceToHost));
+            if (ov) {
+                HIP_TRY(fpd_fill_u32(w, 1, 0u, e->stream));
+                HIP_TRY(hipStreamSynchronize(e->stream));
+                return fail(FP_ERR_INTERNAL, "sort aggregation: observation pairs overflowed their buffer");
+            }
+        }
+    }
     return FP_OK;
 }
 
@@ -985,9 +1065,8 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
         e->epoch++;
         const fp_stats before = e->stats;
         rc = on_device ? run_batch(e, v, 0) : plain_host_batch(e, v);
-        e->eager_filtered += e->stats.filtered_reads - before.filtered_reads;
-        e->eager_baq += e->stats.baq_reads - before.baq_reads;
         e->eager_cells += e->stats.baq_cells - before.baq_cells;
+        e->eager_cells15 += e->stats.baq15_cells - before.baq15_cells;
         return rc;
     }
     e->stats.num_batches += 1;
@@ -1086,7 +1165,7 @@ fp_status plain_host_batch(fp_engine *e, const fp_batch_view *hb)
 // compaction of one dense table into key-ordered rows on the device
 // read groups [rg_base, rg_base + nrg) of table t: rows keyed relative to rg_base
 fp_status compact(fp_engine *e, int t, uint32_t rg_base, uint32_t nrg, DevBuf &okey, DevBuf &ocount, DevBuf &olo,
-                  DevBuf &ohi, uint64_t *nrows)
+                  DevBuf &ohi, uint64_t *nrows, hipStream_t st)
 {
     const uint64_t off0 = (uint64_t)rg_base << (fpd::TABLE_BITS[t] - 8);
     const uint64_t n = (uint64_t)nrg << (fpd::TABLE_BITS[t] - 8);
@@ -1094,50 +1173,70 @@ fp_status compact(fp_engine *e, int t, uint32_t rg_base, uint32_t nrg, DevBuf &o
     const uint64_t *parts[3] = {e->tables.part[t][0] + off0, e->tables.part[t][1] + off0,
                                 e->tables.part[t][2] + off0};
     const uint64_t nb = (n + 255) / 256;
-    DevBuf bc, bo;
+    DevBuf &bc = e->pw[0], &bo = e->pw[1];
     HIP_TRY(bc.ensure(4 * nb + 4));
     HIP_TRY(bo.ensure(8 * nb + 8));
-    HIP_TRY(fpd_compact_count(count, n, bc.as<uint32_t>(), e->stream));
+    HIP_TRY(fpd_compact_count(count, n, bc.as<uint32_t>(), st));
     std::vector<uint32_t> h(nb);
     std::vector<uint64_t> off(nb);
-    HIP_TRY(hipMemcpyAsync(h.data(), bc.p, 4 * nb, hipMemcpyDeviceToHost, e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
+    HIP_TRY(hipMemcpyAsync(h.data(), bc.p, 4 * nb, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
     uint64_t total = 0;
     for (uint64_t i = 0; i < nb; i++) { off[i] = total; total += h[i]; }
-    HIP_TRY(hipMemcpyAsync(bo.p, off.data(), 8 * nb, hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(hipMemcpyAsync(bo.p, off.data(), 8 * nb, hipMemcpyHostToDevice, st));
     HIP_TRY(okey.ensure(4 * total + 4));
     HIP_TRY(ocount.ensure(8 * total + 8));
     HIP_TRY(olo.ensure(8 * total + 8));
     HIP_TRY(ohi.ensure(8 * total + 8));
     HIP_TRY(fpd_compact_scatter(count, parts, n, bo.as<uint64_t>(),
                                 okey.as<uint32_t>(), ocount.as<uint64_t>(), olo.as<uint64_t>(), ohi.as<uint64_t>(),
-                                e->stream));
-    HIP_TRY(hipStreamSynchronize(e->stream));
-    bc.release();
-    bo.release();
+                                st));
+    HIP_TRY(hipStreamSynchronize(st));
     *nrows = total;
     return FP_OK;
 }
 
-// postprocess of read groups [rg_base, rg_base + nrg) into rows (keys relative to rg_base)
-fp_status postprocess_range(fp_engine *e, uint32_t rg_base, uint32_t nrg, TableRows *rows)
+// The estimated quality -10*log10(expected/obs) and the binomial coefficient of the 61-bin
+// posterior (empirical_quality.cu:72-75,151-156,179-189), evaluated with the host libm that the
+// reference's host build uses: the last bit of log10 and lgammaf differs between libm
+// implementations, and either can move a posterior near-tie or a printed rounding boundary.
+void host_terms(uint64_t obs, double mism, double expected, double log10e, double *est, double *coeff)
 {
-    hipStream_t st = e->stream;
+    *est = double(-10.0 * log10(expected / double(obs)));
+    uint64_t nErr = uint64_t(mism + 0.5) + 1;
+    uint64_t nObs = obs + 1 + 1;
+    const uint32_t MAXOBS = 2147483647u - 1;
+    if (nObs > MAXOBS) {
+        const double fraction = double(MAXOBS) / double(nObs);
+        nErr = (uint64_t)round(double(nErr) * fraction);
+        nObs = MAXOBS;
+    }
+    auto lf = [&](uint64_t x) { return (double)lgammaf((float)(double)(x + 1)) * log10e; };
+    *coeff = lf(nObs) - lf(nErr) - lf(nObs - nErr);
+}
+
+// postprocess of read groups [rg_base, rg_base + nrg) into rows (keys relative to rg_base), on stream st
+fp_status postprocess_range(fp_engine *e, uint32_t rg_base, uint32_t nrg, TableRows *rows, hipStream_t st)
+{
     static const int qshift[3] = {2, 12, 13};
+    const double log10e = e->consts.log10e;
     for (int t = 0; t < 3; t++) {
-        DevBuf k, c, lo, hi, m, ex, es, em;
+        DevBuf &k = e->pw[2], &c = e->pw[3], &lo = e->pw[4], &hi = e->pw[5], &m = e->pw[6], &ex = e->pw[7],
+               &es = e->pw[8], &cf = e->pw[9], &em = e->pw[10];
         uint64_t nr = 0;
-        fp_status rc = compact(e, t, rg_base, nrg, k, c, lo, hi, &nr);
+        fp_status rc = compact(e, t, rg_base, nrg, k, c, lo, hi, &nr, st);
         if (rc != FP_OK) return rc;
         HIP_TRY(m.ensure(8 * nr + 8));
         HIP_TRY(ex.ensure(8 * nr + 8));
         HIP_TRY(es.ensure(8 * nr + 8));
+        HIP_TRY(cf.ensure(8 * nr + 8));
         HIP_TRY(em.ensure(8 * nr + 8));
-        HIP_TRY(fpd_launch_empirical(nr, k.as<uint32_t>(), c.as<uint64_t>(), lo.as<uint64_t>(), hi.as<uint64_t>(),
-                                     qshift[t], m.as<double>(), ex.as<double>(), es.as<double>(), em.as<double>(), st));
+        HIP_TRY(fpd_launch_obs_values(nr, k.as<uint32_t>(), c.as<uint64_t>(), lo.as<uint64_t>(), hi.as<uint64_t>(),
+                                      qshift[t], m.as<double>(), ex.as<double>(), st));
         TableRows &R = rows[t + 1];
         R.key.resize(nr); R.obs.resize(nr); R.lo.resize(nr); R.hi.resize(nr);
         R.mism.resize(nr); R.expected.resize(nr); R.estimated.resize(nr); R.empirical.resize(nr);
+        std::vector<double> coeff(nr);
         if (nr) {
             HIP_TRY(hipMemcpyAsync(R.key.data(), k.p, 4 * nr, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipMemcpyAsync(R.obs.data(), c.p, 8 * nr, hipMemcpyDeviceToHost, st));
@@ -1145,36 +1244,48 @@ fp_status postprocess_range(fp_engine *e, uint32_t rg_base, uint32_t nrg, TableR
             HIP_TRY(hipMemcpyAsync(R.hi.data(), hi.p, 8 * nr, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipMemcpyAsync(R.mism.data(), m.p, 8 * nr, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipMemcpyAsync(R.expected.data(), ex.p, 8 * nr, hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipMemcpyAsync(R.estimated.data(), es.p, 8 * nr, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            for (uint64_t i = 0; i < nr; i++) host_terms(R.obs[i], R.mism[i], R.expected[i], log10e, &R.estimated[i], &coeff[i]);
+            HIP_TRY(hipMemcpyAsync(es.p, R.estimated.data(), 8 * nr, hipMemcpyHostToDevice, st));
+            HIP_TRY(hipMemcpyAsync(cf.p, coeff.data(), 8 * nr, hipMemcpyHostToDevice, st));
+            HIP_TRY(fpd_launch_empirical(nr, c.as<uint64_t>(), m.as<double>(), es.as<double>(), cf.as<double>(), 1u,
+                                         em.as<double>(), st));
             HIP_TRY(hipMemcpyAsync(R.empirical.data(), em.p, 8 * nr, hipMemcpyDeviceToHost, st));
         }
         HIP_TRY(hipStreamSynchronize(st));
-        k.release(); c.release(); lo.release(); hi.release(); m.release(); ex.release(); es.release(); em.release();
     }
     // read group table (RecalTable0)
     {
         const uint32_t ns = nrg * 4;
-        DevBuf valid, obs, m, ex, es, em;
+        DevBuf &valid = e->pw[11], &obs = e->pw[3], &m = e->pw[6], &ex = e->pw[7], &es = e->pw[8], &cf = e->pw[9],
+               &em = e->pw[10];
         HIP_TRY(valid.ensure(4 * ns + 4));
         HIP_TRY(obs.ensure(8 * ns + 8));
         HIP_TRY(m.ensure(8 * ns + 8));
         HIP_TRY(ex.ensure(8 * ns + 8));
         HIP_TRY(es.ensure(8 * ns + 8));
+        HIP_TRY(cf.ensure(8 * ns + 8));
         HIP_TRY(em.ensure(8 * ns + 8));
         const uint64_t q0 = (uint64_t)rg_base << 10;
         const uint64_t *qparts[3] = {e->tables.part[0][0] + q0, e->tables.part[0][1] + q0, e->tables.part[0][2] + q0};
-        HIP_TRY(fpd_launch_rg_table(nrg, e->tables.count[0] + q0, qparts,
-                                    valid.as<uint32_t>(), obs.as<uint64_t>(), m.as<double>(), ex.as<double>(),
-                                    es.as<double>(), em.as<double>(), st));
+        HIP_TRY(fpd_launch_rg_table(nrg, e->tables.count[0] + q0, qparts, valid.as<uint32_t>(), obs.as<uint64_t>(),
+                                    m.as<double>(), ex.as<double>(), st));
         std::vector<uint32_t> hv(ns);
         std::vector<uint64_t> ho(ns);
-        std::vector<double> hm(ns), hx(ns), hs(ns), he(ns);
+        std::vector<double> hm(ns), hx(ns), hs(ns, 0.0), hc(ns, 0.0), he(ns);
         if (ns) {
             HIP_TRY(hipMemcpyAsync(hv.data(), valid.p, 4 * ns, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipMemcpyAsync(ho.data(), obs.p, 8 * ns, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipMemcpyAsync(hm.data(), m.p, 8 * ns, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipMemcpyAsync(hx.data(), ex.p, 8 * ns, hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipMemcpyAsync(hs.data(), es.p, 8 * ns, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            for (uint32_t s = 0; s < ns; s++)
+                if (hv[s]) host_terms(ho[s], hm[s], hx[s], log10e, &hs[s], &hc[s]);
+            HIP_TRY(hipMemcpyAsync(es.p, hs.data(), 8 * ns, hipMemcpyHostToDevice, st));
+            HIP_TRY(hipMemcpyAsync(cf.p, hc.data(), 8 * ns, hipMemcpyHostToDevice, st));
+            // slots without rows evaluate harmless zeros and are skipped below
+            HIP_TRY(fpd_launch_empirical(ns, obs.as<uint64_t>(), m.as<double>(), es.as<double>(), cf.as<double>(), 0u,
+                                         em.as<double>(), st));
             HIP_TRY(hipMemcpyAsync(he.data(), em.p, 8 * ns, hipMemcpyDeviceToHost, st));
         }
         HIP_TRY(hipStreamSynchronize(st));
@@ -1197,7 +1308,7 @@ fp_status postprocess_range(fp_engine *e, uint32_t rg_base, uint32_t nrg, TableR
 
 fp_status postprocess(fp_engine *e)
 {
-    fp_status rc = postprocess_range(e, 0, e->num_rg, e->rows);
+    fp_status rc = postprocess_range(e, 0, e->num_rg, e->rows, e->stream);
     if (rc != FP_OK) return rc;
     e->postprocessed = true;
     return FP_OK;
@@ -1279,6 +1390,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     e->wpe = o.baq_waves_per_simd >= 1 && o.baq_waves_per_simd <= 4 ? (int)o.baq_waves_per_simd : 1;
     e->ckpt = o.baq_checkpoint_rows == 2 ? 2 : 1;
     e->sort_bits = o.sort_digit_bits ? (int)o.sort_digit_bits : 8;
+    e->budget_bytes = (uint64_t)o.serve_hbm_budget_mib << 20;
     // the digit width is tuned on the batched engine's first staged batches unless it is given
     e->sort_tuned = o.sort_digit_bits != 0 || o.aggregation != FP_AGG_SORT || o.engine_kind != FP_ENGINE_BATCHED;
     memset(&e->stats, 0, sizeof(e->stats));
@@ -1287,7 +1399,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (he != hipSuccess) { delete e; return fail(FP_ERR_HIP, hipGetErrorString(he)); }
     for (int i = 0; i < 16; i++) (void)hipEventCreate(&e->ev[i]);
-    he = hipHostMalloc((void **)&e->host_counts, 64 * sizeof(uint32_t), hipHostMallocDefault);
+    he = hipHostMalloc((void **)&e->host_counts, 128 * sizeof(uint32_t), hipHostMallocDefault);
     if (he == hipSuccess) he = fpd_cov_prepare();
     if (he == hipSuccess) he = fpd_baq_prepare();
     if (he != hipSuccess) { fp_engine_destroy(e); return fail(FP_ERR_HIP, hipGetErrorString(he)); }
@@ -1404,6 +1516,8 @@ fp_status fp_engine_process_batch(fp_engine *e, const fp_batch_view *hb)
             (uint64_t)hb->read_start[i] + ((hb->read_len[i] + 7) & ~7u) > hb->num_read_bases ||
             (uint64_t)hb->qual_start[i] + hb->qual_len[i] > hb->num_qualities || (hb->read_start[i] & 7))
             return fail(FP_ERR_INVALID_ARG, "read " + std::to_string(i) + " lies outside the batch arrays");
+        if (e->batched && hb->read_len[i] > e->rows_cap)
+            return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(e->rows_cap) + ")");
     }
     if (e->batched) return batched_append(e, hb, false);
     if (n == 0) { e->stats.num_batches++; return FP_OK; }
@@ -1533,7 +1647,7 @@ fp_status fp_engine_get_table(fp_engine *e, int table_id, fp_table_out *out)
         }
         DevBuf k, c, lo, hi;
         uint64_t nr = 0;
-        rc = compact(e, table_id - 1, 0, e->num_rg, k, c, lo, hi, &nr);
+        rc = compact(e, table_id - 1, 0, e->num_rg, k, c, lo, hi, &nr, e->stream);
         if (rc != FP_OK) return rc;
         tmp.key.resize(nr); tmp.obs.resize(nr); tmp.lo.resize(nr); tmp.hi.resize(nr);
         if (nr) {
@@ -1676,16 +1790,27 @@ fp_status fp_engine_reset(fp_engine *e)
 {
     fp_status rc = check_engine(e, true);
     if (rc != FP_OK) return rc;
+    // Reads still staged are dropped, not run: a reset also recovers an engine whose staged batch
+    // failed a check.  Work in flight is waited for.
+    for (auto &s : e->stg)
+        if (s.stream) HIP_TRY(hipStreamSynchronize(s.stream));
+    if (e->side) HIP_TRY(hipStreamSynchronize(e->side));
+    HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->batched) {
-        rc = batched_sync(e);
-        if (rc != FP_OK) return rc;
+        for (auto &s : e->stg) {
+            collect_back_time(e, s);
+            s.n = 0;
+            s.ncig = s.nbases = s.nquals = 0;
+            s.fill += 1;
+            if (e->staged_ready) HIP_TRY(fpd_fill_u32(s.bs.baq_counts.as<uint32_t>(), 144, 0u, e->stream));
+        }
         if (e->staged_ready) HIP_TRY(fpd_fill_u32(e->bcounters.as<uint32_t>(), 16, 0u, e->stream));
     }
     for (int t = 0; t < 3; t++)
         for (int k = 0; k < 4; k++) HIP_TRY(hipMemsetAsync(e->tab[t][k].p, 0, e->tab_size[t] * 8, e->stream));
     HIP_TRY(hipStreamSynchronize(e->stream));
     memset(&e->stats, 0, sizeof(e->stats));
-    e->eager_filtered = e->eager_baq = e->eager_cells = 0;
+    e->eager_cells = e->eager_cells15 = 0;
     e->postprocessed = false;
     for (auto &r : e->rows) r = TableRows();
     return FP_OK;
@@ -1721,6 +1846,10 @@ void fp_engine_destroy(fp_engine *e)
         s.bs.release();
     }
     for (auto &ev : e->bev) if (ev) (void)hipEventDestroy(ev);
+    if (e->post) (void)hipStreamSynchronize(e->post);
+    for (auto &b : e->pw) b.release();
+    e->check.release();
+    if (e->post) (void)hipStreamDestroy(e->post);
     if (e->host_counts) (void)hipHostFree(e->host_counts);
     for (auto &b : e->in) b.release();
     for (auto &t : e->tab) for (auto &b : t) b.release();
@@ -1732,6 +1861,18 @@ void fp_engine_destroy(fp_engine *e)
 
 // --- serving jobs -----------------------------------------------------------
 
+// a job's HBM footprint: the resident tables of its read groups (count and three limb sums per
+// entry of the three direct-indexed tables) plus a reserve for its postprocess rows
+static constexpr uint64_t TABLE_BYTES_PER_RG = 32ull * ((1ull << 10) + (1ull << 20) + (1ull << 21));
+static constexpr uint64_t POST_RESERVE_PER_RG = 8ull << 20;
+
+static fp_status post_stream(fp_engine *e, hipStream_t *out)
+{
+    if (!e->post) HIP_TRY(hipStreamCreateWithFlags(&e->post, hipStreamNonBlocking));
+    *out = e->post;
+    return FP_OK;
+}
+
 fp_status fp_job_open(fp_engine *e, uint32_t num_read_groups, const char *const *read_group_names, fp_job **out)
 {
     if (!out) return fail(FP_ERR_INVALID_ARG, "null output pointer");
@@ -1741,6 +1882,12 @@ fp_status fp_job_open(fp_engine *e, uint32_t num_read_groups, const char *const 
     if (e->slot_owner.empty()) return fail(FP_ERR_STATE, "not a serving engine (serve_rg_slots = 0)");
     const uint32_t n = num_read_groups;
     if (n == 0 || n > e->num_rg) return fail(FP_ERR_INVALID_ARG, "a job needs 1..serve_rg_slots read groups");
+    const uint64_t bytes = n * (TABLE_BYTES_PER_RG + POST_RESERVE_PER_RG);
+    if (e->budget_bytes && bytes > e->budget_bytes)
+        return fail(FP_ERR_INVALID_ARG, "the job's footprint exceeds the whole HBM budget");
+    if (e->budget_bytes && e->bytes_in_use + bytes > e->budget_bytes)
+        return fail(FP_ERR_BUDGET, "no room for the job now (HBM budget: " + std::to_string(e->bytes_in_use) + " of " +
+                                       std::to_string(e->budget_bytes) + " bytes in use)");
     // first fit of n contiguous free slots: the job's keys form one range per table
     uint32_t run = 0, base = 0;
     bool found = false;
@@ -1755,6 +1902,8 @@ fp_status fp_job_open(fp_engine *e, uint32_t num_read_groups, const char *const 
     j->id = e->next_job_id++;
     j->base = base;
     j->nrg = n;
+    j->bytes = bytes;
+    e->bytes_in_use += bytes;
     for (uint32_t i = 0; i < n; i++)
         j->names.push_back(read_group_names && read_group_names[i] ? read_group_names[i] : "");
     for (uint32_t i = 0; i < n; i++) e->slot_owner[base + i] = j->id;
@@ -1771,6 +1920,8 @@ static fp_status job_submit(fp_job *j, const fp_batch_view *b, bool on_device)
     if (rc != FP_OK) return rc;
     rc = validate_view(b);
     if (rc != FP_OK) return rc;
+    // a job's batch is checked before it joins a staging buffer shared with other jobs, so a bad batch
+    // fails its own submit and never another job's
     if (!on_device) {
         for (uint32_t i = 0; i < b->num_reads; i++) {
             if ((uint64_t)b->cigar_start[i] + b->cigar_len[i] > b->num_cigar_ops ||
@@ -1779,10 +1930,27 @@ static fp_status job_submit(fp_job *j, const fp_batch_view *b, bool on_device)
                 return fail(FP_ERR_INVALID_ARG, "read " + std::to_string(i) + " lies outside the batch arrays");
             if (b->read_group[i] >= j->nrg)
                 return fail(FP_ERR_INVALID_ARG, "read group id beyond the job's read groups");
+            if (b->read_len[i] > e->rows_cap)
+                return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(e->rows_cap) + ")");
         }
+    } else if (b->num_reads) {
+        hipStream_t ps;
+        rc = post_stream(e, &ps);
+        if (rc != FP_OK) return rc;
+        HIP_TRY(e->check.ensure(16));
+        HIP_TRY(fpd_fill_u32(e->check.as<uint32_t>(), 2, 0u, ps));
+        const fpd::BatchDev db = to_dev(b);
+        HIP_TRY(fpd_check_batch(&db, j->nrg, e->check.as<uint32_t>(), ps));
+        uint32_t *h = e->host_counts + 96;
+        HIP_TRY(hipMemcpyAsync(h, e->check.p, 8, hipMemcpyDeviceToHost, ps));
+        HIP_TRY(hipStreamSynchronize(ps));
+        if (h[1]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the job's read groups");
+        if (h[0] > e->rows_cap)
+            return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(e->rows_cap) + ")");
     }
     rc = batched_append(e, b, on_device, j->base, j->nrg);
     if (rc != FP_OK) return rc;
+    j->fill[e->cur] = e->stg[e->cur].fill;
     j->reads += b->num_reads;
     j->batches += 1;
     return FP_OK;
@@ -1799,9 +1967,20 @@ fp_status fp_job_close(fp_job *j)
     fp_status rc = check_engine(e, true);
     if (rc != FP_OK) return rc;
     auto t0 = std::chrono::steady_clock::now();
-    rc = batched_sync(e);   // runs whatever is staged, this job's reads included
+    // the job's last reads may still be staging: run that buffer now (with whatever else it holds)
+    if (j->fill[e->cur] == e->stg[e->cur].fill && e->stg[e->cur].n > 0) {
+        rc = batched_flush(e);
+        if (rc != FP_OK) return rc;
+    }
+    // The job's postprocess waits for the covariate stage of the buffers that held its reads (each
+    // buffer's stream is in order, so its latest event covers every earlier fill), not for the whole
+    // engine: other jobs' batches keep running meanwhile.
+    hipStream_t ps;
+    rc = post_stream(e, &ps);
     if (rc != FP_OK) return rc;
-    rc = postprocess_range(e, j->base, j->nrg, j->rows);
+    for (int s = 0; s < fp_engine::NSTAGE; s++)
+        if (j->fill[s] && e->stg[s].cov_done) HIP_TRY(hipStreamWaitEvent(ps, e->stg[s].cov_done, 0));
+    rc = postprocess_range(e, j->base, j->nrg, j->rows, ps);
     if (rc != FP_OK) return rc;
     j->post_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     j->closed = true;
@@ -1863,15 +2042,20 @@ void fp_job_release(fp_job *j)
     fp_engine *e = j->e;
     if (e && check_engine(e, true) == FP_OK) {
         if (!j->closed) (void)batched_sync(e);   // nothing of the job may still be in flight
-        for (int t = 0; t < 3; t++) {
-            const uint64_t off0 = (uint64_t)j->base << (fpd::TABLE_BITS[t] - 8);
-            const uint64_t n = (uint64_t)j->nrg << (fpd::TABLE_BITS[t] - 8);
-            for (int k = 0; k < 4; k++)
-                (void)hipMemsetAsync(e->tab[t][k].as<uint64_t>() + off0, 0, 8 * n, e->stream);
+        hipStream_t ps = nullptr;
+        if (post_stream(e, &ps) == FP_OK) {
+            // a closed job's reads are through their covariate stages (its close waited for them)
+            for (int t = 0; t < 3; t++) {
+                const uint64_t off0 = (uint64_t)j->base << (fpd::TABLE_BITS[t] - 8);
+                const uint64_t n = (uint64_t)j->nrg << (fpd::TABLE_BITS[t] - 8);
+                for (int k = 0; k < 4; k++)
+                    (void)hipMemsetAsync(e->tab[t][k].as<uint64_t>() + off0, 0, 8 * n, ps);
+            }
+            (void)hipStreamSynchronize(ps);
         }
-        (void)hipStreamSynchronize(e->stream);
         for (uint32_t i = 0; i < j->nrg; i++)
             if (e->slot_owner[j->base + i] == j->id) e->slot_owner[j->base + i] = -1;
+        e->bytes_in_use -= std::min(e->bytes_in_use, j->bytes);
     }
     delete j;
 }

@@ -42,6 +42,20 @@ static constexpr uint16_t NONE16 = 0xffff;
 // covariate table geometry: key bits 18 / 28 / 29, RG in the top 8 bits
 static constexpr int TABLE_BITS[3] = {18, 28, 29};
 
+// Per-batch counter words.  Each batch state owns a block of u32 words that the front stage of its
+// batch zeroes first; words 0..3 are the BAQ band-class sizes and 8..10 the band kernels' chunk
+// counters.  The words below are the batch's own read counts and error flags, so an error of one
+// staged batch never outlives that batch.
+enum : uint32_t {
+    BW_OVERFLOW = 5,    // sort aggregation: the batch's pairs overflowed their buffer
+    BW_ACTIVE = 11,     // reads left active by the filters
+    BW_FILTERED = 12,   // reads dropped by the filters
+    BW_BAQ = 13,        // active reads that need BAQ (at least one error)
+    BW_LONGEST = 14,    // longest read of the batch (read_len)
+    BW_BAD_RG = 15,     // active reads whose read group lies beyond the tables
+    BW_WORDS = 16       // words read back after the front stage
+};
+
 struct BatchDev {
     uint32_t num_reads;            // reads, or the capacity when `live` is set
     const uint32_t *live;          // device {reads, qualities lo, qualities hi} of a staged batch; null: num_reads

@@ -901,7 +901,7 @@ __global__ void __launch_bounds__(256) k_rg_count(uint32_t n, const uint32_t *li
     __syncthreads();
     for (uint32_t g = threadIdx.x; g < num_rg; g += blockDim.x)
         if (cnt[g]) atomicAdd(&rg_count[g], cnt[g]);
-    if (threadIdx.x == 0 && nbad) atomicAdd(errors, nbad);
+    if (threadIdx.x == 0 && nbad && errors) atomicAdd(errors, nbad);
 }
 
 __global__ void __launch_bounds__(256) k_rg_offsets(uint32_t num_rg, const uint32_t *rg_count, uint32_t *rg_offset,
@@ -989,7 +989,7 @@ extern "C" hipError_t fpd_cov_prepare()
 }
 
 extern "C" hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
-                                     uint32_t frac_f64, uint32_t keep_frac, uint32_t *counters, uint32_t *rg_buf,
+                                     uint32_t frac_f64, uint32_t keep_frac, uint32_t *batch, uint32_t *rg_buf,
                                      uint32_t max_read_len, uint32_t *const *pkey, double *const *pval,
                                      uint32_t *pcursor, uint64_t pcap, hipStream_t st)
 {
@@ -1002,7 +1002,7 @@ extern "C" hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStat
     hipLaunchKernelGGL(fpd::k_fill_u32c, dim3(1), dim3(256), 0, st, cnt, nrg, 0u);
     const uint32_t rg_blocks = std::min<uint32_t>((n + 255) / 256, fpd::RG_BLOCKS);
     hipLaunchKernelGGL(fpd::k_rg_count, dim3(rg_blocks), dim3(256), 0, st, n, b->live, s->status, b->read_group,
-                       nrg, cnt, counters + 4);
+                       nrg, cnt, (uint32_t *)nullptr);   // the prep kernel counts such reads (BW_BAD_RG)
     // tiles of 1024 reads for large batches; smaller ones (down to 128) keep a small batch spread
     // over the whole device
     const uint32_t tile = std::max<uint32_t>(fpd::TILE_MIN, std::min<uint32_t>(fpd::TILE, (n / 1024 + 63) & ~63u));
@@ -1017,7 +1017,7 @@ extern "C" hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStat
         // pair mode (sort aggregation): pairs out instead of table updates; LDS holds the base stages
         for (int i = 0; i < 3; i++) { a.pkey[i] = pkey[i]; a.pval[i] = pval[i]; }
         a.pcursor = pcursor;
-        a.perr = counters + 6;
+        a.perr = batch + fpd::BW_OVERFLOW;
         a.pcap = pcap;
         hipLaunchKernelGGL(fpd::k_cov<true>, dim3(blocks), dim3(64 * fpd::COV_WAVES), fpd::STAGE_BYTES, st, a);
         return hipGetLastError();

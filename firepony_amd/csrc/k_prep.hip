@@ -113,7 +113,8 @@ struct PrepArgs {
     BatchDev b;
     GenomeDev g;
     ReadStateDev s;
-    uint32_t *counters;   // [0] active reads, [1] filtered, [2] baq reads, [3] max read length
+    uint32_t *batch;      // the batch's counter words (BW_ACTIVE .. BW_BAD_RG)
+    uint32_t num_rg;      // read groups of the tables (active reads beyond them are counted as errors)
     uint32_t lds_len;     // LDS flag row per read (multiple of 8); longer reads take the lane path
 };
 
@@ -840,11 +841,13 @@ __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
     const uint64_t m_act = __ballot(valid && st == RS_ACTIVE);
     const uint64_t m_filt = __ballot(valid && st == RS_FILTERED);
     const uint64_t m_baq = __ballot(valid && st == RS_ACTIVE && nerr != 0);
+    const uint64_t m_rg = __ballot(valid && st == RS_ACTIVE && b.read_group[valid ? r : 0] >= a.num_rg);
     if (lane == 0) {
-        if (m_act) atomicAdd(&a.counters[0], (uint32_t)__popcll(m_act));
-        if (m_filt) atomicAdd(&a.counters[1], (uint32_t)__popcll(m_filt));
-        if (m_baq) atomicAdd(&a.counters[2], (uint32_t)__popcll(m_baq));
-        if (mx) atomicMax(&a.counters[3], mx);
+        if (m_act) atomicAdd(&a.batch[BW_ACTIVE], (uint32_t)__popcll(m_act));
+        if (m_filt) atomicAdd(&a.batch[BW_FILTERED], (uint32_t)__popcll(m_filt));
+        if (m_baq) atomicAdd(&a.batch[BW_BAQ], (uint32_t)__popcll(m_baq));
+        if (m_rg) atomicAdd(&a.batch[BW_BAD_RG], (uint32_t)__popcll(m_rg));
+        if (mx) atomicMax(&a.batch[BW_LONGEST], mx);
     }
 }
 
@@ -852,11 +855,11 @@ __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
 
 // host launcher
 extern "C" hipError_t fpd_launch_prep(const fpd::BatchDev *b, const fpd::GenomeDev *g,
-                                      const fpd::ReadStateDev *s, uint32_t *counters, uint32_t max_read_len,
-                                      hipStream_t st)
+                                      const fpd::ReadStateDev *s, uint32_t *batch, uint32_t num_rg,
+                                      uint32_t max_read_len, hipStream_t st)
 {
     fpd::PrepArgs a{};
-    a.b = *b; a.g = *g; a.s = *s; a.counters = counters;
+    a.b = *b; a.g = *g; a.s = *s; a.batch = batch; a.num_rg = num_rg;
     const uint32_t n = b->num_reads;
     if (n == 0) return hipSuccess;
     // flag rows sized for the longest expected read (256 when unknown)
@@ -950,6 +953,38 @@ __global__ void __launch_bounds__(256) k_append(AppendArgs a)
     for (uint64_t r = head + 4 * words + tid; r < nb; r += stride) d[r] = s[r];
 }
 } // namespace fpd
+
+namespace fpd {
+// a job's device batch before it joins a shared staging buffer: the longest read and the reads whose
+// read group lies at or beyond the job's own (out[0] max read_len, out[1] such reads; out zeroed by
+// the caller)
+__global__ void __launch_bounds__(256) k_check_batch(BatchDev b, uint32_t rg_limit, uint32_t *out)
+{
+    uint32_t mx = 0, bad = 0;
+    for (uint32_t r = blockIdx.x * 256 + threadIdx.x; r < b.num_reads; r += gridDim.x * 256) {
+        const uint32_t L = b.read_len[r];
+        mx = L > mx ? L : mx;
+        bad += b.read_group[r] >= rg_limit ? 1u : 0u;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t v = (uint32_t)__shfl_xor((int)mx, o);
+        mx = v > mx ? v : mx;
+        bad += (uint32_t)__shfl_xor((int)bad, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (mx) atomicMax(&out[0], mx);
+        if (bad) atomicAdd(&out[1], bad);
+    }
+}
+} // namespace fpd
+
+extern "C" hipError_t fpd_check_batch(const fpd::BatchDev *b, uint32_t rg_limit, uint32_t *out, hipStream_t st)
+{
+    if (b->num_reads == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((b->num_reads + 255) / 256, 1024u);
+    hipLaunchKernelGGL(fpd::k_check_batch, dim3(blocks), dim3(256), 0, st, *b, rg_limit, out);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t fpd_append(const void *const *src, void *const *dst, const uint64_t *bytes, uint32_t cig_off,
                                  uint32_t base_off, uint32_t qual_off, uint32_t rg_off, uint32_t rg_limit,

@@ -94,7 +94,9 @@ typedef struct fp_options {
                                         (same results, a quarter less forward-row
                                         traffic, a second forward pass); 0 = 1.  Used
                                         with LDS staging at one wave per SIMD.          */
-    uint32_t reserved[1];
+    uint32_t serve_hbm_budget_mib;   /* serving: HBM admission budget in MiB for the open jobs'
+                                        resident tables and postprocess reserve; 0 = the
+                                        read-group slots alone bound admission          */
 } fp_options;
 
 /* how covariate observations reach the resident tables (identical results) */
@@ -230,6 +232,10 @@ typedef struct fp_stats {
     uint64_t sort_pairs;         /*   slots emitted, real pairs sorted, and radix       */
     uint64_t sort_passes;        /*   passes run (summed over tables and batches)       */
     uint64_t baq_checkpoint_rows; /* rows per forward checkpoint of the 15-cell BAQ kernel */
+    double baq15_kernel_ms;      /* profile_stages: device time of the 15-cell band kernel alone
+                                    (HIP events around each of its launches)                   */
+    uint64_t baq15_launches;     /* profile_stages: its launches                              */
+    uint64_t baq15_cells;        /* band cells of the 15-cell class (its algorithmic work)     */
 } fp_stats;
 
 /* Per-batch intermediates of the last processed batch, for stage-by-stage

@@ -3,8 +3,10 @@
 Stage-by-stage: read filters, windows, error bits, base masks, BAQ tags,
 fractional errors, high-quality windows, the raw covariate tables (counts and
 exact fixed-point mismatch sums), the postprocessed tables and the report.
-Integer and index results must match bit for bit; the estimated quality
-(device log10) within 1e-9 relative.
+Integer and index results must match bit for bit, and so must every floating
+point result: fractional errors, mismatch sums, expected errors, estimated and
+empirical qualities (their libm terms come from the host's libm, as the
+oracle's do).
 """
 import os
 import sys
@@ -87,8 +89,8 @@ def compare_final(o, e, tmp_path):
         np.testing.assert_array_equal(b["observations"], a["observations"])
         np.testing.assert_array_equal(b["mismatches"], a["mismatches"], err_msg=f"final table {t} mismatches")
         np.testing.assert_array_equal(b["expected_errors"], a["expected_errors"], err_msg=f"final table {t} expected")
-        np.testing.assert_allclose(b["estimated_quality"], a["estimated_quality"], rtol=1e-9, atol=0,
-                                   err_msg=f"final table {t} estimated quality")
+        np.testing.assert_array_equal(b["estimated_quality"], a["estimated_quality"],
+                                      err_msg=f"final table {t} estimated quality")
         bad = np.nonzero(b["empirical_quality"] != a["empirical_quality"])[0]
         assert bad.size == 0, (f"table {t}: {bad.size} empirical qualities differ; key {a['keys'][bad[0]]:#x} "
                                f"oracle {a['empirical_quality'][bad[0]]} engine {b['empirical_quality'][bad[0]]}")
