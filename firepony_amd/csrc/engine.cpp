@@ -1047,6 +1047,27 @@ fp_status batched_sync(fp_engine *e)
 
 fp_status plain_host_batch(fp_engine *e, const fp_batch_view *hb);
 
+// Device-side addresses of a host batch's arrays when every one of them is pinned host memory (the
+// device then reads them over the host link itself, in one launch); false when any array is
+// pageable, which only the runtime's copies can move.
+static bool pinned_sources(const void *const *src, const uint64_t *len, const void **dsrc)
+{
+    for (int i = 0; i < 18; i++) {
+        dsrc[i] = nullptr;
+        if (!len[i]) continue;
+        hipPointerAttribute_t at;
+        if (hipPointerGetAttributes(&at, src[i]) != hipSuccess) { (void)hipGetLastError(); return false; }
+        if (at.type != hipMemoryTypeHost) return false;
+        void *dp = nullptr;
+        if (hipHostGetDevicePointer(&dp, const_cast<void *>(src[i]), 0) != hipSuccess || !dp) {
+            (void)hipGetLastError();
+            return false;
+        }
+        dsrc[i] = dp;
+    }
+    return true;
+}
+
 // append one batch (host or device arrays) to the current staging buffer; a serving job's reads move
 // to its read-group slots [rg_off, rg_off + rg_limit)
 fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, uint32_t rg_off = 0,
@@ -1097,10 +1118,13 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
         else { off = elem[i] * s->n; len[i] = elem[i] * k; }
         dst[i] = (char *)s->arr[i].p + off;
     }
-    if (on_device) {
-        // one launch copies and rebases the whole batch (a copy per array costs a launch each)
-        HIP_TRY(fpd_append(src, dst, len, (uint32_t)s->ncig, (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off,
-                           rg_limit, cs));
+    const void *hsrc[18];
+    const bool pinned = !on_device && pinned_sources(src, len, hsrc);
+    if (on_device || pinned) {
+        // one launch copies and rebases the whole batch (a copy per array costs a launch each); a
+        // pinned host batch is read over the host link by the same kernel
+        HIP_TRY(fpd_append(pinned ? hsrc : src, dst, len, (uint32_t)s->ncig, (uint32_t)s->nbases,
+                           (uint32_t)s->nquals, rg_off, rg_limit, cs));
     } else {
         for (int i = 0; i < 18; i++)
             if (len[i]) HIP_TRY(hipMemcpyAsync(dst[i], src[i], len[i], kind, cs));
@@ -1136,9 +1160,17 @@ fp_status plain_host_batch(fp_engine *e, const fp_batch_view *hb)
                                 4 * hb->num_cigar_ops, 4ull * n, 4ull * n,
                                 hb->num_read_bases / 2, 4ull * n, 4ull * n,
                                 hb->num_qualities, 4ull * n, 4ull * n, 2ull * n, 1ull * n, 4ull * n};
-    for (int i = 0; i < 18; i++) {
-        HIP_TRY(e->in[i].ensure(sz[i]));
-        if (copy_sz[i]) HIP_TRY(hipMemcpyAsync(e->in[i].p, src[i], copy_sz[i], hipMemcpyHostToDevice, st));
+    for (int i = 0; i < 18; i++) HIP_TRY(e->in[i].ensure(sz[i]));
+    const void *hsrc[18];
+    uint64_t len[18];
+    void *dst[18];
+    for (int i = 0; i < 18; i++) { len[i] = copy_sz[i]; dst[i] = e->in[i].p; }
+    if (pinned_sources(src, len, hsrc)) {
+        // pinned host arrays: one copy launch reads all of them over the host link
+        HIP_TRY(fpd_append(hsrc, dst, len, 0, 0, 0, 0, 0xffffffffu, st));
+    } else {
+        for (int i = 0; i < 18; i++)
+            if (copy_sz[i]) HIP_TRY(hipMemcpyAsync(e->in[i].p, src[i], copy_sz[i], hipMemcpyHostToDevice, st));
     }
     fp_batch_view dv = *hb;
     dv.chromosome = e->in[0].as<uint16_t>();

@@ -16,6 +16,8 @@ def hip():
         _HIP.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         _HIP.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _HIP.hipFree.argtypes = [C.c_void_p]
+        _HIP.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        _HIP.hipHostFree.argtypes = [C.c_void_p]
     return _HIP
 
 
@@ -52,4 +54,31 @@ class DeviceCopy:
     def close(self):
         for p in self.ptrs:
             hip().hipFree(p)
+        self.ptrs = []
+
+
+class PinnedCopy:
+    """fp_batch_view whose arrays live in pinned host memory (hipHostMalloc); frees them on close.
+    The engine reads such a batch over the host link in one copy launch."""
+
+    def __init__(self, batch):
+        arrs = batch.arrays()
+        self.ptrs = []
+        v = abi.FpBatchView()
+        hv = batch.view
+        v.num_reads, v.num_cigar_ops = hv.num_reads, hv.num_cigar_ops
+        v.num_read_bases, v.num_qualities = hv.num_read_bases, hv.num_qualities
+        for name, ptype in FIELDS:
+            a = np.ascontiguousarray(arrs[name])
+            p = C.c_void_p()
+            if hip().hipHostMalloc(C.byref(p), max(a.nbytes, 4), 0) != 0:
+                raise RuntimeError("hipHostMalloc failed")
+            self.ptrs.append(p)
+            C.memmove(p, a.ctypes.data, a.nbytes)
+            setattr(v, name, C.cast(p, ptype))
+        self.view = v
+
+    def close(self):
+        for p in self.ptrs:
+            hip().hipHostFree(p)
         self.ptrs = []

@@ -103,3 +103,35 @@ def test_batched_device_input_matches_host_input():
     dev.close()
     for d in keep:
         d.close()
+
+
+@pytest.mark.parametrize("kind", [abi.FP_ENGINE_BATCHED, abi.FP_ENGINE_PLAIN])
+def test_pinned_host_input_against_oracle(tmp_path, kind):
+    say(f"{'batched' if kind else 'plain'}: pinned host batches (read by the copy kernel over the host link) vs "
+        f"pageable host batches and the oracle")
+    from devbatch import PinnedCopy
+    g, rs, n = mixed_workload(9000, seed=45)
+    o = Oracle(g)
+    opts = dict(engine_kind=kind, max_batch_reads=2048, max_read_length=250, baq_waves_per_simd=1) if kind else {}
+    pinned = Engine(g, **opts)
+    pageable = Engine(g, **opts)
+    keep = []
+    for first in range(0, n, 1000):
+        b = synth.Batch(g, rs, first, 1000)
+        o.process(b)
+        pageable.process(b)
+        p = PinnedCopy(b)
+        keep.append(p)
+        pinned.process(p)
+    same_tables(pageable, pinned, "pinned vs pageable input")
+    o.postprocess()
+    pinned.postprocess()
+    for t in (abi.FP_TABLE_QUALITY, abi.FP_TABLE_CYCLE, abi.FP_TABLE_CONTEXT):
+        a, b = o.table(t), pinned.table(t)
+        for k in ("keys", "observations", "mismatches_fixed_lo", "mismatches_fixed_hi"):
+            np.testing.assert_array_equal(b[k], a[k], err_msg=f"table {t} {k}")
+    assert o.report(str(tmp_path / "o.txt")) == pinned.report(str(tmp_path / "p.txt"))
+    pinned.close()
+    pageable.close()
+    for p in keep:
+        p.close()
