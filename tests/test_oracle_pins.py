@@ -16,6 +16,7 @@ from oracle import pyoracle
 from oracle.pyoracle import Oracle
 
 from handbatch import HandBatch, HandGenome, random_genome_string
+from refmodel import hmm_independent
 
 COMP = {"A": "T", "C": "G", "G": "C", "T": "A"}
 
@@ -243,70 +244,6 @@ def test_fractional_errors_hand_cases(baq, err, exp):
 
 # ---------------------------------------------------------------------------
 # BAQ: an unbanded glocal HMM written from the model definition
-
-EM, EI = 0.33333333333, 0.25
-
-
-def hmm_independent(ref, query, qual, bw):
-    """Forward-backward over the full (query x reference) lattice with states
-    M/I/D, the BAQ transition model, and cells outside |i - k| <= bw removed.
-    Returns (MAP state, MAP posterior) per query base."""
-    lr, lq = len(ref), len(query)
-    gop, gep = 10 ** -4.0, 0.1
-    sM = 1.0 / (2 * lq + 2)
-    bM, bI = (1 - gop) / lr, gop / lr
-    t = np.array([[(1 - 2 * gop) * (1 - sM), gop * (1 - sM), gop * (1 - sM)],
-                  [(1 - gep) * (1 - sM), gep * (1 - sM), 0.0],
-                  [1 - gep, 0.0, gep]])
-
-    def emis(k, i):
-        if ref[k - 1] == 15 or query[i - 1] == 15:
-            return 1.0
-        p = 10 ** (-max(qual[i - 1], 4) / 10.0)
-        return 1 - p if ref[k - 1] == query[i - 1] else p * EM
-
-    inband = lambda i, k: 1 <= k <= lr and abs(i - k) <= bw
-    F = np.zeros((lq + 1, lr + 2, 3))
-    s = np.zeros(lq + 2)
-    for k in range(1, lr + 1):
-        if inband(1, k):
-            F[1, k, 0] = emis(k, 1) * bM
-            F[1, k, 1] = EI * bI
-    s[1] = F[1].sum()
-    F[1] /= s[1]
-    for i in range(2, lq + 1):
-        for k in range(1, lr + 1):
-            if not inband(i, k):
-                continue
-            F[i, k, 0] = emis(k, i) * (F[i - 1, k - 1] @ t[:, 0])
-            F[i, k, 1] = EI * (F[i - 1, k, 0] * t[0, 1] + F[i - 1, k, 1] * t[1, 1])
-            F[i, k, 2] = F[i, k - 1, 0] * t[0, 2] + F[i, k - 1, 2] * t[2, 2]
-        s[i] = F[i].sum()
-        F[i] /= s[i]
-    s[lq + 1] = (F[lq, :, 0] * sM + F[lq, :, 1] * sM).sum()
-    B = np.zeros((lq + 2, lr + 2, 3))
-    for k in range(1, lr + 1):
-        if inband(lq, k):
-            B[lq, k, 0] = sM / s[lq] / s[lq + 1]
-            B[lq, k, 1] = sM / s[lq] / s[lq + 1]
-    for i in range(lq - 1, 0, -1):
-        for k in range(lr, 0, -1):
-            if not inband(i, k):
-                continue
-            e = emis(k + 1, i + 1) * B[i + 1, k + 1, 0] if k < lr else 0.0
-            B[i, k, 0] = e * t[0, 0] + EI * t[0, 1] * B[i + 1, k, 1] + t[0, 2] * B[i, k + 1, 2]
-            B[i, k, 1] = e * t[1, 0] + EI * t[1, 1] * B[i + 1, k, 1]
-            B[i, k, 2] = (e * t[2, 0] + t[2, 2] * B[i, k + 1, 2]) * (1.0 if i > 1 else 0.0)
-        B[i] /= s[i]
-    states, post = [], []
-    for i in range(1, lq + 1):
-        z = F[i, :, :2] * B[i, :, :2]
-        tot = z.sum()
-        k, st = np.unravel_index(np.argmax(z), z.shape)
-        states.append(int(k - 1) << 2 | int(st))
-        post.append(z[k, st] / tot)
-    return states, post
-
 
 @pytest.mark.parametrize("seed,lq,bw,full", [(1, 25, 7, False), (2, 30, 40, True), (3, 40, 10, False),
                                              (4, 18, 30, True)])
