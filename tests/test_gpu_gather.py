@@ -1,13 +1,16 @@
 """gather_intermediates on the GPU (fp_engine_gather): engines that each took part of the
 batches, merged into one, give the tables and report of a single engine that took them all,
 whatever the split, the order of the merges and the engine forms; the reference's multi-device
-flow (device/pipeline_interface.cu:52-58,137-160) with exact sums."""
+flow (device/pipeline_interface.cu:52-58,137-160) with exact sums.  Both are checked against the
+oracle run over all the batches, and against oracles that each took the same part and were
+gathered the same way (fpo_gather)."""
 import sys
 
 import numpy as np
 
 from firepony_amd import abi, synth
 from firepony_amd.engine import Engine
+from oracle.pyoracle import Oracle
 
 import pytest
 
@@ -43,8 +46,21 @@ def test_gather_equals_one_engine(tmp_path):
     parts[0].gather(parts[1])
     same_tables(one, parts[0], "gathered")
     assert parts[0].stats().total_reads == one.stats().total_reads == n
+    o = Oracle(g)
+    for b in batches:
+        o.process(b)
+    same_tables(o, parts[0], "gathered engines vs the oracle over all batches")
+    ops = [Oracle(g) for _ in range(3)]
+    for i, b in enumerate(batches):
+        ops[i % 3].process(b)
+    ops[1].gather(ops[2])
+    ops[0].gather(ops[1])
+    same_tables(ops[0], parts[0], "gathered engines vs gathered oracles")
     one.postprocess()
     parts[0].postprocess()
-    assert one.report(str(tmp_path / "one.txt")) == parts[0].report(str(tmp_path / "gathered.txt"))
+    o.postprocess()
+    ro = o.report(str(tmp_path / "oracle.txt"))
+    assert one.report(str(tmp_path / "one.txt")) == ro
+    assert parts[0].report(str(tmp_path / "gathered.txt")) == ro
     for e in parts + [one]:
         e.close()

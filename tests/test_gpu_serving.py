@@ -4,7 +4,8 @@ Each job's covariate tables live in its own read-group slots, so its report
 must be byte-identical whether it runs alone, interleaved with other jobs
 (continuous batching: reads of several jobs in the same staging buffers and
 graph replays, jobs opening and closing at different times), or on an
-ordinary engine set up with the job's own header.
+ordinary engine set up with the job's own header, and that engine's report is
+the oracle's for the same reads.
 """
 import ctypes as C
 import sys
@@ -14,6 +15,7 @@ import pytest
 
 from firepony_amd import abi, synth
 from firepony_amd.engine import Engine, FireponyError
+from oracle.pyoracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -64,10 +66,14 @@ def test_job_result_alone_equals_in_mix(tmp_path):
         gk_names = (C.c_char_p * nrg)(*[x.encode() for x in names[k]])
         gk.header.read_group_names = gk_names
         e = Engine(gk)
+        o = Oracle(gk)
         for b in batches(gk, specs[k]):
             e.process(b)
+            o.process(b)
         e.postprocess()
+        o.postprocess()
         ref.append(e.report(str(tmp_path / f"ref{k}.txt")))
+        assert ref[-1] == o.report(str(tmp_path / f"oracle{k}.txt")), f"job {k}: ordinary engine vs oracle"
         e.close()
 
     # alone on a serving engine, one after another
