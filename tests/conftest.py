@@ -8,3 +8,6 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
+
+# the sanitizer run's copy of the tree (tests/sanitizer/run.sh) is not collected here
+collect_ignore_glob = ["sanitizer/_build/*"]
