@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -68,6 +69,17 @@ hipError_t fpd_launch_rg_table(uint32_t num_rg, const uint64_t *count, const uin
                                uint32_t *valid, uint64_t *obs, double *mism, double *expected, hipStream_t st);
 hipError_t fpd_check_batch(const fpd::BatchDev *b, uint32_t rg_limit, uint32_t *out, hipStream_t st);
 }
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free batch entry point (fp_engine_process_batch_device_noalloc): per-batch
+// read state, counters and the BAQ band rows of one batch.  A file-scope device array of a fixed size;
+// each GPU the library runs on holds its own copy until the process ends.  Host code never passes the
+// array's name to a kernel or copy: it takes the array's device address on each device once
+// (hipGetSymbolAddress) and uses that.
+static constexpr uint64_t NOALLOC_SCRATCH_BYTES = 256ull << 20;
+static constexpr uint32_t NOALLOC_MAX_READS = 65536;
+static constexpr uint32_t NOALLOC_MAX_READ_LENGTH = 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -222,6 +234,7 @@ struct fp_engine {
     DevBuf in[18];
     // per-batch state
     BatchState ps;                      // the plain path's (and that of batches run eagerly)
+    BatchState na;                      // allocation-free batches: views into the scratch array
     BatchState *last = &ps;             // the state of the last processed batch (inspection)
     DevBuf counters, scratch, scratch_gen;
     uint32_t *host_counts = nullptr;   // pinned readback area (64 u32)
@@ -431,6 +444,11 @@ struct Plan {
     uint64_t nquals;         // qualities (the capacity for a staged batch)
     const uint32_t *live;    // device live sizes of a staged batch, or null
     uint32_t rows_cap;       // >= the longest read
+    // allocation-free batches: the BAQ band classes take turns in this region, each with as many
+    // resident waves as fit, and no per-quality fractional errors are kept
+    double *band = nullptr;
+    uint64_t band_doubles = 0;
+    bool no_frac = false;
 };
 
 // resident waves of a band class; its per-wave scratch slot in doubles
@@ -504,7 +522,7 @@ fp_status ensure_pairs(fp_engine *e, uint64_t cap)
 fp_status launch_cov_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, int sort_bits, uint32_t *counters,
                            hipStream_t st, hipEvent_t sort_mark)
 {
-    const uint32_t keep = e->opts.debug_arrays ? 1u : 0u;
+    const uint32_t keep = e->opts.debug_arrays && !p.no_frac ? 1u : 0u;
     const uint32_t maxlen = std::min<uint32_t>(p.rows_cap, 499);
     if (e->opts.aggregation != FP_AGG_SORT) {
         HIP_TRY(fpd_launch_cov(&b, &S.rs, &e->tables, e->opts.frac_error_f64, keep, S.baq_counts.as<uint32_t>(),
@@ -558,6 +576,24 @@ fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, 
                                   e->opts.disable_baq_staging ? 0 : 1, ckpt > 1 ? 1 : 0,
                                   S.baq_counts.as<uint32_t>() + BAQ_CHUNK_COUNTERS + c, s);
     };
+    if (p.band) {
+        // allocation-free batch: the classes in turn on one stream, each in the whole band region
+        for (int c : {1, 2, 0}) {
+            const uint32_t cnt = known_counts ? known_counts[c] : 0u;
+            if (cnt == 0) continue;
+            const uint64_t slot = band_slot(c, p.rows_cap, 0);
+            const uint32_t waves = (uint32_t)std::min<uint64_t>((cnt + 63) / 64 + 3, p.band_doubles / slot) / 4 * 4;
+            if (waves == 0) return fail(FP_ERR_INVALID_ARG, "allocation-free batch: a BAQ band does not fit the scratch");
+            HIP_TRY(fpd_launch_baq_hmm(c, &b, &e->genome, &S.rs, S.baq_lists.as<uint32_t>() + (uint64_t)c * cap,
+                                       S.baq_counts.as<uint32_t>() + c, p.band, slot, p.rows_cap, BAQ_CLASS_W[c],
+                                       waves, (unsigned long long *)(counters + (c == 0 ? 10 : 8)), 1,
+                                       e->opts.disable_baq_staging ? 0 : 1, 0,
+                                       S.baq_counts.as<uint32_t>() + BAQ_CHUNK_COUNTERS + c, st));
+        }
+        if (mark) HIP_TRY(hipEventRecord(mark, st));
+        HIP_TRY(fpd_launch_baq_post(&b, &S.rs, 1u, st));
+        return FP_OK;
+    }
     if (e->opts.profile_stages) {
         // per-kernel timing: the classes in turn on one stream, events around the 15-cell class
         for (int c = 1; c < 3; c++)
@@ -611,30 +647,50 @@ fp_status launch_generic(fp_engine *e, BatchState &S, const fpd::BatchDev &b, co
                          uint32_t *counters, hipStream_t st)
 {
     if (count == 0) return FP_OK;
-    const uint32_t waves = std::min<uint32_t>((count + 63) / 64, 256u);
+    uint32_t waves = std::min<uint32_t>((count + 63) / 64, 256u);
     const uint64_t slot = band_slot(3, p.rows_cap, wgen);
-    HIP_TRY(e->scratch_gen.ensure(slot * waves * 8 + 8));
+    double *region;
+    if (p.band) {
+        waves = (uint32_t)std::min<uint64_t>(waves, p.band_doubles / slot);
+        if (waves == 0) return fail(FP_ERR_INVALID_ARG, "allocation-free batch: a BAQ band does not fit the scratch");
+        region = p.band;
+    } else {
+        HIP_TRY(e->scratch_gen.ensure(slot * waves * 8 + 8));
+        region = e->scratch_gen.as<double>();
+    }
     HIP_TRY(fpd_launch_baq_hmm(3, &b, &e->genome, &S.rs,
                                S.baq_lists.as<uint32_t>() + 3ull * std::max<uint32_t>(p.n, 1),
-                               S.baq_counts.as<uint32_t>() + 3, e->scratch_gen.as<double>(), slot, p.rows_cap, wgen,
+                               S.baq_counts.as<uint32_t>() + 3, region, slot, p.rows_cap, wgen,
                                waves, (unsigned long long *)(counters + 8), e->wpe, 0, 0, nullptr, st));
     return FP_OK;
 }
 
 // one batch, eagerly, with the host reading the class counts between the stages (the plain engine)
-fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_hint)
+// na: an allocation-free batch, whose per-batch state, counters and BAQ band region are carved from
+// the caller-independent scratch array (fp_engine_process_batch_device_noalloc); null: the engine's own
+struct NoAllocPlan {
+    uint32_t *counters;
+    double *band;
+    uint64_t band_doubles;
+};
+
+fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_hint,
+                    const NoAllocPlan *na = nullptr)
 {
     const uint32_t n = v->num_reads;
-    BatchState &S = e->ps;
-    fp_status rc = ensure_state(e, S, n, v->num_read_bases, v->num_qualities);
-    if (rc != FP_OK) return rc;
+    BatchState &S = na ? e->na : e->ps;
+    if (!na) {
+        fp_status rc = ensure_state(e, S, n, v->num_read_bases, v->num_qualities);
+        if (rc != FP_OK) return rc;
+    }
+    fp_status rc = FP_OK;
     e->last_num_reads = n;
     e->last = &S;
     e->last_num_bases = v->num_read_bases;
     e->last_num_quals = v->num_qualities;
     hipStream_t st = e->stream;
     fpd::BatchDev b = to_dev(v);
-    uint32_t *counters = e->counters.as<uint32_t>();
+    uint32_t *counters = na ? na->counters : e->counters.as<uint32_t>();
 
     HIP_TRY(hipEventRecord(e->ev[0], st));
     HIP_TRY(fpd_fill_u32(counters, 16, 0u, st));
@@ -648,7 +704,16 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
     HIP_TRY(hipStreamSynchronize(st));
     if (hc[fpd::BW_BAD_RG]) return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups");
     p.rows_cap = std::max<uint32_t>(std::max(max_read_len_hint, hc[fpd::BW_LONGEST]), 1);
-    HIP_TRY(e->scratch.ensure(band_scratch_doubles(e->wpe, n, p.rows_cap) * 8 + 8));
+    if (na) {
+        if (hc[fpd::BW_LONGEST] > NOALLOC_MAX_READ_LENGTH)
+            return fail(FP_ERR_INVALID_ARG, "allocation-free batch: a read is longer than " +
+                                                std::to_string(NOALLOC_MAX_READ_LENGTH) + " bases");
+        p.band = na->band;
+        p.band_doubles = na->band_doubles;
+        p.no_frac = true;
+    } else {
+        HIP_TRY(e->scratch.ensure(band_scratch_doubles(e->wpe, n, p.rows_cap) * 8 + 8));
+    }
     const bool sorting = e->opts.aggregation == FP_AGG_SORT;
     if (sorting) {
         rc = ensure_pairs(e, 3ull * n * p.rows_cap);
@@ -1432,6 +1497,9 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     if (he != hipSuccess) { delete e; return fail(FP_ERR_HIP, hipGetErrorString(he)); }
     for (int i = 0; i < 16; i++) (void)hipEventCreate(&e->ev[i]);
     he = hipHostMalloc((void **)&e->host_counts, 128 * sizeof(uint32_t), hipHostMallocDefault);
+    if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&e->na.fork, hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&e->na.join, hipEventDisableTiming);
     if (he == hipSuccess) he = fpd_cov_prepare();
     if (he == hipSuccess) he = fpd_baq_prepare();
     if (he != hipSuccess) { fp_engine_destroy(e); return fail(FP_ERR_HIP, hipGetErrorString(he)); }
@@ -1809,7 +1877,9 @@ fp_status fp_engine_debug_fetch(fp_engine *e, int item, void *dst, uint64_t byte
     case FP_DBG_BANDWIDTH: src = e->last->rs.bandwidth; break;
     case FP_DBG_BAQ: src = e->last->rs.baq; break;
     case FP_DBG_FRAC_ERRORS:
-        if (!e->opts.debug_arrays) return fail(FP_ERR_STATE, "fractional errors are kept only with debug_arrays = 1");
+        if (!e->opts.debug_arrays || !e->last->rs.frac)
+            return fail(FP_ERR_STATE, "fractional errors are kept only with debug_arrays = 1 (not for "
+                                      "allocation-free batches)");
         src = e->last->rs.frac;
         break;
     default: return fail(FP_ERR_INVALID_ARG, "unknown debug item");
@@ -1863,6 +1933,9 @@ void fp_engine_destroy(fp_engine *e)
     for (DevBuf *b : bufs) b->release();
     for (int t = 0; t < 3; t++) { e->pk[t].release(); e->pv[t].release(); }
     e->ps.release();
+    // the allocation-free state only views the scratch array: nothing of it is freed
+    e->na.baq_counts.p = e->na.baq_lists.p = e->na.rg_buf.p = nullptr;
+    e->na.release();
     e->scratch_gen.release();
     e->bcounters.release();
     for (auto &s : e->stg) {
@@ -1889,6 +1962,81 @@ void fp_engine_destroy(fp_engine *e)
     if (e->side) (void)hipStreamDestroy(e->side);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
+}
+
+// --- allocation-free batches ------------------------------------------------
+
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[64];
+
+// the scratch array's device address on the current device, taken once per device
+static fp_status noalloc_scratch(int dev, unsigned char **out)
+{
+    if (dev < 0 || dev >= 64) return fail(FP_ERR_UNSUPPORTED, "allocation-free batches: device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (he != hipSuccess || !p)
+            return fail(FP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(he));
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = (unsigned char *)g_noalloc_ptr[dev];
+    return FP_OK;
+}
+
+fp_status fp_engine_process_batch_device_noalloc(fp_engine *e, const fp_batch_view *db)
+{
+    fp_status rc = check_engine(e, true);
+    if (rc != FP_OK) return rc;
+    rc = validate_view(db);
+    if (rc != FP_OK) return rc;
+    if (e->batched) return fail(FP_ERR_UNSUPPORTED, "allocation-free batches run on the plain engine");
+    if (e->opts.aggregation == FP_AGG_SORT)
+        return fail(FP_ERR_UNSUPPORTED, "allocation-free batches use the direct table aggregation");
+    const uint32_t n = db->num_reads;
+    if (n == 0) { e->stats.num_batches++; return FP_OK; }
+    if (n > NOALLOC_MAX_READS)
+        return fail(FP_ERR_INVALID_ARG, "allocation-free batch: more than " + std::to_string(NOALLOC_MAX_READS) +
+                                            " reads");
+    if (db->num_qualities > (uint64_t)n * NOALLOC_MAX_READ_LENGTH ||
+        db->num_read_bases > (uint64_t)n * ((NOALLOC_MAX_READ_LENGTH + 7) & ~7u))
+        return fail(FP_ERR_INVALID_ARG, "allocation-free batch: more bases than " +
+                                            std::to_string(NOALLOC_MAX_READ_LENGTH) + " per read");
+    unsigned char *base = nullptr;
+    rc = noalloc_scratch(e->device, &base);
+    if (rc != FP_OK) return rc;
+    // carve the batch's state from the array, 256-byte aligned pieces
+    uint64_t off = 0;
+    auto take = [&](uint64_t bytes) -> void * {
+        void *q = base + off;
+        off += (bytes + 255) & ~255ull;
+        return q;
+    };
+    const uint64_t nb = db->num_read_bases, nq = db->num_qualities;
+    NoAllocPlan na{};
+    na.counters = (uint32_t *)take(64);
+    BatchState &S = e->na;
+    S.baq_counts.p = take(4 * 400);
+    S.baq_lists.p = take(16ull * n);
+    S.rg_buf.p = take(4ull * (4ull * std::max<uint32_t>(e->num_rg, 1) + 2 + n));
+    fpd::ReadStateDev &r = S.rs;
+    r.status = (uint8_t *)take(n);
+    r.rwc = (uint32_t *)take(4ull * n);
+    r.noins = (uint32_t *)take(4ull * n);
+    r.refw = (uint32_t *)take(4ull * n);
+    r.num_errors = (uint16_t *)take(2ull * n);
+    r.hmmw = (uint32_t *)take(4ull * n);
+    r.bandwidth = (uint16_t *)take(2ull * n);
+    r.hqw = (uint32_t *)take(4ull * n);
+    r.base_flags = (uint8_t *)take(nb + 8);
+    r.baq = (uint8_t *)take(nq + 8);
+    r.baq_state = (uint32_t *)take(4 * (nq + 8));
+    r.frac = nullptr;
+    if (off >= NOALLOC_SCRATCH_BYTES) return fail(FP_ERR_INVALID_ARG, "allocation-free batch: larger than the scratch");
+    na.band = (double *)(base + off);
+    na.band_doubles = (NOALLOC_SCRATCH_BYTES - off) / 8;
+    return run_batch(e, db, 0, &na);
 }
 
 // --- serving jobs -----------------------------------------------------------

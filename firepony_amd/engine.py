@@ -24,7 +24,7 @@ EXPORTED = [
     "fp_engine_debug_fetch", "fp_engine_reset", "fp_engine_destroy", "fp_last_error",
     "fp_status_string", "fp_job_open", "fp_job_submit", "fp_job_submit_device", "fp_job_close",
     "fp_job_write_report", "fp_job_get_table", "fp_job_stats", "fp_job_release",
-    "fp_engine_set_read_group_names",
+    "fp_engine_set_read_group_names", "fp_engine_process_batch_device_noalloc",
 ]
 
 
@@ -48,6 +48,7 @@ def lib():
                                       C.POINTER(abi.FpVariantsView)]
         L.fp_engine_process_batch.argtypes = [vp, C.POINTER(abi.FpBatchView)]
         L.fp_engine_process_batch_device.argtypes = [vp, C.POINTER(abi.FpBatchView)]
+        L.fp_engine_process_batch_device_noalloc.argtypes = [vp, C.POINTER(abi.FpBatchView)]
         L.fp_engine_synchronize.argtypes = [vp]
         L.fp_engine_gather.argtypes = [vp, vp]
         L.fp_engine_postprocess.argtypes = [vp]
@@ -114,6 +115,10 @@ class Engine:
     def process_device(self, view):
         """Batch whose arrays already live in this engine's device memory."""
         _check(lib().fp_engine_process_batch_device(self._h, C.byref(view)))
+
+    def process_device_noalloc(self, view):
+        """Device batch through the entry point that allocates no device memory (plain engines)."""
+        _check(lib().fp_engine_process_batch_device_noalloc(self._h, C.byref(view)))
 
     def synchronize(self):
         _check(lib().fp_engine_synchronize(self._h))

@@ -274,6 +274,16 @@ fp_status fp_engine_process_batch(fp_engine *e, const fp_batch_view *host_batch)
  * view pointers are device pointers on this engine's device).  Stream-ordered:
  * returns after enqueueing; fp_engine_synchronize waits. */
 fp_status fp_engine_process_batch_device(fp_engine *e, const fp_batch_view *device_batch);
+/* No counterpart in the reference.  fp_engine_process_batch_device for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses that forbid hipMalloc):
+ * it calls no hipMalloc, hipFree or allocating library routine.  Plain engines with the direct
+ * aggregation; at most 65,536 reads of at most 256 bases per batch (larger batches are refused
+ * with FP_ERR_INVALID_ARG).  The batch's read state, counters and BAQ band rows live in one
+ * 256 MiB file-scope device array per GPU, held until the process ends and shared by every call
+ * on that GPU: calls that overlap on one GPU race; each call clears what must start at zero, and
+ * the last call's intermediates stay there for fp_engine_debug_fetch (fractional errors are not
+ * kept).  Same tables as fp_engine_process_batch_device; returns after the batch has run. */
+fp_status fp_engine_process_batch_device_noalloc(fp_engine *e, const fp_batch_view *device_batch);
 fp_status fp_engine_synchronize(fp_engine *e);
 /* firepony_pipeline::gather_intermediates (device/pipeline_interface.cu:52-58,
  * 137-160): merge src's observation tables into dst's. */
