@@ -221,7 +221,7 @@ struct fp_engine {
     std::string name;
     HostConstants consts;
     int wpe = 1;   // waves per SIMD of the 15-cell BAQ band kernel (1-4)
-    int ckpt = 1;  // rows per forward checkpoint of the 15-cell band kernel (1: every row spilled; 2: recompute)
+    int ckpt = 1;  // rows per forward checkpoint of the 15-cell band kernel (1: every row spilled; 3-5: recompute)
 
     // setup state
     bool is_setup = false;
@@ -573,7 +573,7 @@ fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, 
                                   S.baq_counts.as<uint32_t>() + c, e->scratch.as<double>() + band_offset(wpe, c, p.n, p.rows_cap),
                                   band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c], band_waves(wpe, c, p.n),
                                   (unsigned long long *)(counters + (c == 0 ? 10 : 8)), wpe,
-                                  e->opts.disable_baq_staging ? 0 : 1, ckpt > 1 ? 1 : 0,
+                                  e->opts.disable_baq_staging ? 0 : 1, ckpt > 1 ? ckpt : 0,
                                   S.baq_counts.as<uint32_t>() + BAQ_CHUNK_COUNTERS + c, s);
     };
     if (p.band) {
@@ -767,12 +767,11 @@ fp_status run_batch(fp_engine *e, const fp_batch_view *v, uint32_t max_read_len_
 // ---------------------------------------------------------------------------
 // batched engine
 
-// BAQ candidates of the autotuner: (waves per SIMD, rows per forward checkpoint).  The checkpointed
-// kernel is not among them: measured slower on every workload here (it holds two rows in registers,
-// which saves a quarter of the forward-row traffic for a second forward pass), so it runs only when
-// asked for.
-static const int TUNE_CANDIDATES[3][2] = {{1, 1}, {2, 1}, {3, 1}};
-static const int NUM_TUNE = 3;
+// BAQ candidates of the autotuner: (waves per SIMD, rows per forward checkpoint).  One or two waves
+// per SIMD of the every-row kernel, and the checkpointed kernel (one wave per SIMD) with 3 or 4 rows
+// per checkpoint.
+static const int TUNE_CANDIDATES[4][2] = {{1, 1}, {2, 1}, {1, 3}, {1, 4}};
+static const int NUM_TUNE = 4;
 static const int SORT_CANDIDATES[5] = {4, 5, 6, 7, 8};
 
 fpd::BatchDev stage_dev(fp_engine *e, fp_engine::Stage &s)
@@ -1474,7 +1473,8 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     }
     if (o.aggregation != FP_AGG_DIRECT && o.aggregation != FP_AGG_SORT)
         return fail(FP_ERR_INVALID_ARG, "aggregation must be FP_AGG_DIRECT or FP_AGG_SORT");
-    if (o.baq_checkpoint_rows > 2) return fail(FP_ERR_INVALID_ARG, "baq_checkpoint_rows must be 0, 1 or 2");
+    if (o.baq_checkpoint_rows > 5 || o.baq_checkpoint_rows == 2)
+        return fail(FP_ERR_INVALID_ARG, "baq_checkpoint_rows must be 0 (tuned), 1, 3, 4 or 5");
     if (o.sort_digit_bits != 0 && (o.sort_digit_bits < 4 || o.sort_digit_bits > 8))
         return fail(FP_ERR_INVALID_ARG, "sort_digit_bits must be 0 (autotune) or 4..8");
     int ndev = 0;
@@ -1485,7 +1485,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     e->device = hip_device;
     e->opts = o;
     e->wpe = o.baq_waves_per_simd >= 1 && o.baq_waves_per_simd <= 4 ? (int)o.baq_waves_per_simd : 1;
-    e->ckpt = o.baq_checkpoint_rows == 2 ? 2 : 1;
+    e->ckpt = o.baq_checkpoint_rows >= 3 ? (int)o.baq_checkpoint_rows : 1;
     e->sort_bits = o.sort_digit_bits ? (int)o.sort_digit_bits : 8;
     e->budget_bytes = (uint64_t)o.serve_hbm_budget_mib << 20;
     // the digit width is tuned on the batched engine's first staged batches unless it is given

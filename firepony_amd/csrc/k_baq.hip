@@ -31,6 +31,9 @@ static constexpr double EM = 0.33333333333;
 static constexpr double EI = 0.25;
 static constexpr double GEP = 0.1;
 
+// compile-time form of a row: Steady<true> rows carry no band or N masks
+template <bool B> struct Steady { static constexpr bool value = B; };
+
 // MAP quality from y = 1 - max/sum (baq.cu:673-674).  The exact value is the
 // largest v in [5, 101] with y <= T[v] (T decreasing, computed with the host
 // libm); the device log only provides the starting guess, which the threshold
@@ -918,16 +921,20 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
 // ---------------------------------------------------------------------------
 // Checkpointed band kernel (inputs staged in LDS, one wave per SIMD).
 //
-// The forward pass spills every row's scale factor but the full state (M, I, D) of only every K-th
-// row.  The backward pass walks K-row segments from the top: it recomputes a segment's forward
-// rows from the segment's checkpoint into registers, with the forward pass's own row code (so the
-// values are the same bits), and consumes them in reverse for the MAP step.  Forward-row traffic
-// drops from 32 B per cell (M and I of every row written and read back) to 48/K B; the recompute
-// repeats the forward arithmetic once, on FP64 units the spill-bound form leaves idle.  The
-// segment loop runs on a wave-uniform counter, so all lanes of a wave load the same checkpoint row
-// at once (coalesced) even when their reads' lengths differ.
-template <int W, int K, bool PREFETCH>
-__global__ void __launch_bounds__(256, 1) k_baq_seg(BaqArgs a)
+// The forward pass keeps every row's scale factor but the full state (M, I, D) of only every K-th
+// row, from row 1.  The backward pass then walks K-row segments from the top: it reloads a
+// segment's checkpoint, recomputes the segment's other forward rows into registers with the
+// forward pass's own row code (so they are the same bits), and consumes them in reverse for the
+// MAP steps.  Forward-row traffic falls from 32 B per cell (M and I of every row written and read
+// back) to 48/K B; the recompute repeats the forward arithmetic once.
+//
+// Registers: the K recomputed rows as (M, I) pairs, the forward D of the recompute and the
+// backward row's M and I; the forward pass's own state dies before them.  The next segment's
+// checkpoint is loaded while this segment's backward rows run: its D into the recompute's D once
+// the recompute is done, its (M, I) into the top row's slot once that row's MAP step is done.  The
+// segment loop runs on a wave-uniform counter, so all lanes load the same checkpoint row at once.
+template <int W, int K, bool TAGL>
+__global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
 {
     constexpr int NWIN = (W + 15) / 16;
     constexpr int TOP = (W - 1) - 16 * (NWIN - 1);
@@ -950,14 +957,16 @@ __global__ void __launch_bounds__(256, 1) k_baq_seg(BaqArgs a)
     double2 *Cmi = (double2 *)slot;
     double *Cd = slot + (uint64_t)nck * W * 2 * 64;
     double *Sst = slot + (uint64_t)(a.rows_cap + 1) * W * 2 * 64;         // [row][lane]
-    uint64_t *Ost = (uint64_t *)(Sst + (uint64_t)(a.rows_cap + 2) * 64);  // [row][lane] MAP state | quality << 32
+    uint32_t *Ost = (uint32_t *)(Sst + (uint64_t)(a.rows_cap + 2) * 64);  // [row][lane] cap code, then tag
     const BatchDev &b = a.b;
     uint64_t ncell = 0;
     extern __shared__ unsigned char baq_stage[];
     const uint32_t base_bytes = (a.rows_cap + 1) / 2, ref_bytes = (a.rows_cap + 2 * W + 1) / 2;
-    uint8_t *rowq = baq_stage + (threadIdx.x >> 6) * ((a.rows_cap + base_bytes + ref_bytes) * 64);
+    const uint32_t wave_bytes = (a.rows_cap + base_bytes + ref_bytes + (TAGL ? a.rows_cap : 0u)) * 64;
+    uint8_t *rowq = baq_stage + (threadIdx.x >> 6) * wave_bytes;
     uint8_t *rowb = rowq + (uint64_t)a.rows_cap * 64;
     uint8_t *refd = rowb + (uint64_t)base_bytes * 64;
+    uint8_t *tagl = refd + (uint64_t)ref_bytes * 64;
     auto refn = [&](int i) -> uint32_t { return (refd[(i >> 1) * 64 + lane] >> ((i & 1) * 4)) & 15; };
     auto rowv = [&](int q) -> uint32_t {
         return rowq[q * 64 + lane] | (((rowb[(q >> 1) * 64 + lane] >> ((q & 1) * 4)) & 15) << 8);
@@ -1008,7 +1017,7 @@ __global__ void __launch_bounds__(256, 1) k_baq_seg(BaqArgs a)
         }
     };
 
-    for (uint32_t chunk = gwave; (uint64_t)chunk * 64 < total; chunk += nwaves) {
+    auto run_chunk = [&](uint32_t chunk) {
         const uint32_t idx = chunk * 64 + lane;
         const bool act = idx < total;
         const uint32_t rd = act ? a.list[idx] : 0u;
@@ -1021,7 +1030,7 @@ __global__ void __launch_bounds__(256, 1) k_baq_seg(BaqArgs a)
             const int o = __shfl_xor(qmax, off);
             qmax = o > qmax ? o : qmax;
         }
-        if (!act) continue;
+        if (!act) return;
 
         // ---- setup (baq.cu:260-354)
         const uint32_t hw = a.s.hmmw[rd];
@@ -1067,30 +1076,113 @@ __global__ void __launch_bounds__(256, 1) k_baq_seg(BaqArgs a)
         };
 
         const bool good = cap_walk(b, a.s, rd, [&](uint16_t p, uint32_t kind, uint32_t ep) {
-            Ost[(uint64_t)((int)p - qStart + 1) * 64 + lane] = (uint64_t)ep | ((uint64_t)kind << 32);
+            Ost[(uint64_t)((int)p - qStart + 1) * 64 + lane] = cap_code(kind, ep);
         });
 
-        // this read's inputs into the wave's LDS area (each lane writes and later reads only its own column)
-#pragma unroll 4
-        for (int q = 0; q < qLen; q++) rowq[q * 64 + lane] = b.quals[qqual + q];
-#pragma unroll 4
-        for (int q = 0; q < qLen; q += 2) {
-            const uint32_t w = b.reads[(qbase + q) >> 3];
-            const uint32_t b0 = (w >> (((qbase + q) & 7) * 4)) & 15;
-            const uint64_t i1 = qbase + (q + 1 < qLen ? q + 1 : q);
-            const uint32_t b1 = q + 1 < qLen ? (b.reads[i1 >> 3] >> ((i1 & 7) * 4)) & 15 : 15u;
-            rowb[(q >> 1) * 64 + lane] = (uint8_t)(b0 | (b1 << 4));
+        // This read's inputs into the wave's LDS area (each lane writes and later reads only its own
+        // column).  With one wave per SIMD nothing hides a load's latency but the wave's own work, so
+        // the loads go out 16 at a time and are consumed after one wait.
+        {
+            // qualities: the aligned words holding bytes [qqual, qqual + qLen); every such word holds
+            // at least one of them, so it lies in the same page as the batch's own bytes
+            const int sh = (int)((uintptr_t)(b.quals + qqual) & 3);
+            const uint32_t *qw = (const uint32_t *)(b.quals + qqual - sh);
+            const int nw = (sh + qLen + 3) >> 2;
+            for (int c = 0; c < nw; c += 16) {
+                uint32_t w[16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) w[u] = qw[c + u < nw ? c + u : nw - 1];
+#pragma unroll
+                for (int u = 0; u < 16; u++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int q = 4 * (c + u) + j - sh;
+                        if (q >= 0 && q < qLen) rowq[q * 64 + lane] = (uint8_t)(w[u] >> (8 * j));
+                    }
+            }
         }
-#pragma unroll 4
-        for (int bi = (W - bw) >> 1; bi <= (qLen + 2 * W - 1 - bw) >> 1; bi++) {
-            const int k0 = 2 * bi - W, k1 = k0 + 1;
-            refd[bi * 64 + lane] = (uint8_t)(nibble(ref_word(hoff + k0), k0) | (nibble(ref_word(hoff + k1), k1) << 4));
+        {
+            // query bases: byte q / 2 holds bases q (low nibble) and q + 1, the base stream realigned
+            // to the window start; nibbles past the window are never read
+            const uint64_t w0 = qbase >> 3, wl = (qbase + qLen - 1) >> 3;
+            const int sh = (int)(qbase & 7) * 4;
+            const int nw = (int)(wl - w0) + 1;
+            for (int c = 0; c < nw; c += 16) {
+                uint32_t w[17];
+#pragma unroll
+                for (int u = 0; u < 17; u++) w[u] = b.reads[w0 + (uint64_t)(c + u < nw ? c + u : nw - 1)];
+#pragma unroll
+                for (int u = 0; u < 16; u++) {
+                    const uint32_t v = (uint32_t)((((uint64_t)w[u + 1] << 32) | w[u]) >> sh);
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int bi = 4 * (c + u) + j;
+                        if (2 * bi < qLen) rowb[bi * 64 + lane] = (uint8_t)(v >> (8 * j));
+                    }
+                }
+            }
+        }
+        {
+            // reference cells k = 2 bi - W (low nibble) and 2 bi - W + 1 of byte bi for bi in [bi0, bi1]:
+            // the contig's base stream realigned to cell 2 bi0 - W, 8 cells per group of 4 bytes, N
+            // outside the window and the contig (words clamped into the contig, as ref_word does)
+            const int bi0 = (W - bw) >> 1, bi1 = (qLen + 2 * W - 1 - bw) >> 1;
+            const int ks = 2 * bi0 - W;
+            const int64_t ps = hoff + ks;
+            const int64_t w0 = ps >> 3;                       // floor
+            const int sh = (int)(ps & 7) * 4;
+            const int ng = (bi1 - bi0 + 4) >> 2;
+            // valid cells [kv0, kv1)
+            const int64_t kv0 = 0, kv1 = (int64_t)refLen < clen - hoff ? (int64_t)refLen : clen - hoff;
+            for (int c = 0; c < ng; c += 16) {
+                uint32_t w[17];
+#pragma unroll
+                for (int u = 0; u < 17; u++) w[u] = ref_word((w0 + (c + u < ng ? c + u : ng)) * 8);
+#pragma unroll
+                for (int u = 0; u < 16; u++) {
+                    uint32_t v = (uint32_t)((((uint64_t)w[u + 1] << 32) | w[u]) >> sh);
+                    const int64_t kg = ks + 8 * (c + u);       // cell of nibble 0
+                    int64_t a0 = kv0 - kg, a1 = kv1 - kg;
+                    a0 = a0 < 0 ? 0 : (a0 > 8 ? 8 : a0);
+                    a1 = a1 < 0 ? 0 : (a1 > 8 ? 8 : a1);
+                    const uint32_t below1 = a1 >= 8 ? ~0u : ((1u << (4 * a1)) - 1);
+                    const uint32_t below0 = a0 >= 8 ? ~0u : ((1u << (4 * a0)) - 1);
+                    const uint32_t m = a1 > a0 ? (below1 & ~below0) : 0u;
+                    v = (v & m) | ~m;                          // N = 15 in every invalid cell
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int bi = bi0 + 4 * (c + u) + j;
+                        if (bi <= bi1) refd[bi * 64 + lane] = (uint8_t)(v >> (8 * j));
+                    }
+                }
+            }
         }
 
-        // forward row i >= 2 from row i - 1 (baq.cu:444-512): unscaled values and their sum.  The
-        // forward pass and the segment recompute both run this code, so they produce the same bits.
-        auto fwd_row = [&](int i, double (&M)[W], double (&I)[W], double (&D)[W], const uint64_t (&win)[NWIN],
-                           uint32_t qb, double p) -> double {
+        // A steady row has every band cell inside the band and the HMM window, no N among the band's
+        // reference cells and a query base that is not N: its cells need no masks and its emissions
+        // only the match test.  The kernels take the mask-free code for a row when every lane still
+        // running is steady there (a wave-uniform branch); both forms compute the same expressions.
+        const bool full_band = 2 * bw + 1 == W;
+        auto no_n = [&](const uint64_t (&win)[NWIN], uint32_t qb) -> bool {
+            uint64_t t = 0;
+#pragma unroll
+            for (int w = 0; w < NWIN; w++) t |= win[w] & (win[w] >> 1) & (win[w] >> 2) & (win[w] >> 3);
+            return (t & 0x1111111111111111ull) == 0 && qb != B_N;
+        };
+        // forward row i: cells k = i - bw .. i - bw + W - 1 all in [1, refLen]
+        auto fwd_steady = [&](int i, const uint64_t (&win)[NWIN], uint32_t qb) -> bool {
+            return full_band && i - bw >= 1 && i - bw + W - 1 <= refLen && no_n(win, qb);
+        };
+        // backward row i: the same, and no cell at k = refLen (whose emission term is zero)
+        auto bwd_steady = [&](int i, const uint64_t (&win)[NWIN], uint32_t qb) -> bool {
+            return full_band && i - bw >= 1 && i - bw + W - 1 < refLen && no_n(win, qb);
+        };
+
+        // forward row i >= 2 from row i - 1 (baq.cu:444-512) into O (may be P itself), unscaled, and its
+        // sum.  The forward pass and the segment recompute both run this code: the same bits.
+        auto fwd_row = [&](auto steady, int i, const double2 (&P)[W], double2 (&O)[W], double (&D)[W],
+                           const uint64_t (&win)[NWIN], uint32_t qb, double p) -> double {
+            constexpr bool S = decltype(steady)::value;
             const double em_eq = 1 - p, em_ne = p * EM;
             const int klo = 1 > i - bw ? 1 : i - bw;
             const int khi = refLen < i + bw ? refLen : i + bw;
@@ -1098,112 +1190,143 @@ __global__ void __launch_bounds__(256, 1) k_baq_seg(BaqArgs a)
 #pragma unroll
             for (int r = 0; r < W; r++) {
                 const int k = i - bw + r;
-                const bool in = k >= klo && k <= khi;
-                const double pM = M[r], pI = I[r], pD = D[r];
-                const double nM = (r + 1 < W) ? M[r + 1 < W ? r + 1 : r] : 0.0;
-                const double nI = (r + 1 < W) ? I[r + 1 < W ? r + 1 : r] : 0.0;
+                const bool in = S || (k >= klo && k <= khi);
+                const double pM = P[r].x, pI = P[r].y, pD = D[r];
+                const double nM = (r + 1 < W) ? P[r + 1 < W ? r + 1 : r].x : 0.0;
+                const double nI = (r + 1 < W) ? P[r + 1 < W ? r + 1 : r].y : 0.0;
                 const uint32_t rb = (uint32_t)((win[r >> 4] >> (4 * (r & 15))) & 15);
-                const double e = (rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne);
+                const double e = (!S && (rb == B_N || qb == B_N)) ? 1.0 : (rb == qb ? em_eq : em_ne);
                 const double mv = in ? e * (m0 * pM + m3 * pI + m6 * pD) : 0.0;
                 const double iv = in ? EI * (m1 * nM + m4 * nI) : 0.0;
                 const double dv = in ? m2 * cM + m8 * cD : 0.0;
                 sum += in ? mv + iv + dv : 0.0;
-                M[r] = mv; I[r] = iv; D[r] = dv;
+                O[r] = make_double2(mv, iv);
+                D[r] = dv;
                 cM = mv; cD = dv;
             }
             return sum;
         };
-        auto fwd_scale = [&](double (&M)[W], double (&I)[W], double (&D)[W], double sum) {
-            const double inv = 1. / sum;
-#pragma unroll
-            for (int r = 0; r < W; r++) { M[r] = M[r] * inv; I[r] = I[r] * inv; D[r] = D[r] * inv; }
+        auto fwd_any = [&](int i, const double2 (&P)[W], double2 (&O)[W], double (&D)[W],
+                           const uint64_t (&win)[NWIN], uint32_t qb, double p) -> double {
+            if (__all(fwd_steady(i, win, qb))) return fwd_row(Steady<true>{}, i, P, O, D, win, qb, p);
+            return fwd_row(Steady<false>{}, i, P, O, D, win, qb, p);
         };
-        auto store_ckpt = [&](int j, const double (&M)[W], const double (&I)[W], const double (&D)[W]) {
+        // rows >= 2 are scaled by the reciprocal of their sum (baq.cu:505-510)
+        auto fwd_scale = [&](double2 (&O)[W], double (&D)[W], double inv) {
+#pragma unroll
+            for (int r = 0; r < W; r++) { O[r] = make_double2(O[r].x * inv, O[r].y * inv); D[r] = D[r] * inv; }
+        };
+        auto store_ckpt = [&](int j, const double2 (&O)[W], const double (&D)[W]) {
             double2 *cmi = Cmi + (uint64_t)j * W * 64 + lane;
             double *cd = Cd + (uint64_t)j * W * 64 + lane;
 #pragma unroll
-            for (int r = 0; r < W; r++) { cmi[r * 64] = make_double2(M[r], I[r]); cd[r * 64] = D[r]; }
+            for (int r = 0; r < W; r++) { cmi[r * 64] = O[r]; cd[r * 64] = D[r]; }
         };
 
-        double M[W], I[W], D[W];
-        uint64_t win[NWIN];
-        // ---- forward row 1 (baq.cu:396-442)
+        // ---- forward pass.  The scale factor area keeps 1 / s[i], the factor the backward rows
+        // multiply by (the same division as the forward's own), and s[qLen] stays in a register.
+        double sq1 = 0.0, sq = 0.0;
         {
-            const uint32_t v = rowv(0);
-            const uint8_t qb = (uint8_t)(v >> 8);
-            const double p = qprob((uint8_t)(v & 255));
-            const double em_eq = 1 - p, em_ne = p * EM;
-            const int khi = refLen < bw + 1 ? refLen : bw + 1;
-            ncell += khi >= 1 ? (uint64_t)khi : 0;
-            win_fill(win, -bw);
-            double sum = 0.0;
+            double2 MI[W];
+            double D[W];
+            uint64_t win[NWIN];
+            // row 1 (baq.cu:396-442)
+            {
+                const uint32_t v = rowv(0);
+                const uint8_t qb = (uint8_t)(v >> 8);
+                const double p = qprob((uint8_t)(v & 255));
+                const double em_eq = 1 - p, em_ne = p * EM;
+                const int khi = refLen < bw + 1 ? refLen : bw + 1;
+                ncell += khi >= 1 ? (uint64_t)khi : 0;
+                win_fill(win, -bw);
+                double sum = 0.0;
 #pragma unroll
-            for (int r = 0; r < W; r++) {
-                const int k = 1 - bw + r;
-                const bool in = k >= 1 && k <= khi;
-                const uint8_t rb = (uint8_t)((win[r >> 4] >> (4 * (r & 15))) & 15);
-                const double e = (rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne);
-                const double mv = in ? e * bM : 0.0;
-                const double iv = in ? EI * bI : 0.0;
-                sum += in ? mv + iv : 0.0;
-                M[r] = mv; I[r] = iv; D[r] = 0.0;
+                for (int r = 0; r < W; r++) {
+                    const int k = 1 - bw + r;
+                    const bool in = k >= 1 && k <= khi;
+                    const uint8_t rb = (uint8_t)((win[r >> 4] >> (4 * (r & 15))) & 15);
+                    const double e = (rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne);
+                    const double mv = in ? e * bM : 0.0;
+                    const double iv = in ? EI * bI : 0.0;
+                    sum += in ? mv + iv : 0.0;
+                    MI[r] = make_double2(mv, iv); D[r] = 0.0;
+                }
+                Sst[64 + lane] = 1. / sum;
+                sq = sum;
+#pragma unroll
+                for (int r = 0; r < W; r++) {
+                    MI[r] = make_double2(MI[r].x / sum, MI[r].y / sum); D[r] = D[r] / sum;
+                }
+                store_ckpt(0, MI, D);
             }
-            Sst[64 + lane] = sum;
-#pragma unroll
-            for (int r = 0; r < W; r++) { M[r] = M[r] / sum; I[r] = I[r] / sum; D[r] = D[r] / sum; }
-            store_ckpt(0, M, I, D);
-        }
-        // ---- forward rows 2..qLen: next row's LDS inputs read a row ahead
-        const int qlast = qLen - 1;
-        const int q1 = 1 < qlast ? 1 : qlast;
-        uint32_t cv = rowv(q1);
-        uint32_t crb = refn(2 - bw - 1 + (W - 1) + W);
-        for (int i = 2; i <= qLen; i++) {
-            const int in_ = i < qlast ? i : qlast;
-            const uint32_t nv = rowv(in_);
-            const uint32_t nrw = refn(i + 1 - bw - 1 + (W - 1) + W);
-            win_shr(win, crb);
-            const int klo = 1 > i - bw ? 1 : i - bw;
-            const int khi = refLen < i + bw ? refLen : i + bw;
-            if (khi >= klo) ncell += (uint64_t)(khi - klo + 1);
-            const double sum = fwd_row(i, M, I, D, win, cv >> 8, qprob((uint8_t)(cv & 255)));
-            cv = nv;
-            crb = nrw;
-            pin(cv); pin(crb);
-            Sst[(uint64_t)i * 64 + lane] = sum;
-            fwd_scale(M, I, D, sum);
-            if ((i - 1) % K == 0) store_ckpt((i - 1) / K, M, I, D);
-        }
-        // ---- s[qLen+1] (baq.cu:514-525)
-        double sq1 = 0.0;
-        {
+            // rows 2..qLen: the next row's LDS inputs read a row ahead
+            const int qlast = qLen - 1;
+            const int q1 = 1 < qlast ? 1 : qlast;
+            uint32_t cv = rowv(q1);
+            uint32_t crb = refn(2 - bw - 1 + (W - 1) + W);
+            for (int i = 2; i <= qLen; i++) {
+                const int in_ = i < qlast ? i : qlast;
+                const uint32_t nv = rowv(in_);
+                const uint32_t nrw = refn(i + 1 - bw - 1 + (W - 1) + W);
+                win_shr(win, crb);
+                const int klo = 1 > i - bw ? 1 : i - bw;
+                const int khi = refLen < i + bw ? refLen : i + bw;
+                if (khi >= klo) ncell += (uint64_t)(khi - klo + 1);
+                const double sum = fwd_any(i, MI, MI, D, win, cv >> 8, qprob((uint8_t)(cv & 255)));
+                cv = nv;
+                crb = nrw;
+                pin(cv); pin(crb);
+                const double inv = 1. / sum;
+                Sst[(uint64_t)i * 64 + lane] = inv;
+                sq = sum;
+                fwd_scale(MI, D, inv);
+                if ((i - 1) % K == 0) store_ckpt((i - 1) / K, MI, D);
+            }
+            // s[qLen+1] (baq.cu:514-525)
             const int klo = 1 > qLen - bw ? 1 : qLen - bw;
             const int khi = refLen < qLen + bw ? refLen : qLen + bw;
 #pragma unroll
             for (int r = 0; r < W; r++) {
                 const int k = qLen - bw + r;
-                const double t = M[r] * sM + I[r] * sI;
+                const double t = MI[r].x * sM + MI[r].y * sI;
                 sq1 += (k >= klo && k <= khi) ? t : 0.0;
             }
         }
 
         // ---- backward + MAP (baq.cu:528-681), K rows at a time from the top
-        auto load_ckpt = [&](int j, double (&M)[W], double (&I)[W], double (&D)[W]) {
-            const double2 *cmi = Cmi + (uint64_t)j * W * 64 + lane;
-            const double *cd = Cd + (uint64_t)j * W * 64 + lane;
-#pragma unroll
-            for (int r = 0; r < W; r++) {
-                const double2 f = cmi[r * 64];
-                M[r] = f.x; I[r] = f.y; D[r] = cd[r * 64];
-            }
-        };
-        const int nseg = (qLen - 1) / K + 1;
         const int nseg_w = (qmax - 1) / K + 1;
-        // backward row qLen (baq.cu:528-545), before the segment walk reaches it
+        const int last_seg = (qLen - 1) / K;
+        double2 F[K][W];
+        double Dr[W];
         double bMv[W], bIv[W];
         uint64_t bwin[NWIN];
+        // backward row i < qLen from row i + 1 (baq.cu:547-604) in place in (bMv, bIv), unscaled
+        auto bwd_row = [&](auto steady, int i, uint32_t qb, double p) {
+            constexpr bool S = decltype(steady)::value;
+            const double y = (i > 1) ? 1. : 0.;             // a steady row has i > bw >= 1
+            const double em_eq = 1 - p, em_ne = p * EM;
+            const int klo = 1 > i - bw ? 1 : i - bw;
+            const int khi = refLen < i + bw ? refLen : i + bw;
+            double cD = 0.0;
+#pragma unroll
+            for (int r = W - 1; r >= 0; r--) {
+                const int k = i - bw + r;
+                const bool in = S || (k >= klo && k <= khi);
+                const double pM = bMv[r];
+                const double pIl = r >= 1 ? bIv[r >= 1 ? r - 1 : 0] : 0.0;
+                const uint8_t rb = (uint8_t)((bwin[r >> 4] >> (4 * (r & 15))) & 15);
+                const double e0 = ((!S && (rb == B_N || qb == B_N)) ? 1.0 : (rb == qb ? em_eq : em_ne)) * pM;
+                const double e = (!S && k >= refLen) ? 0.0 : e0;
+                const double mv = in ? e * m0 + EIm1 * pIl + m2 * cD : 0.0;
+                const double iv = in ? e * m3 + EIm4 * pIl : 0.0;
+                // (x * 1.0 is x: a steady row leaves the factor out)
+                const double dv = in ? (S ? e * m6 + m8 * cD : (e * m6 + m8 * cD) * y) : 0.0;
+                bMv[r] = mv; bIv[r] = iv;
+                cD = dv;
+            }
+        };
         {
-            const double sq = Sst[(uint64_t)qLen * 64 + lane];
+            // backward row qLen (baq.cu:528-545)
             const int klo = 1 > qLen - bw ? 1 : qLen - bw;
             const int khi = refLen < qLen + bw ? refLen : qLen + bw;
             const double vM = sM / sq / sq1, vI = sI / sq / sq1;
@@ -1215,66 +1338,63 @@ __global__ void __launch_bounds__(256, 1) k_baq_seg(BaqArgs a)
             }
             win_fill(bwin, qLen - bw);
         }
-        load_ckpt(nseg - 1, M, I, D);          // the forward state is dead: its registers hold the recompute
+        // the first segment's checkpoint (a lane whose window ends below the wave's top segment loads
+        // its own last one, which the walk reaches when its rows begin)
+        {
+            const int j0 = (nseg_w - 1) < last_seg ? (nseg_w - 1) : last_seg;
+            const double2 *cmi = Cmi + (uint64_t)j0 * W * 64 + lane;
+            const double *cd = Cd + (uint64_t)j0 * W * 64 + lane;
+#pragma unroll
+            for (int r = 0; r < W; r++) { F[K - 1][r] = cmi[r * 64]; Dr[r] = cd[r * 64]; }
+        }
         for (int j = nseg_w - 1; j >= 0; j--) {
             const int s0 = 1 + j * K;
-            if (s0 > qLen) continue;
-            double2 F[K][W];
+            if (j > last_seg) continue;          // rows above this lane's window
+            // The segment's scale factors and cap codes are loaded first: loads retire in issue order,
+            // so a load issued after the next checkpoint's prefetch would wait for the prefetch.
+            double rinv[K];
+            uint32_t rcode[K];
 #pragma unroll
-            for (int r = 0; r < W; r++) F[0][r] = make_double2(M[r], I[r]);
+            for (int t = 0; t < K; t++) {
+                const int i = s0 + t <= qLen ? s0 + t : qLen;
+                rinv[t] = Sst[(uint64_t)i * 64 + lane];
+                rcode[t] = Ost[(uint64_t)i * 64 + lane];
+            }
+            // the checkpoint row, loaded into the top slot during the previous segment
+#pragma unroll
+            for (int r = 0; r < W; r++) F[0][r] = F[K - 1][r];
             uint64_t fwin[NWIN];
-            win_fill(fwin, s0 - bw);           // forward row s0 + 1: cell r at window index s0 - bw + r
+            win_fill(fwin, s0 - bw);             // forward row s0 + 1: cell r at window index s0 - bw + r
 #pragma unroll
             for (int t = 1; t < K; t++) {
                 const int i = s0 + t;
                 if (i <= qLen) {
                     if (t > 1) win_shr(fwin, refn(i - bw + W - 2 + W));
                     const uint32_t v = rowv(i - 1);
-                    const double sum = fwd_row(i, M, I, D, fwin, v >> 8, qprob((uint8_t)(v & 255)));
-                    fwd_scale(M, I, D, sum);
-#pragma unroll
-                    for (int r = 0; r < W; r++) F[t][r] = make_double2(M[r], I[r]);
+                    const double sum = fwd_any(i, F[t - 1], F[t], Dr, fwin, v >> 8, qprob((uint8_t)(v & 255)));
+                    fwd_scale(F[t], Dr, 1. / sum);
                 }
             }
-            double Sv[K];
-            uint64_t Cv[K];
+            // the forward D is dead: the next checkpoint's D comes in now
+            const int jn = j > 0 ? j - 1 : 0;
+            {
+                const double *cd = Cd + (uint64_t)jn * W * 64 + lane;
 #pragma unroll
-            for (int t = 0; t < K; t++) {
-                const int i = s0 + t <= qLen ? s0 + t : qLen;
-                Sv[t] = Sst[(uint64_t)i * 64 + lane];
-                Cv[t] = Ost[(uint64_t)i * 64 + lane];
+                for (int r = 0; r < W; r++) Dr[r] = cd[r * 64];
             }
-            if (PREFETCH && j > 0) load_ckpt(j - 1, M, I, D);   // arrives while this segment's backward rows run
 #pragma unroll
             for (int t = K - 1; t >= 0; t--) {
                 const int i = s0 + t;
                 if (i > qLen) continue;
+                const uint32_t code = rcode[t];
                 if (i < qLen) {
                     win_shl(bwin, refn(i - bw + W));
                     const uint32_t v = rowv(i);
                     const uint8_t qb = (uint8_t)(v >> 8);
                     const double p = qprob((uint8_t)(v & 255));
-                    const double y = (i > 1) ? 1. : 0.;
-                    const double em_eq = 1 - p, em_ne = p * EM;
-                    const int klo = 1 > i - bw ? 1 : i - bw;
-                    const int khi = refLen < i + bw ? refLen : i + bw;
-                    double cD = 0.0;
-#pragma unroll
-                    for (int r = W - 1; r >= 0; r--) {
-                        const int k = i - bw + r;
-                        const bool in = k >= klo && k <= khi;
-                        const double pM = bMv[r];
-                        const double pIl = r >= 1 ? bIv[r >= 1 ? r - 1 : 0] : 0.0;
-                        const uint8_t rb = (uint8_t)((bwin[r >> 4] >> (4 * (r & 15))) & 15);
-                        const double e0 = ((rb == B_N || qb == B_N) ? 1.0 : (rb == qb ? em_eq : em_ne)) * pM;
-                        const double e = k >= refLen ? 0.0 : e0;
-                        const double mv = in ? e * m0 + EIm1 * pIl + m2 * cD : 0.0;
-                        const double iv = in ? e * m3 + EIm4 * pIl : 0.0;
-                        const double dv = in ? (e * m6 + m8 * cD) * y : 0.0;
-                        bMv[r] = mv; bIv[r] = iv;
-                        cD = dv;
-                    }
-                    const double yy = 1.0 / Sv[t];
+                    if (__all(bwd_steady(i, bwin, qb))) bwd_row(Steady<true>{}, i, qb, p);
+                    else bwd_row(Steady<false>{}, i, qb, p);
+                    const double yy = rinv[t];
 #pragma unroll
                     for (int r = 0; r < W; r++) { bMv[r] = bMv[r] * yy; bIv[r] = bIv[r] * yy; }
                 }
@@ -1293,23 +1413,43 @@ __global__ void __launch_bounds__(256, 1) k_baq_seg(BaqArgs a)
                     if (z > mx) { mx = z; stt = (k - 1) << 2 | 1; }
                 }
                 mx /= sum;
-                const uint64_t code = Cv[t];
                 const uint8_t oq = (uint8_t)(rowv(i - 1) & 255);
-                Ost[(uint64_t)i * 64 + lane] = cap_tag((uint32_t)(code >> 32), (uint32_t)code, oq, mapq(1.0 - mx), (uint32_t)stt);
+                const uint8_t tag = cap_tag(code >> 30, code & CAP_EP_MASK, oq, mapq(1.0 - mx), (uint32_t)stt);
+                if constexpr (TAGL) tagl[(i - 1) * 64 + lane] = tag;
+                else Ost[(uint64_t)i * 64 + lane] = tag;
+                if (t == K - 1) {
+                    // the top row is consumed: its slot takes the next checkpoint's (M, I)
+                    const double2 *cmi = Cmi + (uint64_t)jn * W * 64 + lane;
+#pragma unroll
+                    for (int r = 0; r < W; r++) F[K - 1][r] = cmi[r * 64];
+                }
+            }
+            if (s0 + K - 1 > qLen) {
+                // the top row was not this lane's (its window ends inside the segment): load here
+                const double2 *cmi = Cmi + (uint64_t)jn * W * 64 + lane;
+#pragma unroll
+                for (int r = 0; r < W; r++) F[K - 1][r] = cmi[r * 64];
             }
         }
+        // the read's tags, row by row in lockstep across the wave
         uint8_t *out = a.s.baq + b.qual_start[rd] + qStart;
-        const uint64_t *orow = Ost + 64 + lane;
-        int j = 0;
-        for (; j + 8 <= qLen; j += 8) {
-            uint64_t t[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) t[u] = orow[(uint64_t)(j + u) * 64];
-#pragma unroll
-            for (int u = 0; u < 8; u++) out[j + u] = good ? (uint8_t)t[u] : (uint8_t)64;
+        if constexpr (TAGL) {
+            const uint8_t *trow = tagl + lane;
+            for (int j = 0; j < qLen; j++) out[j] = good ? trow[j * 64] : (uint8_t)64;
+        } else {
+            const uint32_t *orow = Ost + 64 + lane;
+            for (int j = 0; j < qLen; j++) out[j] = good ? (uint8_t)orow[(uint64_t)j * 64] : (uint8_t)64;
         }
-        for (; j < qLen; j++) out[j] = good ? (uint8_t)orow[(uint64_t)j * 64] : (uint8_t)64;
-    }
+    };
+    // chunks from the counter (zero at launch), longest windows first, or round-robin without it
+    const uint32_t nchunks = (uint32_t)((total + 63) / 64);
+    auto take = [&]() -> uint32_t {
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(a.next, 1u);
+        return __builtin_amdgcn_readfirstlane(c);
+    };
+    for (uint32_t k = a.next ? take() : gwave; k < nchunks; k = a.next ? take() : k + nwaves)
+        run_chunk(a.next ? nchunks - 1 - k : k);
     for (int off = 32; off >= 1; off >>= 1) ncell += __shfl_xor(ncell, off);
     if (lane == 0 && ncell) atomicAdd(a.cells, (unsigned long long)ncell);
 }
@@ -1501,6 +1641,7 @@ extern "C" hipError_t fpd_launch_baq_classify(uint32_t num_reads, const uint32_t
 }
 
 static int g_baq_lds_limit = 64 * 1024;
+static int g_baq_cu_lds = 64 * 1024;     // LDS of one CU
 
 // LDS bytes of a STAGE band workgroup (4 waves): per wave rows_cap quality bytes, rows_cap query
 // base nibbles and rows_cap + 2W reference nibbles, and with tags rows_cap tag bytes
@@ -1519,22 +1660,33 @@ extern "C" hipError_t fpd_baq_prepare()
     if (e != hipSuccess) return e;
     // the static tables (qualities and thresholds, 3 KiB) share the limit
     g_baq_lds_limit = (v > 0 ? v : 64 * 1024) - 4096;
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess && cu > 0)
+        g_baq_cu_lds = cu;
     const int lim = g_baq_lds_limit;
     (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<15, 1, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<15, 2, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<23, 1, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<33, 1, true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<15, 1, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<23, 1, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     (void)hipFuncSetAttribute((const void *)fpd::k_baq_band<33, 1, true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
-    (void)hipFuncSetAttribute((const void *)fpd::k_baq_seg<15, 2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_ckpt<15, 3, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_ckpt<15, 3, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_ckpt<15, 4, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_ckpt<15, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_ckpt<15, 5, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void *)fpd::k_baq_ckpt<15, 5, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     return hipSuccess;
 }
 
-// whether a band class runs its STAGE kernel for this row capacity (class 0 only at one wave per SIMD)
+// whether a band class runs its STAGE kernel for this row capacity (class 0 at one wave per SIMD, or at
+// two when two workgroups' staging areas fit one CU's LDS)
 extern "C" int fpd_baq_staged(int cls, uint32_t rows_cap, int wpe, int allow)
 {
-    if (!allow || cls > 2 || (cls == 0 && wpe != 1)) return 0;
+    if (!allow || cls > 2 || (cls == 0 && wpe > 2)) return 0;
     const int W = cls == 0 ? 15 : (cls == 1 ? 23 : 33);
+    if (cls == 0 && wpe == 2) return 2 * (baq_stage_bytes(W, rows_cap) + 4096) <= (size_t)g_baq_cu_lds;
     return baq_stage_bytes(W, rows_cap) <= (size_t)g_baq_lds_limit;
 }
 
@@ -1556,14 +1708,24 @@ extern "C" hipError_t fpd_launch_baq_hmm(int cls, const fpd::BatchDev *b, const 
     const bool stage = fpd_baq_staged(cls, rows_cap, wpe, allow_stage) != 0;
     const int W = cls == 0 ? 15 : (cls == 1 ? 23 : 33);
     // the tags stay in LDS too when the budget holds them (short reads)
-    const bool tags = stage && cls <= 2 && baq_stage_bytes(W, rows_cap, true) <= (size_t)g_baq_lds_limit;
+    const bool tags = stage && cls <= 2 && !(cls == 0 && wpe == 2) &&
+                      baq_stage_bytes(W, rows_cap, true) <= (size_t)g_baq_lds_limit;
     const size_t lds = stage ? baq_stage_bytes(W, rows_cap, tags) : 0;
     switch (cls) {
     case 0:
-        if (stage && recompute) {
-            hipLaunchKernelGGL((fpd::k_baq_seg<15, 2, true>), g4, b4, baq_stage_bytes(15, rows_cap), st, a);
+        if (stage && recompute >= 2) {
+            // checkpoint every `recompute` rows; the tags stay in LDS when the budget holds them
+            switch (recompute) {
+            case 3: if (tags) hipLaunchKernelGGL((fpd::k_baq_ckpt<15, 3, true>), g4, b4, lds, st, a);
+                    else hipLaunchKernelGGL((fpd::k_baq_ckpt<15, 3, false>), g4, b4, lds, st, a); break;
+            case 5: if (tags) hipLaunchKernelGGL((fpd::k_baq_ckpt<15, 5, true>), g4, b4, lds, st, a);
+                    else hipLaunchKernelGGL((fpd::k_baq_ckpt<15, 5, false>), g4, b4, lds, st, a); break;
+            default: if (tags) hipLaunchKernelGGL((fpd::k_baq_ckpt<15, 4, true>), g4, b4, lds, st, a);
+                     else hipLaunchKernelGGL((fpd::k_baq_ckpt<15, 4, false>), g4, b4, lds, st, a); break;
+            }
             break;
         }
+        if (stage && wpe == 2) { hipLaunchKernelGGL((fpd::k_baq_band<15, 2, true, false>), g4, b4, lds, st, a); break; }
         if (tags) { hipLaunchKernelGGL((fpd::k_baq_band<15, 1, true, true>), g4, b4, lds, st, a); break; }
         if (stage) { hipLaunchKernelGGL((fpd::k_baq_band<15, 1, true, false>), g4, b4, lds, st, a); break; }
         switch (wpe) {

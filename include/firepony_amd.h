@@ -89,11 +89,14 @@ typedef struct fp_options {
                                         in LDS once per read (0 = stage when the batch's
                                         longest read fits the LDS budget)              */
     uint32_t baq_checkpoint_rows;    /* 15-cell BAQ band kernel: 1 = every forward row's
-                                        state spilled to HBM for the backward pass; 2 =
-                                        every second row's, the rows between recomputed
-                                        (same results, a quarter less forward-row
-                                        traffic, a second forward pass); 0 = 1.  Used
-                                        with LDS staging at one wave per SIMD.          */
+                                        state spilled to HBM for the backward pass; 3, 4
+                                        or 5 = the full state of every K-th row only, the
+                                        rows between recomputed into registers segment by
+                                        segment (same results, 48/K instead of 32 bytes of
+                                        forward-row traffic per band cell, a second
+                                        forward pass); 0 = chosen by the autotuner on the
+                                        batched engine (1 otherwise).  Used with LDS staging
+                                        at one wave per SIMD.                            */
     uint32_t serve_hbm_budget_mib;   /* serving: HBM admission budget in MiB for the open jobs'
                                         resident tables and postprocess reserve; 0 = the
                                         read-group slots alone bound admission          */

@@ -36,7 +36,7 @@ def mixed_workload(n=20000, seed=21):
                                 contigs=(400000, 300000), sites=800, clip=1.0, indel=0.05)
 
 
-@pytest.mark.parametrize("wpe,ckpt", [(1, 1), (0, 0), (1, 2)])
+@pytest.mark.parametrize("wpe,ckpt", [(1, 1), (0, 0), (1, 3), (1, 4)])
 def test_batched_equals_plain_with_replays(tmp_path, wpe, ckpt):
     say(f"batched vs plain: mixed lengths, staging buffers of 4096 reads, BAQ waves/SIMD {wpe or 'autotuned'}, "
         f"forward checkpoint rows {ckpt or 'autotuned'}")

@@ -119,7 +119,7 @@ def run_parity(tmp_path, genome, rs, n, batch_size, per_batch=True, **engine_opt
     e.close()
 
 
-@pytest.mark.parametrize("wpe,nostage,ckpt", [(1, 0, 1), (1, 1, 1), (2, 0, 1), (3, 0, 1), (4, 0, 1), (1, 0, 2)])
+@pytest.mark.parametrize("wpe,nostage,ckpt", [(1, 0, 1), (1, 1, 1), (2, 0, 1), (3, 0, 1), (4, 0, 1), (1, 0, 3), (1, 0, 4), (1, 0, 5)])
 def test_config0_slice(tmp_path, wpe, nostage, ckpt):
     say(f"parity: config 0 first 4000 reads, BAQ register budget for {wpe} waves/SIMD, LDS staging {1 - nostage}, "
         f"forward checkpoint every {ckpt} rows")
@@ -129,7 +129,7 @@ def test_config0_slice(tmp_path, wpe, nostage, ckpt):
                baq_checkpoint_rows=ckpt)
 
 
-@pytest.mark.parametrize("ckpt", [1, 2])
+@pytest.mark.parametrize("ckpt", [1, 3, 4])
 def test_edge_cases(tmp_path, ckpt):
     say(f"parity: edge-case menu, paired, N runs, multi-base sites, forward checkpoint every {ckpt} rows")
     g, rs, n = synth.small_workload(seed=7, num_reads=3000, edge=0.35, lengths=(36, 100, 151), read_groups=3,
@@ -138,7 +138,7 @@ def test_edge_cases(tmp_path, ckpt):
     run_parity(tmp_path, g, rs, n, 1000, baq_checkpoint_rows=ckpt)
 
 
-@pytest.mark.parametrize("nostage,ckpt", [(0, 1), (1, 1), (0, 2)])
+@pytest.mark.parametrize("nostage,ckpt", [(0, 1), (1, 1), (0, 3), (0, 4)])
 def test_mixed_lengths_f64(tmp_path, nostage, ckpt):
     say(f"parity: mixed lengths, large indels, fp64 fractional errors, LDS staging {1 - nostage}, "
         f"forward checkpoint every {ckpt} rows")
