@@ -81,6 +81,13 @@ struct BatchDev {
 
 __device__ inline uint32_t live_reads(const BatchDev &b) { return b.live ? b.live[0] : b.num_reads; }
 
+// Wave votes over the lanes still running, as ballots, which the compiler keeps where they are
+// written.  HIP's __all and __any call library functions declared const, which the optimiser may
+// move to a point where other lanes run, so their answer need not describe the lanes that branch
+// on it.
+__device__ inline bool wave_all(bool p) { return __ballot(p) == __ballot(1); }
+__device__ inline bool wave_any(bool p) { return __ballot(p) != 0; }
+
 struct GenomeDev {
     uint32_t num_contigs;
     const uint64_t *word_offset;

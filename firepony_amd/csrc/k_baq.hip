@@ -1079,83 +1079,21 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
             Ost[(uint64_t)((int)p - qStart + 1) * 64 + lane] = cap_code(kind, ep);
         });
 
-        // This read's inputs into the wave's LDS area (each lane writes and later reads only its own
-        // column).  With one wave per SIMD nothing hides a load's latency but the wave's own work, so
-        // the loads go out 16 at a time and are consumed after one wait.
-        {
-            // qualities: the aligned words holding bytes [qqual, qqual + qLen); every such word holds
-            // at least one of them, so it lies in the same page as the batch's own bytes
-            const int sh = (int)((uintptr_t)(b.quals + qqual) & 3);
-            const uint32_t *qw = (const uint32_t *)(b.quals + qqual - sh);
-            const int nw = (sh + qLen + 3) >> 2;
-            for (int c = 0; c < nw; c += 16) {
-                uint32_t w[16];
-#pragma unroll
-                for (int u = 0; u < 16; u++) w[u] = qw[c + u < nw ? c + u : nw - 1];
-#pragma unroll
-                for (int u = 0; u < 16; u++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int q = 4 * (c + u) + j - sh;
-                        if (q >= 0 && q < qLen) rowq[q * 64 + lane] = (uint8_t)(w[u] >> (8 * j));
-                    }
-            }
+        // this read's inputs into the wave's LDS area (each lane writes and later reads only its own column)
+#pragma unroll 4
+        for (int q = 0; q < qLen; q++) rowq[q * 64 + lane] = b.quals[qqual + q];
+#pragma unroll 4
+        for (int q = 0; q < qLen; q += 2) {
+            const uint32_t w = b.reads[(qbase + q) >> 3];
+            const uint32_t b0 = (w >> (((qbase + q) & 7) * 4)) & 15;
+            const uint64_t i1 = qbase + (q + 1 < qLen ? q + 1 : q);
+            const uint32_t b1 = q + 1 < qLen ? (b.reads[i1 >> 3] >> ((i1 & 7) * 4)) & 15 : 15u;
+            rowb[(q >> 1) * 64 + lane] = (uint8_t)(b0 | (b1 << 4));
         }
-        {
-            // query bases: byte q / 2 holds bases q (low nibble) and q + 1, the base stream realigned
-            // to the window start; nibbles past the window are never read
-            const uint64_t w0 = qbase >> 3, wl = (qbase + qLen - 1) >> 3;
-            const int sh = (int)(qbase & 7) * 4;
-            const int nw = (int)(wl - w0) + 1;
-            for (int c = 0; c < nw; c += 16) {
-                uint32_t w[17];
-#pragma unroll
-                for (int u = 0; u < 17; u++) w[u] = b.reads[w0 + (uint64_t)(c + u < nw ? c + u : nw - 1)];
-#pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    const uint32_t v = (uint32_t)((((uint64_t)w[u + 1] << 32) | w[u]) >> sh);
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int bi = 4 * (c + u) + j;
-                        if (2 * bi < qLen) rowb[bi * 64 + lane] = (uint8_t)(v >> (8 * j));
-                    }
-                }
-            }
-        }
-        {
-            // reference cells k = 2 bi - W (low nibble) and 2 bi - W + 1 of byte bi for bi in [bi0, bi1]:
-            // the contig's base stream realigned to cell 2 bi0 - W, 8 cells per group of 4 bytes, N
-            // outside the window and the contig (words clamped into the contig, as ref_word does)
-            const int bi0 = (W - bw) >> 1, bi1 = (qLen + 2 * W - 1 - bw) >> 1;
-            const int ks = 2 * bi0 - W;
-            const int64_t ps = hoff + ks;
-            const int64_t w0 = ps >> 3;                       // floor
-            const int sh = (int)(ps & 7) * 4;
-            const int ng = (bi1 - bi0 + 4) >> 2;
-            // valid cells [kv0, kv1)
-            const int64_t kv0 = 0, kv1 = (int64_t)refLen < clen - hoff ? (int64_t)refLen : clen - hoff;
-            for (int c = 0; c < ng; c += 16) {
-                uint32_t w[17];
-#pragma unroll
-                for (int u = 0; u < 17; u++) w[u] = ref_word((w0 + (c + u < ng ? c + u : ng)) * 8);
-#pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    uint32_t v = (uint32_t)((((uint64_t)w[u + 1] << 32) | w[u]) >> sh);
-                    const int64_t kg = ks + 8 * (c + u);       // cell of nibble 0
-                    int64_t a0 = kv0 - kg, a1 = kv1 - kg;
-                    a0 = a0 < 0 ? 0 : (a0 > 8 ? 8 : a0);
-                    a1 = a1 < 0 ? 0 : (a1 > 8 ? 8 : a1);
-                    const uint32_t below1 = a1 >= 8 ? ~0u : ((1u << (4 * a1)) - 1);
-                    const uint32_t below0 = a0 >= 8 ? ~0u : ((1u << (4 * a0)) - 1);
-                    const uint32_t m = a1 > a0 ? (below1 & ~below0) : 0u;
-                    v = (v & m) | ~m;                          // N = 15 in every invalid cell
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int bi = bi0 + 4 * (c + u) + j;
-                        if (bi <= bi1) refd[bi * 64 + lane] = (uint8_t)(v >> (8 * j));
-                    }
-                }
-            }
+#pragma unroll 4
+        for (int bi = (W - bw) >> 1; bi <= (qLen + 2 * W - 1 - bw) >> 1; bi++) {
+            const int k0 = 2 * bi - W, k1 = k0 + 1;
+            refd[bi * 64 + lane] = (uint8_t)(nibble(ref_word(hoff + k0), k0) | (nibble(ref_word(hoff + k1), k1) << 4));
         }
 
         // A steady row has every band cell inside the band and the HMM window, no N among the band's
@@ -1208,7 +1146,7 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
         };
         auto fwd_any = [&](int i, const double2 (&P)[W], double2 (&O)[W], double (&D)[W],
                            const uint64_t (&win)[NWIN], uint32_t qb, double p) -> double {
-            if (__all(fwd_steady(i, win, qb))) return fwd_row(Steady<true>{}, i, P, O, D, win, qb, p);
+            if (wave_all(fwd_steady(i, win, qb))) return fwd_row(Steady<true>{}, i, P, O, D, win, qb, p);
             return fwd_row(Steady<false>{}, i, P, O, D, win, qb, p);
         };
         // rows >= 2 are scaled by the reciprocal of their sum (baq.cu:505-510)
@@ -1392,7 +1330,7 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
                     const uint32_t v = rowv(i);
                     const uint8_t qb = (uint8_t)(v >> 8);
                     const double p = qprob((uint8_t)(v & 255));
-                    if (__all(bwd_steady(i, bwin, qb))) bwd_row(Steady<true>{}, i, qb, p);
+                    if (wave_all(bwd_steady(i, bwin, qb))) bwd_row(Steady<true>{}, i, qb, p);
                     else bwd_row(Steady<false>{}, i, qb, p);
                     const double yy = rinv[t];
 #pragma unroll

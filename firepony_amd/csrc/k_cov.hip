@@ -777,7 +777,7 @@ __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
         const uint32_t cnt = min(64u, w1 - g);
         const ReadMeta ml = meta_lane(a, g + lane, lane < cnt);
         const int len_l = (int)hi16(ml.rw) - (int)lo16(ml.rw) + 1;
-        if (!__any(lane < cnt && len_l > 128)) {
+        if (!wave_any(lane < cnt && len_l > 128)) {
             // reads of up to 128 window bases: software-pipelined, the next read's inputs are
             // in flight while this one is processed
             ReadMeta m = meta_of(ml, 0);

@@ -765,7 +765,7 @@ __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
 
     uint8_t st = RS_FILTERED;
     uint16_t nerr = 0;
-    if (!__all(L <= RL)) {
+    if (!wave_all(L <= RL)) {
         if (valid) st = prep_read_lane(a, r, nerr);
     } else {
         const bool ok = valid && prep_filter(a, r);
@@ -801,7 +801,7 @@ __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
             p.L = rl(L, j); p.pj = (p.L + 7) & ~7u;
             return p;
         };
-        if (!__any(ok && L > 128)) prep_pass<2>(a, okmask, read_of, prep_rows, RL, nerr);
+        if (!wave_any(ok && L > 128)) prep_pass<2>(a, okmask, read_of, prep_rows, RL, nerr);
         else prep_pass<4>(a, okmask, read_of, prep_rows, RL, nerr);
         __syncthreads();
 

@@ -147,6 +147,18 @@ def test_mixed_lengths_f64(tmp_path, nostage, ckpt):
     run_parity(tmp_path, g, rs, n, 1500, frac_error_f64=1, disable_baq_staging=nostage, baq_checkpoint_rows=ckpt)
 
 
+@pytest.mark.parametrize("ckpt", [1, 3])
+def test_waves_taking_several_chunks(tmp_path, ckpt):
+    """One 150k-read batch of the mixed-length workload (50-250 bp): the 15-cell class then has more
+    64-read chunks than the band kernel has resident waves (1024 at one wave per SIMD), so each wave
+    takes several chunks in turn and reuses its LDS staging area and scratch slot.  Stage by stage
+    against the oracle, per forward checkpoint interval."""
+    say(f"parity: 150k mixed-length reads in one batch, waves taking several chunks, checkpoint every {ckpt} rows")
+    gs, rs, _ = synth.named_spec(2)
+    g = synth.Genome(gs, rs.num_read_groups)
+    run_parity(tmp_path, g, rs, 150000, 150000, frac_error_f64=1, baq_waves_per_simd=1, baq_checkpoint_rows=ckpt)
+
+
 def test_long_reads(tmp_path):
     say("parity: reads longer than 256 bases next to short ones (per-lane flag path of the prep kernel)")
     g, rs, n = synth.small_workload(seed=13, num_reads=1500, edge=0.1, lengths=(90, 300, 420), read_groups=2,
