@@ -457,7 +457,7 @@ def main():
                     help="covariate aggregation of the profile-only pass")
     ap.add_argument("--no-sort", action="store_true", help="skip the sort-aggregation pass")
     ap.add_argument("--batched-only", action="store_true",
-                    help="only the batched engine's timed pass (graph replays), for kernel-overlap traces")
+                    help="only the batched engine's timed passes (host-fed and HBM-resident), for kernel-overlap traces")
     ap.add_argument("--workload", choices=("config1", "mixed"), default="config1",
                     help="workload of the profile-only pass (mixed: the first --mixed-reads reads of the "
                          "50-250 bp set)")
@@ -473,6 +473,8 @@ def main():
     ap.add_argument("--stage-reads", type=int, default=1 << 21,
                     help="reads per staging buffer of the batched passes (0 = the engine's default, 2^20); "
                          "the per-kernel profile and roofline passes keep the default")
+    ap.add_argument("--staging-buffers", type=int, default=0,
+                    help="staging buffers of the batched passes, 2 or 3 (0 = the engine's default, 3)")
     ap.add_argument("--wpe", type=int, default=0, help="BAQ waves per SIMD (0 = autotuned)")
     ap.add_argument("--ckpt", type=int, default=0,
                     help="rows per BAQ forward checkpoint, 1 or 2 (0 = autotuned; profile-only pass: 1)")
@@ -510,6 +512,8 @@ def main():
     batched_opts = dict(profile_opts)
     if args.stage_reads:
         batched_opts.update(max_batch_reads=args.stage_reads)
+    if args.staging_buffers:
+        batched_opts.update(staging_buffers=args.staging_buffers)
     if args.profile_only:
         log(f"generating {(total + B - 1) // B} input batches of {B} reads")
         dev = upload(genome, rs, total, B, torch)
@@ -552,12 +556,13 @@ def main():
                                  baq_waves_per_simd=wpe, baq_checkpoint_rows=ckpt, **batched_opts)
     value = reads / dt
     log(f"batched, host batches: {reads} reads in {dt:.3f} s -> {value:.4g} reads/s")
-    if args.batched_only:
-        print(json.dumps({"batched_only": True, "reads": reads, "seconds": dt, "value": value}), flush=True)
-        return
     ddt, _, dreads = stepped_pass(genome, dev, args.steps, args.warmup, torch, False, per_step,
                                   baq_waves_per_simd=wpe, baq_checkpoint_rows=ckpt, **batched_opts)
     log(f"batched, HBM-resident batches: {dreads / ddt:.4g} reads/s")
+    if args.batched_only:
+        print(json.dumps({"batched_only": True, "reads": reads, "seconds": dt, "value": value,
+                          "device_resident": dreads / ddt}), flush=True)
+        return
     out_plain = None
     if not args.no_plain:
         pdt, pst_, preads = stepped_pass(genome, pin, args.steps, 1, torch, True, per_step,
@@ -619,6 +624,8 @@ def main():
         mopts = dict(engine_kind=abi.FP_ENGINE_BATCHED, max_read_length=250, frac_error_f64=1)
         if args.stage_reads:
             mopts.update(max_batch_reads=args.stage_reads)
+        if args.staging_buffers:
+            mopts.update(staging_buffers=args.staging_buffers)
         pdt2, _ = timed_pass(genome2, dev2, 1, torch, baq_waves_per_simd=1, frac_error_f64=1)
         bdt2, bst2 = timed_pass(genome2, dev2, 1, torch, **mopts)
         out["mixed_length"] = {"reads": n2, "workload": "50-250 bp reads, 8 read groups, 5 % indel reads of "

@@ -5,7 +5,7 @@ Structures and enums here must match the header field for field; the
 """
 import ctypes as C
 
-FP_ABI_VERSION = 1
+FP_ABI_VERSION = 2
 
 FP_OK = 0
 FP_ERR_INVALID_ARG = 1
@@ -59,6 +59,7 @@ class FpOptions(C.Structure):
         ("disable_baq_staging", C.c_uint32),
         ("baq_checkpoint_rows", C.c_uint32),
         ("serve_hbm_budget_mib", C.c_uint32),
+        ("staging_buffers", C.c_uint32),
     ]
 
 

@@ -218,6 +218,11 @@ struct fp_engine {
     // the BAQ stage's second stream, shared by every batch state (the BAQ stages of the staging buffers
     // take turns): a process has few hardware queues, and streams beyond them share one
     hipStream_t side = nullptr;
+    // host batches of the batched engine are copied by the DMA engines on a stream of their own: the
+    // copies then run while the BAQ kernels hold every CU's registers (a copy kernel would wait for
+    // them), and the host waits for its own batch's copies only
+    hipStream_t copy = nullptr;
+    hipEvent_t copied = nullptr;
     std::string name;
     HostConstants consts;
     int wpe = 1;   // waves per SIMD of the 15-cell BAQ band kernel (1-4)
@@ -281,12 +286,15 @@ struct fp_engine {
         bool back_pending = false;      // stage-2 time not yet collected
         uint64_t fill = 1;              // which fill of this buffer is staging now (a job records the
                                         // fills that hold its reads)
+        bool host_fed = false;          // this fill holds a batch copied from host memory
+        uint32_t g_reserve = 0;         // reserved waves the BAQ graph was captured with
     };
-    // staging buffers: one fills and runs its prep and covariates while the other's BAQ runs (a third
-    // buffer lets more kernels overlap but measured no faster: the BAQ kernels hold the HBM
-    // bandwidth, so kernels running beside them slow them down)
-    static constexpr int NSTAGE = 2;
-    Stage stg[NSTAGE];
+    // staging buffers, used in turn: one fills while the others run.  A buffer refills only once its
+    // previous batch has run, so with two a host-fed buffer's copies wait for its own last batch;
+    // with three (the default) they overlap the other two buffers' kernels.
+    static constexpr int MAX_STAGE = 3;
+    Stage stg[MAX_STAGE];
+    int nstage = 3;
     bool batched = false;
     int cur = 0;
     uint32_t cap_reads = 0, rows_cap = 0;
@@ -327,7 +335,7 @@ struct fp_job {
     uint64_t reads = 0, batches = 0;
     double post_ms = 0;
     TableRows rows[4];
-    uint64_t fill[2] = {0, 0};   // per staging buffer: the last fill that holds the job's reads (0: none)
+    uint64_t fill[fp_engine::MAX_STAGE] = {};   // per staging buffer: the last fill that holds the job's reads (0: none)
     uint64_t bytes = 0;          // HBM footprint counted against the engine's budget
 };
 
@@ -449,13 +457,24 @@ struct Plan {
     double *band = nullptr;
     uint64_t band_doubles = 0;
     bool no_frac = false;
+    // resident BAQ waves left out of every band class's grid (a multiple of 4: whole CUs)
+    uint32_t reserve_waves = 0;
 };
 
+// A band kernel workgroup takes all of a CU's registers (the 15-cell checkpoint kernel 512 per lane,
+// the wider classes over 350), so while the persistent band grids run, no other kernel starts
+// anywhere.  A staging buffer filled from host memory therefore leaves an eighth of the CUs (4 per
+// XCD) out of its band grids: the copy kernels reading the next host batches over the host link run
+// there instead of waiting for the BAQ stage to end.  Measured on the 10M-read workload fed in 20k-read
+// host batches: 7.1e7 reads/s with every CU in the grid, 8.2e7 with 32 left out (7.2e7 with 24, 8.1e7
+// with 48); batches already in HBM keep every CU (1.09e8; 1.03e8 with 32 left out).
+static constexpr uint32_t HOST_FED_RESERVE_WAVES = 4 * 32;
+
 // resident waves of a band class; its per-wave scratch slot in doubles
-static uint32_t band_waves(uint32_t wpe_class0, int c, uint32_t n)
+static uint32_t band_waves(uint32_t wpe_class0, int c, uint32_t n, uint32_t reserve = 0)
 {
     const uint32_t wpe = c == 0 ? wpe_class0 : 1u;
-    const uint32_t w = std::min<uint32_t>((n + 63) / 64, 1024u * wpe);
+    const uint32_t w = std::min<uint32_t>((n + 63) / 64, (1024u - reserve) * wpe);
     return (w + 3) / 4 * 4;
 }
 static uint64_t band_slot(int c, uint32_t rows_cap, uint32_t wgen)
@@ -465,10 +484,10 @@ static uint64_t band_slot(int c, uint32_t rows_cap, uint32_t wgen)
     return ((uint64_t)(rows_cap + 1) * W * 2 + (rows_cap + 2) + (c == 3 ? 3ull * W : rows_cap + 1ull)) * 64;
 }
 // the band classes run at the same time (two streams), so each has its own region of the scratch
-static uint64_t band_offset(int wpe, int c, uint32_t n, uint32_t rows_cap)
+static uint64_t band_offset(int wpe, int c, uint32_t n, uint32_t rows_cap, uint32_t reserve = 0)
 {
     uint64_t off = 0;
-    for (int k = 0; k < c; k++) off += band_slot(k, rows_cap, 0) * band_waves(wpe, k, n);
+    for (int k = 0; k < c; k++) off += band_slot(k, rows_cap, 0) * band_waves(wpe, k, n, reserve);
     return off;
 }
 static uint64_t band_scratch_doubles(int wpe, uint32_t n, uint32_t rows_cap)
@@ -570,8 +589,10 @@ fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, 
     auto launch_class = [&](int c, hipStream_t s) -> hipError_t {
         // the class's chunk counter: a word of the class counts that the front stage zeroes
         return fpd_launch_baq_hmm(c, &b, &e->genome, &S.rs, S.baq_lists.as<uint32_t>() + (uint64_t)c * cap,
-                                  S.baq_counts.as<uint32_t>() + c, e->scratch.as<double>() + band_offset(wpe, c, p.n, p.rows_cap),
-                                  band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c], band_waves(wpe, c, p.n),
+                                  S.baq_counts.as<uint32_t>() + c,
+                                  e->scratch.as<double>() + band_offset(wpe, c, p.n, p.rows_cap, p.reserve_waves),
+                                  band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c],
+                                  band_waves(wpe, c, p.n, p.reserve_waves),
                                   (unsigned long long *)(counters + (c == 0 ? 10 : 8)), wpe,
                                   e->opts.disable_baq_staging ? 0 : 1, ckpt > 1 ? ckpt : 0,
                                   S.baq_counts.as<uint32_t>() + BAQ_CHUNK_COUNTERS + c, s);
@@ -808,7 +829,8 @@ fp_status batched_alloc(fp_engine *e)
     const uint64_t C = e->cap_reads;
     const uint64_t sz[18] = {2 * C, 4 * C, 4 * C, 4 * C, 4 * C, 4 * C, 4 * e->cap_cig, 4 * C, 4 * C,
                              e->cap_bases / 2, 4 * C, 4 * C, e->cap_quals, 4 * C, 4 * C, 2 * C, C, 4 * C};
-    for (auto &s : e->stg) {
+    for (int si = 0; si < e->nstage; si++) {
+        fp_engine::Stage &s = e->stg[si];
         for (int i = 0; i < 18; i++) HIP_TRY(s.arr[i].ensure(sz[i] + 64));
         HIP_TRY(s.live.ensure(16));
         if (!s.live_host) HIP_TRY(hipHostMalloc((void **)&s.live_host, 16, hipHostMallocDefault));
@@ -903,6 +925,7 @@ static void stage_drop(fp_engine *e, fp_engine::Stage &s)
 {
     e->stats.total_reads -= s.n;
     s.n = 0;
+    s.host_fed = false;
     s.ncig = s.nbases = s.nquals = 0;
     s.fill += 1;
 }
@@ -916,8 +939,9 @@ static void stage_done(fp_engine *e, fp_engine::Stage &s)
     e->last_num_quals = s.nquals;
     e->last = &s.bs;
     s.n = 0;
+    s.host_fed = false;
     s.ncig = s.nbases = s.nquals = 0;
-    e->cur = (e->cur + 1) % fp_engine::NSTAGE;
+    e->cur = (e->cur + 1) % e->nstage;
     e->postprocessed = false;
 }
 
@@ -930,7 +954,7 @@ static void stage_done(fp_engine *e, fp_engine::Stage &s)
 fp_status batched_flush(fp_engine *e)
 {
     fp_engine::Stage &s = e->stg[e->cur];
-    fp_engine::Stage &o = e->stg[(e->cur + fp_engine::NSTAGE - 1) % fp_engine::NSTAGE];
+    fp_engine::Stage &o = e->stg[(e->cur + e->nstage - 1) % e->nstage];
     if (s.n == 0) return FP_OK;
     fp_status rc = batched_alloc(e);
     if (rc != FP_OK) return rc;
@@ -948,7 +972,8 @@ fp_status batched_flush(fp_engine *e)
     HIP_TRY(hipMemcpyAsync(prev, S.baq_counts.as<uint32_t>() + fpd::BW_OVERFLOW, 4, hipMemcpyDeviceToHost, st));
 
     const fpd::BatchDev b = stage_dev(e, s);
-    const Plan p{e->cap_reads, e->cap_quals, s.live.as<uint32_t>(), e->rows_cap};
+    Plan p{e->cap_reads, e->cap_quals, s.live.as<uint32_t>(), e->rows_cap};
+    p.reserve_waves = s.host_fed ? HOST_FED_RESERVE_WAVES : 0u;
     uint32_t *counters = e->bcounters.as<uint32_t>();
     if (s.g_epoch != e->epoch) {
         rc = capture(st, &s.gfront, [&] { return launch_front(e, S, b, p, counters, st); });
@@ -1041,7 +1066,8 @@ fp_status batched_flush(fp_engine *e)
     rc = launch_generic(e, S, b, p, hc[3], hc[4], counters, st);
     if (rc != FP_OK) return rc;
     if (cand < 0) {
-        if (!s.gbaq || s.g_wpe != wpe || s.g_ckpt != ckpt || !s.gcov || s.g_bits != bits) {
+        if (!s.gbaq || s.g_wpe != wpe || s.g_ckpt != ckpt || !s.gcov || s.g_bits != bits ||
+            s.g_reserve != p.reserve_waves) {
             rc = capture(st, &s.gbaq,
                          [&] { return launch_baq_stage(e, S, b, p, wpe, ckpt, counters, st, nullptr, nullptr); });
             if (rc != FP_OK) return rc;
@@ -1051,6 +1077,7 @@ fp_status batched_flush(fp_engine *e)
             s.g_wpe = wpe;
             s.g_ckpt = ckpt;
             s.g_bits = bits;
+            s.g_reserve = p.reserve_waves;
         }
         HIP_TRY(hipGraphLaunch(s.gbaq, st));
         HIP_TRY(hipEventRecord(s.baq_done, st));
@@ -1095,7 +1122,8 @@ fp_status batched_sync(fp_engine *e)
     e->stats.baq15_cells = e->eager_cells15 + cells15;
     if (e->opts.aggregation == FP_AGG_SORT) {
         // a staged batch's pair overflow: reported once, then cleared
-        for (auto &s : e->stg) {
+        for (int si = 0; si < e->nstage; si++) {
+            fp_engine::Stage &s = e->stg[si];
             uint32_t *w = s.bs.baq_counts.as<uint32_t>() + fpd::BW_OVERFLOW;
             uint32_t ov = 0;
             HIP_TRY(hipMemcpy(&ov, w, 4, hipMemcpyDeviceToHost));
@@ -1182,26 +1210,45 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
         else { off = elem[i] * s->n; len[i] = elem[i] * k; }
         dst[i] = (char *)s->arr[i].p + off;
     }
-    const void *hsrc[18];
-    const bool pinned = !on_device && pinned_sources(src, len, hsrc);
-    if (on_device || pinned) {
-        // one launch copies and rebases the whole batch (a copy per array costs a launch each); a
-        // pinned host batch is read over the host link by the same kernel
-        HIP_TRY(fpd_append(pinned ? hsrc : src, dst, len, (uint32_t)s->ncig, (uint32_t)s->nbases,
-                           (uint32_t)s->nquals, rg_off, rg_limit, cs));
+    if (on_device) {
+        // one launch copies and rebases the whole batch (a copy per array costs a launch each)
+        HIP_TRY(fpd_append(src, dst, len, (uint32_t)s->ncig, (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off,
+                           rg_limit, cs));
     } else {
-        for (int i = 0; i < 18; i++)
-            if (len[i]) HIP_TRY(hipMemcpyAsync(dst[i], src[i], len[i], kind, cs));
-        HIP_TRY(fpd_rebase(k, s->arr[7].as<uint32_t>() + s->n, s->arr[10].as<uint32_t>() + s->n,
-                           s->arr[13].as<uint32_t>() + s->n, s->arr[17].as<uint32_t>() + s->n, (uint32_t)s->ncig,
-                           (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off, rg_limit, cs));
+        // Host arrays, on the copy stream, which first waits for the buffer's previous batch (its
+        // kernels read these arrays) when this is the fill's first batch; the host then waits for the
+        // copy alone, never for other kernels on the buffer's stream.  Pinned arrays are read over the
+        // host link by one copy-and-rebase kernel (18 DMA copies cost ~25 us each in the runtime:
+        // 4.3e7 reads/s against 7.1e7 with the kernel); pageable ones take the runtime's copies and a
+        // rebase kernel on the buffer's stream.
+        s->host_fed = true;
+        if (!e->copy) {
+            HIP_TRY(hipStreamCreateWithFlags(&e->copy, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&e->copied, hipEventDisableTiming));
+        }
+        if (s->n == 0 && s->cov_done) HIP_TRY(hipStreamWaitEvent(e->copy, s->cov_done, 0));
+        const void *hsrc[18];
+        if (pinned_sources(src, len, hsrc)) {
+            HIP_TRY(fpd_append(hsrc, dst, len, (uint32_t)s->ncig, (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off,
+                               rg_limit, e->copy));
+            HIP_TRY(hipEventRecord(e->copied, e->copy));
+            HIP_TRY(hipStreamWaitEvent(cs, e->copied, 0));
+        } else {
+            for (int i = 0; i < 18; i++)
+                if (len[i]) HIP_TRY(hipMemcpyAsync(dst[i], src[i], len[i], kind, e->copy));
+            HIP_TRY(hipEventRecord(e->copied, e->copy));
+            HIP_TRY(hipStreamWaitEvent(cs, e->copied, 0));
+            HIP_TRY(fpd_rebase(k, s->arr[7].as<uint32_t>() + s->n, s->arr[10].as<uint32_t>() + s->n,
+                               s->arr[13].as<uint32_t>() + s->n, s->arr[17].as<uint32_t>() + s->n, (uint32_t)s->ncig,
+                               (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off, rg_limit, cs));
+        }
     }
     s->n += k;
     s->ncig += v->num_cigar_ops;
     s->nbases += v->num_read_bases;
     s->nquals += v->num_qualities;
     e->stats.total_reads += k;
-    if (!on_device) HIP_TRY(hipStreamSynchronize(cs));   // host arrays are borrowed for the call only
+    if (!on_device) HIP_TRY(hipEventSynchronize(e->copied));   // host arrays are borrowed for the call only
     return FP_OK;
 }
 
@@ -1477,6 +1524,8 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
         return fail(FP_ERR_INVALID_ARG, "baq_checkpoint_rows must be 0 (tuned), 1, 3, 4 or 5");
     if (o.sort_digit_bits != 0 && (o.sort_digit_bits < 4 || o.sort_digit_bits > 8))
         return fail(FP_ERR_INVALID_ARG, "sort_digit_bits must be 0 (autotune) or 4..8");
+    if (o.staging_buffers != 0 && (o.staging_buffers < 2 || o.staging_buffers > 3))
+        return fail(FP_ERR_INVALID_ARG, "staging_buffers must be 0 (three), 2 or 3");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (hip_device < 0 || hip_device >= ndev) return fail(FP_ERR_INVALID_ARG, "no such HIP device");
@@ -1488,6 +1537,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     e->ckpt = o.baq_checkpoint_rows >= 3 ? (int)o.baq_checkpoint_rows : 1;
     e->sort_bits = o.sort_digit_bits ? (int)o.sort_digit_bits : 8;
     e->budget_bytes = (uint64_t)o.serve_hbm_budget_mib << 20;
+    e->nstage = o.staging_buffers ? (int)o.staging_buffers : fp_engine::MAX_STAGE;
     // the digit width is tuned on the batched engine's first staged batches unless it is given
     e->sort_tuned = o.sort_digit_bits != 0 || o.aggregation != FP_AGG_SORT || o.engine_kind != FP_ENGINE_BATCHED;
     memset(&e->stats, 0, sizeof(e->stats));
@@ -1897,11 +1947,14 @@ fp_status fp_engine_reset(fp_engine *e)
     for (auto &s : e->stg)
         if (s.stream) HIP_TRY(hipStreamSynchronize(s.stream));
     if (e->side) HIP_TRY(hipStreamSynchronize(e->side));
+    if (e->copy) HIP_TRY(hipStreamSynchronize(e->copy));
     HIP_TRY(hipStreamSynchronize(e->stream));
     if (e->batched) {
-        for (auto &s : e->stg) {
+        for (int si = 0; si < e->nstage; si++) {
+            fp_engine::Stage &s = e->stg[si];
             collect_back_time(e, s);
             s.n = 0;
+            s.host_fed = false;
             s.ncig = s.nbases = s.nquals = 0;
             s.fill += 1;
             if (e->staged_ready) HIP_TRY(fpd_fill_u32(s.bs.baq_counts.as<uint32_t>(), 144, 0u, e->stream));
@@ -1928,6 +1981,7 @@ void fp_engine_destroy(fp_engine *e)
     for (auto &s : e->stg)
         if (s.stream) (void)hipStreamSynchronize(s.stream);
     if (e->side) (void)hipStreamSynchronize(e->side);
+    if (e->copy) (void)hipStreamSynchronize(e->copy);
     DevBuf *bufs[] = {&e->g_word_offset, &e->g_length, &e->g_words, &e->g_foff, &e->g_fstart, &e->g_fstop,
                       &e->g_mel, &e->counters, &e->scratch, &e->tk, &e->tv, &e->shist, &e->stot, &e->sctr};
     for (DevBuf *b : bufs) b->release();
@@ -1959,6 +2013,8 @@ void fp_engine_destroy(fp_engine *e)
     for (auto &b : e->in) b.release();
     for (auto &t : e->tab) for (auto &b : t) b.release();
     for (int i = 0; i < 16; i++) if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+    if (e->copied) (void)hipEventDestroy(e->copied);
+    if (e->copy) (void)hipStreamDestroy(e->copy);
     if (e->side) (void)hipStreamDestroy(e->side);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
@@ -2158,7 +2214,7 @@ fp_status fp_job_close(fp_job *j)
     hipStream_t ps;
     rc = post_stream(e, &ps);
     if (rc != FP_OK) return rc;
-    for (int s = 0; s < fp_engine::NSTAGE; s++)
+    for (int s = 0; s < e->nstage; s++)
         if (j->fill[s] && e->stg[s].cov_done) HIP_TRY(hipStreamWaitEvent(ps, e->stg[s].cov_done, 0));
     rc = postprocess_range(e, j->base, j->nrg, j->rows, ps);
     if (rc != FP_OK) return rc;

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define FP_ABI_VERSION 1
+#define FP_ABI_VERSION 2   /* bumped whenever fp_options or a view changes layout */
 
 typedef enum fp_status {
     FP_OK = 0,
@@ -100,6 +100,10 @@ typedef struct fp_options {
     uint32_t serve_hbm_budget_mib;   /* serving: HBM admission budget in MiB for the open jobs'
                                         resident tables and postprocess reserve; 0 = the
                                         read-group slots alone bound admission          */
+    uint32_t staging_buffers;        /* batched engine: 2 or 3 staging buffers (0 = 3).
+                                        A buffer refills once its previous batch has
+                                        run; with three, host batches copy in while the
+                                        other two buffers' kernels run               */
 } fp_options;
 
 /* how covariate observations reach the resident tables (identical results) */
