@@ -154,11 +154,11 @@ def roofline(stats):
     avg_s = stats.baq15_kernel_ms / 1e3 / launches
     flops = BAQ_FLOPS_PER_CELL * stats.baq15_cells / launches
     ach = flops / avg_s / 1e12
-    traffic = load_traffic().get("baq_band15")
+    traffic = load_traffic().get("baq15")
     out = {"bound": "valu_fp64", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
            "frac": ach / FP64_PEAK_TFLOPS, "traffic": traffic, "avg_launch_ms": avg_s * 1e3,
            "launches": launches, "cells_per_launch": stats.baq15_cells / launches,
-           "kernel": "fpd::k_baq_band<15,...> (1M-read launches)",
+           "kernel": f"the 15-cell BAQ class ({'fpd::k_baq_ckpt<15, %d, ...>' % stats.baq_checkpoint_rows if stats.baq_checkpoint_rows > 1 else 'fpd::k_baq_band<15, ...>'}), 1M-read launches",
            "note": "vector FP64 (the recurrence has no dense-contraction form, so no MFMA); achieved = 43 "
                    "flops x band cells per launch / average launch time"}
     if traffic:
@@ -477,7 +477,8 @@ def main():
                     help="staging buffers of the batched passes, 2 or 3 (0 = the engine's default, 3)")
     ap.add_argument("--wpe", type=int, default=0, help="BAQ waves per SIMD (0 = autotuned)")
     ap.add_argument("--ckpt", type=int, default=0,
-                    help="rows per BAQ forward checkpoint, 1 or 2 (0 = autotuned; profile-only pass: 1)")
+                    help="rows per BAQ forward checkpoint: 1 (every row spilled) or 3-5 (0 = autotuned; the "
+                         "profile-only pass uses 3, the autotuner's choice on config 1)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -524,7 +525,7 @@ def main():
         if args.aggregation == "sort":
             popts.update(aggregation=abi.FP_AGG_SORT, sort_digit_bits=8)
         pdt, pst = timed_pass(genome, dev, 0, torch, baq_waves_per_simd=args.wpe or 1,
-                              baq_checkpoint_rows=args.ckpt or 1, profile_stages=1, **popts)
+                              baq_checkpoint_rows=args.ckpt or 3, profile_stages=1, **popts)
         log(f"profile: {reads} reads in {pdt:.3f} s; prep {pst.prep_kernel_ms:.1f} ms, baq_hmm "
             f"{pst.baq_hmm_kernel_ms:.1f} ms ({pst.baq_hmm_launches} launches), 15-cell band "
             f"{pst.baq15_kernel_ms:.1f} ms ({pst.baq15_launches} launches), cov {pst.cov_kernel_ms:.1f} ms")
