@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -287,7 +288,18 @@ struct fp_engine {
         uint64_t fill = 1;              // which fill of this buffer is staging now (a job records the
                                         // fills that hold its reads)
         bool host_fed = false;          // this fill holds a batch copied from host memory
+        bool copies_pending = false;    // copy-stream work the buffer's stream has not waited for yet
         uint32_t g_reserve = 0;         // reserved waves the BAQ graph was captured with
+        // A flushed buffer's front stage (prep, classification) runs first; its small readback (band
+        // class counts, longest read, error words) lands in hc, and the rest of the batch is launched
+        // once the host sees it there: the host goes on staging the next batches meanwhile.
+        uint32_t *hc = nullptr;         // pinned: [0, BW_WORDS) the front stage's counts; [64] the pair
+                                        // overflow word of the buffer's previous batch
+        hipEvent_t f0 = nullptr, f1 = nullptr;   // around its front stage (timing)
+        hipEvent_t front_done = nullptr;         // after the front stage and its readback
+        bool pending = false;           // flushed; the stages after the front not launched yet
+        uint32_t pend_n = 0;            // reads of the pending batch
+        bool pend_host_fed = false;
     };
     // staging buffers, used in turn: one fills while the others run.  A buffer refills only once its
     // previous batch has run, so with two a host-fed buffer's copies wait for its own last batch;
@@ -297,6 +309,7 @@ struct fp_engine {
     int nstage = 3;
     bool batched = false;
     int cur = 0;
+    std::deque<int> pending;            // flushed buffers waiting for their readback, in flush order
     uint32_t cap_reads = 0, rows_cap = 0;
     uint64_t cap_cig = 0, cap_bases = 0, cap_quals = 0;
     DevBuf bcounters;                   // prep / BAQ counters accumulated over staged batches
@@ -838,6 +851,13 @@ fp_status batched_alloc(fp_engine *e)
         if (!s.cov_done) HIP_TRY(hipEventCreateWithFlags(&s.cov_done, hipEventDisableTiming));
         if (!s.t0) HIP_TRY(hipEventCreate(&s.t0));
         if (!s.t1) HIP_TRY(hipEventCreate(&s.t1));
+        if (!s.f0) HIP_TRY(hipEventCreate(&s.f0));
+        if (!s.f1) HIP_TRY(hipEventCreate(&s.f1));
+        if (!s.front_done) HIP_TRY(hipEventCreateWithFlags(&s.front_done, hipEventDisableTiming));
+        if (!s.hc) {
+            HIP_TRY(hipHostMalloc((void **)&s.hc, 80 * sizeof(uint32_t), hipHostMallocDefault));
+            memset(s.hc, 0, 80 * sizeof(uint32_t));
+        }
         // The first buffer runs on the engine stream and the others on streams of their own, which
         // also carry their appends (so a buffer refills only once its last batch is consumed).  A
         // process has four hardware queues; streams beyond them share one and run in turn.
@@ -920,16 +940,6 @@ void tune_record(fp_engine *e, int kind, int cand, double per_read)
     }
 }
 
-// empty a staging buffer without running its reads (its batch failed a check); the buffer refills
-static void stage_drop(fp_engine *e, fp_engine::Stage &s)
-{
-    e->stats.total_reads -= s.n;
-    s.n = 0;
-    s.host_fed = false;
-    s.ncig = s.nbases = s.nquals = 0;
-    s.fill += 1;
-}
-
 // finish a staging buffer's turn: the next buffer fills
 static void stage_done(fp_engine *e, fp_engine::Stage &s)
 {
@@ -947,33 +957,66 @@ static void stage_done(fp_engine *e, fp_engine::Stage &s)
 
 // Run the current staging buffer (if it holds reads) and switch to the next one.  Each buffer has
 // its own compute stream and per-batch state, so one buffer's covariate stage and prep run at the
-// same time as the other's BAQ.  The BAQ kernels of all buffers share the band
-// scratch and take turns (each waits for the previous buffer's event baq_done); so do the covariate
-// stages of the sort aggregation (its pair buffers are shared, event cov_done).  The direct tables
-// take every buffer's atomic adds in any order.
+// same time as another's BAQ.  The BAQ kernels of all buffers share the band scratch and take turns
+// (each waits for the previous buffer's event baq_done); so do the covariate stages of the sort
+// aggregation (its pair buffers are shared, event cov_done).  The direct tables take every buffer's
+// atomic adds in any order.
+//
+// The flush launches the front stage and its readback and returns; batched_back launches the rest
+// once the readback has landed (batched_poll from later calls, or batched_drain when a call must
+// wait).  Buffers go through batched_back in flush order, so the BAQ turns keep that order.
+fp_status batched_back(fp_engine *e);
+
+fp_status batched_poll(fp_engine *e)
+{
+    while (!e->pending.empty()) {
+        const hipError_t q = hipEventQuery(e->stg[e->pending.front()].front_done);
+        if (q == hipErrorNotReady) return FP_OK;
+        if (q != hipSuccess) { HIP_TRY(q); }
+        fp_status rc = batched_back(e);
+        if (rc != FP_OK) return rc;
+    }
+    return FP_OK;
+}
+
+// every flushed buffer's stages launched (waits for their readbacks); the first error is returned after
+// all of them have been handled
+fp_status batched_drain(fp_engine *e)
+{
+    fp_status first = FP_OK;
+    std::string msg;
+    while (!e->pending.empty()) {
+        fp_status rc = batched_back(e);
+        if (rc != FP_OK && first == FP_OK) { first = rc; msg = g_last_error; }
+    }
+    if (first != FP_OK) return fail(first, msg);
+    return FP_OK;
+}
+
 fp_status batched_flush(fp_engine *e)
 {
     fp_engine::Stage &s = e->stg[e->cur];
-    fp_engine::Stage &o = e->stg[(e->cur + e->nstage - 1) % e->nstage];
     if (s.n == 0) return FP_OK;
     fp_status rc = batched_alloc(e);
     if (rc != FP_OK) return rc;
     const bool prof = e->opts.profile_stages != 0;
     hipStream_t st = prof ? e->stream : s.stream;
     BatchState &S = s.bs;
-    collect_back_time(e, s);
     s.live_host[0] = s.n;
     s.live_host[1] = (uint32_t)s.nquals;
     s.live_host[2] = (uint32_t)(s.nquals >> 32);
     s.live_host[3] = 0;
+    if (s.copies_pending) {
+        // the copy stream is in order: its latest event covers every copy into this buffer
+        HIP_TRY(hipStreamWaitEvent(st, e->copied, 0));
+        s.copies_pending = false;
+    }
     HIP_TRY(hipMemcpyAsync(s.live.p, s.live_host, 16, hipMemcpyHostToDevice, st));
     // the pair-overflow word of this buffer's previous batch, read before the front stage zeroes it
-    uint32_t *prev = e->host_counts + 64;
-    HIP_TRY(hipMemcpyAsync(prev, S.baq_counts.as<uint32_t>() + fpd::BW_OVERFLOW, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(s.hc + 64, S.baq_counts.as<uint32_t>() + fpd::BW_OVERFLOW, 4, hipMemcpyDeviceToHost, st));
 
     const fpd::BatchDev b = stage_dev(e, s);
     Plan p{e->cap_reads, e->cap_quals, s.live.as<uint32_t>(), e->rows_cap};
-    p.reserve_waves = s.host_fed ? HOST_FED_RESERVE_WAVES : 0u;
     uint32_t *counters = e->bcounters.as<uint32_t>();
     if (s.g_epoch != e->epoch) {
         rc = capture(st, &s.gfront, [&] { return launch_front(e, S, b, p, counters, st); });
@@ -982,38 +1025,69 @@ fp_status batched_flush(fp_engine *e)
         if (s.gcov) { (void)hipGraphExecDestroy(s.gcov); s.gcov = nullptr; }
         s.g_epoch = e->epoch;
     }
-    HIP_TRY(hipEventRecord(e->bev[2], st));
+    HIP_TRY(hipEventRecord(s.f0, st));
     if (prof) {
         rc = launch_front(e, S, b, p, counters, st);
         if (rc != FP_OK) return rc;
     } else {
         HIP_TRY(hipGraphLaunch(s.gfront, st));
     }
-    HIP_TRY(hipEventRecord(e->bev[3], st));
+    HIP_TRY(hipEventRecord(s.f1, st));
     // the band classes, the batch's read counts, longest read and error words: one small readback
-    uint32_t *hc = e->host_counts;
-    HIP_TRY(hipMemcpyAsync(hc, S.baq_counts.p, 4 * fpd::BW_WORDS, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    e->stats.read_filter_ms += elapsed(e->bev[2], e->bev[3]);
-    e->stats.prep_kernel_ms += elapsed(e->bev[2], e->bev[3]);
+    HIP_TRY(hipMemcpyAsync(s.hc, S.baq_counts.p, 4 * fpd::BW_WORDS, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(s.front_done, st));
+    s.pending = true;
+    s.pend_n = s.n;
+    s.pend_host_fed = s.host_fed;
+    e->pending.push_back(e->cur);
+    stage_done(e, s);
+    // the per-kernel profile runs every stage eagerly, in turn
+    if (prof) return batched_drain(e);
+    return FP_OK;
+}
+
+// the stages after the front of the oldest flushed buffer: waits for its readback, checks it, then
+// launches the BAQ classes, cap/recode and the covariates
+fp_status batched_back(fp_engine *e)
+{
+    const int si = e->pending.front();
+    e->pending.pop_front();
+    fp_engine::Stage &s = e->stg[si];
+    fp_engine::Stage &o = e->stg[(si + e->nstage - 1) % e->nstage];
+    s.pending = false;
+    const bool prof = e->opts.profile_stages != 0;
+    hipStream_t st = prof ? e->stream : s.stream;
+    BatchState &S = s.bs;
+    HIP_TRY(hipEventSynchronize(s.front_done));
+    collect_back_time(e, s);
+    const uint32_t *hc = s.hc;
+    e->stats.read_filter_ms += elapsed(s.f0, s.f1);
+    e->stats.prep_kernel_ms += elapsed(s.f0, s.f1);
     e->stats.prep_launches += 1;
     // A failed check drops this buffer's reads before they touch the tables; the error is reported
     // once and the engine goes on with the next batch.
-    if (prev[0]) {
-        stage_drop(e, s);
+    auto drop = [&]() { e->stats.total_reads -= s.pend_n; };
+    if (hc[64]) {
+        drop();
         return fail(FP_ERR_INTERNAL, "sort aggregation: observation pairs overflowed their buffer");
     }
     if (hc[fpd::BW_BAD_RG]) {
-        stage_drop(e, s);
+        drop();
         return fail(FP_ERR_INVALID_ARG, "read group id beyond the header's read groups (staged batch dropped)");
     }
     if (hc[fpd::BW_LONGEST] > e->rows_cap) {
-        stage_drop(e, s);
+        drop();
         return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(e->rows_cap) +
                                             "; staged batch dropped)");
     }
     e->stats.filtered_reads += hc[fpd::BW_FILTERED];
     e->stats.baq_reads += hc[fpd::BW_BAQ];
+
+    const fpd::BatchDev b = stage_dev(e, s);
+    Plan p{e->cap_reads, e->cap_quals, s.live.as<uint32_t>(), e->rows_cap};
+    p.reserve_waves = s.pend_host_fed ? HOST_FED_RESERVE_WAVES : 0u;
+    uint32_t *counters = e->bcounters.as<uint32_t>();
+    fp_status rc = FP_OK;
 
     // autotuner: BAQ waves per SIMD first, then the radix digit width of the sort aggregation; each
     // candidate runs one staged batch eagerly, alone on the device, and is timed per read
@@ -1054,13 +1128,12 @@ fp_status batched_flush(fp_engine *e)
         e->stats.cov_launches += 1;
         e->stats.baq_ms += elapsed(e->ev[8], e->ev[2]);
         e->stats.covariates_ms += elapsed(e->ev[2], e->ev[3]);
-        stage_done(e, s);
         return FP_OK;
     }
     if (cand >= 0)   // the candidate runs alone
         for (auto &x : e->stg)
             if (&x != &s && x.stream) HIP_TRY(hipStreamSynchronize(x.stream));
-    // the band scratch is free once the other buffer's BAQ kernels are done
+    // the band scratch is free once the previous buffer's BAQ kernels are done
     HIP_TRY(hipStreamWaitEvent(st, o.baq_done, 0));
     HIP_TRY(hipEventRecord(s.t0, st));
     rc = launch_generic(e, S, b, p, hc[3], hc[4], counters, st);
@@ -1097,9 +1170,8 @@ fp_status batched_flush(fp_engine *e)
         HIP_TRY(hipEventSynchronize(s.t1));
         const double ms = elapsed(s.t0, s.t1);
         e->stats.baq_ms += ms;
-        tune_record(e, kind, cand, ms / std::max<uint32_t>(s.n, 1));
+        tune_record(e, kind, cand, ms / std::max<uint32_t>(s.pend_n, 1));
     }
-    stage_done(e, s);
     return FP_OK;
 }
 
@@ -1107,6 +1179,8 @@ fp_status batched_flush(fp_engine *e)
 fp_status batched_sync(fp_engine *e)
 {
     fp_status rc = batched_flush(e);
+    if (rc != FP_OK) return rc;
+    rc = batched_drain(e);
     if (rc != FP_OK) return rc;
     for (auto &s : e->stg)
         if (s.stream) HIP_TRY(hipStreamSynchronize(s.stream));
@@ -1190,6 +1264,13 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
         if (rc != FP_OK) return rc;
         s = &e->stg[e->cur];
     }
+    // a buffer refills once its previous batch's stages are all launched (the copies wait for them)
+    fp_status first_err = FP_OK;
+    std::string first_msg;
+    while (s->pending) {
+        fp_status rc = batched_back(e);
+        if (rc != FP_OK && first_err == FP_OK) { first_err = rc; first_msg = g_last_error; }
+    }
     fp_status rc = batched_alloc(e);
     if (rc != FP_OK) return rc;
     // the buffer's own stream: its appends follow its last batch's kernels
@@ -1232,7 +1313,7 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
             HIP_TRY(fpd_append(hsrc, dst, len, (uint32_t)s->ncig, (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off,
                                rg_limit, e->copy));
             HIP_TRY(hipEventRecord(e->copied, e->copy));
-            HIP_TRY(hipStreamWaitEvent(cs, e->copied, 0));
+            s->copies_pending = true;   // the buffer's stream waits for them once, at the flush
         } else {
             for (int i = 0; i < 18; i++)
                 if (len[i]) HIP_TRY(hipMemcpyAsync(dst[i], src[i], len[i], kind, e->copy));
@@ -1248,7 +1329,12 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
     s->nbases += v->num_read_bases;
     s->nquals += v->num_qualities;
     e->stats.total_reads += k;
+    // launch the stages of flushed buffers whose readback has landed; an earlier buffer's failed check
+    // is reported here (this call's batch is staged all the same)
+    rc = batched_poll(e);
+    if (first_err == FP_OK && rc != FP_OK) { first_err = rc; first_msg = g_last_error; }
     if (!on_device) HIP_TRY(hipEventSynchronize(e->copied));   // host arrays are borrowed for the call only
+    if (first_err != FP_OK) return fail(first_err, first_msg);
     return FP_OK;
 }
 
@@ -1943,7 +2029,9 @@ fp_status fp_engine_reset(fp_engine *e)
     fp_status rc = check_engine(e, true);
     if (rc != FP_OK) return rc;
     // Reads still staged are dropped, not run: a reset also recovers an engine whose staged batch
-    // failed a check.  Work in flight is waited for.
+    // failed a check.  Flushed buffers run (a failed check drops one, as always); work in flight is
+    // waited for.
+    if (e->batched) (void)batched_drain(e);
     for (auto &s : e->stg)
         if (s.stream) HIP_TRY(hipStreamSynchronize(s.stream));
     if (e->side) HIP_TRY(hipStreamSynchronize(e->side));
@@ -1996,10 +2084,11 @@ void fp_engine_destroy(fp_engine *e)
         for (auto &b : s.arr) b.release();
         s.live.release();
         if (s.live_host) (void)hipHostFree(s.live_host);
+        if (s.hc) (void)hipHostFree(s.hc);
         if (s.gfront) (void)hipGraphExecDestroy(s.gfront);
         if (s.gbaq) (void)hipGraphExecDestroy(s.gbaq);
         if (s.gcov) (void)hipGraphExecDestroy(s.gcov);
-        hipEvent_t evs[] = {s.baq_done, s.cov_done, s.t0, s.t1};
+        hipEvent_t evs[] = {s.baq_done, s.cov_done, s.t0, s.t1, s.f0, s.f1, s.front_done};
         for (hipEvent_t ev : evs) if (ev) (void)hipEventDestroy(ev);
         if (s.stream && s.stream != e->stream) (void)hipStreamDestroy(s.stream);
         s.bs.release();
@@ -2208,6 +2297,9 @@ fp_status fp_job_close(fp_job *j)
         rc = batched_flush(e);
         if (rc != FP_OK) return rc;
     }
+    // every flushed buffer's stages launched, so the events below cover the job's reads
+    rc = batched_drain(e);
+    if (rc != FP_OK) return rc;
     // The job's postprocess waits for the covariate stage of the buffers that held its reads (each
     // buffer's stream is in order, so its latest event covers every earlier fill), not for the whole
     // engine: other jobs' batches keep running meanwhile.

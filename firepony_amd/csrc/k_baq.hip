@@ -921,18 +921,22 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
 // ---------------------------------------------------------------------------
 // Checkpointed band kernel (inputs staged in LDS, one wave per SIMD).
 //
-// The forward pass keeps every row's scale factor but the full state (M, I, D) of only every K-th
-// row, from row 1.  The backward pass then walks K-row segments from the top: it reloads a
-// segment's checkpoint, recomputes the segment's other forward rows into registers with the
-// forward pass's own row code (so they are the same bits), and consumes them in reverse for the
-// MAP steps.  Forward-row traffic falls from 32 B per cell (M and I of every row written and read
-// back) to 48/K B; the recompute repeats the forward arithmetic once.
+// The forward pass keeps every row's scale factor but the state of only every K-th row, from row 1.
+// The backward pass then walks K-row segments from the top: it reloads a segment's checkpoint,
+// recomputes the segment's other forward rows into registers with the forward pass's own row code
+// (so they are the same bits), and consumes them in reverse for the MAP steps.
+//
+// A checkpoint row i >= 2 is kept as its unscaled M and I (16 B per cell): the row's D follows from
+// its own M along the band (D[k] = m2 M[k-1] + m8 D[k-1], the recurrence of the forward pass), and
+// scaling M, I and D by the stored 1 / s[i] gives the forward pass's scaled row bit for bit.  Row 1
+// (scaled by division, D = 0) is kept scaled.  Forward-row traffic falls from 32 B per cell (M and I
+// of every row written and read back) to 32/K B; the recompute repeats the forward arithmetic once.
 //
 // Registers: the K recomputed rows as (M, I) pairs, the forward D of the recompute and the
 // backward row's M and I; the forward pass's own state dies before them.  The next segment's
-// checkpoint is loaded while this segment's backward rows run: its D into the recompute's D once
-// the recompute is done, its (M, I) into the top row's slot once that row's MAP step is done.  The
-// segment loop runs on a wave-uniform counter, so all lanes load the same checkpoint row at once.
+// checkpoint (M, I) and its scale factor are loaded while this segment's backward rows run, into the
+// top row's slot once that row's MAP step is done.  The segment loop runs on a wave-uniform counter,
+// so all lanes load the same checkpoint row at once.
 template <int W, int K, bool TAGL>
 __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
 {
@@ -951,11 +955,9 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
     const uint32_t nwaves = gridDim.x * (blockDim.x >> 6);
     const uint32_t total = *a.count;
     double *slot = a.scratch + (uint64_t)gwave * a.slot_doubles;
-    // checkpoints of rows 1, 1 + K, 1 + 2K, ...: (M, I) [j][r][lane], then D [j][r][lane]; they fit in
-    // the forward-row area of the every-row kernel, and the scale factors and cap codes keep its offsets
-    const uint32_t nck = (a.rows_cap + K - 1) / K;
+    // checkpoints of rows 1, 1 + K, 1 + 2K, ...: (M, I) [j][r][lane]; they fit in the forward-row area
+    // of the every-row kernel, and the scale factors and cap codes keep its offsets
     double2 *Cmi = (double2 *)slot;
-    double *Cd = slot + (uint64_t)nck * W * 2 * 64;
     double *Sst = slot + (uint64_t)(a.rows_cap + 1) * W * 2 * 64;         // [row][lane]
     uint32_t *Ost = (uint32_t *)(Sst + (uint64_t)(a.rows_cap + 2) * 64);  // [row][lane] cap code, then tag
     const BatchDev &b = a.b;
@@ -1154,11 +1156,28 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
 #pragma unroll
             for (int r = 0; r < W; r++) { O[r] = make_double2(O[r].x * inv, O[r].y * inv); D[r] = D[r] * inv; }
         };
-        auto store_ckpt = [&](int j, const double2 (&O)[W], const double (&D)[W]) {
+        auto store_ckpt = [&](int j, const double2 (&O)[W]) {
             double2 *cmi = Cmi + (uint64_t)j * W * 64 + lane;
-            double *cd = Cd + (uint64_t)j * W * 64 + lane;
 #pragma unroll
-            for (int r = 0; r < W; r++) { cmi[r * 64] = O[r]; cd[r * 64] = D[r]; }
+            for (int r = 0; r < W; r++) cmi[r * 64] = O[r];
+        };
+        // a checkpoint row i >= 2 from its unscaled (M, I) in O: its D along the band as the forward
+        // row code computes it (cells outside the band hold M = 0 and get D = 0), then the row scaled
+        // by inv = 1 / s[i]
+        auto restore_ckpt = [&](int i, double2 (&O)[W], double (&D)[W], double inv) {
+            const int klo = 1 > i - bw ? 1 : i - bw;
+            const int khi = refLen < i + bw ? refLen : i + bw;
+            double cM = 0.0, cD = 0.0;
+#pragma unroll
+            for (int r = 0; r < W; r++) {
+                const int k = i - bw + r;
+                const bool in = k >= klo && k <= khi;
+                const double dv = in ? m2 * cM + m8 * cD : 0.0;
+                D[r] = dv;
+                cM = O[r].x; cD = dv;
+            }
+#pragma unroll
+            for (int r = 0; r < W; r++) { O[r] = make_double2(O[r].x * inv, O[r].y * inv); D[r] = D[r] * inv; }
         };
 
         // ---- forward pass.  The scale factor area keeps 1 / s[i], the factor the backward rows
@@ -1195,7 +1214,7 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
                 for (int r = 0; r < W; r++) {
                     MI[r] = make_double2(MI[r].x / sum, MI[r].y / sum); D[r] = D[r] / sum;
                 }
-                store_ckpt(0, MI, D);
+                store_ckpt(0, MI);
             }
             // rows 2..qLen: the next row's LDS inputs read a row ahead
             const int qlast = qLen - 1;
@@ -1217,8 +1236,8 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
                 const double inv = 1. / sum;
                 Sst[(uint64_t)i * 64 + lane] = inv;
                 sq = sum;
+                if ((i - 1) % K == 0) store_ckpt((i - 1) / K, MI);   // unscaled
                 fwd_scale(MI, D, inv);
-                if ((i - 1) % K == 0) store_ckpt((i - 1) / K, MI, D);
             }
             // s[qLen+1] (baq.cu:514-525)
             const int klo = 1 > qLen - bw ? 1 : qLen - bw;
@@ -1278,12 +1297,13 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
         }
         // the first segment's checkpoint (a lane whose window ends below the wave's top segment loads
         // its own last one, which the walk reaches when its rows begin)
+        double ck_inv;                           // 1 / s of the checkpoint row in the top slot
         {
             const int j0 = (nseg_w - 1) < last_seg ? (nseg_w - 1) : last_seg;
             const double2 *cmi = Cmi + (uint64_t)j0 * W * 64 + lane;
-            const double *cd = Cd + (uint64_t)j0 * W * 64 + lane;
 #pragma unroll
-            for (int r = 0; r < W; r++) { F[K - 1][r] = cmi[r * 64]; Dr[r] = cd[r * 64]; }
+            for (int r = 0; r < W; r++) F[K - 1][r] = cmi[r * 64];
+            ck_inv = Sst[(uint64_t)(1 + j0 * K) * 64 + lane];
         }
         for (int j = nseg_w - 1; j >= 0; j--) {
             const int s0 = 1 + j * K;
@@ -1298,9 +1318,16 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
                 rinv[t] = Sst[(uint64_t)i * 64 + lane];
                 rcode[t] = Ost[(uint64_t)i * 64 + lane];
             }
-            // the checkpoint row, loaded into the top slot during the previous segment
+            // the checkpoint row, loaded into the top slot during the previous segment: row 1 as it is
+            // (D = 0), a later row restored from its unscaled (M, I)
 #pragma unroll
             for (int r = 0; r < W; r++) F[0][r] = F[K - 1][r];
+            if (j > 0) {
+                restore_ckpt(s0, F[0], Dr, ck_inv);
+            } else {
+#pragma unroll
+                for (int r = 0; r < W; r++) Dr[r] = 0.0;
+            }
             uint64_t fwin[NWIN];
             win_fill(fwin, s0 - bw);             // forward row s0 + 1: cell r at window index s0 - bw + r
 #pragma unroll
@@ -1313,13 +1340,7 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
                     fwd_scale(F[t], Dr, 1. / sum);
                 }
             }
-            // the forward D is dead: the next checkpoint's D comes in now
             const int jn = j > 0 ? j - 1 : 0;
-            {
-                const double *cd = Cd + (uint64_t)jn * W * 64 + lane;
-#pragma unroll
-                for (int r = 0; r < W; r++) Dr[r] = cd[r * 64];
-            }
 #pragma unroll
             for (int t = K - 1; t >= 0; t--) {
                 const int i = s0 + t;
@@ -1345,6 +1366,7 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
                     const double2 f = F[t][r];
                     double z = f.x * bMv[r];
                     sum += z;
+                    // the first strictly larger value wins (values are finite and >= 0, so fmax is z > mx ? z : mx)
                     if (z > mx) { mx = z; stt = (k - 1) << 2 | 0; }
                     z = f.y * bIv[r];
                     sum += z;
@@ -1360,6 +1382,7 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
                     const double2 *cmi = Cmi + (uint64_t)jn * W * 64 + lane;
 #pragma unroll
                     for (int r = 0; r < W; r++) F[K - 1][r] = cmi[r * 64];
+                    ck_inv = Sst[(uint64_t)(1 + jn * K) * 64 + lane];
                 }
             }
             if (s0 + K - 1 > qLen) {
@@ -1367,6 +1390,7 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
                 const double2 *cmi = Cmi + (uint64_t)jn * W * 64 + lane;
 #pragma unroll
                 for (int r = 0; r < W; r++) F[K - 1][r] = cmi[r * 64];
+                ck_inv = Sst[(uint64_t)(1 + jn * K) * 64 + lane];
             }
         }
         // the read's tags, row by row in lockstep across the wave
