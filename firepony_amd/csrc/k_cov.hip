@@ -164,6 +164,7 @@ struct LdsAcc {
     uint32_t *cnt;
     uint32_t *mcnt;
     uint32_t mcodes;
+    uint32_t dump, mdump;   // first of 64 per-lane count words (and packed-count words) never flushed
 };
 
 __device__ inline bool lds_cycle_m(const LdsAcc &L, uint32_t code, uint32_t q)
@@ -267,6 +268,67 @@ __device__ inline void emit_base(const CovArgs &a, const LdsAcc &L, uint32_t rg,
     }
     if (xid != 2047u) {
         lds_add(L, E_CID + (xid >> 4), f1);
+        lds_add_sums(L, E_CID + 64 + (xid >> 4), f2);
+    }
+}
+
+// The same keys and updates as emit_base for the wave path, with the five count updates of a base
+// issued unconditionally: a base whose key has no LDS entry (no key at all, or one kept in the
+// resident tables) counts into its lane's dump word, which the flush never reads.  The sums (nonzero
+// errors only) and the resident-table fallbacks (qualities >= 64, cycle codes beyond the LDS entries)
+// run under one wave-uniform test; most bases have neither.
+template <class BASE>
+__device__ inline void emit_base_wave(const CovArgs &a, const LdsAcc &L, uint32_t rg, uint32_t ncodes,
+                                      uint16_t flags, bool neg, BASE base, uint16_t rwx, uint16_t rwy,
+                                      uint16_t hx, uint16_t hy, uint16_t bp, uint32_t q, const Limbs &f0,
+                                      const Limbs &f1, const Limbs &f2)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t cm, cid, xm, xid;
+    cycle_keys(flags, rwx, rwy, bp, cm, cid);
+    context_keys(base, hx, hy, neg, bp, xm, xid);
+    const bool qok = q < 64;                            // quality and context M entries in LDS
+    const bool mok = qok && cm < L.mcodes;              // cycle M count in LDS
+    const bool yid = cid != 1023u, yok = yid && cid < ncodes;
+    const bool xmk = xm != 2047u, xik = xid != 2047u;
+    const uint32_t dump = L.dump + lane;
+    const uint32_t mi = q * L.mcodes + cm;
+    atomicAdd(&L.cnt[qok ? q : dump], 1u);
+    atomicAdd(&L.mcnt[mok ? (mi >> 1) : L.mdump + lane], mok ? 1u << (16 * (mi & 1)) : 0u);
+    atomicAdd(&L.cnt[yok ? E_YID + cid : dump], 1u);
+    atomicAdd(&L.cnt[xmk && qok ? E_CM + q * 16 + (xm >> 4) : dump], 1u);
+    atomicAdd(&L.cnt[xik ? E_CID + (xid >> 4) : dump], 1u);
+    const bool sums = nonzero(f0) || nonzero(f1) || nonzero(f2);
+    const bool rare = !qok || !mok || (yid && !yok);
+    if (!wave_any(sums || rare)) return;
+    // RecalTable1: EventTracker(2) | QualityScore(8) | ReadGroup(8)
+    if (qok) lds_add_sums(L, q, f0);
+    else global_add(a.t, 0, 0u | (q << 2) | (rg << 10), 1, f0);
+    // cycle: EventTracker(2) | Cycle(10) | QualityScore(8) | ReadGroup(8)
+    const uint32_t key = 0u | (cm << 2) | (q << 12) | (rg << 20);
+    if (mok) {
+        if (nonzero(f0)) {
+            atomicAdd((unsigned long long *)&a.t.part[1][0][key], (unsigned long long)f0.p0);
+            atomicAdd((unsigned long long *)&a.t.part[1][1][key], (unsigned long long)f0.p1);
+            atomicAdd((unsigned long long *)&a.t.part[1][2][key], (unsigned long long)f0.p2);
+        }
+    } else {
+        global_add(a.t, 1, key, 1, f0);
+    }
+    if (yok) {
+        lds_add_sums(L, E_YID + cid, f1);
+        lds_add_sums(L, E_YID + ncodes + cid, f2);
+    } else if (yid) {
+        global_add(a.t, 1, 1u | (cid << 2) | (45u << 12) | (rg << 20), 1, f1);
+        global_add(a.t, 1, 2u | (cid << 2) | (45u << 12) | (rg << 20), 1, f2);
+    }
+    // context: EventTracker(2) | Context(11) | QualityScore(8) | ReadGroup(8)
+    if (xmk) {
+        if (qok) lds_add_sums(L, E_CM + q * 16 + (xm >> 4), f0);
+        else global_add(a.t, 2, 0u | (xm << 2) | (q << 13) | (rg << 21), 1, f0);
+    }
+    if (xik) {
+        lds_add_sums(L, E_CID + (xid >> 4), f1);
         lds_add_sums(L, E_CID + 64 + (xid >> 4), f2);
     }
 }
@@ -683,7 +745,7 @@ __device__ __forceinline__ void cov_read_wave(const CovArgs &a, const LdsAcc &L,
                 em = true;
                 i0 += f1.p0; i1 += f1.p1; i2 += f1.p2;
                 d0 += f2.p0; d1 += f2.p1; d2 += f2.p2;
-                emit_base(a, L, rg, ncodes, flags, neg, base, rwx, rwy, hx, hy, bp, qv[c], f0, f1, f2, false);
+                emit_base_wave(a, L, rg, ncodes, flags, neg, base, rwx, rwy, hx, hy, bp, qv[c], f0, f1, f2);
             }
         }
         nid += (uint32_t)__popcll(__ballot(em));
@@ -717,11 +779,13 @@ __global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
     L.p0 = lds_raw;
     L.p1 = lds_raw + E;
     L.p2 = lds_raw + 2 * E;
-    L.cnt = (uint32_t *)(lds_raw + 3 * E);
-    L.mcnt = L.cnt + E;
+    L.cnt = (uint32_t *)(lds_raw + 3 * E);          // E entries, then 64 dump words
+    L.mcnt = L.cnt + E + 64;                        // MW words, then 64 dump words
     L.mcodes = a.mcodes;
+    L.dump = E;
+    L.mdump = MW;
     uint8_t *stage = PAIRS ? (uint8_t *)lds_raw + (threadIdx.x >> 6) * 256
-                           : (uint8_t *)(L.mcnt + MW) + (threadIdx.x >> 6) * 256;
+                           : (uint8_t *)(L.mcnt + MW + 64) + (threadIdx.x >> 6) * 256;
     const PairRegion pr{pr_used, pr_base};
 
     // which read group / tile this block covers
@@ -963,8 +1027,8 @@ static void cov_layout(uint32_t max_read_len, uint32_t *ncodes, uint32_t *mcodes
     const int limit = g_cov_lds_limit > 0 ? g_cov_lds_limit : 64 * 1024;
     const uint32_t budget = (uint32_t)limit - fpd::STAGE_BYTES - 256;   // 256: the kernel's static LDS
     uint32_t nc = 2 * max_read_len + 2, mc = 2 * max_read_len + 2;
-    auto need = [&](uint32_t n, uint32_t m) {
-        return (fpd::E_YID + 2 * n) * fpd::LDS_ENTRY_BYTES + 4 * ((m * 64 + 1) / 2);
+    auto need = [&](uint32_t n, uint32_t m) {   // entries, packed counts, 2 x 64 dump words
+        return (fpd::E_YID + 2 * n) * fpd::LDS_ENTRY_BYTES + 4 * ((m * 64 + 1) / 2) + 512;
     };
     while (need(nc, mc) > budget && mc > 0) mc = mc > 16 ? mc - 16 : 0;
     while (need(nc, mc) > budget && nc > 0) nc = nc > 16 ? nc - 16 : 0;
