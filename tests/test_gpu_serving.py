@@ -131,3 +131,36 @@ def test_admission_by_slot_budget():
     b.release()
     b1.release()
     e.close()
+
+
+def test_admission_by_hbm_budget(tmp_path):
+    """Jobs are admitted by their HBM footprint (resident tables of their read groups plus a
+    postprocess reserve, about 104 MiB per read group) against serve_hbm_budget_mib; a job admitted
+    after another's release runs to the oracle's report."""
+    say("serving: admission by the HBM byte budget")
+    g = genome(1)
+    e = Engine(g, engine_kind=abi.FP_ENGINE_BATCHED, max_batch_reads=4096, max_read_length=150,
+               serve_rg_slots=8, serve_hbm_budget_mib=250, baq_waves_per_simd=1)
+    a = e.open_job(["a0"])
+    b = e.open_job(["b0"])
+    assert a is not None and b is not None             # 2 x 104 MiB of 250
+    assert e.open_job(["c0"]) is None                   # a third would need 312 MiB: refused now
+    with pytest.raises(FireponyError):
+        e.open_job(["x0", "x1", "x2"])                  # larger than the whole budget: an error
+    a.release()
+    c = e.open_job(["c0"])                              # room again after the release
+    assert c is not None
+    spec = job_spec(611, 1, 3000)
+    for bt in batches(g, spec):
+        c.submit(bt)
+    c.close()
+    names = (C.c_char_p * 1)(b"c0")   # the job's read-group name, for the oracle's report
+    g.header.read_group_names = names
+    o2 = Oracle(g)
+    for bt in batches(g, spec):
+        o2.process(bt)
+    o2.postprocess()
+    assert c.report(str(tmp_path / "c.txt")) == o2.report(str(tmp_path / "o.txt")), "budget-admitted job vs oracle"
+    c.release()
+    b.release()
+    e.close()
