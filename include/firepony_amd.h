@@ -275,7 +275,12 @@ fp_status fp_engine_setup(fp_engine *e, const fp_header_view *header,
                           const fp_reference_view *reference,
                           const fp_variants_view *variants);
 /* worker loop body (device/pipeline_interface.cu:195-224): H2D copy of a host
- * batch followed by firepony_process_batch (device/pipeline.cu:56-148). */
+ * batch followed by firepony_process_batch (device/pipeline.cu:56-148).  The
+ * host arrays are borrowed for the call: it returns once the batch is in device
+ * memory.  On a batched engine the batch's checks (read length, read group) run
+ * with its staging buffer; a failing buffer's reads are dropped and the error is
+ * returned by the call that sees it (this or a later process, synchronize,
+ * postprocess or job close), which still stages its own batch. */
 fp_status fp_engine_process_batch(fp_engine *e, const fp_batch_view *host_batch);
 /* firepony_process_batch on a batch already resident in device memory (all
  * view pointers are device pointers on this engine's device).  Stream-ordered:
