@@ -967,6 +967,18 @@ static void stage_done(fp_engine *e, fp_engine::Stage &s)
 // wait).  Buffers go through batched_back in flush order, so the BAQ turns keep that order.
 fp_status batched_back(fp_engine *e);
 
+// true when every flushed buffer's stages have been launched and have completed
+static bool batched_idle(fp_engine *e)
+{
+    if (!e->pending.empty()) return false;
+    for (int si = 0; si < e->nstage; si++) {
+        const fp_engine::Stage &s = e->stg[si];
+        if (s.cov_done && hipEventQuery(s.cov_done) == hipErrorNotReady) return false;
+    }
+    (void)hipGetLastError();
+    return true;
+}
+
 fp_status batched_poll(fp_engine *e)
 {
     while (!e->pending.empty()) {
@@ -1213,6 +1225,38 @@ fp_status batched_sync(fp_engine *e)
 
 fp_status plain_host_batch(fp_engine *e, const fp_batch_view *hb);
 
+// Host-side checks of a host batch: every read inside the batch arrays (and starting on a word of
+// bases), no read longer than max_len, no read group at or beyond rg_limit (0xffffffff: read groups
+// unchecked; the prep kernel filters reads without one, as the reference does).  One pass ORs the
+// conditions over all reads (the loop vectorises); the failing read is searched for only when one
+// fails, with the same messages in the same order as a per-read walk.
+static fp_status check_host_reads(const fp_batch_view *b, uint32_t max_len, uint32_t rg_limit)
+{
+    const uint32_t n = b->num_reads;
+    const uint32_t rgl = rg_limit == 0xffffffffu ? 0xffffffffu : rg_limit;
+    const uint32_t rg_on = rg_limit != 0xffffffffu;
+    uint32_t bad = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint64_t ce = (uint64_t)b->cigar_start[i] + b->cigar_len[i];
+        const uint64_t re = (uint64_t)b->read_start[i] + ((b->read_len[i] + 7) & ~7u);
+        const uint64_t qe = (uint64_t)b->qual_start[i] + b->qual_len[i];
+        bad |= (uint32_t)(ce > b->num_cigar_ops) | (uint32_t)(re > b->num_read_bases) |
+               (uint32_t)(qe > b->num_qualities) | (b->read_start[i] & 7) | (uint32_t)(b->read_len[i] > max_len) |
+               (rg_on & (uint32_t)(b->read_group[i] >= rgl));
+    }
+    if (!bad) return FP_OK;
+    for (uint32_t i = 0; i < n; i++) {
+        if ((uint64_t)b->cigar_start[i] + b->cigar_len[i] > b->num_cigar_ops ||
+            (uint64_t)b->read_start[i] + ((b->read_len[i] + 7) & ~7u) > b->num_read_bases ||
+            (uint64_t)b->qual_start[i] + b->qual_len[i] > b->num_qualities || (b->read_start[i] & 7))
+            return fail(FP_ERR_INVALID_ARG, "read " + std::to_string(i) + " lies outside the batch arrays");
+        if (rg_on && b->read_group[i] >= rgl) return fail(FP_ERR_INVALID_ARG, "read group id beyond the job's read groups");
+        if (b->read_len[i] > max_len)
+            return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(max_len) + ")");
+    }
+    return FP_OK;
+}
+
 // Device-side addresses of a host batch's arrays when every one of them is pinned host memory (the
 // device then reads them over the host link itself, in one launch); false when any array is
 // pageable, which only the runtime's copies can move.
@@ -1236,8 +1280,10 @@ static bool pinned_sources(const void *const *src, const uint64_t *len, const vo
 
 // append one batch (host or device arrays) to the current staging buffer; a serving job's reads move
 // to its read-group slots [rg_off, rg_off + rg_limit)
+// A host batch's checks (check_host_reads: read bounds, read length, read groups below
+// host_check_rg) run while its copy is in flight; a failing batch is not staged.
 fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, uint32_t rg_off = 0,
-                         uint32_t rg_limit = 0xffffffffu)
+                         uint32_t rg_limit = 0xffffffffu, uint32_t host_check_rg = 0xffffffffu)
 {
     const uint32_t k = v->num_reads;
     if (k == 0) { e->stats.num_batches += 1; return FP_OK; }
@@ -1251,6 +1297,10 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
         if (rc != FP_OK) return rc;
         e->epoch++;
         const fp_stats before = e->stats;
+        if (!on_device) {
+            rc = check_host_reads(v, e->rows_cap, host_check_rg);
+            if (rc != FP_OK) return rc;
+        }
         rc = on_device ? run_batch(e, v, 0) : plain_host_batch(e, v);
         e->eager_cells += e->stats.baq_cells - before.baq_cells;
         e->eager_cells15 += e->stats.baq15_cells - before.baq15_cells;
@@ -1314,7 +1364,17 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
                                rg_limit, e->copy));
             HIP_TRY(hipEventRecord(e->copied, e->copy));
             s->copies_pending = true;   // the buffer's stream waits for them once, at the flush
+            // the batch's checks while the copy runs: a failing batch is not staged (its bytes sit past
+            // the buffer's reads and are overwritten by the next batch)
+            rc = check_host_reads(v, e->rows_cap, host_check_rg);
+            if (rc != FP_OK) {
+                const std::string msg = g_last_error;
+                HIP_TRY(hipEventSynchronize(e->copied));   // the host arrays are still being read
+                return fail(rc, msg);
+            }
         } else {
+            rc = check_host_reads(v, e->rows_cap, host_check_rg);
+            if (rc != FP_OK) return rc;
             for (int i = 0; i < 18; i++)
                 if (len[i]) HIP_TRY(hipMemcpyAsync(dst[i], src[i], len[i], kind, e->copy));
             HIP_TRY(hipEventRecord(e->copied, e->copy));
@@ -1333,6 +1393,13 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
     // is reported here (this call's batch is staged all the same)
     rc = batched_poll(e);
     if (first_err == FP_OK && rc != FP_OK) { first_err = rc; first_msg = g_last_error; }
+    // With nothing of the engine left to run on the device (the start of a stream, or a stream that
+    // arrives slower than it runs), a buffer holding an eighth of its capacity runs at once rather
+    // than when it is full: the device starts early, and full buffers follow while it works.
+    if (first_err == FP_OK && !e->opts.profile_stages && s->n >= e->cap_reads / 8 && batched_idle(e)) {
+        rc = batched_flush(e);
+        if (rc != FP_OK) { first_err = rc; first_msg = g_last_error; }
+    }
     if (!on_device) HIP_TRY(hipEventSynchronize(e->copied));   // host arrays are borrowed for the call only
     if (first_err != FP_OK) return fail(first_err, first_msg);
     return FP_OK;
@@ -1746,16 +1813,12 @@ fp_status fp_engine_process_batch(fp_engine *e, const fp_batch_view *hb)
     rc = validate_view(hb);
     if (rc != FP_OK) return rc;
     const uint32_t n = hb->num_reads;
-    // bounds of the host arrays (host-side validation: reads must lie inside the arrays)
-    for (uint32_t i = 0; i < n; i++) {
-        if ((uint64_t)hb->cigar_start[i] + hb->cigar_len[i] > hb->num_cigar_ops ||
-            (uint64_t)hb->read_start[i] + ((hb->read_len[i] + 7) & ~7u) > hb->num_read_bases ||
-            (uint64_t)hb->qual_start[i] + hb->qual_len[i] > hb->num_qualities || (hb->read_start[i] & 7))
-            return fail(FP_ERR_INVALID_ARG, "read " + std::to_string(i) + " lies outside the batch arrays");
-        if (e->batched && hb->read_len[i] > e->rows_cap)
-            return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(e->rows_cap) + ")");
-    }
-    if (e->batched) return batched_append(e, hb, false);
+    // bounds of the host arrays (reads must lie inside the arrays) and, batched, the read lengths: on
+    // the batched engine they run while the batch's copy is in flight.  Read groups are checked on the
+    // device (a bad one drops its staging buffer), as before.
+    if (e->batched) return batched_append(e, hb, false, 0, 0xffffffffu, 0xffffffffu);
+    rc = check_host_reads(hb, 0xffffffffu, 0xffffffffu);
+    if (rc != FP_OK) return rc;
     if (n == 0) { e->stats.num_batches++; return FP_OK; }
     return plain_host_batch(e, hb);
 }
@@ -2248,16 +2311,7 @@ static fp_status job_submit(fp_job *j, const fp_batch_view *b, bool on_device)
     // a job's batch is checked before it joins a staging buffer shared with other jobs, so a bad batch
     // fails its own submit and never another job's
     if (!on_device) {
-        for (uint32_t i = 0; i < b->num_reads; i++) {
-            if ((uint64_t)b->cigar_start[i] + b->cigar_len[i] > b->num_cigar_ops ||
-                (uint64_t)b->read_start[i] + ((b->read_len[i] + 7) & ~7u) > b->num_read_bases ||
-                (uint64_t)b->qual_start[i] + b->qual_len[i] > b->num_qualities || (b->read_start[i] & 7))
-                return fail(FP_ERR_INVALID_ARG, "read " + std::to_string(i) + " lies outside the batch arrays");
-            if (b->read_group[i] >= j->nrg)
-                return fail(FP_ERR_INVALID_ARG, "read group id beyond the job's read groups");
-            if (b->read_len[i] > e->rows_cap)
-                return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(e->rows_cap) + ")");
-        }
+        // host batch: checked by batched_append while its copy is in flight (read group included)
     } else if (b->num_reads) {
         hipStream_t ps;
         rc = post_stream(e, &ps);
@@ -2273,7 +2327,7 @@ static fp_status job_submit(fp_job *j, const fp_batch_view *b, bool on_device)
         if (h[0] > e->rows_cap)
             return fail(FP_ERR_INVALID_ARG, "a read is longer than max_read_length (" + std::to_string(e->rows_cap) + ")");
     }
-    rc = batched_append(e, b, on_device, j->base, j->nrg);
+    rc = batched_append(e, b, on_device, j->base, j->nrg, j->nrg);
     if (rc != FP_OK) return rc;
     j->fill[e->cur] = e->stg[e->cur].fill;
     j->reads += b->num_reads;

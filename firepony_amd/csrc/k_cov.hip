@@ -605,7 +605,6 @@ __device__ __forceinline__ void cov_read_wave(const CovArgs &a, const LdsAcc &L,
     const uint32_t r = m.r;
     const uint32_t rw = m.rw;
     const uint16_t rwx = lo16(rw), rwy = hi16(rw);
-    const uint32_t rs = m.rs;
     const uint32_t qs = m.qs;
     const uint32_t L_read = m.len;
     const int len = (int)rwy - (int)rwx + 1;

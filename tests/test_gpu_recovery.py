@@ -66,9 +66,10 @@ def test_batched_drops_the_failed_buffer_and_goes_on(kind):
                       0, 500)
     dgood = [DeviceCopy(b) for b in good]
     dbad = DeviceCopy(bad)
-    e = Engine(g, engine_kind=abi.FP_ENGINE_BATCHED, max_batch_reads=4096, max_read_length=120,
+    # staging buffers of 16,384 reads: an idle engine runs a buffer once it holds 2,048 reads, so
+    # good 0, good 1 and the bad batch share one buffer, dropped as a whole when it runs
+    e = Engine(g, engine_kind=abi.FP_ENGINE_BATCHED, max_batch_reads=16384, max_read_length=120,
                baq_waves_per_simd=1)
-    # staging buffer: good 0, good 1, the bad batch -> dropped when it runs
     e.process_device(dgood[0].view)
     e.process_device(dgood[1].view)
     e.process_device(dbad.view)
