@@ -138,6 +138,28 @@ __device__ inline void context_keys(BASE base, uint16_t hx, uint16_t hy, bool ne
     if (n >= 3) kid = ((ctx >> ((n - 3) * 2)) << 4) | 3;
 }
 
+// context_keys without the loop: the (up to) three bases are read at once, positions past the
+// high-quality window clamped to the base itself and masked out; the same keys
+template <class BASE>
+__device__ inline void context_keys3(BASE base, uint16_t hx, uint16_t hy, bool neg, uint16_t bp, uint32_t &km,
+                                     uint32_t &kid)
+{
+    const int avail = neg ? (int)hy - (int)bp + 1 : (int)bp - (int)hx + 1;
+    const int dir = neg ? 1 : -1;
+    const int p1 = avail >= 2 ? (int)bp + dir : (int)bp, p2 = avail >= 3 ? (int)bp + 2 * dir : (int)bp;
+    const uint32_t b0 = base((int)bp), b1 = base(p1), b2 = base(p2);
+    auto acgt = [](uint32_t b) { return b == B_A || b == B_C || b == B_G || b == B_T; };
+    auto code = [neg](uint32_t b) {   // A C G T -> 0 1 2 3, complemented on the reverse strand
+        const uint32_t c = 31u - (uint32_t)__builtin_clz(b | 1u);
+        return neg ? 3u - c : c;
+    };
+    const bool v2 = acgt(b0) && avail >= 2 && acgt(b1);
+    const bool v3 = v2 && avail >= 3 && acgt(b2);
+    const uint32_t c0 = code(b0), c1 = code(b1), c2 = code(b2);
+    km = v2 ? ((((c0 << 2) | c1) << 4) | 2u) : 2047u;
+    kid = v3 ? ((((c0 << 4) | (c1 << 2) | c2) << 4) | 3u) : 2047u;
+}
+
 // an exact per-base value as limbs
 struct Limbs {
     uint64_t p0, p1, p2;
@@ -286,7 +308,7 @@ __device__ inline void emit_base_wave(const CovArgs &a, const LdsAcc &L, uint32_
     const uint32_t lane = threadIdx.x & 63;
     uint32_t cm, cid, xm, xid;
     cycle_keys(flags, rwx, rwy, bp, cm, cid);
-    context_keys(base, hx, hy, neg, bp, xm, xid);
+    context_keys3(base, hx, hy, neg, bp, xm, xid);
     const bool qok = q < 64;                            // quality and context M entries in LDS
     const bool mok = qok && cm < L.mcodes;              // cycle M count in LDS
     const bool yid = cid != 1023u, yok = yid && cid < ncodes;
@@ -739,8 +761,16 @@ __device__ __forceinline__ void cov_read_wave(const CovArgs &a, const LdsAcc &L,
             frac_at(j, c, vs, vi, vd);
             const uint16_t bp = (uint16_t)(rwx + j);
             if (!(fl[c] & BF_MASKED) && !bit_at(SM, c)) {
-                const Limbs f0 = to_limbs(vs, a.frac_f64), f1 = to_limbs(vi, a.frac_f64),
-                            f2 = to_limbs(vd, a.frac_f64);
+                // outside BAQ blocks every value is exactly 0 or 1 (limbs 0, 0 and 0 or 2^26)
+                Limbs f0, f1, f2;
+                auto unit = [](double v) { return v == 0.0 || v == 1.0; };
+                if (wave_all(unit(vs) && unit(vi) && unit(vd))) {
+                    f0 = {0, 0, vs == 1.0 ? 1ull << 26 : 0ull};
+                    f1 = {0, 0, vi == 1.0 ? 1ull << 26 : 0ull};
+                    f2 = {0, 0, vd == 1.0 ? 1ull << 26 : 0ull};
+                } else {
+                    f0 = to_limbs(vs, a.frac_f64); f1 = to_limbs(vi, a.frac_f64); f2 = to_limbs(vd, a.frac_f64);
+                }
                 em = true;
                 i0 += f1.p0; i1 += f1.p1; i2 += f1.p2;
                 d0 += f2.p0; d1 += f2.p1; d2 += f2.p2;
