@@ -661,9 +661,11 @@ __device__ __forceinline__ void cov_read_wave(const CovArgs &a, const LdsAcc &L,
         LQ[c] = __ballot(v && qv[c] <= 2);
     }
     // high quality window (covariates.cu:215-246)
-    int hxr = next_zero(LQ, 0);
+    // (no base of quality <= 2: the whole window)
+    const bool lowq = (LQ[0] | LQ[1] | LQ[2] | LQ[3]) != 0;
+    int hxr = lowq ? next_zero(LQ, 0) : 0;
     if (hxr > len - 1) hxr = len - 1;
-    int hyr = prev_zero(LQ, len);
+    int hyr = lowq ? prev_zero(LQ, len) : len - 1;
     if (hyr < hxr) hyr = hxr;
     const uint16_t hx = (uint16_t)(rwx + hxr), hy = (uint16_t)(rwx + hyr);
     if (lane == 0) a.s.hqw[r] = pack2(hx, hy);
@@ -752,13 +754,21 @@ __device__ __forceinline__ void cov_read_wave(const CovArgs &a, const LdsAcc &L,
 
     uint32_t nid = 0;
     uint64_t i0 = 0, i1 = 0, i2 = 0, d0 = 0, d1 = 0, d2 = 0;
+    // a read without BAQ blocks has every error value 0 or 1: its bits (a wave-uniform test)
+    const bool blocks = (IB[0] | IB[1] | IB[2] | IB[3]) != 0;
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int j = lane + 64 * c;
         bool em = false;
         if (j < len) {
             double vs, vi, vd;
-            frac_at(j, c, vs, vi, vd);
+            if (blocks || a.keep_frac) {
+                frac_at(j, c, vs, vi, vd);
+            } else {
+                vs = bit_at(ES, c) ? 1.0 : 0.0;
+                vi = bit_at(EI_, c) ? 1.0 : 0.0;
+                vd = bit_at(ED, c) ? 1.0 : 0.0;
+            }
             const uint16_t bp = (uint16_t)(rwx + j);
             if (!(fl[c] & BF_MASKED) && !bit_at(SM, c)) {
                 // outside BAQ blocks every value is exactly 0 or 1 (limbs 0, 0 and 0 or 2^26)
