@@ -287,9 +287,7 @@ struct fp_engine {
         bool back_pending = false;      // stage-2 time not yet collected
         uint64_t fill = 1;              // which fill of this buffer is staging now (a job records the
                                         // fills that hold its reads)
-        bool host_fed = false;          // this fill holds a batch copied from host memory
         bool copies_pending = false;    // copy-stream work the buffer's stream has not waited for yet
-        uint32_t g_reserve = 0;         // reserved waves the BAQ graph was captured with
         // A flushed buffer's front stage (prep, classification) runs first; its small readback (band
         // class counts, longest read, error words) lands in hc, and the rest of the batch is launched
         // once the host sees it there: the host goes on staging the next batches meanwhile.
@@ -299,7 +297,6 @@ struct fp_engine {
         hipEvent_t front_done = nullptr;         // after the front stage and its readback
         bool pending = false;           // flushed; the stages after the front not launched yet
         uint32_t pend_n = 0;            // reads of the pending batch
-        bool pend_host_fed = false;
     };
     // staging buffers, used in turn: one fills while the others run.  A buffer refills only once its
     // previous batch has run, so with two a host-fed buffer's copies wait for its own last batch;
@@ -470,24 +467,13 @@ struct Plan {
     double *band = nullptr;
     uint64_t band_doubles = 0;
     bool no_frac = false;
-    // resident BAQ waves left out of every band class's grid (a multiple of 4: whole CUs)
-    uint32_t reserve_waves = 0;
 };
 
-// A band kernel workgroup takes all of a CU's registers (the 15-cell checkpoint kernel 512 per lane,
-// the wider classes over 350), so while the persistent band grids run, no other kernel starts
-// anywhere.  A staging buffer filled from host memory therefore leaves an eighth of the CUs (4 per
-// XCD) out of its band grids: the copy kernels reading the next host batches over the host link run
-// there instead of waiting for the BAQ stage to end.  Measured on the 10M-read workload fed in 20k-read
-// host batches: 7.1e7 reads/s with every CU in the grid, 8.2e7 with 32 left out (7.2e7 with 24, 8.1e7
-// with 48); batches already in HBM keep every CU (1.09e8; 1.03e8 with 32 left out).
-static constexpr uint32_t HOST_FED_RESERVE_WAVES = 4 * 32;
-
 // resident waves of a band class; its per-wave scratch slot in doubles
-static uint32_t band_waves(uint32_t wpe_class0, int c, uint32_t n, uint32_t reserve = 0)
+static uint32_t band_waves(uint32_t wpe_class0, int c, uint32_t n)
 {
     const uint32_t wpe = c == 0 ? wpe_class0 : 1u;
-    const uint32_t w = std::min<uint32_t>((n + 63) / 64, (1024u - reserve) * wpe);
+    const uint32_t w = std::min<uint32_t>((n + 63) / 64, 1024u * wpe);
     return (w + 3) / 4 * 4;
 }
 static uint64_t band_slot(int c, uint32_t rows_cap, uint32_t wgen)
@@ -497,10 +483,10 @@ static uint64_t band_slot(int c, uint32_t rows_cap, uint32_t wgen)
     return ((uint64_t)(rows_cap + 1) * W * 2 + (rows_cap + 2) + (c == 3 ? 3ull * W : rows_cap + 1ull)) * 64;
 }
 // the band classes run at the same time (two streams), so each has its own region of the scratch
-static uint64_t band_offset(int wpe, int c, uint32_t n, uint32_t rows_cap, uint32_t reserve = 0)
+static uint64_t band_offset(int wpe, int c, uint32_t n, uint32_t rows_cap)
 {
     uint64_t off = 0;
-    for (int k = 0; k < c; k++) off += band_slot(k, rows_cap, 0) * band_waves(wpe, k, n, reserve);
+    for (int k = 0; k < c; k++) off += band_slot(k, rows_cap, 0) * band_waves(wpe, k, n);
     return off;
 }
 static uint64_t band_scratch_doubles(int wpe, uint32_t n, uint32_t rows_cap)
@@ -603,9 +589,9 @@ fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, 
         // the class's chunk counter: a word of the class counts that the front stage zeroes
         return fpd_launch_baq_hmm(c, &b, &e->genome, &S.rs, S.baq_lists.as<uint32_t>() + (uint64_t)c * cap,
                                   S.baq_counts.as<uint32_t>() + c,
-                                  e->scratch.as<double>() + band_offset(wpe, c, p.n, p.rows_cap, p.reserve_waves),
+                                  e->scratch.as<double>() + band_offset(wpe, c, p.n, p.rows_cap),
                                   band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c],
-                                  band_waves(wpe, c, p.n, p.reserve_waves),
+                                  band_waves(wpe, c, p.n),
                                   (unsigned long long *)(counters + (c == 0 ? 10 : 8)), wpe,
                                   e->opts.disable_baq_staging ? 0 : 1, ckpt > 1 ? ckpt : 0,
                                   S.baq_counts.as<uint32_t>() + BAQ_CHUNK_COUNTERS + c, s);
@@ -949,7 +935,6 @@ static void stage_done(fp_engine *e, fp_engine::Stage &s)
     e->last_num_quals = s.nquals;
     e->last = &s.bs;
     s.n = 0;
-    s.host_fed = false;
     s.ncig = s.nbases = s.nquals = 0;
     e->cur = (e->cur + 1) % e->nstage;
     e->postprocessed = false;
@@ -1050,7 +1035,6 @@ fp_status batched_flush(fp_engine *e)
     HIP_TRY(hipEventRecord(s.front_done, st));
     s.pending = true;
     s.pend_n = s.n;
-    s.pend_host_fed = s.host_fed;
     e->pending.push_back(e->cur);
     stage_done(e, s);
     // the per-kernel profile runs every stage eagerly, in turn
@@ -1096,8 +1080,7 @@ fp_status batched_back(fp_engine *e)
     e->stats.baq_reads += hc[fpd::BW_BAQ];
 
     const fpd::BatchDev b = stage_dev(e, s);
-    Plan p{e->cap_reads, e->cap_quals, s.live.as<uint32_t>(), e->rows_cap};
-    p.reserve_waves = s.pend_host_fed ? HOST_FED_RESERVE_WAVES : 0u;
+    const Plan p{e->cap_reads, e->cap_quals, s.live.as<uint32_t>(), e->rows_cap};
     uint32_t *counters = e->bcounters.as<uint32_t>();
     fp_status rc = FP_OK;
 
@@ -1151,8 +1134,7 @@ fp_status batched_back(fp_engine *e)
     rc = launch_generic(e, S, b, p, hc[3], hc[4], counters, st);
     if (rc != FP_OK) return rc;
     if (cand < 0) {
-        if (!s.gbaq || s.g_wpe != wpe || s.g_ckpt != ckpt || !s.gcov || s.g_bits != bits ||
-            s.g_reserve != p.reserve_waves) {
+        if (!s.gbaq || s.g_wpe != wpe || s.g_ckpt != ckpt || !s.gcov || s.g_bits != bits) {
             rc = capture(st, &s.gbaq,
                          [&] { return launch_baq_stage(e, S, b, p, wpe, ckpt, counters, st, nullptr, nullptr); });
             if (rc != FP_OK) return rc;
@@ -1162,7 +1144,6 @@ fp_status batched_back(fp_engine *e)
             s.g_wpe = wpe;
             s.g_ckpt = ckpt;
             s.g_bits = bits;
-            s.g_reserve = p.reserve_waves;
         }
         HIP_TRY(hipGraphLaunch(s.gbaq, st));
         HIP_TRY(hipEventRecord(s.baq_done, st));
@@ -1352,7 +1333,6 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
         // host link by one copy-and-rebase kernel (18 DMA copies cost ~25 us each in the runtime:
         // 4.3e7 reads/s against 7.1e7 with the kernel); pageable ones take the runtime's copies and a
         // rebase kernel on the buffer's stream.
-        s->host_fed = true;
         if (!e->copy) {
             HIP_TRY(hipStreamCreateWithFlags(&e->copy, hipStreamNonBlocking));
             HIP_TRY(hipEventCreateWithFlags(&e->copied, hipEventDisableTiming));
@@ -2105,7 +2085,6 @@ fp_status fp_engine_reset(fp_engine *e)
             fp_engine::Stage &s = e->stg[si];
             collect_back_time(e, s);
             s.n = 0;
-            s.host_fed = false;
             s.ncig = s.nbases = s.nquals = 0;
             s.fill += 1;
             if (e->staged_ready) HIP_TRY(fpd_fill_u32(s.bs.baq_counts.as<uint32_t>(), 144, 0u, e->stream));
