@@ -45,7 +45,8 @@ hipError_t fpd_rebase(uint32_t n, uint32_t *cigar_start, uint32_t *read_start, u
                       uint32_t *read_group, uint32_t cig_off, uint32_t base_off, uint32_t qual_off, uint32_t rg_off,
                       uint32_t rg_limit, hipStream_t st);
 hipError_t fpd_append(const void *const *src, void *const *dst, const uint64_t *bytes, uint32_t cig_off,
-                      uint32_t base_off, uint32_t qual_off, uint32_t rg_off, uint32_t rg_limit, hipStream_t st);
+                      uint32_t base_off, uint32_t qual_off, uint32_t rg_off, uint32_t rg_limit, uint64_t block_bytes,
+                      hipStream_t st);
 hipError_t fpd_cov_prepare();
 hipError_t fpd_launch_cov(const fpd::BatchDev *b, const fpd::ReadStateDev *s, const fpd::TablesDev *t,
                           uint32_t frac_f64, uint32_t keep_frac, uint32_t *batch, uint32_t *rg_buf,
@@ -276,13 +277,14 @@ struct fp_engine {
         uint32_t *live_host = nullptr;  // pinned
         uint32_t n = 0;
         uint64_t ncig = 0, nbases = 0, nquals = 0;
-        hipGraphExec_t gfront = nullptr, gbaq = nullptr, gcov = nullptr;
+        // front stage; 15-cell BAQ class; 23- and 33-cell classes; cap/recode + covariates
+        hipGraphExec_t gfront = nullptr, gbaq = nullptr, gside = nullptr, gcov = nullptr;
         int g_wpe = 0, g_ckpt = 0, g_bits = 0;
         uint64_t g_epoch = 0;
         BatchState bs;                  // this buffer's per-batch state
-        hipStream_t stream = nullptr;   // this buffer's compute stream
-        hipEvent_t baq_done = nullptr;  // after its BAQ kernels (they share the band scratch)
-        hipEvent_t cov_done = nullptr;  // after its covariate stage (sort buffers are shared)
+        hipStream_t stream = nullptr;   // its appends and front stage (the engine stream)
+        hipEvent_t baq_done = nullptr;  // after its 15-cell and generic BAQ kernels
+        hipEvent_t cov_done = nullptr;  // after its covariate stage (the buffer may refill)
         hipEvent_t t0 = nullptr, t1 = nullptr;   // around its stage 2 (timing)
         bool back_pending = false;      // stage-2 time not yet collected
         uint64_t fill = 1;              // which fill of this buffer is staging now (a job records the
@@ -301,12 +303,22 @@ struct fp_engine {
     // staging buffers, used in turn: one fills while the others run.  A buffer refills only once its
     // previous batch has run, so with two a host-fed buffer's copies wait for its own last batch;
     // with three (the default) they overlap the other two buffers' kernels.
-    static constexpr int MAX_STAGE = 3;
+    static constexpr int MAX_STAGE = 4;
     Stage stg[MAX_STAGE];
     int nstage = 3;
     bool batched = false;
+    // The batched engine's streams follow the stages, not the buffers: the engine stream carries
+    // every buffer's device appends and front stage, bq the 15-cell and generic BAQ kernels, side the
+    // 23- and 33-cell classes, cap/recode and the covariates, copy the host batches.  Each stream
+    // takes the buffers in flush order, and a process has four hardware queues (queues run their
+    // work in order): with one stream per stage, one buffer's front stage never waits behind
+    // another buffer's BAQ or covariate kernels in a shared queue.
+    hipStream_t bq = nullptr;
     int cur = 0;
     std::deque<int> pending;            // flushed buffers waiting for their readback, in flush order
+    int last_back = -1;                 // the buffer whose stages were launched last (-1: none yet)
+    uint32_t last_flush_n = 0;          // reads of the buffer flushed last (0 after a sync)
+    double idle_since = -1;             // host time (ms) the device was first seen idle, or -1
     uint32_t cap_reads = 0, rows_cap = 0;
     uint64_t cap_cig = 0, cap_bases = 0, cap_quals = 0;
     DevBuf bcounters;                   // prep / BAQ counters accumulated over staged batches
@@ -581,23 +593,30 @@ fp_status collect_sort_counts(fp_engine *e, int sort_bits)
 
 // stage 2a (graph-safe): the BAQ band classes and cap/recode.  The 15-cell class counts its band
 // cells in counter words 10..11, the other classes in 8..9.
+// one band class's kernel over its read list, in the class's own region of the band scratch
+static hipError_t launch_band_class(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, int wpe,
+                                    int ckpt, uint32_t *counters, int c, hipStream_t s)
+{
+    const uint32_t cap = std::max<uint32_t>(p.n, 1);
+    // the class's chunk counter: a word of the class counts that the front stage zeroes
+    return fpd_launch_baq_hmm(c, &b, &e->genome, &S.rs, S.baq_lists.as<uint32_t>() + (uint64_t)c * cap,
+                              S.baq_counts.as<uint32_t>() + c,
+                              e->scratch.as<double>() + band_offset(wpe, c, p.n, p.rows_cap),
+                              band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c], band_waves(wpe, c, p.n),
+                              (unsigned long long *)(counters + (c == 0 ? 10 : 8)), wpe,
+                              e->opts.disable_baq_staging ? 0 : 1, ckpt > 1 ? ckpt : 0,
+                              S.baq_counts.as<uint32_t>() + BAQ_CHUNK_COUNTERS + c, s);
+}
+
 fp_status launch_baq_stage(fp_engine *e, BatchState &S, const fpd::BatchDev &b, const Plan &p, int wpe, int ckpt,
                            uint32_t *counters, hipStream_t st, hipEvent_t mark, const uint32_t *known_counts)
 {
-    const uint32_t cap = std::max<uint32_t>(p.n, 1);
     auto launch_class = [&](int c, hipStream_t s) -> hipError_t {
-        // the class's chunk counter: a word of the class counts that the front stage zeroes
-        return fpd_launch_baq_hmm(c, &b, &e->genome, &S.rs, S.baq_lists.as<uint32_t>() + (uint64_t)c * cap,
-                                  S.baq_counts.as<uint32_t>() + c,
-                                  e->scratch.as<double>() + band_offset(wpe, c, p.n, p.rows_cap),
-                                  band_slot(c, p.rows_cap, 0), p.rows_cap, BAQ_CLASS_W[c],
-                                  band_waves(wpe, c, p.n),
-                                  (unsigned long long *)(counters + (c == 0 ? 10 : 8)), wpe,
-                                  e->opts.disable_baq_staging ? 0 : 1, ckpt > 1 ? ckpt : 0,
-                                  S.baq_counts.as<uint32_t>() + BAQ_CHUNK_COUNTERS + c, s);
+        return launch_band_class(e, S, b, p, wpe, ckpt, counters, c, s);
     };
     if (p.band) {
         // allocation-free batch: the classes in turn on one stream, each in the whole band region
+        const uint32_t cap = std::max<uint32_t>(p.n, 1);
         for (int c : {1, 2, 0}) {
             const uint32_t cnt = known_counts ? known_counts[c] : 0u;
             if (cnt == 0) continue;
@@ -844,19 +863,15 @@ fp_status batched_alloc(fp_engine *e)
             HIP_TRY(hipHostMalloc((void **)&s.hc, 80 * sizeof(uint32_t), hipHostMallocDefault));
             memset(s.hc, 0, 80 * sizeof(uint32_t));
         }
-        // The first buffer runs on the engine stream and the others on streams of their own, which
-        // also carry their appends (so a buffer refills only once its last batch is consumed).  A
-        // process has four hardware queues; streams beyond them share one and run in turn.
-        // profile_stages keeps every kernel on the engine stream (clean per-kernel times).
-        if (!s.stream) {
-            if (&s == &e->stg[0] || e->opts.profile_stages) s.stream = e->stream;
-            else HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
-        }
+        // every buffer's appends and front stage run on the engine stream (profile_stages keeps all
+        // of its kernels there, for clean per-kernel times)
+        s.stream = e->stream;
         fp_status rc = ensure_state(e, s.bs, e->cap_reads, e->cap_bases, e->cap_quals,
                                     e->opts.debug_arrays != 0 || e->rows_cap > 256);
         if (rc != FP_OK) return rc;
         HIP_TRY(fpd_fill_u32(s.bs.baq_counts.as<uint32_t>(), 144, 0u, e->stream));   // read before the first batch
     }
+    if (!e->bq) HIP_TRY(hipStreamCreateWithFlags(&e->bq, hipStreamNonBlocking));
     fp_status rc = FP_OK;
     HIP_TRY(e->scratch.ensure(band_scratch_doubles(std::max(3, e->wpe), e->cap_reads, e->rows_cap) * 8 + 8));
     if (e->opts.aggregation == FP_AGG_SORT) {
@@ -941,27 +956,54 @@ static void stage_done(fp_engine *e, fp_engine::Stage &s)
 }
 
 // Run the current staging buffer (if it holds reads) and switch to the next one.  Each buffer has
-// its own compute stream and per-batch state, so one buffer's covariate stage and prep run at the
-// same time as another's BAQ.  The BAQ kernels of all buffers share the band scratch and take turns
-// (each waits for the previous buffer's event baq_done); so do the covariate stages of the sort
-// aggregation (its pair buffers are shared, event cov_done).  The direct tables take every buffer's
-// atomic adds in any order.
+// its own per-batch state, and each stage its own stream (engine stream: appends and front stage;
+// bq: 15-cell and generic BAQ; side: wider BAQ classes, cap/recode and covariates), so one buffer's
+// front stage and another's covariate stage run at the same time as a third's BAQ.  On each stream
+// the buffers take turns in flush order, which is what the shared band scratch regions and the sort
+// aggregation's shared pair buffers need.  The direct tables take every buffer's atomic adds in any
+// order.
 //
 // The flush launches the front stage and its readback and returns; batched_back launches the rest
 // once the readback has landed (batched_poll from later calls, or batched_drain when a call must
 // wait).  Buffers go through batched_back in flush order, so the BAQ turns keep that order.
 fp_status batched_back(fp_engine *e);
 
-// true when every flushed buffer's stages have been launched and have completed
-static bool batched_idle(fp_engine *e)
+static double host_ms()
 {
-    if (!e->pending.empty()) return false;
-    for (int si = 0; si < e->nstage; si++) {
-        const fp_engine::Stage &s = e->stg[si];
-        if (s.cov_done && hipEventQuery(s.cov_done) == hipErrorNotReady) return false;
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// How many reads the current buffer needs before it runs without being full (0xffffffff: only when
+// full).  Nothing queued behind the BAQ kernels now running (every flushed buffer's stages launched,
+// and the BAQ stage of the last one begun: its start event t0 follows the previous buffer's BAQ
+// kernels and its own front stage) means the BAQ stage is about to run dry; a buffer flushed then
+// has its front stage done before those kernels end.  After a sync (the start of a stream) an eighth
+// of a buffer is enough; after that, twice the last buffer's reads, so that the buffers grow to full
+// within a few turns and a full buffer waits while the device works (buffers that stayed small would
+// each arrive as late as the last one, and a buffer refills only once its last batch has run).  A
+// device left idle for 2 ms (a stream that arrives slower than the device runs it) again takes an
+// eighth of a buffer.
+static uint32_t batched_early_need(fp_engine *e)
+{
+    const uint32_t never = 0xffffffffu, eighth = e->cap_reads / 8;
+    if (!e->pending.empty()) return never;
+    if (e->last_flush_n == 0) return eighth;
+    const hipEvent_t t0 = e->last_back >= 0 ? e->stg[e->last_back].t0 : nullptr;
+    if (t0 && hipEventQuery(t0) == hipErrorNotReady) return never;
+    bool idle = true;
+    for (int si = 0; si < e->nstage && idle; si++) {
+        const hipEvent_t c = e->stg[si].cov_done;
+        idle = !(c && hipEventQuery(c) == hipErrorNotReady);
     }
     (void)hipGetLastError();
-    return true;
+    if (!idle) {
+        e->idle_since = -1;
+    } else {
+        const double now = host_ms();
+        if (e->idle_since < 0) e->idle_since = now;
+        if (now - e->idle_since > 2.0) return eighth;
+    }
+    return (uint32_t)std::min<uint64_t>(e->cap_reads, std::max<uint64_t>(eighth, 2ull * e->last_flush_n));
 }
 
 fp_status batched_poll(fp_engine *e)
@@ -1019,6 +1061,7 @@ fp_status batched_flush(fp_engine *e)
         rc = capture(st, &s.gfront, [&] { return launch_front(e, S, b, p, counters, st); });
         if (rc != FP_OK) return rc;
         if (s.gbaq) { (void)hipGraphExecDestroy(s.gbaq); s.gbaq = nullptr; }
+        if (s.gside) { (void)hipGraphExecDestroy(s.gside); s.gside = nullptr; }
         if (s.gcov) { (void)hipGraphExecDestroy(s.gcov); s.gcov = nullptr; }
         s.g_epoch = e->epoch;
     }
@@ -1035,6 +1078,8 @@ fp_status batched_flush(fp_engine *e)
     HIP_TRY(hipEventRecord(s.front_done, st));
     s.pending = true;
     s.pend_n = s.n;
+    e->last_flush_n = s.n;
+    e->idle_since = -1;
     e->pending.push_back(e->cur);
     stage_done(e, s);
     // the per-kernel profile runs every stage eagerly, in turn
@@ -1049,10 +1094,9 @@ fp_status batched_back(fp_engine *e)
     const int si = e->pending.front();
     e->pending.pop_front();
     fp_engine::Stage &s = e->stg[si];
-    fp_engine::Stage &o = e->stg[(si + e->nstage - 1) % e->nstage];
     s.pending = false;
     const bool prof = e->opts.profile_stages != 0;
-    hipStream_t st = prof ? e->stream : s.stream;
+    hipStream_t st = e->stream;
     BatchState &S = s.bs;
     HIP_TRY(hipEventSynchronize(s.front_done));
     collect_back_time(e, s);
@@ -1125,41 +1169,60 @@ fp_status batched_back(fp_engine *e)
         e->stats.covariates_ms += elapsed(e->ev[2], e->ev[3]);
         return FP_OK;
     }
-    if (cand >= 0)   // the candidate runs alone
-        for (auto &x : e->stg)
-            if (&x != &s && x.stream) HIP_TRY(hipStreamSynchronize(x.stream));
-    // the band scratch is free once the previous buffer's BAQ kernels are done
-    HIP_TRY(hipStreamWaitEvent(st, o.baq_done, 0));
-    HIP_TRY(hipEventRecord(s.t0, st));
-    rc = launch_generic(e, S, b, p, hc[3], hc[4], counters, st);
+    // BAQ queue: this buffer's 15-cell and generic BAQ kernels follow its front stage and, in queue
+    // order, the previous buffer's (they share the band scratch regions of their classes)
+    hipStream_t bq = e->bq, sd = e->side;
+    if (cand >= 0) {   // the candidate runs alone
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        HIP_TRY(hipStreamSynchronize(bq));
+        HIP_TRY(hipStreamSynchronize(sd));
+    }
+    HIP_TRY(hipStreamWaitEvent(bq, s.front_done, 0));
+    HIP_TRY(hipEventRecord(s.t0, bq));
+    e->last_back = si;
+    rc = launch_generic(e, S, b, p, hc[3], hc[4], counters, bq);
     if (rc != FP_OK) return rc;
     if (cand < 0) {
-        if (!s.gbaq || s.g_wpe != wpe || s.g_ckpt != ckpt || !s.gcov || s.g_bits != bits) {
-            rc = capture(st, &s.gbaq,
-                         [&] { return launch_baq_stage(e, S, b, p, wpe, ckpt, counters, st, nullptr, nullptr); });
+        if (!s.gbaq || s.g_wpe != wpe || s.g_ckpt != ckpt || !s.gside || !s.gcov || s.g_bits != bits) {
+            rc = capture(bq, &s.gbaq, [&]() -> fp_status {
+                HIP_TRY(launch_band_class(e, S, b, p, wpe, ckpt, counters, 0, bq));
+                return FP_OK;
+            });
             if (rc != FP_OK) return rc;
-            rc = capture(st, &s.gcov,
-                         [&] { return launch_cov_stage(e, S, b, p, bits, counters, st, nullptr); });
+            rc = capture(sd, &s.gside, [&]() -> fp_status {
+                for (int c = 1; c < 3; c++) HIP_TRY(launch_band_class(e, S, b, p, wpe, ckpt, counters, c, sd));
+                return FP_OK;
+            });
+            if (rc != FP_OK) return rc;
+            rc = capture(sd, &s.gcov, [&]() -> fp_status {
+                HIP_TRY(fpd_launch_baq_post(&b, &S.rs, 1u, sd));
+                return launch_cov_stage(e, S, b, p, bits, counters, sd, nullptr);
+            });
             if (rc != FP_OK) return rc;
             s.g_wpe = wpe;
             s.g_ckpt = ckpt;
             s.g_bits = bits;
         }
-        HIP_TRY(hipGraphLaunch(s.gbaq, st));
-        HIP_TRY(hipEventRecord(s.baq_done, st));
-        if (sorting) HIP_TRY(hipStreamWaitEvent(st, o.cov_done, 0));
-        HIP_TRY(hipGraphLaunch(s.gcov, st));
-        HIP_TRY(hipEventRecord(s.cov_done, st));
-        HIP_TRY(hipEventRecord(s.t1, st));
+        HIP_TRY(hipGraphLaunch(s.gbaq, bq));
+        HIP_TRY(hipEventRecord(s.baq_done, bq));
+        // side queue: the wider classes beside the 15-cell class (their own scratch regions, in queue
+        // order), then cap/recode and the covariates once every class is done; the covariate stages
+        // of the sort aggregation share its pair buffers and keep queue order as well
+        HIP_TRY(hipStreamWaitEvent(sd, s.front_done, 0));
+        HIP_TRY(hipGraphLaunch(s.gside, sd));
+        HIP_TRY(hipStreamWaitEvent(sd, s.baq_done, 0));
+        HIP_TRY(hipGraphLaunch(s.gcov, sd));
+        HIP_TRY(hipEventRecord(s.cov_done, sd));
+        HIP_TRY(hipEventRecord(s.t1, sd));
         s.back_pending = true;
     } else {
-        rc = launch_baq_stage(e, S, b, p, wpe, ckpt, counters, st, nullptr, nullptr);   // timed candidate, eager
+        rc = launch_baq_stage(e, S, b, p, wpe, ckpt, counters, bq, nullptr, nullptr);   // timed candidate, eager
         if (rc != FP_OK) return rc;
-        HIP_TRY(hipEventRecord(s.baq_done, st));
-        rc = launch_cov_stage(e, S, b, p, bits, counters, st, nullptr);
+        HIP_TRY(hipEventRecord(s.baq_done, bq));
+        rc = launch_cov_stage(e, S, b, p, bits, counters, bq, nullptr);
         if (rc != FP_OK) return rc;
-        HIP_TRY(hipEventRecord(s.cov_done, st));
-        HIP_TRY(hipEventRecord(s.t1, st));
+        HIP_TRY(hipEventRecord(s.cov_done, bq));
+        HIP_TRY(hipEventRecord(s.t1, bq));
         HIP_TRY(hipEventSynchronize(s.t1));
         const double ms = elapsed(s.t0, s.t1);
         e->stats.baq_ms += ms;
@@ -1177,7 +1240,10 @@ fp_status batched_sync(fp_engine *e)
     if (rc != FP_OK) return rc;
     for (auto &s : e->stg)
         if (s.stream) HIP_TRY(hipStreamSynchronize(s.stream));
+    if (e->bq) HIP_TRY(hipStreamSynchronize(e->bq));
+    if (e->side) HIP_TRY(hipStreamSynchronize(e->side));
     HIP_TRY(hipStreamSynchronize(e->stream));
+    e->last_flush_n = 0;   // the next batch starts a stream
     if (!e->staged_ready) return FP_OK;
     for (auto &s : e->stg) collect_back_time(e, s);
     uint32_t *hc = e->host_counts + 32;
@@ -1205,6 +1271,11 @@ fp_status batched_sync(fp_engine *e)
 }
 
 fp_status plain_host_batch(fp_engine *e, const fp_batch_view *hb);
+
+// Bytes of a batch array per workgroup of the append kernel.  Over the host link a few hundred waves
+// keep the link busy (the kernel runs beside the other buffers' kernels, so it takes no more); from
+// HBM more workgroups finish the copy sooner.
+static constexpr uint64_t APPEND_HOST_BLOCK_BYTES = 256 << 10, APPEND_DEVICE_BLOCK_BYTES = 32 << 10;
 
 // Host-side checks of a host batch: every read inside the batch arrays (and starting on a word of
 // bases), no read longer than max_len, no read group at or beyond rg_limit (0xffffffff: read groups
@@ -1304,8 +1375,10 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
     }
     fp_status rc = batched_alloc(e);
     if (rc != FP_OK) return rc;
-    // the buffer's own stream: its appends follow its last batch's kernels
-    hipStream_t cs = e->opts.profile_stages ? e->stream : s->stream;
+    // device appends run on the engine stream; a fill's first one waits until the buffer's last batch
+    // has been consumed (its covariate stage)
+    hipStream_t cs = e->stream;
+    if (on_device && s->n == 0 && s->cov_done) HIP_TRY(hipStreamWaitEvent(cs, s->cov_done, 0));
     const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const void *src[18] = {v->chromosome, v->alignment_start, v->alignment_stop, v->mate_chromosome,
                            v->mate_alignment_start, v->inferred_insert_size, v->cigars, v->cigar_start,
@@ -1325,7 +1398,7 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
     if (on_device) {
         // one launch copies and rebases the whole batch (a copy per array costs a launch each)
         HIP_TRY(fpd_append(src, dst, len, (uint32_t)s->ncig, (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off,
-                           rg_limit, cs));
+                           rg_limit, APPEND_DEVICE_BLOCK_BYTES, cs));
     } else {
         // Host arrays, on the copy stream, which first waits for the buffer's previous batch (its
         // kernels read these arrays) when this is the fill's first batch; the host then waits for the
@@ -1341,7 +1414,7 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
         const void *hsrc[18];
         if (pinned_sources(src, len, hsrc)) {
             HIP_TRY(fpd_append(hsrc, dst, len, (uint32_t)s->ncig, (uint32_t)s->nbases, (uint32_t)s->nquals, rg_off,
-                               rg_limit, e->copy));
+                               rg_limit, APPEND_HOST_BLOCK_BYTES, e->copy));
             HIP_TRY(hipEventRecord(e->copied, e->copy));
             s->copies_pending = true;   // the buffer's stream waits for them once, at the flush
             // the batch's checks while the copy runs: a failing batch is not staged (its bytes sit past
@@ -1373,10 +1446,9 @@ fp_status batched_append(fp_engine *e, const fp_batch_view *v, bool on_device, u
     // is reported here (this call's batch is staged all the same)
     rc = batched_poll(e);
     if (first_err == FP_OK && rc != FP_OK) { first_err = rc; first_msg = g_last_error; }
-    // With nothing of the engine left to run on the device (the start of a stream, or a stream that
-    // arrives slower than it runs), a buffer holding an eighth of its capacity runs at once rather
-    // than when it is full: the device starts early, and full buffers follow while it works.
-    if (first_err == FP_OK && !e->opts.profile_stages && s->n >= e->cap_reads / 8 && batched_idle(e)) {
+    // When the BAQ stage is about to run dry, a buffer runs before it is full (batched_early_need):
+    // its front stage then runs beside the BAQ kernels in flight and its BAQ kernels follow theirs.
+    if (first_err == FP_OK && !e->opts.profile_stages && s->n >= e->cap_reads / 8 && s->n >= batched_early_need(e)) {
         rc = batched_flush(e);
         if (rc != FP_OK) { first_err = rc; first_msg = g_last_error; }
     }
@@ -1411,7 +1483,7 @@ fp_status plain_host_batch(fp_engine *e, const fp_batch_view *hb)
     for (int i = 0; i < 18; i++) { len[i] = copy_sz[i]; dst[i] = e->in[i].p; }
     if (pinned_sources(src, len, hsrc)) {
         // pinned host arrays: one copy launch reads all of them over the host link
-        HIP_TRY(fpd_append(hsrc, dst, len, 0, 0, 0, 0, 0xffffffffu, st));
+        HIP_TRY(fpd_append(hsrc, dst, len, 0, 0, 0, 0, 0xffffffffu, APPEND_HOST_BLOCK_BYTES, st));
     } else {
         for (int i = 0; i < 18; i++)
             if (copy_sz[i]) HIP_TRY(hipMemcpyAsync(e->in[i].p, src[i], copy_sz[i], hipMemcpyHostToDevice, st));
@@ -1657,8 +1729,8 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
         return fail(FP_ERR_INVALID_ARG, "baq_checkpoint_rows must be 0 (tuned), 1, 3, 4 or 5");
     if (o.sort_digit_bits != 0 && (o.sort_digit_bits < 4 || o.sort_digit_bits > 8))
         return fail(FP_ERR_INVALID_ARG, "sort_digit_bits must be 0 (autotune) or 4..8");
-    if (o.staging_buffers != 0 && (o.staging_buffers < 2 || o.staging_buffers > 3))
-        return fail(FP_ERR_INVALID_ARG, "staging_buffers must be 0 (three), 2 or 3");
+    if (o.staging_buffers != 0 && (o.staging_buffers < 2 || o.staging_buffers > fp_engine::MAX_STAGE))
+        return fail(FP_ERR_INVALID_ARG, "staging_buffers must be 0 (three) or 2 to 4");
     int ndev = 0;
     HIP_TRY(hipGetDeviceCount(&ndev));
     if (hip_device < 0 || hip_device >= ndev) return fail(FP_ERR_INVALID_ARG, "no such HIP device");
@@ -1670,7 +1742,7 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     e->ckpt = o.baq_checkpoint_rows >= 3 ? (int)o.baq_checkpoint_rows : 1;
     e->sort_bits = o.sort_digit_bits ? (int)o.sort_digit_bits : 8;
     e->budget_bytes = (uint64_t)o.serve_hbm_budget_mib << 20;
-    e->nstage = o.staging_buffers ? (int)o.staging_buffers : fp_engine::MAX_STAGE;
+    e->nstage = o.staging_buffers ? (int)o.staging_buffers : 3;
     // the digit width is tuned on the batched engine's first staged batches unless it is given
     e->sort_tuned = o.sort_digit_bits != 0 || o.aggregation != FP_AGG_SORT || o.engine_kind != FP_ENGINE_BATCHED;
     memset(&e->stats, 0, sizeof(e->stats));
@@ -1681,6 +1753,13 @@ fp_status fp_engine_create(int hip_device, const fp_options *opts, fp_engine **o
     for (int i = 0; i < 16; i++) (void)hipEventCreate(&e->ev[i]);
     he = hipHostMalloc((void **)&e->host_counts, 128 * sizeof(uint32_t), hipHostMallocDefault);
     if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking);
+    if (o.engine_kind == FP_ENGINE_BATCHED) {
+        // the batched engine's four stage streams are created together, so that they take four
+        // hardware queues of their own as far as the process has them free
+        if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->bq, hipStreamNonBlocking);
+        if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->copy, hipStreamNonBlocking);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&e->copied, hipEventDisableTiming);
+    }
     if (he == hipSuccess) he = hipEventCreateWithFlags(&e->na.fork, hipEventDisableTiming);
     if (he == hipSuccess) he = hipEventCreateWithFlags(&e->na.join, hipEventDisableTiming);
     if (he == hipSuccess) he = fpd_cov_prepare();
@@ -2077,6 +2156,7 @@ fp_status fp_engine_reset(fp_engine *e)
     if (e->batched) (void)batched_drain(e);
     for (auto &s : e->stg)
         if (s.stream) HIP_TRY(hipStreamSynchronize(s.stream));
+    if (e->bq) HIP_TRY(hipStreamSynchronize(e->bq));
     if (e->side) HIP_TRY(hipStreamSynchronize(e->side));
     if (e->copy) HIP_TRY(hipStreamSynchronize(e->copy));
     HIP_TRY(hipStreamSynchronize(e->stream));
@@ -2110,6 +2190,7 @@ void fp_engine_destroy(fp_engine *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (auto &s : e->stg)
         if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (e->bq) (void)hipStreamSynchronize(e->bq);
     if (e->side) (void)hipStreamSynchronize(e->side);
     if (e->copy) (void)hipStreamSynchronize(e->copy);
     DevBuf *bufs[] = {&e->g_word_offset, &e->g_length, &e->g_words, &e->g_foff, &e->g_fstart, &e->g_fstop,
@@ -2129,6 +2210,7 @@ void fp_engine_destroy(fp_engine *e)
         if (s.hc) (void)hipHostFree(s.hc);
         if (s.gfront) (void)hipGraphExecDestroy(s.gfront);
         if (s.gbaq) (void)hipGraphExecDestroy(s.gbaq);
+        if (s.gside) (void)hipGraphExecDestroy(s.gside);
         if (s.gcov) (void)hipGraphExecDestroy(s.gcov);
         hipEvent_t evs[] = {s.baq_done, s.cov_done, s.t0, s.t1, s.f0, s.f1, s.front_done};
         for (hipEvent_t ev : evs) if (ev) (void)hipEventDestroy(ev);
@@ -2146,6 +2228,7 @@ void fp_engine_destroy(fp_engine *e)
     for (int i = 0; i < 16; i++) if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
     if (e->copied) (void)hipEventDestroy(e->copied);
     if (e->copy) (void)hipStreamDestroy(e->copy);
+    if (e->bq) (void)hipStreamDestroy(e->bq);
     if (e->side) (void)hipStreamDestroy(e->side);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;

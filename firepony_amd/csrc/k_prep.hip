@@ -891,50 +891,73 @@ __global__ void __launch_bounds__(256) k_rebase(uint32_t n, uint32_t *cigar_star
 } // namespace fpd
 
 namespace fpd {
-// append of a device-resident batch in one launch: blockIdx.y picks one of the 18 batch arrays, whose
-// bytes the blocks along x copy; the three start-offset arrays and the read groups are rebased on the
-// way (as k_rebase does), everything else is copied in 16-, 4- or 1-byte units as alignment allows
+// Append of a batch in one launch: the 18 batch arrays get runs of workgroups in proportion to their
+// bytes (first[i] .. first[i + 1] - 1 copy array i); the three start-offset arrays and the read
+// groups are rebased on the way (as k_rebase does), everything else is copied in 16-, 4- or 1-byte
+// units as alignment allows.  Each lane keeps APPEND_UNROLL loads in flight: for a batch read from
+// pinned host memory over the host link, a few hundred waves then keep the link busy, where one
+// load per lane would need thousands of resident waves (which crowd out the kernels running beside
+// the copy).  The kernel keeps to 24 registers: the checkpointed BAQ kernel holds 488 of a SIMD's
+// 512 with one wave, and a copy of a host batch that did not fit beside it would wait for it.
+static constexpr int APPEND_UNROLL = 2;
 struct AppendArgs {
     const uint8_t *src[18];
     uint8_t *dst[18];
     uint64_t bytes[18];
+    uint32_t first[19];
     uint32_t cig_off, base_off, qual_off, rg_off, rg_limit;
 };
 
-__global__ void __launch_bounds__(256) k_append(AppendArgs a)
+// d[r] = f(s[r]) for r < n, APPEND_UNROLL loads in flight per lane
+template <class T, class F>
+__device__ inline void append_copy(const T *s, T *d, uint64_t n, uint64_t tid, uint64_t stride, F f)
 {
-    const int i = blockIdx.y;
+    for (uint64_t r0 = tid; r0 < n; r0 += APPEND_UNROLL * stride) {
+        T v[APPEND_UNROLL];
+#pragma unroll
+        for (int u = 0; u < APPEND_UNROLL; u++) {
+            const uint64_t r = r0 + u * stride;
+            if (r < n) v[u] = s[r];
+        }
+#pragma unroll
+        for (int u = 0; u < APPEND_UNROLL; u++) {
+            const uint64_t r = r0 + u * stride;
+            if (r < n) d[r] = f(v[u]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_num_vgpr(24))) k_append(AppendArgs a)
+{
+    int i = 0;
+    while (i < 17 && a.first[i + 1] <= blockIdx.x) i++;
     const uint64_t nb = a.bytes[i];
-    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tid = (uint64_t)(blockIdx.x - a.first[i]) * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)(a.first[i + 1] - a.first[i]) * blockDim.x;
     const uint8_t *s = a.src[i];
     uint8_t *d = a.dst[i];
+    auto same = [](auto v) { return v; };
     if (i == 7 || i == 10 || i == 13 || i == 17) {
-        const uint32_t *s4 = (const uint32_t *)s;
-        uint32_t *d4 = (uint32_t *)d;
         const bool remap = a.rg_off || a.rg_limit != 0xffffffffu;
-        for (uint64_t r = tid; r < nb / 4; r += stride) {
-            uint32_t v = s4[r];
-            if (i == 7) v += a.cig_off;
-            else if (i == 10) v += a.base_off;
-            else if (i == 13) v += a.qual_off;
-            else if (remap) v = v < a.rg_limit ? v + a.rg_off : 0xffffffffu;
-            d4[r] = v;
+        const uint32_t off = i == 7 ? a.cig_off : (i == 10 ? a.base_off : (i == 13 ? a.qual_off : 0u));
+        const uint32_t rg_off = a.rg_off, rg_limit = a.rg_limit;
+        if (i == 17) {
+            append_copy((const uint32_t *)s, (uint32_t *)d, nb / 4, tid, stride, [&](uint32_t v) {
+                return remap ? (v < rg_limit ? v + rg_off : 0xffffffffu) : v;
+            });
+        } else {
+            append_copy((const uint32_t *)s, (uint32_t *)d, nb / 4, tid, stride, [&](uint32_t v) { return v + off; });
         }
         return;
     }
     const uintptr_t al = (uintptr_t)s | (uintptr_t)d;
     if ((al & 15) == 0) {
-        const uint4 *s16 = (const uint4 *)s;
-        uint4 *d16 = (uint4 *)d;
-        for (uint64_t r = tid; r < nb / 16; r += stride) d16[r] = s16[r];
+        append_copy((const uint4 *)s, (uint4 *)d, nb / 16, tid, stride, same);
         for (uint64_t r = nb / 16 * 16 + tid; r < nb; r += stride) d[r] = s[r];
         return;
     }
     if ((al & 3) == 0) {
-        const uint32_t *s4 = (const uint32_t *)s;
-        uint32_t *d4 = (uint32_t *)d;
-        for (uint64_t r = tid; r < nb / 4; r += stride) d4[r] = s4[r];
+        append_copy((const uint32_t *)s, (uint32_t *)d, nb / 4, tid, stride, same);
         for (uint64_t r = nb / 4 * 4 + tid; r < nb; r += stride) d[r] = s[r];
         return;
     }
@@ -946,9 +969,21 @@ __global__ void __launch_bounds__(256) k_append(AppendArgs a)
     const uint8_t *sw = s + head;
     uint32_t *dw = (uint32_t *)(d + head);
     const uint64_t words = (nb - head) / 4;
-    for (uint64_t r = tid; r < words; r += stride) {
-        const uint8_t *p = sw + 4 * r;
-        dw[r] = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+    for (uint64_t r0 = tid; r0 < words; r0 += APPEND_UNROLL * stride) {
+        uint32_t w[APPEND_UNROLL];
+#pragma unroll
+        for (int u = 0; u < APPEND_UNROLL; u++) {
+            const uint64_t r = r0 + u * stride;
+            if (r < words) {
+                const uint8_t *p = sw + 4 * r;
+                w[u] = (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < APPEND_UNROLL; u++) {
+            const uint64_t r = r0 + u * stride;
+            if (r < words) dw[r] = w[u];
+        }
     }
     for (uint64_t r = head + 4 * words + tid; r < nb; r += stride) d[r] = s[r];
 }
@@ -986,22 +1021,24 @@ extern "C" hipError_t fpd_check_batch(const fpd::BatchDev *b, uint32_t rg_limit,
     return hipGetLastError();
 }
 
+// block_bytes: bytes of an array per workgroup (at most 64 workgroups per array)
 extern "C" hipError_t fpd_append(const void *const *src, void *const *dst, const uint64_t *bytes, uint32_t cig_off,
                                  uint32_t base_off, uint32_t qual_off, uint32_t rg_off, uint32_t rg_limit,
-                                 hipStream_t st)
+                                 uint64_t block_bytes, hipStream_t st)
 {
     fpd::AppendArgs a{};
-    uint64_t most = 0;
+    uint32_t blocks = 0;
     for (int i = 0; i < 18; i++) {
         a.src[i] = (const uint8_t *)src[i];
         a.dst[i] = (uint8_t *)dst[i];
         a.bytes[i] = src[i] && dst[i] ? bytes[i] : 0;
-        most = std::max(most, a.bytes[i]);
+        a.first[i] = blocks;
+        if (a.bytes[i]) blocks += (uint32_t)std::min<uint64_t>(64, (a.bytes[i] + block_bytes - 1) / block_bytes);
     }
-    if (most == 0) return hipSuccess;
+    a.first[18] = blocks;
+    if (blocks == 0) return hipSuccess;
     a.cig_off = cig_off; a.base_off = base_off; a.qual_off = qual_off; a.rg_off = rg_off; a.rg_limit = rg_limit;
-    const uint32_t gx = (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, most / 16384));
-    hipLaunchKernelGGL(fpd::k_append, dim3(gx, 18), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(fpd::k_append, dim3(blocks), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

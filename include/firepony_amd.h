@@ -100,10 +100,10 @@ typedef struct fp_options {
     uint32_t serve_hbm_budget_mib;   /* serving: HBM admission budget in MiB for the open jobs'
                                         resident tables and postprocess reserve; 0 = the
                                         read-group slots alone bound admission          */
-    uint32_t staging_buffers;        /* batched engine: 2 or 3 staging buffers (0 = 3).
+    uint32_t staging_buffers;        /* batched engine: 2 to 4 staging buffers (0 = 3).
                                         A buffer refills once its previous batch has
-                                        run; with three, host batches copy in while the
-                                        other two buffers' kernels run               */
+                                        run; with three or more, host batches copy in
+                                        while the other buffers' kernels run          */
 } fp_options;
 
 /* how covariate observations reach the resident tables (identical results) */
