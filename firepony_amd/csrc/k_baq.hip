@@ -34,15 +34,23 @@ static constexpr double GEP = 0.1;
 // compile-time form of a row: Steady<true> rows carry no band or N masks
 template <bool B> struct Steady { static constexpr bool value = B; };
 
+// Starting guess in [4, 100] for the MAP quality of y > 0: -4.343 ln(y) + 0.499 in single precision
+// (one hardware log2, -4.343 ln 2 = -3.0103; y below the float range gives +inf, clamped to 100)
+__device__ inline int map_guess(double y)
+{
+    const float g = -3.0103f * __builtin_amdgcn_logf((float)y) + 0.499f;
+    return (int)fminf(fmaxf(g, 4.0f), 100.0f);
+}
+
 // MAP quality from y = 1 - max/sum (baq.cu:673-674).  The exact value is the
 // largest v in [5, 101] with y <= T[v] (T decreasing, computed with the host
-// libm); the device log only provides the starting guess, which the threshold
-// comparisons then correct, so the result never depends on the device log.
+// libm); the guess only provides the starting point, which the threshold
+// comparisons then correct (from any start), so the result never depends on
+// the device's logarithm.
 __device__ inline uint8_t map_quality(double y)
 {
     if (y == 0.0) return 4;                 // log(0): (int)+inf is INT_MIN on the host
-    int k = (int)(-4.343 * log(y) + 0.499);
-    k = k < 4 ? 4 : (k > 101 ? 101 : k);
+    int k = map_guess(y);
     while (k >= 5 && !(y <= c_map_thr[k])) k--;
     while (k + 1 <= 101 && y <= c_map_thr[k + 1]) k++;
     if (k <= 4) return 4;
@@ -475,8 +483,7 @@ __global__ void __launch_bounds__(256, WPE) k_baq_band(BaqArgs a)
     // MAP quality through the thresholds (largest v in [5, 101] with y <= T[v])
     auto mapq = [&](double y) -> uint8_t {
         if (y == 0.0) return 4;
-        int k = (int)(-4.343 * log(y) + 0.499);
-        k = k < 4 ? 4 : (k > 100 ? 100 : k);
+        int k = map_guess(y);
         const double t0 = s_thr[k], t1 = s_thr[k + 1];
         if (k >= 5 && !(y <= t0)) {
             k--;
@@ -977,6 +984,8 @@ __global__ void __launch_bounds__(256, 1) k_baq_ckpt(BaqArgs a)
     auto pin = [](auto &v) { asm volatile("" : "+v"(v)); };
     auto mapq = [&](double y) -> uint8_t {
         if (y == 0.0) return 4;
+        // the double-precision guess here: with the single-precision one (map_guess) this kernel's
+        // register allocation goes past 512 into spills
         int k = (int)(-4.343 * log(y) + 0.499);
         k = k < 4 ? 4 : (k > 100 ? 100 : k);
         const double t0 = s_thr[k], t1 = s_thr[k + 1];
