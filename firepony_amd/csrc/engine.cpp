@@ -234,7 +234,7 @@ struct fp_engine {
     bool is_setup = false;
     std::vector<std::string> rg_names;
     uint32_t num_rg = 0;
-    DevBuf g_word_offset, g_length, g_words, g_foff, g_fstart, g_fstop, g_mel;
+    DevBuf g_word_offset, g_length, g_words, g_foff, g_fstart, g_fstop, g_mel, g_boff, g_slb, g_ssb;
     fpd::GenomeDev genome{};
 
     // batch buffers (host batches are copied here)
@@ -1834,6 +1834,36 @@ fp_status fp_engine_setup(fp_engine *e, const fp_header_view *h, const fp_refere
         HIP_TRY(hipMemcpy(e->g_fstop.p, v->feature_stop, 4 * nf, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(e->g_mel.p, mel.data(), 4 * nf, hipMemcpyHostToDevice));
     }
+    // the bucket index of the sites: the prep kernel's lower bound over max_end_left and upper bound
+    // over the starts search one bucket's sites instead of the contig's
+    std::vector<uint64_t> boff(v->num_sequences + 1, 0);
+    std::vector<uint32_t> slb, ssb;
+    for (uint32_t s = 0; s < v->num_sequences; s++) {
+        const uint64_t a = v->feature_offset[s], b = v->feature_offset[s + 1];
+        uint64_t span = s < r->num_sequences ? r->length[s] : 0;
+        if (b > a) span = std::max<uint64_t>(span, std::max<uint64_t>(mel[b - 1], v->feature_start[b - 1]));
+        const uint64_t nb = (span >> fpd::SITE_BUCKET_BITS) + 2;   // the last bucket: +infinity
+        boff[s] = slb.size();
+        uint64_t il = a, is = a;
+        for (uint64_t j = 0; j + 1 < nb; j++) {
+            const uint64_t t = j << fpd::SITE_BUCKET_BITS;
+            while (il < b && mel[il] < t) il++;
+            while (is < b && v->feature_start[is] < t) is++;
+            slb.push_back((uint32_t)(il - a));
+            ssb.push_back((uint32_t)(is - a));
+        }
+        slb.push_back((uint32_t)(b - a));
+        ssb.push_back((uint32_t)(b - a));
+    }
+    boff[v->num_sequences] = slb.size();
+    HIP_TRY(e->g_boff.ensure(8ull * boff.size()));
+    HIP_TRY(e->g_slb.ensure(4ull * slb.size() + 4));
+    HIP_TRY(e->g_ssb.ensure(4ull * ssb.size() + 4));
+    HIP_TRY(hipMemcpy(e->g_boff.p, boff.data(), 8ull * boff.size(), hipMemcpyHostToDevice));
+    if (!slb.empty()) {
+        HIP_TRY(hipMemcpy(e->g_slb.p, slb.data(), 4ull * slb.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->g_ssb.p, ssb.data(), 4ull * ssb.size(), hipMemcpyHostToDevice));
+    }
     fpd::GenomeDev &g = e->genome;
     g.num_contigs = r->num_sequences;
     g.word_offset = e->g_word_offset.as<uint64_t>();
@@ -1844,6 +1874,9 @@ fp_status fp_engine_setup(fp_engine *e, const fp_header_view *h, const fp_refere
     g.feature_start = e->g_fstart.as<uint32_t>();
     g.feature_stop = e->g_fstop.as<uint32_t>();
     g.max_end_left = e->g_mel.as<uint32_t>();
+    g.bucket_offset = e->g_boff.as<uint64_t>();
+    g.site_lb = e->g_slb.as<uint32_t>();
+    g.site_sb = e->g_ssb.as<uint32_t>();
 
     rc = alloc_tables(e);
     if (rc != FP_OK) return rc;
@@ -2194,7 +2227,7 @@ void fp_engine_destroy(fp_engine *e)
     if (e->side) (void)hipStreamSynchronize(e->side);
     if (e->copy) (void)hipStreamSynchronize(e->copy);
     DevBuf *bufs[] = {&e->g_word_offset, &e->g_length, &e->g_words, &e->g_foff, &e->g_fstart, &e->g_fstop,
-                      &e->g_mel, &e->counters, &e->scratch, &e->tk, &e->tv, &e->shist, &e->stot, &e->sctr};
+                      &e->g_mel, &e->g_boff, &e->g_slb, &e->g_ssb, &e->counters, &e->scratch, &e->tk, &e->tv, &e->shist, &e->stot, &e->sctr};
     for (DevBuf *b : bufs) b->release();
     for (int t = 0; t < 3; t++) { e->pk[t].release(); e->pv[t].release(); }
     e->ps.release();

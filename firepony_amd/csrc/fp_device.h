@@ -98,7 +98,15 @@ struct GenomeDev {
     const uint32_t *feature_start;
     const uint32_t *feature_stop;
     const uint32_t *max_end_left;
+    // per-contig bucket index of the known sites (buckets of 2^SITE_BUCKET_BITS reference
+    // positions): bucket j of contig c holds, relative to the contig's first site, the first site
+    // whose max_end_left is >= j * 2^bits (site_lb) and the first whose start is >= j * 2^bits
+    // (site_sb); a contig's last entry stands for +infinity (its site count)
+    const uint64_t *bucket_offset;   // num_variant_contigs + 1
+    const uint32_t *site_lb;
+    const uint32_t *site_sb;
 };
+static constexpr uint32_t SITE_BUCKET_BITS = 9;
 
 struct ReadStateDev {
     uint8_t *status;

@@ -424,12 +424,16 @@ __device__ void known_sites(const PrepArgs &a, uint32_t r, const Windows &w, Mas
         const uint32_t *fstop = a.g.feature_stop + f0;
         const uint32_t *mel = a.g.max_end_left + f0;
         const uint32_t qx = astart + fwx, qy = astart + fwy;
+        // the contig's bucket index brackets both searches: the bound for a position in bucket j lies
+        // between the bucket's entry and the next one (the last entry stands for +infinity)
+        const uint64_t b0 = a.g.bucket_offset[chrom];
+        const uint32_t last = (uint32_t)(a.g.bucket_offset[chrom + 1] - b0) - 2;
+        const uint32_t jx = qx >> SITE_BUCKET_BITS < last ? qx >> SITE_BUCKET_BITS : last;
+        const uint32_t jy = qy >> SITE_BUCKET_BITS < last ? qy >> SITE_BUCKET_BITS : last;
         // lower_bound over max_end_point_left (primitives/algorithms.h:39-70)
         uint32_t vx;
-        if (nf == 0 || qx < mel[0]) vx = 0;
-        else if (mel[nf - 1] < qx) vx = nf;
-        else {
-            uint32_t lo = 0, cnt = nf;
+        {
+            uint32_t lo = a.g.site_lb[b0 + jx], cnt = a.g.site_lb[b0 + jx + 1] - lo;
             while (cnt > 0) {
                 const uint32_t h = cnt / 2;
                 if (mel[lo + h] < qx) { lo = lo + h + 1; cnt -= h + 1; }
@@ -439,7 +443,7 @@ __device__ void known_sites(const PrepArgs &a, uint32_t r, const Windows &w, Mas
         }
         if (vx < nf) {
             // upper_bound over feature_start (primitives/algorithms.h:77-100)
-            uint32_t lo = 0, cnt = nf;
+            uint32_t lo = a.g.site_sb[b0 + jy], cnt = a.g.site_sb[b0 + jy + 1] - lo;
             while (cnt > 0) {
                 const uint32_t h = cnt / 2;
                 if (!(qy < fstart[lo + h])) { lo = lo + h + 1; cnt -= h + 1; }
