@@ -115,7 +115,6 @@ struct PrepArgs {
     ReadStateDev s;
     uint32_t *batch;      // the batch's counter words (BW_ACTIVE .. BW_BAD_RG)
     uint32_t num_rg;      // read groups of the tables (active reads beyond them are counted as errors)
-    uint32_t lds_len;     // LDS flag row per read (multiple of 8); longer reads take the lane path
 };
 
 // filter_mapq + filter_if_any_set + filter_if_cigar_malformed (read_filters.cu:43-258),
@@ -158,7 +157,7 @@ struct Windows {
 };
 
 // filter_invalid_reads + filter_malformed_reads (read_filters.cu:43-154); false: the read is filtered
-__device__ bool prep_filter(const PrepArgs &a, uint32_t r)
+__device__ __forceinline__ bool prep_filter(const PrepArgs &a, uint32_t r)
 {
     const BatchDev &b = a.b;
     const uint32_t L = b.read_len[r];
@@ -205,7 +204,7 @@ __device__ bool prep_filter(const PrepArgs &a, uint32_t r)
     return ok;
 }
 
-__device__ Windows prep_windows(const BatchDev &b, uint32_t r)
+__device__ __forceinline__ Windows prep_windows(const BatchDev &b, uint32_t r)
 {
     const uint32_t L = b.read_len[r];
     const uint32_t *ops = b.cigars + b.cigar_start[r];
@@ -408,7 +407,7 @@ __device__ uint16_t error_walk_lane(const PrepArgs &a, uint32_t r, const Windows
 // known-site masking (snp_filter.cu:162-395): mask(d0, d1) masks read offsets d0..d1 (inside the
 // read window)
 template <class Mask>
-__device__ void known_sites(const PrepArgs &a, uint32_t r, const Windows &w, Mask mask)
+__device__ __forceinline__ void known_sites(const PrepArgs &a, uint32_t r, const Windows &w, Mask mask)
 {
     const BatchDev &b = a.b;
     const uint32_t *ops = b.cigars + b.cigar_start[r];
@@ -513,7 +512,7 @@ __device__ void known_sites(const PrepArgs &a, uint32_t r, const Windows &w, Mas
 
 // HMM reference window + band width (baq.cu:89-187); RS_ACTIVE, or RS_BAQ_DROPPED when the window
 // runs past the contig
-__device__ uint8_t hmm_window(const PrepArgs &a, uint32_t r, const Windows &w, uint16_t nerr)
+__device__ __forceinline__ uint8_t hmm_window(const PrepArgs &a, uint32_t r, const Windows &w, uint16_t nerr)
 {
     const BatchDev &b = a.b;
     const ReadStateDev &s = a.s;
@@ -554,7 +553,7 @@ __device__ uint8_t hmm_window(const PrepArgs &a, uint32_t r, const Windows &w, u
 
 // One read by one lane, every per-base flag read-modify-written in HBM: the path of waves that hold
 // a read longer than the LDS rows.  Returns the read's status.
-__device__ uint8_t prep_read_lane(const PrepArgs &a, uint32_t r, uint16_t &nerr)
+__device__ __forceinline__ uint8_t prep_read_lane(const PrepArgs &a, uint32_t r, uint16_t &nerr)
 {
     const BatchDev &b = a.b;
     const ReadStateDev &s = a.s;
@@ -692,14 +691,15 @@ __device__ uint32_t error_walk(const PrepRead &p, const BaseLoads<NC> &x, uint8_
 }
 
 // The per-base pass over the wave's reads that passed the filters (read_filters.cu:376-438,
-// cigar.cu:612-733), one read at a time across the wave: flags into the read's LDS row and its
-// error count (SNPs inside the read window plus the walk's insertions and deletions) into nerr of
-// the read's lane.  Software-pipelined: the next read's bases, qualities and ops are in flight
-// while the current read's reference words return and are compared, so a read costs about one
-// round trip to memory.
+// cigar.cu:612-733), one read at a time across the wave: the read's flags, with the known-site
+// ranges of its lane (ks_n of them, up to two packed as d0 | d1 << 16 in ks0 and ks1) masked in,
+// written once to HBM (padded to a whole number of 8-byte words), and its error count (SNPs inside
+// the read window plus the walk's insertions and deletions) into nerr of the read's lane.
+// Software-pipelined: the next read's bases, qualities and ops are in flight while the current
+// read's reference words return and are compared, so a read costs about one round trip to memory.
 template <int NC, class ReadOf>
-__device__ void prep_pass(const PrepArgs &a, uint64_t okmask, ReadOf read_of, uint8_t *rows, uint32_t RL,
-                          uint16_t &nerr)
+__device__ __forceinline__ void prep_pass(const PrepArgs &a, uint64_t okmask, ReadOf read_of, uint32_t ks_n, uint32_t ks0,
+                          uint32_t ks1, uint16_t &nerr)
 {
     const uint32_t lane = threadIdx.x & 63;
     uint64_t m = okmask;
@@ -724,7 +724,9 @@ __device__ void prep_pass(const PrepArgs &a, uint64_t okmask, ReadOf read_of, ui
         const int jn = more ? __builtin_ctzll(m) : j;
         const PrepRead pn = read_of(jn);
         const BaseLoads<NC> nxt = load_bases<NC>(a.b, pn);   // in flight during the comparisons
-        uint8_t *row = rows + (uint32_t)j * RL;
+        const uint32_t kn = rl(ks_n, j), k0 = rl(ks0, j), k1 = rl(ks1, j);
+        uint8_t *row = a.s.base_flags + p.rs;
+        const bool words = (p.rs & 3) == 0;   // 4-byte stores of 4 lanes' flags, else one byte per lane
 #pragma unroll
         for (int c = 0; c < NC; c++) {
             const uint32_t i = lane + 64 * c;
@@ -738,7 +740,16 @@ __device__ void prep_pass(const PrepArgs &a, uint64_t okmask, ReadOf read_of, ui
                 }
             }
             cnt += (uint32_t)__popcll(__ballot(snp_in));
-            if (i < p.pj) row[i] = f[c];
+            const bool ks = (kn > 0 && i >= lo16(k0) && i <= hi16(k0)) || (kn > 1 && i >= lo16(k1) && i <= hi16(k1));
+            if (ks) f[c] |= BF_MASKED;
+            // lanes 4k gather positions 4k .. 4k + 3 (pj is a multiple of 8, so a word below pj is whole)
+            const uint32_t g2 = (uint32_t)f[c] | ((uint32_t)__shfl_down((int)f[c], 1) << 8);
+            const uint32_t g4 = g2 | ((uint32_t)__shfl_down((int)g2, 2) << 16);
+            if (words) {
+                if ((lane & 3) == 0 && i < p.pj) *(uint32_t *)(row + i) = g4;
+            } else if (i < p.pj) {
+                row[i] = f[c];
+            }
         }
         if ((int)lane == j) nerr = (uint16_t)cnt;
         if (!more) break;
@@ -750,12 +761,14 @@ __device__ void prep_pass(const PrepArgs &a, uint64_t okmask, ReadOf read_of, ui
 
 // One wave per 64 reads.  The CIGAR walks (filters, windows, known sites, HMM window) run one lane
 // per read; the per-base work (base masks, SNP / insertion / deletion flags) runs one read at a time
-// across the wave, lanes over bases, so its loads are coalesced.  Flags are built in LDS rows of
-// lds_len bytes per read and leave as whole 8-byte words; nothing is read-modify-written in HBM.
-// A wave that holds a read longer than lds_len takes prep_read_lane for all its reads.
+// across the wave, lanes over bases, so its loads and its flag stores are coalesced.  The known-site
+// walk runs first and keeps each read's masked ranges in its lane (two of them; a read with more
+// ORs the rest into its stored flags afterwards, the only read-modify-write in HBM), so a read's
+// flags are complete when the pass writes them.  No LDS: the wave count per SIMD is set by the
+// registers alone.  A wave that holds a read longer than 256 bases takes prep_read_lane for all its
+// reads.
 __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
 {
-    extern __shared__ __align__(16) uint8_t prep_rows[];
     const BatchDev &b = a.b;
     const ReadStateDev &s = a.s;
     const uint32_t lane = threadIdx.x;
@@ -765,11 +778,10 @@ __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
     const bool valid = r < n;
     const uint32_t L = valid ? b.read_len[r] : 0u;
     const uint32_t rs = valid ? b.read_start[r] : 0u;
-    const uint32_t RL = a.lds_len;
 
     uint8_t st = RS_FILTERED;
     uint16_t nerr = 0;
-    if (!wave_all(L <= RL)) {
+    if (!wave_all(L <= 256)) {
         if (valid) st = prep_read_lane(a, r, nerr);
     } else {
         const bool ok = valid && prep_filter(a, r);
@@ -788,12 +800,22 @@ __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
         const uint32_t clen = has_contig ? a.g.length[chrom] : 0u;
         const uint32_t cwo = has_contig ? a.g.word_offset[chrom] : 0u;
 
-        // rows of valid reads that the filters dropped are all zero
+        // flags of valid reads that the filters dropped are all zero
         for (uint64_t m = vmask & ~okmask; m; m &= m - 1) {
             const int j = __builtin_ctzll(m);
             const uint32_t pj = (rl(L, j) + 7) & ~7u;
-            uint8_t *row = prep_rows + (uint32_t)j * RL;
+            uint8_t *row = s.base_flags + rl(rs, j);
             for (uint32_t i = lane; i < pj; i += 64) row[i] = 0;
+        }
+        // known-site ranges of each lane's read, first (read offsets d0 | d1 << 16)
+        uint32_t ks_n = 0, ks0 = 0, ks1 = 0;
+        if (ok) {
+            known_sites(a, r, w, [&](uint32_t d0, uint32_t d1) {
+                const uint32_t v = d0 | (d1 << 16);
+                ks1 = ks_n == 1 ? v : ks1;   // selects: an indexed store would go through scratch memory
+                ks0 = ks_n == 0 ? v : ks0;
+                ks_n++;
+            });
         }
         auto read_of = [&](int j) {
             PrepRead p;
@@ -805,15 +827,26 @@ __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
             p.L = rl(L, j); p.pj = (p.L + 7) & ~7u;
             return p;
         };
-        if (!wave_any(ok && L > 128)) prep_pass<2>(a, okmask, read_of, prep_rows, RL, nerr);
-        else prep_pass<4>(a, okmask, read_of, prep_rows, RL, nerr);
-        __syncthreads();
+        if (!wave_any(ok && L > 128)) prep_pass<2>(a, okmask, read_of, ks_n, ks0, ks1, nerr);
+        else prep_pass<4>(a, okmask, read_of, ks_n, ks0, ks1, nerr);
 
-        if (ok) {
-            uint8_t *own = prep_rows + lane * RL;
-            known_sites(a, r, w, [&](uint32_t d0, uint32_t d1) { for (uint32_t d = d0; d <= d1; d++) own[d] |= BF_MASKED; });
-            st = hmm_window(a, r, w, nerr);
+        // a read with more than two known-site ranges ORs the rest into its stored flags (atomically, on
+        // the aligned words that hold them, after the wave's stores have completed)
+        if (wave_any(ks_n > 2)) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __threadfence();
+            if (ks_n > 2) {
+                uint32_t k = 0;
+                known_sites(a, r, w, [&](uint32_t d0, uint32_t d1) {
+                    if (k++ < 2) return;
+                    for (uint32_t d = d0; d <= d1; d++) {
+                        const uintptr_t addr = (uintptr_t)(s.base_flags + rs + d);
+                        atomicOr((uint32_t *)(addr & ~(uintptr_t)3), (uint32_t)BF_MASKED << (8 * (addr & 3)));
+                    }
+                });
+            }
         }
+        if (ok) st = hmm_window(a, r, w, nerr);
         if (valid) {
             s.rwc[r] = ok ? pack2(w.rwx, w.rwy) : 0u;
             s.noins[r] = ok ? pack2(w.nix, w.niy) : 0u;
@@ -822,20 +855,6 @@ __global__ void __launch_bounds__(64) k_prep(PrepArgs a)
             s.hqw[r] = 0;
             s.status[r] = st;
             if (!ok) { s.hmmw[r] = 0; s.bandwidth[r] = 0; }
-        }
-        __syncthreads();
-
-        // flags out: two reads per step, 32 lanes x 8 bytes each
-        for (uint32_t jj = 0; jj < 64; jj += 2) {
-            const uint32_t j = jj + (lane >> 5), k = (lane & 31) * 8;
-            const uint32_t pj = ((uint32_t)__shfl((int)L, (int)j) + 7) & ~7u;
-            const uint32_t rsj = (uint32_t)__shfl((int)rs, (int)j);
-            if (k < pj) {
-                const uint64_t v = *(const uint64_t *)(prep_rows + j * RL + k);
-                uint8_t *dst = s.base_flags + rsj + k;
-                if ((rsj & 7) == 0) *(uint64_t *)dst = v;
-                else for (int t = 0; t < 8; t++) dst[t] = (uint8_t)(v >> (8 * t));
-            }
         }
     }
 
@@ -866,10 +885,8 @@ extern "C" hipError_t fpd_launch_prep(const fpd::BatchDev *b, const fpd::GenomeD
     a.b = *b; a.g = *g; a.s = *s; a.batch = batch; a.num_rg = num_rg;
     const uint32_t n = b->num_reads;
     if (n == 0) return hipSuccess;
-    // flag rows sized for the longest expected read (256 when unknown)
-    uint32_t rows = max_read_len ? std::min<uint32_t>(max_read_len, 256u) : 256u;
-    a.lds_len = (rows + 7) & ~7u;
-    hipLaunchKernelGGL(fpd::k_prep, dim3((n + 63) / 64), dim3(64), (size_t)a.lds_len * 64, st, a);
+    (void)max_read_len;
+    hipLaunchKernelGGL(fpd::k_prep, dim3((n + 63) / 64), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

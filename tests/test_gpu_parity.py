@@ -159,6 +159,17 @@ def test_waves_taking_several_chunks(tmp_path, ckpt):
     run_parity(tmp_path, g, rs, 150000, 150000, frac_error_f64=1, baq_waves_per_simd=1, baq_checkpoint_rows=ckpt)
 
 
+def test_dense_known_sites(tmp_path):
+    """A known site every 20 reference positions (a third of them multi-base): most reads overlap three
+    or more sites, so the prep kernel masks the ranges beyond its two in-register ones by atomic ORs
+    into the stored flags.  Stage by stage against the oracle (base flags included)."""
+    say("parity: dense known sites (one per 20 bp), reads with three or more masked ranges")
+    g, rs, n = synth.small_workload(seed=17, num_reads=3000, edge=0.1, lengths=(101, 151), read_groups=2,
+                                    contigs=(40000, 20000), sites=3000, clip=0.3, indel=0.2, multi_sites=0.3)
+    run_parity(tmp_path, g, rs, n, 1000)
+    run_parity(tmp_path, g, rs, n, 1000, per_batch=False, engine_kind=abi.FP_ENGINE_BATCHED, max_batch_reads=4096)
+
+
 def test_long_reads(tmp_path):
     say("parity: reads longer than 256 bases next to short ones (per-lane flag path of the prep kernel)")
     g, rs, n = synth.small_workload(seed=13, num_reads=1500, edge=0.1, lengths=(90, 300, 420), read_groups=2,
