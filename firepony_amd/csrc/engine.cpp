@@ -978,11 +978,11 @@ static double host_ms()
 // and the BAQ stage of the last one begun: its start event t0 follows the previous buffer's BAQ
 // kernels and its own front stage) means the BAQ stage is about to run dry; a buffer flushed then
 // has its front stage done before those kernels end.  After a sync (the start of a stream) an eighth
-// of a buffer is enough; after that, twice the last buffer's reads, so that the buffers grow to full
-// within a few turns and a full buffer waits while the device works (buffers that stayed small would
-// each arrive as late as the last one, and a buffer refills only once its last batch has run).  A
-// device left idle for 2 ms (a stream that arrives slower than the device runs it) again takes an
-// eighth of a buffer.
+// of a buffer is enough; after that, 1.5 times the last buffer's reads (about the ratio of the host
+// link's batch rate to the device's), so that the buffers grow to full within a few turns and a full
+// buffer waits while the device works (buffers that stayed small would each arrive as late as the
+// last one, and a buffer refills only once its last batch has run).  A device left idle for 0.5 ms
+// (a stream that arrives slower than the device runs it) again takes an eighth of a buffer.
 static uint32_t batched_early_need(fp_engine *e)
 {
     const uint32_t never = 0xffffffffu, eighth = e->cap_reads / 8;
@@ -1001,9 +1001,9 @@ static uint32_t batched_early_need(fp_engine *e)
     } else {
         const double now = host_ms();
         if (e->idle_since < 0) e->idle_since = now;
-        if (now - e->idle_since > 2.0) return eighth;
+        if (now - e->idle_since > 0.5) return eighth;
     }
-    return (uint32_t)std::min<uint64_t>(e->cap_reads, std::max<uint64_t>(eighth, 2ull * e->last_flush_n));
+    return (uint32_t)std::min<uint64_t>(e->cap_reads, std::max<uint64_t>(eighth, 3ull * e->last_flush_n / 2));
 }
 
 fp_status batched_poll(fp_engine *e)
