@@ -513,6 +513,19 @@ __device__ inline int next_zero(const uint64_t *m, int j)
     return res;
 }
 
+// lowest index >= j whose bit is set, or 256
+__device__ inline int next_one(const uint64_t *m, int j)
+{
+    int res = 256;
+#pragma unroll
+    for (int w = 3; w >= 0; w--) {
+        const int lo = j - 64 * w;
+        const uint64_t z = lo >= 64 ? 0ull : (lo <= 0 ? m[w] : (m[w] & (~0ull << (lo & 63))));
+        if (z) res = w * 64 + __builtin_ctzll(z);
+    }
+    return res;
+}
+
 // number of set bits in [a, b]
 __device__ inline int pop_range(const uint64_t *m, int a, int b)
 {
@@ -689,41 +702,40 @@ __device__ __forceinline__ void cov_read_wave(const CovArgs &a, const LdsAcc &L,
     __builtin_amdgcn_wave_barrier();
     auto base = [&](int i) -> uint8_t { return stage[i - (int)rwx]; };
 
-    // bits of window position j = lane + 64c and of its neighbours, with c known at compile time
-    // after unrolling (one shift of a wave-uniform word instead of a lookup over four words)
+    // bit of window position j = lane + 64c, with c known at compile time after unrolling (one
+    // shift of a wave-uniform word instead of a lookup over four words)
     auto bit_at = [&](const uint64_t *M, int c) -> bool { return (M[c] >> lane) & 1; };
-    auto bit_next = [&](const uint64_t *M, int c) -> bool {
-        const uint64_t up = c < 3 ? M[c < 3 ? c + 1 : 3] : 0ull;
-        return lane < 63 ? ((M[c] >> ((lane + 1) & 63)) & 1) : (up & 1);
-    };
-    auto bit_prev = [&](const uint64_t *M, int c) -> bool {
-        const uint64_t down = c > 0 ? M[c > 0 ? c - 1 : 0] : 0ull;
-        return lane > 0 ? ((M[c] >> ((lane - 1) & 63)) & 1) : (down >> 63);
-    };
 
-    // GATK block-averaged fractional errors of window position j = lane + 64c
-    // (calculateFractionalErrorArray)
-    auto frac_at = [&](int j, int c, double &vs, double &vi, double &vd) {
-        int s0 = -1, e = -1;
-        if (bit_at(IB, c)) {
-            const int bs = prev_zero(IB, j) + 1;
-            const int be = next_zero(IB, j) - 1;
-            e = be + 1 < len ? be + 1 : be;
-            s0 = bs - 1 > 0 ? bs - 1 : 0;
-        } else if (j + 1 < len && bit_next(IB, c)) {
-            const int be = next_zero(IB, j + 1) - 1;
-            e = be + 1 < len ? be + 1 : be;
-            s0 = j;
-        } else if (j > 0 && bit_prev(IB, c)) {
-            const int bs = prev_zero(IB, j - 1) + 1;
-            s0 = bs - 1 > 0 ? bs - 1 : 0;
-            e = j;
+    // GATK block-averaged fractional errors (calculateFractionalErrorArray): a BAQ block [bs, be]
+    // (positions whose tag is not 64) averages the error bits over [max(bs - 1, 0), min(be + 1,
+    // len - 1)] and every position of that range takes the average; a position in two ranges takes
+    // the later block's, as the reference assigns the ranges in order.  The blocks are walked in
+    // order with wave-uniform arithmetic on the masks; each lane keeps the size and error counts of
+    // the last range over its positions j = lane + 64c.
+    uint32_t rn[NC], rc[NC];   // range size (0: none) and its SNP | INS << 10 | DEL << 20 error counts
+#pragma unroll
+    for (int c = 0; c < NC; c++) { rn[c] = 0; rc[c] = 0; }
+    for (int pos = next_one(IB, 0); pos < len;) {
+        const int be = next_zero(IB, pos) - 1;
+        const int s0 = pos > 0 ? pos - 1 : 0, e = be + 1 < len ? be + 1 : be;
+        const uint32_t n = (uint32_t)(e - s0 + 1);
+        const uint32_t cnt = (uint32_t)pop_range(ES, s0, e) | ((uint32_t)pop_range(EI_, s0, e) << 10) |
+                             ((uint32_t)pop_range(ED, s0, e) << 20);
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const int j = lane + 64 * c;
+            const bool in = j >= s0 && j <= e;
+            rn[c] = in ? n : rn[c];
+            rc[c] = in ? cnt : rc[c];
         }
-        if (s0 >= 0) {
-            const double n = (double)(e - s0 + 1);
-            vs = (double)pop_range(ES, s0, e) / n;
-            vi = (double)pop_range(EI_, s0, e) / n;
-            vd = (double)pop_range(ED, s0, e) / n;
+        pos = next_one(IB, be + 2);   // be + 1 is outside the block
+    }
+    auto frac_at = [&](int j, int c, double &vs, double &vi, double &vd) {
+        if (rn[c]) {
+            const double n = (double)rn[c];
+            vs = (double)(rc[c] & 1023u) / n;
+            vi = (double)((rc[c] >> 10) & 1023u) / n;
+            vd = (double)(rc[c] >> 20) / n;
         } else {
             vs = bit_at(ES, c) ? 1.0 : 0.0;
             vi = bit_at(EI_, c) ? 1.0 : 0.0;
@@ -754,21 +766,13 @@ __device__ __forceinline__ void cov_read_wave(const CovArgs &a, const LdsAcc &L,
 
     uint32_t nid = 0;
     uint64_t i0 = 0, i1 = 0, i2 = 0, d0 = 0, d1 = 0, d2 = 0;
-    // a read without BAQ blocks has every error value 0 or 1: its bits (a wave-uniform test)
-    const bool blocks = (IB[0] | IB[1] | IB[2] | IB[3]) != 0;
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int j = lane + 64 * c;
         bool em = false;
         if (j < len) {
             double vs, vi, vd;
-            if (blocks || a.keep_frac) {
-                frac_at(j, c, vs, vi, vd);
-            } else {
-                vs = bit_at(ES, c) ? 1.0 : 0.0;
-                vi = bit_at(EI_, c) ? 1.0 : 0.0;
-                vd = bit_at(ED, c) ? 1.0 : 0.0;
-            }
+            frac_at(j, c, vs, vi, vd);
             const uint16_t bp = (uint16_t)(rwx + j);
             if (!(fl[c] & BF_MASKED) && !bit_at(SM, c)) {
                 // outside BAQ blocks every value is exactly 0 or 1 (limbs 0, 0 and 0 or 2^26)
