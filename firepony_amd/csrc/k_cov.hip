@@ -25,8 +25,9 @@
  * Each wavefront handles one read at a time with its 64 lanes spread over the
  * read's bases (positions lane, lane+64, lane+128, lane+192 of the clipped
  * window), so every per-base load is coalesced.  The GATK block-averaged
- * fractional errors are evaluated per base from 256-bit ballot masks of the
- * BAQ blocks and error bits; the context keys read neighbouring bases from a
+ * fractional errors come from 256-bit ballot masks of the BAQ blocks and error
+ * bits: one wave-uniform walk over a read's blocks gives each lane the error
+ * counts of the block range over its positions.  The context keys read neighbouring bases from a
  * per-wave LDS copy of the window.  Windows longer than 256 bases take a
  * serial path on one lane.
  *
@@ -153,8 +154,9 @@ __device__ inline void context_keys3(BASE base, uint16_t hx, uint16_t hy, bool n
         const uint32_t c = 31u - (uint32_t)__builtin_clz(b | 1u);
         return neg ? 3u - c : c;
     };
-    const bool v2 = acgt(b0) && avail >= 2 && acgt(b1);
-    const bool v3 = v2 && avail >= 3 && acgt(b2);
+    // bitwise ands: all three tests are cheap, and a short-circuit would branch on the exec mask
+    const bool v2 = acgt(b0) & (avail >= 2) & acgt(b1);
+    const bool v3 = v2 & (avail >= 3) & acgt(b2);
     const uint32_t c0 = code(b0), c1 = code(b1), c2 = code(b2);
     km = v2 ? ((((c0 << 2) | c1) << 4) | 2u) : 2047u;
     kid = v3 ? ((((c0 << 4) | (c1 << 2) | c2) << 4) | 3u) : 2047u;
@@ -247,16 +249,13 @@ static constexpr uint32_t STAGE_BYTES = COV_WAVES * 256;   // per-wave copies of
 template <class BASE>
 __device__ inline void emit_base(const CovArgs &a, const LdsAcc &L, uint32_t rg, uint32_t ncodes, uint16_t flags,
                                  bool neg, BASE base, uint16_t rwx, uint16_t rwy, uint16_t hx, uint16_t hy,
-                                 uint16_t bp, uint32_t q, const Limbs &f0, const Limbs &f1, const Limbs &f2,
-                                 bool id_in_lds)
+                                 uint16_t bp, uint32_t q, const Limbs &f0, const Limbs &f1, const Limbs &f2)
 {
     // RecalTable1: EventTracker(2) | QualityScore(8) | ReadGroup(8)
     if (q < 64) lds_add(L, q, f0);
     else global_add(a.t, 0, 0u | (q << 2) | (rg << 10), 1, f0);
-    if (id_in_lds) {   // the wave path reduces these across lanes itself
-        lds_add(L, 64, f1);
-        lds_add_sums(L, 65, f2);
-    }
+    lds_add(L, 64, f1);
+    lds_add_sums(L, 65, f2);
     // cycle: EventTracker(2) | Cycle(10) | QualityScore(8) | ReadGroup(8)
     uint32_t cm, cid;
     cycle_keys(flags, rwx, rwy, bp, cm, cid);
@@ -323,7 +322,10 @@ __device__ inline void emit_base_wave(const CovArgs &a, const LdsAcc &L, uint32_
     const bool sums = nonzero(f0) || nonzero(f1) || nonzero(f2);
     const bool rare = !qok || !mok || (yid && !yok);
     if (!wave_any(sums || rare)) return;
-    // RecalTable1: EventTracker(2) | QualityScore(8) | ReadGroup(8)
+    // RecalTable1: EventTracker(2) | QualityScore(8) | ReadGroup(8); the I and D keys are the same
+    // for every base of the read (their counts are added once per read)
+    lds_add_sums(L, 64, f1);
+    lds_add_sums(L, 65, f2);
     if (qok) lds_add_sums(L, q, f0);
     else global_add(a.t, 0, 0u | (q << 2) | (rg << 10), 1, f0);
     // cycle: EventTracker(2) | Cycle(10) | QualityScore(8) | ReadGroup(8)
@@ -473,7 +475,7 @@ __device__ __forceinline__ void cov_read_serial(const CovArgs &a, const LdsAcc &
                 else
                     emit_base(a, L, rg, ncodes, flags, neg, base, rwx, rwy, hx, hy, bp, b.quals[qs + bp],
                               to_limbs(fe[0], a.frac_f64), to_limbs(fe[1], a.frac_f64),
-                              to_limbs(fe[2], a.frac_f64), true);
+                              to_limbs(fe[2], a.frac_f64));
             }
         }
         ro = (uint16_t)(ro + l);
@@ -765,7 +767,6 @@ __device__ __forceinline__ void cov_read_wave(const CovArgs &a, const LdsAcc &L,
     }
 
     uint32_t nid = 0;
-    uint64_t i0 = 0, i1 = 0, i2 = 0, d0 = 0, d1 = 0, d2 = 0;
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         const int j = lane + 64 * c;
@@ -786,24 +787,13 @@ __device__ __forceinline__ void cov_read_wave(const CovArgs &a, const LdsAcc &L,
                     f0 = to_limbs(vs, a.frac_f64); f1 = to_limbs(vi, a.frac_f64); f2 = to_limbs(vd, a.frac_f64);
                 }
                 em = true;
-                i0 += f1.p0; i1 += f1.p1; i2 += f1.p2;
-                d0 += f2.p0; d1 += f2.p1; d2 += f2.p2;
                 emit_base_wave(a, L, rg, ncodes, flags, neg, base, rwx, rwy, hx, hy, bp, qv[c], f0, f1, f2);
             }
         }
         nid += (uint32_t)__popcll(__ballot(em));
     }
-    // the quality-table I and D keys are the same for every base of the read: one LDS update per read;
-    // most reads have no insertion or deletion error anywhere and skip the cross-lane sums
-    Limbs ti = {0, 0, 0}, td = {0, 0, 0};
-    if (__ballot((i0 | i1 | i2 | d0 | d1 | d2) != 0)) {
-        ti = {wave_sum(i0), wave_sum(i1), wave_sum(i2)};
-        td = {wave_sum(d0), wave_sum(d1), wave_sum(d2)};
-    }
-    if (lane == 0 && nid) {
-        lds_add_n(L, 64, nid, ti);
-        lds_add_sums(L, 65, td);
-    }
+    // the quality-table I and D keys are the same for every base of the read: one count update per read
+    if (lane == 0 && nid) atomicAdd(&L.cnt[64], nid);
     // the stage is reused by the wave's next read
     __builtin_amdgcn_wave_barrier();
 }
