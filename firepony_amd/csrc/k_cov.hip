@@ -14,7 +14,7 @@
  * launch order or the order atomics land in, and no update waits for a
  * returned value.
  *
- * Work layout: reads are grouped by read group; a workgroup of 8 waves takes a
+ * Work layout: reads are grouped by read group; a workgroup of 12 waves takes a
  * tile of up to 1024 reads of one group and keeps that group's table entries in
  * LDS: every count, and the exact sums of the quality, context and cycle I/D
  * entries.  Only the sums of the cycle table's mismatch entries (few nonzero,
@@ -240,7 +240,7 @@ __device__ inline void global_add(const TablesDev &t, int tab, uint32_t key, uin
 //   [1218, 1218+2*ncodes)  cycle I/D: 2 x cycle codes
 static constexpr uint32_t E_CM = 66, E_CID = 66 + 1024, E_YID = 66 + 1024 + 128;
 static constexpr uint32_t LDS_ENTRY_BYTES = 3 * 8 + 4;
-static constexpr uint32_t COV_WAVES = 8;
+static constexpr uint32_t COV_WAVES = 12;
 static constexpr uint32_t TILE = 1024;                     // reads per workgroup tile (large batches)
 static constexpr uint32_t TILE_MIN = 128;                  // small batches: more, smaller tiles
 static constexpr uint32_t STAGE_BYTES = COV_WAVES * 256;   // per-wave copies of the window's read bases
@@ -798,9 +798,11 @@ __device__ __forceinline__ void cov_read_wave(const CovArgs &a, const LdsAcc &L,
     __builtin_amdgcn_wave_barrier();
 }
 
-// one workgroup per (read group, tile of a.tile reads of that group)
+// one workgroup per (read group, tile of a.tile reads of that group).  Two 12-wave workgroups share a
+// CU (their LDS fits twice in the CU's 160 KB), six waves per SIMD, for which the registers are held
+// to 80 per lane; eight waves per workgroup left four waves per SIMD, sixteen spilled.
 template <bool PAIRS>
-__global__ void __launch_bounds__(64 * COV_WAVES) k_cov(CovArgs a)
+__global__ void __launch_bounds__(64 * COV_WAVES) __attribute__((amdgpu_waves_per_eu(6, 8))) k_cov(CovArgs a)
 {
     extern __shared__ unsigned long long lds_raw[];
     __shared__ uint32_t pr_used[3], pr_bound, pr_over;
