@@ -74,6 +74,23 @@ struct EventCursor {
         advance_op();
         return true;
     }
+    // step forward to the first event at or after the cursor whose reference coordinate is at least
+    // fs (events without one skipped), a whole op at a time; false (cursor on the last op) if none
+    __device__ bool seek_ref(int fs)
+    {
+        for (;;) {
+            const uint32_t c = code(), l = len();
+            if (op_refs(c) && (int)fo0 + (int)l - 1 >= fs) {
+                const int kk = fs - (int)fo0;
+                if (kk > (int)k) k = (uint32_t)kk;
+                return true;
+            }
+            int32_t j = oi + 1;
+            while (j < (int32_t)nops && op_len(ops[j]) == 0) j++;
+            if (j >= (int32_t)nops) return false;
+            advance_op();
+        }
+    }
     // step to the previous event; false (cursor unchanged) at the first event
     __device__ bool prev()
     {
@@ -313,18 +330,33 @@ __device__ __forceinline__ Windows prep_windows(const BatchDev &b, uint32_t r)
         }
     }
     // ---- no-insertions window (cigar.cu:392-451)
+    // the first event with a read coordinate of at least rwx that is not an insertion, and the last
+    // with one of at most rwy: inside an op the coordinate moves by one per event, so whole ops are
+    // passed over and the event is found by arithmetic in the op that holds it
     uint16_t nix = 0, niy = 0;
     if (cur.first(ops, nops)) {
         for (;;) {
-            const uint16_t ro = cur.rc();
-            if (!(ro == NONE16 || ro < rwx || cur.type() == EV_I)) { nix = ro; break; }
-            if (!cur.next()) break;
+            const uint32_t cd = cur.code(), l = cur.len();
+            if (op_reads(cd) && op_event(cd) != EV_I && (int)cur.ro0 + (int)l - 1 >= (int)rwx) {
+                const int kk = (int)rwx - (int)cur.ro0;
+                nix = (uint16_t)(cur.ro0 + (kk > 0 ? (uint32_t)kk : 0u));
+                break;
+            }
+            int32_t j = cur.oi + 1;
+            while (j < (int32_t)nops && op_len(ops[j]) == 0) j++;
+            if (j >= (int32_t)nops) break;
+            cur.advance_op();
         }
     }
     if (cur.last(ops, nops)) {
         for (;;) {
-            const uint16_t ro = cur.rc();
-            if (!(ro == NONE16 || ro > rwy || cur.type() == EV_I)) { niy = ro; break; }
+            const uint32_t cd = cur.code();
+            if (op_reads(cd) && op_event(cd) != EV_I && (int)cur.ro0 <= (int)rwy) {
+                const int kk = (int)rwy - (int)cur.ro0;
+                niy = (uint16_t)(cur.ro0 + (kk < (int)cur.k ? (uint32_t)kk : cur.k));
+                break;
+            }
+            cur.k = 0;   // prev() from the op's first event moves to the previous op's last
             if (!cur.prev()) break;
         }
     }
@@ -455,15 +487,9 @@ __device__ __forceinline__ void known_sites(const PrepArgs &a, uint32_t r, const
                     const int fs = (int)(fstart[f] - astart);
                     const int fe = (int)(fstop[f] - astart);
                     EventCursor ev;
-                    if (!ev.first(ops, nops)) continue;
-                    bool found = false;
-                    uint16_t refc = NONE16;
-                    for (;;) {
-                        refc = ev.fc();
-                        if (refc != NONE16 && (int)refc >= fs) { found = true; break; }
-                        if (!ev.next()) break;
-                    }
-                    if (!found) continue;
+                    // the first event whose reference coordinate reaches the feature
+                    if (!ev.first(ops, nops) || !ev.seek_ref(fs)) continue;
+                    const uint16_t refc = ev.fc();
                     int left = (int)refc - fs;
                     EventCursor es = ev;
                     bool es_end = false;   // es one past the last event
