@@ -129,3 +129,61 @@ def test_long_cigars_batched_engine(tmp_path):
     e.postprocess()
     compare_final(o, e, tmp_path)
     e.close()
+
+
+def layout_read(contig, rng):
+    """A short read with a random layout of soft clips, insertions and deletions around one long
+    M run: clips of 0-3 bases at either end, then 1-4 ops of 1-3 bases drawn from M, I and D on
+    each side of the run, so insertions and deletions sit next to the clips and to each other."""
+    ops = []
+    lead = int(rng.integers(0, 4))
+    if lead:
+        ops.append(("S", lead))
+    for _ in range(int(rng.integers(1, 5))):
+        ops.append((str(rng.choice(list("MID"))), int(rng.integers(1, 4))))
+    ops.append(("M", int(rng.integers(8, 30))))
+    for _ in range(int(rng.integers(1, 5))):
+        ops.append((str(rng.choice(list("MID"))), int(rng.integers(1, 4))))
+    trail = int(rng.integers(0, 4))
+    if trail:
+        ops.append(("S", trail))
+    span = sum(n for op, n in ops if op in "MD")
+    start = int(rng.integers(20, len(contig) - span - 20))
+    seq, pos = [], start
+    for op, n in ops:
+        if op == "M":
+            for _ in range(n):
+                b = contig[pos]
+                seq.append(OTHER[b] if rng.random() < 0.05 else b)
+                pos += 1
+        elif op == "D":
+            pos += n
+        else:
+            seq += list(rng.choice(list("ACGT"), size=n))
+    length = len(seq)
+    qual = [int(q) for q in rng.integers(5, 41, size=length)]
+    return {"seq": "".join(seq), "qual": qual, "cigar": "".join(f"{n}{op}" for op, n in ops), "start": start,
+            "flags": int(rng.choice([0, 16])), "rg": int(rng.integers(0, 2))}
+
+
+def test_short_cigar_layouts_stage_by_stage(tmp_path):
+    """Windows, known-site masks and error bits of reads whose clips, insertions and deletions sit
+    side by side (the walks that place an event inside its CIGAR op by arithmetic), with a known
+    site at every third reference position, plain engine vs oracle stage by stage."""
+    say("short CIGAR layouts around clips and indels, dense known sites, plain engine vs oracle")
+    rng = np.random.default_rng(41)
+    contig = random_genome_string(rng, 4000)
+    sites = [(0, p, p) for p in range(10, 3990, 3)]
+    g = HandGenome([contig], sites=sites, read_groups=("rg0", "rg1"))
+    o, e = Oracle(g), Engine(g)
+    for i in range(3):
+        b = HandBatch([layout_read(contig, rng) for _ in range(400)])
+        say(f"  batch {i}: {b.view.num_reads} reads")
+        o.process(b)
+        e.process(b)
+        compare_batch(o, e, b)
+    compare_obs_tables(o, e)
+    o.postprocess()
+    e.postprocess()
+    compare_final(o, e, tmp_path)
+    e.close()
