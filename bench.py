@@ -471,8 +471,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=400000, help="reads of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-reads", type=int, default=1 << 21,
-                    help="reads per staging buffer of the batched passes (0 = the engine's default, 2^20); "
-                         "the per-kernel profile and roofline passes keep the default")
+                    help="reads per staging buffer of the HBM-resident, mixed-length and sort batched passes "
+                         "(0 = the engine's default, 2^20); the per-kernel profile and roofline passes keep "
+                         "the default")
+    ap.add_argument("--host-stage-reads", type=int, default=1 << 20,
+                    help="reads per staging buffer of the host-fed batched pass (the value): the staging size "
+                         "at which it measured fastest (0 = --stage-reads)")
     ap.add_argument("--staging-buffers", type=int, default=0,
                     help="staging buffers of the batched passes, 2 or 3 (0 = the engine's default, 3)")
     ap.add_argument("--wpe", type=int, default=0, help="BAQ waves per SIMD (0 = autotuned)")
@@ -553,8 +557,11 @@ def main():
     log(f"BAQ variant: {wpe} waves per SIMD, checkpoint rows {ckpt or 1}")
 
     # value: host-fed batched engine, exactly --steps steps
+    host_opts = dict(batched_opts)
+    if args.host_stage_reads:
+        host_opts.update(max_batch_reads=args.host_stage_reads)
     dt, st, reads = stepped_pass(genome, pin, args.steps, args.warmup, torch, True, per_step,
-                                 baq_waves_per_simd=wpe, baq_checkpoint_rows=ckpt, **batched_opts)
+                                 baq_waves_per_simd=wpe, baq_checkpoint_rows=ckpt, **host_opts)
     value = reads / dt
     log(f"batched, host batches: {reads} reads in {dt:.3f} s -> {value:.4g} reads/s")
     ddt, _, dreads = stepped_pass(genome, dev, args.steps, args.warmup, torch, False, per_step,
@@ -598,12 +605,13 @@ def main():
         "config": {"workload": "30 Mbp / 3 contigs, 10M SE 101 bp reads, 4 read groups, 30k known sites, "
                                "BAQ band 7, input batches of 20,000 reads", "reads": reads, "batch_reads": B,
                    "step_reads": per_step * B,
-                   "engine": f"batched ({(args.stage_reads or (1 << 20)) / 2**20:g}M-read staging buffers, hipGraph "
-                             "replays), host batches through fp_engine_process_batch; per-kernel profile and "
-                             "roofline at 1M-read launches", "baq_waves_per_simd": wpe,
+                   "engine": f"batched ({(args.host_stage_reads or args.stage_reads or (1 << 20)) / 2**20:g}M-read "
+                             "staging buffers, hipGraph replays), host batches through fp_engine_process_batch; "
+                             "per-kernel profile and roofline at 1M-read launches", "baq_waves_per_simd": wpe,
                    "baq_checkpoint_rows": ckpt or 1},
         "device_resident": {"value": dreads / ddt, "seconds": ddt,
-                            "note": "the same steps with the batches already in HBM (no PCIe)"},
+                            "note": f"the same steps with the batches already in HBM (no PCIe), "
+                                    f"{(args.stage_reads or (1 << 20)) / 2**20:g}M-read staging buffers"},
         "kernel_ms": kernel_ms(pst),
         "roofline": roof,
         "end_to_end": end_to_end(value, reads, bytes_per_read, cells_per_read, h2d),
